@@ -1,0 +1,144 @@
+/*
+ * eac_hip.h -- C ABI of libeac_hip.so, the MI355X (gfx950) implementation of
+ * the evidence-accumulation-clustering (EAC) hot path of diogo-aos/masters_final.
+ *
+ * Conventions (every entry point):
+ *   - returns int status: EAC_OK (0) or a negative EAC_ERR_* code; the message
+ *     is available from eac_last_error() (thread-local, valid until the next
+ *     failing call on the same host thread). No C++ exception crosses the ABI.
+ *   - all array arguments are DEVICE pointers owned by the caller, unless the
+ *     parameter name starts with h_ (host pointer).
+ *   - scratch memory is a caller-provided device workspace of at least the
+ *     size returned by the matching *_workspace_bytes() query (no allocation
+ *     happens inside the library).
+ *   - work is enqueued on `stream` (a hipStream_t; NULL = default stream).
+ *     Entry points that return host scalars (h_*) synchronise that stream.
+ *   - vertex ids are int32, CSR offsets int64 (the reference uses int32
+ *     offsets; its Python shims widen them).
+ *
+ * The Python reference passes numpy / Numba device arrays; the shims in
+ * masters_final_amd/ keep its function names and signatures and call these.
+ */
+#ifndef EAC_HIP_H
+#define EAC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define EAC_OK 0
+#define EAC_ERR_ARG (-1)        /* bad argument (size, null pointer, dtype)   */
+#define EAC_ERR_HIP (-2)        /* HIP runtime error                          */
+#define EAC_ERR_WORKSPACE (-3)  /* workspace smaller than the query returned  */
+#define EAC_ERR_KEY_WIDTH (-4)  /* weight range + vertex ids exceed 64 bits   */
+#define EAC_ERR_NOT_FOUND (-5)  /* an edge id is not inside the input graph   */
+#define EAC_ERR_UNSUPPORTED (-6)
+#define EAC_ERR_VALUE (-7)      /* the reference would raise ValueError       */
+
+/* ---- weight dtypes ----------------------------------------------------- */
+#define EAC_W_I32 0
+#define EAC_W_F32 1
+#define EAC_W_U16 2
+#define EAC_W_U8 3
+#define EAC_W_I64 4
+#define EAC_W_I16 5
+#define EAC_W_U32 6
+
+const char* eac_last_error(void);
+int eac_abi_version(void);
+
+/* ---- kernel profiling (no counterpart in the reference) -----------------
+ * When enabled, the dominant kernels are bracketed by HIP events on the
+ * stream they run on; eac_profile_read returns their summed duration, the
+ * number of launches and the algorithmic bytes those launches had to move
+ * (the per-unit byte figures are documented with each slot). Enabling
+ * resets the counters. Host-thread-local use only. */
+#define EAC_PROF_SLOTS 16
+#define EAC_PROF_MST_ROUND0 0 /* CSR stream: 8 B/entry + 16 B/vertex + 8 B/edge out */
+#define EAC_PROF_MST_ROUND 1  /* list pass: 8 B/edge in + 8 B/edge out + 4 B/vertex */
+#define EAC_PROF_MST_HOOK 2   /* 12 B per representative */
+#define EAC_PROF_MST_JUMP 3   /* 8 B/vertex read + 8 B/representative reset */
+#define EAC_PROF_MST_FINAL 4  /* forest output + labels */
+#define EAC_PROF_CC 5         /* union-find over the CSR: 4 B/entry + 8 B/vertex */
+int eac_profile_enable(int32_t on);
+int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t* alg_bytes);
+
+/* ---- exclusive prefix sum ----------------------------------------------
+ * out[i] = init + sum(in[0:i]); *d_total (device, may be NULL) = init + sum(in).
+ * in == out is allowed (in place).
+ * Replaces MyML/MyML/utils/scan.py:144-249 scan_gpu (in-place Blelloch scan
+ * returning the device total) and the CPU scans scan.py:41-53
+ * exprefixsumNumba / scan.py:56-72 exprefixsumNumbaSingle.
+ */
+size_t eac_scan_workspace_bytes(int64_t n, int32_t elem_bytes);
+int eac_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
+                           int32_t init, int32_t* d_total, void* ws,
+                           size_t ws_bytes, void* stream);
+int eac_scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n,
+                           int64_t init, int64_t* d_total, void* ws,
+                           size_t ws_bytes, void* stream);
+/* int32 counts -> int64 offsets (row offsets of CSRs with nnz > 2^31). */
+int eac_scan_exclusive_i32_i64(const int32_t* in, int64_t* out, int64_t n,
+                               int64_t init, int64_t* d_total, void* ws,
+                               size_t ws_bytes, void* stream);
+
+/* ---- Boruvka minimum spanning forest -----------------------------------
+ * Graph in the reference's CSR form: for vertex v its entries are
+ * [first_edge[v], first_edge[v] + out_degree[v]) of dest/weight
+ * (MyML/MyML/graph/mst.py:143 boruvka_minho_seq(dest, weight, firstEdge,
+ * outDegree)). first_edge must be non-decreasing. out_degree may be NULL, in
+ * which case first_edge holds n+1 offsets.
+ * symmetric = 1: every undirected edge {u,v} is stored in both rows (the
+ *   reference's Boruvka input); only the entries with dest > row are read
+ *   as edges.
+ * symmetric = 0: every entry u != v is an undirected edge (e.g. the
+ *   condensed upper-triangular co-association of MyML/MyML/EAC/sparse.py).
+ * Ties are broken by the strict total order (weight, min(u,v), max(u,v)), so
+ * the forest is unique.
+ * Outputs (device, each may be NULL):
+ *   mst_edge_ids[n-1]: CSR position of each forest edge, taken in the row of
+ *     its smaller endpoint (replaces the `mst` array of mst.py:143-232 and
+ *     mst.py:417-564 boruvka_minho_gpu);
+ *   mst_weight[n-1]: weight of each forest edge, in the input dtype;
+ *   labels[n]: connected-component labels, numbered in order of first
+ *     occurrence (vertex 0 gets label 0);
+ *   h_out[4] (host): n_mst, n_components, boruvka rounds, max_weight_key_bits.
+ */
+size_t eac_boruvka_workspace_bytes(int32_t n, int64_t m);
+int eac_boruvka_mst(const int32_t* dest, const void* weight,
+                    int32_t weight_dtype, const int64_t* first_edge,
+                    const int32_t* out_degree, int32_t n, int64_t m,
+                    int32_t symmetric, int64_t* mst_edge_ids,
+                    void* mst_weight, int32_t* labels, int64_t* h_out,
+                    void* ws, size_t ws_bytes, void* stream);
+
+/* ---- connected components ----------------------------------------------
+ * labels[v] = component of v numbered in first-occurrence order. Replaces
+ * MyML/MyML/graph/connected_components.py:46-137 connected_comps_seq and
+ * :176-314 connected_comps_gpu (whose labels are supervertex ids; they equal
+ * ours after first-occurrence relabelling). Weights are not needed.
+ * h_ncomp (host, may be NULL) receives the number of components.
+ */
+size_t eac_cc_workspace_bytes(int32_t n, int64_t m);
+int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
+                             const int32_t* out_degree, int32_t n, int64_t m,
+                             int32_t* labels, int64_t* h_ncomp, void* ws,
+                             size_t ws_bytes, void* stream);
+
+/* ---- first-occurrence relabelling --------------------------------------
+ * out[v] = rank of labels_in[v] among distinct values ordered by first
+ * occurrence. Requires 0 <= labels_in[v] < n_values.
+ */
+size_t eac_relabel_workspace_bytes(int32_t n, int32_t n_values);
+int eac_canonical_relabel(const int32_t* labels_in, int32_t n,
+                          int32_t n_values, int32_t* out, int64_t* h_count,
+                          void* ws, size_t ws_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EAC_HIP_H */

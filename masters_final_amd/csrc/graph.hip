@@ -1,0 +1,1049 @@
+// Boruvka minimum spanning forest, connected components and first-occurrence
+// relabelling for gfx950.
+//
+// Reference algorithm: MyML/MyML/graph/mst.py:143-407 (boruvka_minho_seq and
+// its Numba helpers) and mst.py:417-809 (boruvka_minho_gpu and its kernels),
+// after Sousa et al. Each round every supervertex picks its lightest edge,
+// mirrored picks are resolved by keeping the smaller id as representative
+// (mst.py:282-305), the picks are added to the forest (mst.py:308-317), and
+// supervertices are merged by pointer jumping (mst.py:333-351).
+//
+// MI355X design (not a translation of the reference kernels):
+//  * The reference rebuilds a contracted CSR every round with atomic
+//    counters (countNewEdges/assignInsert, mst.py:755-809), which costs a
+//    count pass, a scan and an atomic scatter per round and leaves the edge
+//    order nondeterministic. Here the graph is kept as a flat list of
+//    undirected edges, each one 64-bit key
+//        key = wkey << 2b | min(u,v) << b | max(u,v)      (b = bits(n-1))
+//    where wkey is the weight mapped to an order-preserving unsigned integer
+//    minus the smallest such value. The key is unique per undirected edge,
+//    so (weight, min-id, max-id) is a strict total order: the forest is
+//    unique and no cycle longer than 2 can form. One coalesced 8-byte read
+//    per surviving edge per round serves both min-edge selection and
+//    filtering; endpoints are decoded from the key.
+//  * Round 0 streams the CSR itself in balanced 2048-entry chunks (row
+//    offsets of the chunk staged in LDS), so vertex degree skew does not
+//    unbalance waves. For a symmetric CSR the per-vertex minimum is an LDS
+//    segmented min with no global atomics for rows inside the chunk.
+//  * Later rounds: per surviving edge, look up both endpoint components and
+//    atomicMin the key into each component's slot (agent-scope 64-bit
+//    atomics, skipped when a plain read already shows a smaller key), and
+//    compact the cross-component edges into the next list (one atomic per
+//    block; list order is irrelevant because atomicMin over unique keys is
+//    order independent, so every output is deterministic).
+//  * Hooking writes a separate hook[] array; a pointer-jumping pass then
+//    updates every vertex's component and resets the per-root minima.
+//  * Component labels use a lock-free union-find (hook larger root under
+//    smaller root with atomicCAS, path halving) over the forest edges, so
+//    every root is its component's smallest vertex and first-occurrence
+//    labels follow from one scan of root flags.
+#include <climits>
+#include <cstring>
+
+#include "prims.hpp"
+
+namespace eac {
+namespace {
+
+constexpr uint64_t kInf = ~0ull;
+constexpr int kGB = 256;             // threads per block
+constexpr int kGI = 8;               // edges per thread per chunk
+constexpr int kGTile = kGB * kGI;    // entries per chunk
+constexpr int kMaxRows = 1024;       // rows staged in LDS per chunk
+
+// ------------------------------------------------------------ weight order
+// Map a weight to an unsigned integer whose order matches the numeric order
+// of the weight (the reference compares weights with < and ==).
+template <typename W>
+__device__ __forceinline__ uint64_t word_of(W w);
+template <>
+__device__ __forceinline__ uint64_t word_of<int32_t>(int32_t w) {
+  return (uint64_t)(int64_t)w ^ (1ull << 63);
+}
+template <>
+__device__ __forceinline__ uint64_t word_of<int64_t>(int64_t w) {
+  return (uint64_t)w ^ (1ull << 63);
+}
+template <>
+__device__ __forceinline__ uint64_t word_of<int16_t>(int16_t w) {
+  return (uint64_t)(int64_t)w ^ (1ull << 63);
+}
+template <>
+__device__ __forceinline__ uint64_t word_of<uint8_t>(uint8_t w) { return w; }
+template <>
+__device__ __forceinline__ uint64_t word_of<uint16_t>(uint16_t w) { return w; }
+template <>
+__device__ __forceinline__ uint64_t word_of<uint32_t>(uint32_t w) { return w; }
+template <>
+__device__ __forceinline__ uint64_t word_of<float>(float w) {
+  uint32_t u = __float_as_uint(w == 0.0f ? 0.0f : w);  // -0 == +0
+  u = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+  return u;
+}
+
+__device__ __forceinline__ uint64_t pack_key(uint64_t wk, uint32_t u, uint32_t v, int vb) {
+  uint32_t lo = u < v ? u : v, hi = u < v ? v : u;
+  return (wk << (2 * vb)) | ((uint64_t)lo << vb) | (uint64_t)hi;
+}
+__device__ __forceinline__ void unpack_ends(uint64_t key, int vb, int32_t* lo, int32_t* hi) {
+  uint64_t mask = (1ull << vb) - 1;
+  *hi = (int32_t)(key & mask);
+  *lo = (int32_t)((key >> vb) & mask);
+}
+
+// ------------------------------------------------------- chunk row mapping
+// chunk_row[c] = row holding entry c*kGTile (the last row whose first entry
+// is <= it), for c in [0, nchunks]; entries of chunk c belong to rows
+// chunk_row[c] .. chunk_row[c+1].
+__global__ void k_chunk_rows(const int64_t* __restrict__ fe, int32_t n, int64_t m,
+                             int64_t nchunks, int32_t* __restrict__ chunk_row) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nchunks) return;
+  int64_t e = c * kGTile;
+  if (e > m - 1) e = m - 1;
+  int32_t lo = 0, hi = n;  // upper_bound over fe[0:n]
+  while (lo < hi) {
+    int32_t mid = lo + ((hi - lo) >> 1);
+    if (fe[mid] <= e) lo = mid + 1; else hi = mid;
+  }
+  int32_t r = lo - 1;
+  chunk_row[c] = r < 0 ? 0 : r;
+}
+
+struct ChunkCtx {
+  int64_t e0, e1;
+  int32_t r_lo, r_cnt;
+  bool staged;
+};
+
+// Stage row bounds of one chunk in LDS (offsets relative to e0).
+template <bool HAS_OD>
+__device__ __forceinline__ ChunkCtx stage_chunk(int64_t c, int64_t m, const int64_t* fe,
+                                                const int32_t* od, int32_t n,
+                                                const int32_t* chunk_row, int32_t* s_beg,
+                                                int32_t* s_end) {
+  ChunkCtx cx;
+  cx.e0 = c * kGTile;
+  cx.e1 = cx.e0 + kGTile < m ? cx.e0 + kGTile : m;
+  cx.r_lo = chunk_row[c];
+  int32_t r_hi = chunk_row[c + 1];
+  cx.r_cnt = r_hi - cx.r_lo + 1;
+  cx.staged = cx.r_cnt <= kMaxRows;
+  if (cx.staged) {
+    for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) {
+      int32_t r = cx.r_lo + i;
+      int64_t b = fe[r];
+      int64_t e = HAS_OD ? b + od[r] : fe[r + 1];
+      s_beg[i] = (int32_t)(b - cx.e0);
+      s_end[i] = (int32_t)(e - cx.e0);
+    }
+  }
+  return cx;
+}
+
+// Row (index into the staged rows) of entry e, or -1 if e lies in a gap.
+template <bool HAS_OD>
+__device__ __forceinline__ int32_t row_of(const ChunkCtx& cx, int64_t e, const int64_t* fe,
+                                          const int32_t* od, const int32_t* s_beg,
+                                          const int32_t* s_end) {
+  if (cx.staged) {
+    int32_t rel = (int32_t)(e - cx.e0);
+    int32_t lo = 0, hi = cx.r_cnt;
+    while (lo < hi) {
+      int32_t mid = (lo + hi) >> 1;
+      if (s_beg[mid] <= rel) lo = mid + 1; else hi = mid;
+    }
+    int32_t i = lo - 1;
+    if (i < 0 || rel >= s_end[i]) return -1;
+    return i;
+  }
+  int32_t lo = cx.r_lo, hi = cx.r_lo + cx.r_cnt;
+  while (lo < hi) {
+    int32_t mid = lo + ((hi - lo) >> 1);
+    if (fe[mid] <= e) lo = mid + 1; else hi = mid;
+  }
+  int32_t r = lo - 1;
+  if (r < 0) return -1;
+  int64_t end = HAS_OD ? fe[r] + od[r] : fe[r + 1];
+  if (e >= end) return -1;
+  return r - cx.r_lo;
+}
+
+// Reserve `cnt` output slots per thread with one atomic per block. Returns
+// this thread's offset inside the block's run; *s_base holds the run start.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t cnt, unsigned long long* counter,
+                                                  uint32_t* s_tmp,
+                                                  unsigned long long* s_base,
+                                                  uint32_t* total_out) {
+  uint32_t total;
+  uint32_t pre = block_excl_scan<uint32_t>(cnt, s_tmp, &total);
+  if (threadIdx.x == 0) *s_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+  __syncthreads();
+  *total_out = total;
+  return pre;
+}
+
+// ----------------------------------------------------- weight range pass
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weight,
+                                                      const int64_t* __restrict__ fe,
+                                                      const int32_t* __restrict__ od,
+                                                      int32_t n, int64_t m,
+                                                      const int32_t* __restrict__ chunk_row,
+                                                      int64_t nchunks,
+                                                      unsigned long long* range) {
+  __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
+  __shared__ uint64_t s_mm[2 * (kGB / kWave)];
+  uint64_t lo = kInf, hi = 0;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      int64_t e = cx.e0 + j * kGB + threadIdx.x;
+      if (e < cx.e1 && row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end) >= 0) {
+        uint64_t w = word_of<W>(weight[e]);
+        lo = w < lo ? w : lo;
+        hi = w > hi ? w : hi;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    uint64_t a = __shfl_xor(lo, off, kWave), b = __shfl_xor(hi, off, kWave);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  int w = threadIdx.x / kWave;
+  if (lane_id() == 0) { s_mm[2 * w] = lo; s_mm[2 * w + 1] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kGB / kWave; ++i) {
+      lo = s_mm[2 * i] < lo ? s_mm[2 * i] : lo;
+      hi = s_mm[2 * i + 1] > hi ? s_mm[2 * i + 1] : hi;
+    }
+    if (lo != kInf) {
+      atomicMin(&range[0], (unsigned long long)lo);
+      atomicMax(&range[1], (unsigned long long)hi);
+    }
+  }
+}
+
+// -------------------------------------------------------- Boruvka round 0
+// Streams the CSR: builds the undirected edge list and every vertex's
+// lightest edge key.
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_round0(
+    const int32_t* __restrict__ dest, const W* __restrict__ weight,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int32_t n, int64_t m,
+    const int32_t* __restrict__ chunk_row, int64_t nchunks, int vb, uint64_t word_lo,
+    int symmetric, unsigned long long* __restrict__ best, uint64_t* __restrict__ list,
+    unsigned long long* __restrict__ list_count, int64_t list_cap,
+    unsigned int* __restrict__ overflow) {
+  __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
+  __shared__ unsigned long long s_best[kMaxRows];
+  __shared__ uint64_t s_out[kGTile];
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  __shared__ unsigned long long s_base;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+    if (cx.staged)
+      for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) s_best[i] = kInf;
+    __syncthreads();
+    uint64_t keys[kGI];
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      keys[j] = kInf;
+      int64_t e = cx.e0 + j * kGB + threadIdx.x;
+      if (e >= cx.e1) continue;
+      int32_t ri = row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end);
+      if (ri < 0) continue;
+      int32_t u = cx.r_lo + ri;
+      int32_t v = dest[e];
+      if (v < 0 || v >= n) {  // invalid id: reported, never dereferenced
+        atomicOr(overflow, 2u);
+        continue;
+      }
+      if (v == u) continue;  // self loops never join a forest
+      uint64_t key = pack_key(word_of<W>(weight[e]) - word_lo, (uint32_t)u, (uint32_t)v, vb);
+      keys[j] = key;
+      if (cx.staged) {
+        atomicMin(&s_best[ri], (unsigned long long)key);
+      } else if (key < best[u]) {
+        atomicMin(&best[u], (unsigned long long)key);
+      }
+      if (!symmetric) {
+        if (key < best[v]) atomicMin(&best[v], (unsigned long long)key);
+      }
+      if (!symmetric || v > u) keep |= 1u << j;
+    }
+    uint32_t total;
+    uint32_t pre = block_reserve(__popc(keep), list_count, s_tmp, &s_base, &total);
+#pragma unroll
+    for (int j = 0; j < kGI; ++j)
+      if (keep & (1u << j)) s_out[pre++] = keys[j];
+    __syncthreads();
+    unsigned long long base = s_base;
+    if (base + total > (unsigned long long)list_cap) {
+      if (threadIdx.x == 0) atomicOr(overflow, 1u);
+    } else {
+      for (uint32_t i = threadIdx.x; i < total; i += kGB) list[base + i] = s_out[i];
+    }
+    if (cx.staged) {
+      for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) {
+        unsigned long long b = s_best[i];
+        if (b == kInf) continue;
+        int32_t r = cx.r_lo + i;
+        bool inside = s_beg[i] >= 0 && s_end[i] <= (int32_t)(cx.e1 - cx.e0);
+        if (inside && symmetric) best[r] = b;
+        else atomicMin(&best[r], b);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------ Boruvka round >= 1
+__global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list_in,
+                                               int64_t m_in, uint64_t* __restrict__ list_out,
+                                               unsigned long long* __restrict__ out_count,
+                                               const int32_t* __restrict__ comp,
+                                               unsigned long long* __restrict__ best, int vb) {
+  __shared__ uint64_t s_out[kGTile];
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  __shared__ unsigned long long s_base;
+  const int64_t nchunks = (m_in + kGTile - 1) / kGTile;
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    const int64_t e0 = c * kGTile;
+    uint64_t keys[kGI];
+    uint32_t keep = 0;
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      int64_t e = e0 + j * kGB + threadIdx.x;
+      keys[j] = e < m_in ? list_in[e] : kInf;
+    }
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      if (keys[j] == kInf) continue;
+      int32_t a, b;
+      unpack_ends(keys[j], vb, &a, &b);
+      int32_t ra = comp[a], rb = comp[b];
+      if (ra == rb) continue;
+      keep |= 1u << j;
+      if (keys[j] < best[ra]) atomicMin(&best[ra], (unsigned long long)keys[j]);
+      if (keys[j] < best[rb]) atomicMin(&best[rb], (unsigned long long)keys[j]);
+    }
+    uint32_t total;
+    uint32_t pre = block_reserve(__popc(keep), out_count, s_tmp, &s_base, &total);
+#pragma unroll
+    for (int j = 0; j < kGI; ++j)
+      if (keep & (1u << j)) s_out[pre++] = keys[j];
+    __syncthreads();
+    unsigned long long base = s_base;
+    for (uint32_t i = threadIdx.x; i < total; i += kGB) list_out[base + i] = s_out[i];
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ hooking
+// For each current representative r: follow its lightest edge to the other
+// component s. If s picked the same edge and r < s, r stays representative
+// (mst.py:282-305: the smaller id keeps no edge); otherwise r hooks under s
+// and the edge joins the forest (recorded in r's slot: every vertex hooks at
+// most once in its lifetime, so slots are written once).
+__global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots_in,
+                                              int64_t nroots,
+                                              const unsigned long long* __restrict__ best,
+                                              const int32_t* __restrict__ comp,
+                                              int32_t* __restrict__ hook,
+                                              uint64_t* __restrict__ mst_key,
+                                              int32_t* __restrict__ roots_out,
+                                              unsigned long long* __restrict__ roots_out_count,
+                                              unsigned long long* __restrict__ hooks_count,
+                                              int vb) {
+  __shared__ int32_t s_out[kGB];
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  __shared__ unsigned long long s_base;
+  __shared__ uint32_t s_hooks;
+  if (threadIdx.x == 0) s_hooks = 0;
+  __syncthreads();
+  for (int64_t base_i = (int64_t)blockIdx.x * kGB; base_i < nroots;
+       base_i += (int64_t)gridDim.x * kGB) {
+    int64_t i = base_i + threadIdx.x;
+    uint32_t stay = 0;
+    int32_t r = -1;
+    if (i < nroots) {
+      r = roots_in ? roots_in[i] : (int32_t)i;
+      uint64_t k = best[r];
+      if (k == kInf) {
+        hook[r] = r;  // component complete: no edge leaves it
+      } else {
+        int32_t a, b;
+        unpack_ends(k, vb, &a, &b);
+        int32_t ra = comp[a], rb = comp[b];
+        int32_t s = (ra == r) ? rb : ra;
+        if (best[s] == k && r < s) {
+          hook[r] = r;
+          stay = 1;
+        } else {
+          hook[r] = s;
+          mst_key[r] = k;
+          atomicAdd(&s_hooks, 1u);
+        }
+      }
+    }
+    uint32_t total;
+    uint32_t pre = block_reserve(stay, roots_out_count, s_tmp, &s_base, &total);
+    if (stay) s_out[pre] = r;
+    __syncthreads();
+    unsigned long long base = s_base;
+    for (uint32_t t = threadIdx.x; t < total; t += kGB) roots_out[base + t] = s_out[t];
+    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_hooks) atomicAdd(hooks_count, (unsigned long long)s_hooks);
+}
+
+// ---------------------------------------------------------- pointer jumping
+// Every vertex moves to the representative at the end of its component's
+// hook chain; representatives of this round reset their minimum.
+// A hook chain visits each vertex at most once, so a walk longer than n
+// means the hooks form a cycle, which happens only when a graph passed as
+// symmetric is not: the walk stops and the call reports the error.
+__global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ comp,
+                                              int32_t* hook,
+                                              unsigned long long* __restrict__ best,
+                                              unsigned int* __restrict__ overflow) {
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
+    int32_t x = comp[v];
+    int32_t y = x;
+    for (int32_t steps = 0;; ++steps) {
+      int32_t h = hook[y];
+      if (h == y) break;
+      if (steps > n) {
+        atomicOr(overflow, 4u);
+        break;
+      }
+      y = h;
+    }
+    if (y != x) {
+      comp[v] = y;
+      hook[x] = y;  // shortcut: any ancestor is a valid value
+    }
+    if (x == (int32_t)v) best[v] = kInf;
+  }
+}
+
+// --------------------------------------------------------- forest output
+// Tile-wise compaction of non-sentinel keys in index order (deterministic):
+// pass 1 counts per tile, a scan turns counts into offsets, pass 2 writes.
+__global__ __launch_bounds__(kGB) void k_count_keys(const uint64_t* __restrict__ keys,
+                                                    int64_t n, int32_t* __restrict__ tile_cnt) {
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  int64_t base = (int64_t)blockIdx.x * kGTile;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j) {
+    int64_t i = base + j * kGB + threadIdx.x;
+    if (i < n && keys[i] != kInf) ++c;
+  }
+  uint32_t total;
+  (void)block_excl_scan<uint32_t>(c, s_tmp, &total);
+  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = (int32_t)total;
+}
+
+__global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__ keys,
+                                                    int64_t n,
+                                                    const int32_t* __restrict__ tile_off,
+                                                    uint64_t* __restrict__ out) {
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  // blocked layout inside the tile keeps index order
+  int64_t base = (int64_t)blockIdx.x * kGTile + (int64_t)threadIdx.x * kGI;
+  uint64_t v[kGI];
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j) {
+    int64_t i = base + j;
+    v[j] = i < n ? keys[i] : kInf;
+    c += v[j] != kInf;
+  }
+  uint32_t total;
+  uint32_t pre = block_excl_scan<uint32_t>(c, s_tmp, &total);
+  int64_t o = (int64_t)tile_off[blockIdx.x] + pre;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j)
+    if (v[j] != kInf) out[o++] = v[j];
+}
+
+// CSR position (in the smaller endpoint's row) and weight of each forest
+// edge. Short rows are scanned; long rows are binary-searched by column and
+// verified, falling back to a scan when the row is not sorted.
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_forest_positions(
+    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ dest,
+    const W* __restrict__ weight, const int64_t* __restrict__ fe,
+    const int32_t* __restrict__ od, int vb, uint64_t word_lo, int64_t* __restrict__ ids,
+    W* __restrict__ wout, unsigned int* __restrict__ missing) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
+       i += (int64_t)gridDim.x * kGB) {
+    uint64_t k = fkeys[i];
+    int32_t a, b;
+    unpack_ends(k, vb, &a, &b);
+    uint64_t wk = k >> (2 * vb);
+    int64_t beg = fe[a];
+    int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
+    int64_t pos = -1;
+    if (end - beg > 32) {
+      int64_t lo = beg, hi = end;
+      while (lo < hi) {
+        int64_t mid = lo + ((hi - lo) >> 1);
+        if (dest[mid] < b) lo = mid + 1; else hi = mid;
+      }
+      for (int64_t p = lo; p < end && dest[p] == b; ++p)
+        if (word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
+    }
+    if (pos < 0) {
+      for (int64_t p = beg; p < end; ++p)
+        if (dest[p] == b && word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
+    }
+    if (pos < 0) {
+      atomicOr(missing, 1u);
+      pos = 0;
+    }
+    if (ids) ids[i] = pos;
+    if (wout) wout[i] = weight[pos];
+  }
+}
+
+// ------------------------------------------------------------- union-find
+__device__ __forceinline__ int32_t uf_ld(int32_t* p, int32_t i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void uf_st(int32_t* p, int32_t i, int32_t v) {
+  __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// parent[x] <= x always holds, so every walk terminates at the smallest
+// vertex of the component.
+__device__ int32_t uf_find(int32_t* P, int32_t x) {
+  for (;;) {
+    int32_t p = uf_ld(P, x);
+    if (p == x) return x;
+    int32_t gp = uf_ld(P, p);
+    if (gp == p) return p;
+    uf_st(P, x, gp);  // path halving
+    x = gp;
+  }
+}
+__device__ void uf_unite(int32_t* P, int32_t a, int32_t b) {
+  a = uf_find(P, a);
+  b = uf_find(P, b);
+  while (a != b) {
+    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    int32_t old = atomicCAS(P + hi, hi, lo);
+    if (old == hi) return;
+    a = uf_find(P, old);
+    b = uf_find(P, lo);
+  }
+}
+
+__global__ void k_iota(int32_t* p, int32_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(kGB) void k_unite_keys(const uint64_t* __restrict__ keys,
+                                                    int64_t nk, int vb, int32_t* P) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < nk;
+       i += (int64_t)gridDim.x * kGB) {
+    int32_t a, b;
+    unpack_ends(keys[i], vb, &a, &b);
+    uf_unite(P, a, b);
+  }
+}
+
+template <bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_unite_csr(const int32_t* __restrict__ dest,
+                                                   const int64_t* __restrict__ fe,
+                                                   const int32_t* __restrict__ od,
+                                                   int32_t n, int64_t m,
+                                                   const int32_t* __restrict__ chunk_row,
+                                                   int64_t nchunks, int32_t* P,
+                                                   unsigned int* bad) {
+  __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
+  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      int64_t e = cx.e0 + j * kGB + threadIdx.x;
+      if (e >= cx.e1) continue;
+      int32_t ri = row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end);
+      if (ri < 0) continue;
+      int32_t u = cx.r_lo + ri;
+      int32_t v = dest[e];
+      if (v < 0 || v >= n) {
+        atomicOr(bad, 1u);
+        continue;
+      }
+      if (v != u) uf_unite(P, u, v);
+    }
+    __syncthreads();
+  }
+}
+
+// Root of every vertex into a separate array (storing it back into P would
+// race with other threads' path-halving stores, which may overwrite it with
+// a non-root ancestor), plus root flags as int32 0/1 for the scan.
+__global__ void k_uf_flatten(int32_t* P, int32_t n, int32_t* __restrict__ root,
+                             int32_t* __restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t r = uf_find(P, (int32_t)i);
+    root[i] = r;
+    flag[i] = (r == (int32_t)i);
+  }
+}
+
+__global__ void k_gather_rank(const int32_t* __restrict__ root,
+                              const int32_t* __restrict__ rank, int32_t n,
+                              int32_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = rank[root[i]];
+}
+
+// First occurrence of each label value (atomicMin with a read-first filter so
+// large clusters settle after their first few waves).
+__global__ void k_first_occ(const int32_t* __restrict__ lab, int32_t n, int32_t nv,
+                            int32_t* first, unsigned int* bad) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = lab[i];
+    if (l < 0 || l >= nv) { atomicOr(bad, 1u); continue; }
+    if ((int32_t)i < first[l]) atomicMin(first + l, (int32_t)i);
+  }
+}
+__global__ void k_first_flag(const int32_t* __restrict__ lab, const int32_t* __restrict__ first,
+                             int32_t n, int32_t nv, int32_t* __restrict__ root,
+                             int32_t* __restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t l = lab[i];
+    int32_t f = (l >= 0 && l < nv) ? first[l] : (int32_t)i;
+    root[i] = f;
+    flag[i] = (f == (int32_t)i);
+  }
+}
+
+// ---------------------------------------------------------------- helpers
+inline int persistent_blocks() { return 256 * 8; }
+
+int d2h_sync(void* host, const void* dev, size_t bytes, hipStream_t s) {
+  void* pin = pinned_scratch(bytes);
+  if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+  EAC_HIP_CHECK(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  memcpy(host, pin, bytes);
+  return EAC_OK;
+}
+
+int weight_bytes(int32_t dt) {
+  switch (dt) {
+    case EAC_W_I32: case EAC_W_F32: case EAC_W_U32: return 4;
+    case EAC_W_U16: case EAC_W_I16: return 2;
+    case EAC_W_U8: return 1;
+    case EAC_W_I64: return 8;
+    default: return 0;
+  }
+}
+
+// Device state counters of one MST call.
+struct MstState {
+  unsigned long long range[2];
+  unsigned long long list_count[2];
+  unsigned long long root_count[2];
+  unsigned long long hooks;
+  unsigned int overflow;
+  unsigned int missing;
+};
+
+struct MstBuffers {
+  int32_t* chunk_row;
+  unsigned long long* best;
+  uint64_t* mst_key;
+  int32_t* comp;
+  int32_t* hook;
+  int32_t* roots[2];
+  uint64_t* list[2];
+  int64_t list_cap;
+  uint64_t* fkeys;
+  int32_t* tile_cnt;
+  int32_t* flag;
+  int32_t* rank;
+  MstState* st;
+};
+
+int64_t list_capacity(int64_t m, int symmetric) {
+  // For a symmetric CSR at most half of the entries have dest > row, plus
+  // slack for inputs that are not exactly symmetric (overflow is detected).
+  return symmetric ? m / 2 + 4096 : m;
+}
+
+MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
+  MstBuffers b;
+  int64_t nchunks = (m + kGTile - 1) / kGTile;
+  b.chunk_row = ar.take<int32_t>(nchunks + 1);
+  b.best = ar.take<unsigned long long>(n);
+  b.mst_key = ar.take<uint64_t>(n);
+  b.comp = ar.take<int32_t>(n);
+  b.hook = ar.take<int32_t>(n);
+  b.roots[0] = ar.take<int32_t>(n);
+  b.roots[1] = ar.take<int32_t>(n);
+  b.list_cap = list_capacity(m, symmetric);
+  b.list[0] = ar.take<uint64_t>(b.list_cap);
+  b.list[1] = ar.take<uint64_t>(b.list_cap);
+  b.fkeys = ar.take<uint64_t>(n);
+  b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 1);
+  b.flag = ar.take<int32_t>(n);
+  b.rank = ar.take<int32_t>(n);
+  b.st = ar.take<MstState>(1);
+  return b;
+}
+
+// union-find labels over a key list, numbered by first occurrence
+int labels_from_keys(const uint64_t* keys, int64_t nk, int vb, int32_t n, int32_t* parent,
+                     int32_t* root, int32_t* flag, int32_t* rank, int32_t* labels,
+                     int32_t* d_ncomp, Arena& ar, hipStream_t s) {
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n);
+  EAC_LAUNCH_CHECK();
+  if (nk > 0) {
+    k_unite_keys<<<grid_for(nk, kGB, persistent_blocks()), kGB, 0, s>>>(keys, nk, vb, parent);
+    EAC_LAUNCH_CHECK();
+  }
+  k_uf_flatten<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n, root, flag);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(flag, rank, n, 0, d_ncomp, ar, s)));
+  if (labels) {
+    k_gather_rank<<<grid_for(n, 256, 4096), 256, 0, s>>>(root, rank, n, labels);
+    EAC_LAUNCH_CHECK();
+  }
+  return EAC_OK;
+}
+
+template <typename W, bool HAS_OD>
+int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, int64_t m, int symmetric, int64_t* ids, W* wout, int32_t* labels,
+             int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  MstBuffers B = carve_mst(ar, n, m, symmetric);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
+  const int64_t nchunks = (m + kGTile - 1) / kGTile;
+  MstState init_st;
+  memset(&init_st, 0, sizeof(init_st));
+  init_st.range[0] = kInf;
+  // state init through a pinned copy (stream ordered)
+  {
+    void* pin = pinned_scratch(sizeof(MstState));
+    if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+    EAC_HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(pin, &init_st, sizeof(init_st));
+    EAC_HIP_CHECK(hipMemcpyAsync(B.st, pin, sizeof(MstState), hipMemcpyHostToDevice, s));
+  }
+  k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(fe, n, m, nchunks, B.chunk_row);
+  EAC_LAUNCH_CHECK();
+  const int gchunks = grid_for(nchunks, 1, persistent_blocks());
+  k_weight_range<W, HAS_OD><<<gchunks, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
+                                                     B.st->range);
+  EAC_LAUNCH_CHECK();
+  MstState hs;
+  EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
+  uint64_t wlo = hs.range[0], whi = hs.range[1];
+  if (wlo == kInf) { wlo = 0; whi = 0; }  // no entries
+  const int vb = bits_for((uint64_t)(n - 1)) < 1 ? 1 : bits_for((uint64_t)(n - 1));
+  const int wbits = bits_for(whi - wlo);
+  if (wbits + 2 * vb > 64)
+    return fail(EAC_ERR_KEY_WIDTH,
+                "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)", wbits, vb);
+
+  EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
+  EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);
+  EAC_LAUNCH_CHECK();
+
+  // round 0 from the CSR
+  const int64_t round0_bytes =
+      (int64_t)(4 + sizeof(W)) * m + 8 * ((int64_t)n + 1) + (HAS_OD ? 4 * (int64_t)n : 0) +
+      8 * (int64_t)n;
+  int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
+  k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
+                                               vb, wlo, symmetric, B.best, B.list[0],
+                                               &B.st->list_count[0], B.list_cap,
+                                               &B.st->overflow);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr, s);
+  int pending = pr;               // record whose output-list bytes are still unknown
+  int64_t pending_bytes = round0_bytes;
+  int cur = 0;
+  int64_t m_cur = 0, nroots = n;
+  const int32_t* roots_in = nullptr;  // round 0: every vertex
+  int rounds = 0;
+  for (;;) {
+    ++rounds;
+    pr = prof_begin(EAC_PROF_MST_HOOK, 12 * nroots, s);
+    k_hook<<<grid_for(nroots, kGB, persistent_blocks()), kGB, 0, s>>>(
+        roots_in, nroots, B.best, B.comp, B.hook, B.mst_key, B.roots[cur],
+        &B.st->root_count[cur], &B.st->hooks, vb);
+    EAC_LAUNCH_CHECK();
+    prof_end(pr, s);
+    pr = prof_begin(EAC_PROF_MST_JUMP, 8 * (int64_t)n + 8 * nroots, s);
+    k_jump<<<grid_for(n, kGB, persistent_blocks()), kGB, 0, s>>>(n, B.comp, B.hook, B.best,
+                                                                    &B.st->overflow);
+    EAC_LAUNCH_CHECK();
+    prof_end(pr, s);
+    EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
+    prof_set_bytes(pending, pending_bytes + 8 * (int64_t)hs.list_count[cur]);
+    if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+    if (hs.overflow & 4u)
+      return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
+                               "(pass symmetric=0)");
+    if (hs.overflow)
+      return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
+                               "pass symmetric=0");
+    m_cur = (int64_t)hs.list_count[cur];
+    nroots = (int64_t)hs.root_count[cur];
+    if (nroots == 0 || m_cur == 0 || rounds > 64) break;
+    // next round: filter list[cur] -> list[nxt], recompute minima
+    int nxt = cur ^ 1;
+    // reset the counters that the next round appends to
+    EAC_TRY(fill<uint64_t>((uint64_t*)&B.st->list_count[nxt], 1, 0ull, s));
+    EAC_TRY(fill<uint64_t>((uint64_t*)&B.st->root_count[nxt], 1, 0ull, s));
+    pending_bytes = 8 * m_cur + 4 * (int64_t)n;
+    pending = prof_begin(EAC_PROF_MST_ROUND, pending_bytes, s);
+    k_round<<<grid_for((m_cur + kGTile - 1) / kGTile, 1, persistent_blocks()), kGB, 0, s>>>(
+        B.list[cur], m_cur, B.list[nxt], &B.st->list_count[nxt], B.comp, B.best, vb);
+    EAC_LAUNCH_CHECK();
+    prof_end(pending, s);
+    roots_in = B.roots[cur];
+    cur = nxt;
+  }
+  if (rounds > 64) return fail(EAC_ERR_HIP, "boruvka: no convergence after 64 rounds");
+
+  // forest keys in vertex order
+  const int64_t ftiles = (n + kGTile - 1) / kGTile;
+  int32_t* d_nf = B.tile_cnt + ftiles;
+  k_count_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(B.tile_cnt, B.tile_cnt, ftiles, 0, d_nf, ar, s)));
+  k_write_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, B.fkeys);
+  EAC_LAUNCH_CHECK();
+  int32_t n_f = 0;
+  EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
+  if (n_f > 0 && (ids || wout)) {
+    k_forest_positions<W, HAS_OD><<<grid_for(n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
+        B.fkeys, n_f, dest, weight, fe, od, vb, wlo, ids, wout, &B.st->missing);
+    EAC_LAUNCH_CHECK();
+  }
+  int32_t n_comp = n - n_f;
+  pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n_f + 12 * (int64_t)n, s);
+  if (labels) {
+    // B.hook is free now: reuse it as the union-find parent array
+    // hook[] and comp[] are free after the rounds: union-find parent / roots
+    EAC_TRY(labels_from_keys(B.fkeys, n_f, vb, n, B.hook, B.comp, B.flag, B.rank, labels,
+                             B.tile_cnt + ftiles, ar, s));
+  }
+  prof_end(pr, s);
+  EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
+  prof_flush();
+  if (hs.missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
+  if (h_out) {
+    h_out[0] = n_f;
+    h_out[1] = n_comp;
+    h_out[2] = rounds;
+    h_out[3] = wbits;
+  }
+  return EAC_OK;
+}
+
+template <typename W>
+int mst_dispatch_od(const int32_t* dest, const void* weight, const int64_t* fe,
+                    const int32_t* od, int32_t n, int64_t m, int symmetric, int64_t* ids,
+                    void* wout, int32_t* labels, int64_t* h_out, void* ws, size_t ws_bytes,
+                    hipStream_t s) {
+  if (od)
+    return mst_impl<W, true>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
+                             labels, h_out, ws, ws_bytes, s);
+  return mst_impl<W, false>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
+                            labels, h_out, ws, ws_bytes, s);
+}
+
+}  // namespace
+
+size_t mst_ws_bytes(int32_t n, int64_t m) {
+  Arena ar(nullptr, 0, true);
+  (void)carve_mst(ar, n, m, 0);
+  return ar.used + scan_ws_bytes(n, 4) + 4096;
+}
+
+}  // namespace eac
+
+// ------------------------------------------------------------------ C ABI
+using namespace eac;
+
+extern "C" size_t eac_boruvka_workspace_bytes(int32_t n, int64_t m) {
+  return eac::mst_ws_bytes(n < 1 ? 1 : n, m < 1 ? 1 : m);
+}
+
+extern "C" int eac_boruvka_mst(const int32_t* dest, const void* weight, int32_t weight_dtype,
+                               const int64_t* first_edge, const int32_t* out_degree,
+                               int32_t n, int64_t m, int32_t symmetric,
+                               int64_t* mst_edge_ids, void* mst_weight, int32_t* labels,
+                               int64_t* h_out, void* ws, size_t ws_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0 || m < 0) return fail(EAC_ERR_ARG, "boruvka: negative size");
+  if (weight_bytes(weight_dtype) == 0)
+    return fail(EAC_ERR_ARG, "boruvka: unsupported weight dtype %d", weight_dtype);
+  if (n == 0) {
+    if (h_out) { h_out[0] = 0; h_out[1] = 0; h_out[2] = 0; h_out[3] = 0; }
+    return EAC_OK;
+  }
+  if (!first_edge || (m > 0 && (!dest || !weight)))
+    return fail(EAC_ERR_ARG, "boruvka: null graph array");
+  if (m == 0) {
+    if (labels) {
+      k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(labels, n);
+      EAC_LAUNCH_CHECK();
+    }
+    if (h_out) { h_out[0] = 0; h_out[1] = n; h_out[2] = 0; h_out[3] = 0; }
+    return EAC_OK;
+  }
+  size_t need = eac_boruvka_workspace_bytes(n, m);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "boruvka: workspace %zu < %zu bytes", ws_bytes, need);
+  symmetric = symmetric ? 1 : 0;
+  switch (weight_dtype) {
+    case EAC_W_I32: return mst_dispatch_od<int32_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_F32: return mst_dispatch_od<float>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_U16: return mst_dispatch_od<uint16_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_U8: return mst_dispatch_od<uint8_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_I64: return mst_dispatch_od<int64_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_I16: return mst_dispatch_od<int16_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+    case EAC_W_U32: return mst_dispatch_od<uint32_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
+  }
+  return fail(EAC_ERR_ARG, "boruvka: unsupported weight dtype");
+}
+
+// ------------------------------------------------- connected components
+extern "C" size_t eac_cc_workspace_bytes(int32_t n, int64_t m) {
+  int64_t nchunks = (m + kGTile - 1) / kGTile;
+  Arena ar(nullptr, 0, true);
+  ar.take<int32_t>(nchunks + 2);
+  ar.take<int32_t>(n > 0 ? n : 1);  // parent
+  ar.take<int32_t>(n > 0 ? n : 1);  // root
+  ar.take<int32_t>(n > 0 ? n : 1);  // flag
+  ar.take<int32_t>(n > 0 ? n : 1);  // rank
+  ar.take<int32_t>(4);
+  return ar.used + eac::scan_ws_bytes(n, 4) + 4096;
+}
+
+extern "C" int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
+                                        const int32_t* out_degree, int32_t n, int64_t m,
+                                        int32_t* labels, int64_t* h_ncomp, void* ws,
+                                        size_t ws_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0 || m < 0) return fail(EAC_ERR_ARG, "cc: negative size");
+  if (n == 0) {
+    if (h_ncomp) *h_ncomp = 0;
+    return EAC_OK;
+  }
+  if (!first_edge || !labels || (m > 0 && !dest)) return fail(EAC_ERR_ARG, "cc: null array");
+  size_t need = eac_cc_workspace_bytes(n, m);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "cc: workspace %zu < %zu bytes", ws_bytes, need);
+  Arena ar(ws, ws_bytes);
+  int64_t nchunks = (m + kGTile - 1) / kGTile;
+  int32_t* chunk_row = ar.take<int32_t>(nchunks + 2);
+  int32_t* parent = ar.take<int32_t>(n);
+  int32_t* root = ar.take<int32_t>(n);
+  int32_t* flag = ar.take<int32_t>(n);
+  int32_t* rank = ar.take<int32_t>(n);
+  int32_t* d_nc = ar.take<int32_t>(4);
+  EAC_TRY(fill<int32_t>(d_nc, 4, 0, s));
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n);
+  EAC_LAUNCH_CHECK();
+  if (m > 0) {
+    k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(first_edge, n, m, nchunks, chunk_row);
+    EAC_LAUNCH_CHECK();
+    int g = grid_for(nchunks, 1, persistent_blocks());
+    int pr = prof_begin(EAC_PROF_CC, 4 * m + 8 * (int64_t)n, s);
+    if (out_degree)
+      k_unite_csr<true><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
+                                          parent, (unsigned int*)(d_nc + 1));
+    else
+      k_unite_csr<false><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
+                                           parent, (unsigned int*)(d_nc + 1));
+    EAC_LAUNCH_CHECK();
+    prof_end(pr, s);
+  }
+  k_uf_flatten<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n, root, flag);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(flag, rank, n, 0, d_nc, ar, s)));
+  k_gather_rank<<<grid_for(n, 256, 4096), 256, 0, s>>>(root, rank, n, labels);
+  EAC_LAUNCH_CHECK();
+  int32_t hc[2] = {0, 0};
+  EAC_TRY(d2h_sync(hc, d_nc, sizeof(hc), s));
+  prof_flush();
+  if (hc[1]) return fail(EAC_ERR_ARG, "cc: dest holds ids outside [0, n)");
+  if (h_ncomp) *h_ncomp = hc[0];
+  return EAC_OK;
+}
+
+// ------------------------------------------------ first-occurrence relabel
+extern "C" size_t eac_relabel_workspace_bytes(int32_t n, int32_t n_values) {
+  Arena ar(nullptr, 0, true);
+  ar.take<int32_t>(n_values > 0 ? n_values : 1);
+  ar.take<int32_t>(n > 0 ? n : 1);
+  ar.take<int32_t>(n > 0 ? n : 1);
+  ar.take<int32_t>(n > 0 ? n : 1);
+  ar.take<uint32_t>(4);
+  return ar.used + eac::scan_ws_bytes(n, 4) + 4096;
+}
+
+extern "C" int eac_canonical_relabel(const int32_t* labels_in, int32_t n, int32_t n_values,
+                                     int32_t* out, int64_t* h_count, void* ws,
+                                     size_t ws_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 0 || n_values < 0) return fail(EAC_ERR_ARG, "relabel: negative size");
+  if (n == 0) {
+    if (h_count) *h_count = 0;
+    return EAC_OK;
+  }
+  if (!labels_in || !out) return fail(EAC_ERR_ARG, "relabel: null array");
+  size_t need = eac_relabel_workspace_bytes(n, n_values);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "relabel: workspace %zu < %zu bytes", ws_bytes, need);
+  Arena ar(ws, ws_bytes);
+  int32_t* first = ar.take<int32_t>(n_values > 0 ? n_values : 1);
+  int32_t* root = ar.take<int32_t>(n);
+  int32_t* flag = ar.take<int32_t>(n);
+  int32_t* rank = ar.take<int32_t>(n);
+  uint32_t* misc = ar.take<uint32_t>(4);
+  EAC_TRY(fill<int32_t>(first, n_values, INT32_MAX, s));
+  EAC_TRY(fill<int32_t>((int32_t*)misc, 4, 0, s));
+  int g = grid_for(n, 256, 4096);
+  k_first_occ<<<g, 256, 0, s>>>(labels_in, n, n_values, first, misc);
+  EAC_LAUNCH_CHECK();
+  k_first_flag<<<g, 256, 0, s>>>(labels_in, first, n, n_values, root, flag);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(flag, rank, n, 0, (int32_t*)misc + 1, ar, s)));
+  k_gather_rank<<<g, 256, 0, s>>>(root, rank, n, out);
+  EAC_LAUNCH_CHECK();
+  uint32_t hm[4];
+  EAC_TRY(d2h_sync(hm, misc, sizeof(hm), s));
+  if (hm[0]) return fail(EAC_ERR_ARG, "relabel: label outside [0, n_values)");
+  if (h_count) *h_count = (int32_t)hm[1];
+  return EAC_OK;
+}

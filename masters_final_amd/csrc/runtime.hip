@@ -1,0 +1,125 @@
+// Error reporting and small host-side runtime pieces shared by every entry
+// point of libeac_hip.so.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+
+namespace eac {
+
+static thread_local char g_err[1024] = {0};
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int fail(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+// One pinned block per host thread; grown on demand, never freed before
+// process exit (pinned memory is allocated, never registered).
+struct Pinned {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+static thread_local Pinned g_pinned;
+
+void* pinned_scratch(size_t bytes) {
+  if (g_pinned.bytes >= bytes) return g_pinned.p;
+  size_t want = bytes < 4096 ? 4096 : bytes;
+  void* p = nullptr;
+  if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  if (g_pinned.p) (void)hipHostFree(g_pinned.p);
+  g_pinned.p = p;
+  g_pinned.bytes = want;
+  return p;
+}
+
+// ---------------------------------------------------------------- profiler
+struct ProfSlot {
+  double ms = 0.0;
+  int64_t launches = 0;
+  int64_t bytes = 0;
+};
+static bool g_prof = false;
+static ProfSlot g_slots[EAC_PROF_SLOTS];
+static std::vector<ProfRec> g_recs;
+static std::vector<hipEvent_t> g_pool;
+
+bool prof_on() { return g_prof; }
+
+static hipEvent_t ev_get() {
+  if (!g_pool.empty()) {
+    hipEvent_t e = g_pool.back();
+    g_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int prof_begin(int slot, int64_t bytes, hipStream_t s) {
+  if (!g_prof || slot < 0 || slot >= EAC_PROF_SLOTS) return -1;
+  ProfRec r{slot, ev_get(), ev_get(), bytes};
+  if (!r.a || !r.b) return -1;
+  (void)hipEventRecord(r.a, s);
+  g_recs.push_back(r);
+  return (int)g_recs.size() - 1;
+}
+
+void prof_end(int rec, hipStream_t s) {
+  if (rec < 0 || rec >= (int)g_recs.size()) return;
+  (void)hipEventRecord(g_recs[rec].b, s);
+}
+
+void prof_set_bytes(int rec, int64_t bytes) {
+  if (rec >= 0 && rec < (int)g_recs.size()) g_recs[rec].bytes = bytes;
+}
+
+void prof_flush() {
+  for (auto& r : g_recs) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      g_slots[r.slot].ms += ms;
+      g_slots[r.slot].launches += 1;
+      g_slots[r.slot].bytes += r.bytes;
+    }
+    g_pool.push_back(r.a);
+    g_pool.push_back(r.b);
+  }
+  g_recs.clear();
+}
+
+}  // namespace eac
+
+extern "C" int eac_profile_enable(int32_t on) {
+  eac::prof_flush();
+  eac::g_prof = on != 0;
+  for (auto& s : eac::g_slots) s = eac::ProfSlot();
+  return EAC_OK;
+}
+
+extern "C" int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches,
+                                int64_t* alg_bytes) {
+  if (slot < 0 || slot >= EAC_PROF_SLOTS) return eac::fail(EAC_ERR_ARG, "profile: bad slot");
+  eac::prof_flush();
+  if (total_ms) *total_ms = eac::g_slots[slot].ms;
+  if (launches) *launches = eac::g_slots[slot].launches;
+  if (alg_bytes) *alg_bytes = eac::g_slots[slot].bytes;
+  return EAC_OK;
+}
+
+extern "C" const char* eac_last_error(void) { return eac::g_err; }
+
+extern "C" int eac_abi_version(void) { return 1; }

@@ -1,0 +1,113 @@
+"""Seeded synthetic inputs for the EAC hot-path workloads (numpy PCG64).
+
+  eac_small()        n=10,000 2-D points from a 6-component Gaussian mixture
+  knn_graph()        symmetrized 3-D k-NN graph with int32 weights (the
+                     1,000,000-vertex MST/CC workload and its smaller cases)
+  gmm_points()       Gaussian-mixture points for the K-Means workloads
+  random_graph()     small random symmetric CSR graphs for parity tests
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def csr_from_undirected(n, lo, hi, w):
+    """Symmetric CSR (both directions, rows sorted by column) from unique
+    undirected edges lo < hi. Returns (indptr int64[n+1], dest int32, weight)."""
+    rows = np.concatenate([lo, hi]).astype(np.int64)
+    cols = np.concatenate([hi, lo]).astype(np.int64)
+    ww = np.concatenate([w, w])
+    order = np.argsort(rows * n + cols, kind="stable")
+    rows, cols, ww = rows[order], cols[order], ww[order]
+    indptr = np.zeros(n + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=n), out=indptr[1:])
+    return indptr, cols.astype(np.int32), ww
+
+
+def knn_graph(n=1_000_000, k=20, seed=1, frac_islands=0.10, n_islands=4, wmin=1, wmax=100,
+              dtype=np.int32):
+    """3-D uniform points, exact k-NN (k neighbours besides the point itself),
+    symmetrized as a union; `frac_islands` of the points form `n_islands`
+    separate clusters translated by +10 per island along x, so their k-NN
+    stays inside. Weights uniform integers in [wmin, wmax] per undirected
+    edge. Returns dict(indptr, dest, weight, n, n_undirected)."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, 3))
+    n_isl = int(round(n * frac_islands))
+    if n_islands > 0 and n_isl >= n_islands * (k + 1):
+        owner = rng.permutation(n)[:n_isl]
+        grp = np.arange(n_isl) % n_islands
+        pts[owner, 0] += 10.0 * (grp + 1)
+    kk = min(k + 1, n)
+    tree = cKDTree(pts)
+    _, nb = tree.query(pts, k=kk, workers=-1)
+    nb = nb[:, 1:].astype(np.int64)
+    src = np.repeat(np.arange(n, dtype=np.int64), nb.shape[1])
+    dst = nb.ravel()
+    lo = np.minimum(src, dst)
+    hi = np.maximum(src, dst)
+    key = np.unique(lo * n + hi)
+    lo, hi = key // n, key % n
+    w = rng.integers(wmin, wmax + 1, size=lo.size).astype(dtype)
+    indptr, dest, weight = csr_from_undirected(n, lo, hi, w)
+    return {"indptr": indptr, "dest": dest, "weight": weight, "n": n,
+            "n_undirected": int(lo.size)}
+
+
+def random_graph(n, avg_deg, seed, wmax=10, n_isolated=0, n_parts=1, dtype=np.int32):
+    """Small random symmetric graph: Erdos-Renyi edges inside `n_parts`
+    vertex blocks (so the graph has at least n_parts components), a few
+    isolated vertices, integer weights in [1, wmax] (many ties)."""
+    rng = np.random.default_rng(seed)
+    m_target = max(int(n * avg_deg / 2), 1)
+    part = rng.integers(0, n_parts, size=n)
+    iso = rng.permutation(n)[:n_isolated] if n_isolated else np.zeros(0, np.int64)
+    u = rng.integers(0, n, size=3 * m_target)
+    v = rng.integers(0, n, size=3 * m_target)
+    ok = (u != v) & (part[u] == part[v])
+    if iso.size:
+        bad = np.zeros(n, bool)
+        bad[iso] = True
+        ok &= ~bad[u] & ~bad[v]
+    u, v = u[ok], v[ok]
+    lo, hi = np.minimum(u, v), np.maximum(u, v)
+    key = np.unique(lo * n + hi)[:m_target]
+    lo, hi = key // n, key % n
+    w = rng.integers(1, wmax + 1, size=lo.size).astype(dtype)
+    indptr, dest, weight = csr_from_undirected(n, lo, hi, w)
+    return {"indptr": indptr, "dest": dest, "weight": weight, "n": n,
+            "n_undirected": int(lo.size)}
+
+
+def gmm_points(n, d, n_comp, seed, mean_scale=None, mean_std=None, std=1.0, anisotropic=None):
+    """Gaussian mixture, equal weights. Means uniform in [-mean_scale, mean_scale]^d
+    or N(0, mean_std^2 I); isotropic `std`, or anisotropic covariances
+    Q diag(lam) Q^T with lam ~ U[a, b] when anisotropic=(a, b). float32."""
+    rng = np.random.default_rng(seed)
+    if mean_std is not None:
+        means = rng.normal(0.0, mean_std, size=(n_comp, d))
+    else:
+        s = 10.0 if mean_scale is None else mean_scale
+        means = rng.uniform(-s, s, size=(n_comp, d))
+    comp = rng.integers(0, n_comp, size=n)
+    X = np.empty((n, d), np.float64)
+    if anisotropic is None:
+        X[:] = means[comp] + std * rng.standard_normal((n, d))
+    else:
+        a, b = anisotropic
+        for c in range(n_comp):
+            idx = np.nonzero(comp == c)[0]
+            q, r = np.linalg.qr(rng.standard_normal((d, d)))
+            q = q * np.sign(np.diag(r))
+            lam = rng.uniform(a, b, size=d)
+            L = q * np.sqrt(lam)
+            X[idx] = means[c] + rng.standard_normal((idx.size, d)) @ L.T
+    return X.astype(np.float32), comp
+
+
+def eac_small(seed=0):
+    """n=10,000 points in 2-D, 6-component GMM, means U[-10,10]^2, unit std."""
+    X, y = gmm_points(10_000, 2, 6, seed, mean_scale=10.0, std=1.0)
+    return X, y

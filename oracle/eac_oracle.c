@@ -1,0 +1,725 @@
+/*
+ * eac_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A single-threaded C restatement of the reference's CPU path for the EAC
+ * hot path (diogo-aos/masters_final, Python 2 + Numba, read as source text;
+ * numba is not installed and the reference is never run). Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, as a
+ * checker or a timed baseline. The product (masters_final_amd/) never calls
+ * it.
+ *
+ * Each function cites the reference lines it follows. Where the reference's
+ * result depends on an unstable sort or on thread timing, the function
+ * documents the canonical choice made instead ("canonical:" notes).
+ *
+ * Weights are carried as double: every integer weight of the reference's
+ * graphs (uint8/uint16/int32) is exact in double and comparisons are the
+ * reference's < and ==.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_OK 0
+#define ORC_ERR_VALUE (-7)
+#define ORC_ERR_NOT_FOUND (-5)
+#define ORC_ERR_ARG (-1)
+
+/* ------------------------------------------------------------------ scans */
+
+/* MyML/MyML/utils/scan.py:41-53 exprefixsumNumba: out[i] = carry; carry +=
+ * in[i]; returns the final carry (init + total). */
+int64_t orc_exprefixsum_i64(const int64_t* in, int64_t* out, int64_t n, int64_t init) {
+  int64_t carry = init;
+  for (int64_t i = 0; i < n; ++i) {
+    out[i] = carry;
+    carry += in[i];
+  }
+  return carry;
+}
+
+/* MyML/MyML/utils/scan.py:56-72 exprefixsumNumbaSingle: in place, keeper
+ * holds the value being overwritten. Reads in[0] even for n == 0 in the
+ * reference; here n == 0 returns init. */
+int64_t orc_exprefixsum_single_i64(int64_t* a, int64_t n, int64_t init) {
+  if (n <= 0) return init;
+  int64_t carry = init;
+  int64_t keeper = a[0];
+  a[0] = init;
+  for (int64_t i = 1; i < n; ++i) {
+    carry += keeper;
+    keeper = a[i];
+    a[i] = carry;
+  }
+  carry += keeper;
+  return carry;
+}
+
+/* ------------------------------------------------- Boruvka (reference CPU) */
+
+/* mst.py:249-279 findMinEdgeNumba: strict < on weight, ties to the smaller
+ * (contracted) destination; the first CSR entry wins among equal pairs. */
+static void find_min_edge(int64_t* minedge, const double* w, const int64_t* fe,
+                          const int64_t* od, const int64_t* dest, int64_t nv) {
+  for (int64_t v = 0; v < nv; ++v) {
+    int64_t deg = od[v];
+    if (deg == 0) { minedge[v] = -1; continue; }
+    int64_t start = fe[v], end = start + deg;
+    int64_t me = start;
+    double mw = w[start];
+    int64_t md = dest[start];
+    for (int64_t e = start + 1; e < end; ++e) {
+      double ew = w[e];
+      int64_t ed = dest[e];
+      if (ew < mw) { me = e; mw = ew; md = ed; }
+      else if (ew == mw && ed < md) { me = e; mw = ew; md = ed; }
+    }
+    minedge[v] = me;
+  }
+}
+
+/* mst.py:282-305 removeMirroredNumba (sequential, in place). */
+static void remove_mirrored(int64_t* minedge, const int64_t* dest, int64_t nv) {
+  for (int64_t v = 0; v < nv; ++v) {
+    int64_t my = minedge[v];
+    if (my == -1) continue;
+    int64_t succ = dest[my];
+    int64_t se = minedge[succ];
+    if (se == -1) continue;
+    int64_t ss = dest[se];
+    if (v == ss && v < succ) minedge[v] = -1;
+  }
+}
+
+/* mst.py:320-351 initColorsNumba + propagateColorsNumba until converged. */
+static void colors_to_roots(const int64_t* minedge, const int64_t* dest, int64_t* colors,
+                            int64_t nv) {
+  for (int64_t v = 0; v < nv; ++v) colors[v] = minedge[v] != -1 ? dest[minedge[v]] : v;
+  int converged = 0;
+  while (!converged) {
+    converged = 1;
+    for (int64_t v = 0; v < nv; ++v) {
+      int64_t c = colors[v];
+      if (c != v) {
+        int64_t nc = colors[c];
+        if (nc != c) converged = 0;
+        colors[v] = nc;
+      }
+    }
+  }
+}
+
+/* mst.py:143-232 boruvka_minho_seq with helpers mst.py:235-407.
+ * Inputs: CSR (dest, w, fe, od) with nv vertices and ne entries.
+ * Output: mst[0:n_mst] = original edge ids (CSR positions) in the order the
+ * reference appends them (round by round, vertex order). Returns n_mst.
+ * components (nullable, [nv]) receives the final supervertex id of every
+ * vertex, following connected_components.py:88-94,162-173. */
+int64_t orc_boruvka_minho_seq(const int64_t* dest_in, const double* w_in, const int64_t* fe_in,
+                              const int64_t* od_in, int64_t nv, int64_t ne, int64_t* mst,
+                              int64_t* components) {
+  if (nv <= 0) return 0;
+  int64_t* dest = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+  double* w = malloc(sizeof(double) * (ne > 0 ? ne : 1));
+  int64_t* eid = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+  int64_t* fe = malloc(sizeof(int64_t) * nv);
+  int64_t* od = malloc(sizeof(int64_t) * nv);
+  memcpy(dest, dest_in, sizeof(int64_t) * ne);
+  memcpy(w, w_in, sizeof(double) * ne);
+  memcpy(fe, fe_in, sizeof(int64_t) * nv);
+  memcpy(od, od_in, sizeof(int64_t) * nv);
+  for (int64_t e = 0; e < ne; ++e) eid[e] = e;
+  int64_t* labels = NULL;
+  if (components) {
+    labels = components;
+    for (int64_t v = 0; v < nv; ++v) labels[v] = v;
+  }
+  int64_t ncomp = nv, n_mst = 0;
+  for (;;) {
+    int64_t* minedge = malloc(sizeof(int64_t) * ncomp);
+    int64_t* colors = malloc(sizeof(int64_t) * ncomp);
+    find_min_edge(minedge, w, fe, od, dest, ncomp);
+    remove_mirrored(minedge, dest, ncomp);
+    for (int64_t v = 0; v < ncomp; ++v)  /* mst.py:308-317 */
+      if (minedge[v] != -1) mst[n_mst++] = eid[minedge[v]];
+    colors_to_roots(minedge, dest, colors, ncomp);
+    /* mst.py:235-243 buildFlag + exclusive scan (scan.py:56-72) */
+    int64_t* newv = minedge;
+    for (int64_t v = 0; v < ncomp; ++v) newv[v] = (colors[v] == v);
+    int64_t nnew = orc_exprefixsum_single_i64(newv, ncomp, 0);
+    if (labels)  /* connected_components.py:162-173 */
+      for (int64_t v = 0; v < nv; ++v) labels[v] = newv[colors[labels[v]]];
+    if (nnew == 1) { free(minedge); free(colors); break; }
+    /* mst.py:354-372 countNewEdgesNumba */
+    int64_t* nod = calloc(nnew, sizeof(int64_t));
+    for (int64_t v = 0; v < ncomp; ++v) {
+      int64_t c = colors[v], cid = newv[c];
+      for (int64_t e = fe[v]; e < fe[v] + od[v]; ++e)
+        if (colors[dest[e]] != c) nod[cid] += 1;
+    }
+    int64_t* nfe = malloc(sizeof(int64_t) * nnew);
+    int64_t nne = orc_exprefixsum_i64(nod, nfe, nnew, 0);
+    if (nne == 0) { free(minedge); free(colors); free(nod); free(nfe); break; }
+    /* mst.py:378-407 assignInsertNumba */
+    int64_t* ndest = malloc(sizeof(int64_t) * nne);
+    double* nw = malloc(sizeof(double) * nne);
+    int64_t* neid = malloc(sizeof(int64_t) * nne);
+    int64_t* top = malloc(sizeof(int64_t) * nnew);
+    memcpy(top, nfe, sizeof(int64_t) * nnew);
+    for (int64_t v = 0; v < ncomp; ++v) {
+      int64_t c = colors[v];
+      for (int64_t e = fe[v]; e < fe[v] + od[v]; ++e) {
+        int64_t sc = colors[dest[e]];
+        if (c != sc) {
+          int64_t sv = newv[c], ssv = newv[sc];
+          int64_t p = top[sv]++;
+          ndest[p] = ssv;
+          nw[p] = w[e];
+          neid[p] = eid[e];
+        }
+      }
+    }
+    free(dest); free(w); free(eid); free(fe); free(od); free(top);
+    free(minedge); free(colors);
+    dest = ndest; w = nw; eid = neid; fe = nfe; od = nod;
+    ncomp = nnew;
+  }
+  free(dest); free(w); free(eid); free(fe); free(od);
+  return n_mst;
+}
+
+/* connected_components.py:46-137 connected_comps_seq: same contraction,
+ * labels are final supervertex ids (not first-occurrence numbered). */
+void orc_connected_comps_seq(const int64_t* dest, const double* w, const int64_t* fe,
+                             const int64_t* od, int64_t nv, int64_t ne, int64_t* labels) {
+  int64_t* mst = malloc(sizeof(int64_t) * (nv > 1 ? nv - 1 : 1));
+  orc_boruvka_minho_seq(dest, w, fe, od, nv, ne, mst, labels);
+  free(mst);
+}
+
+/* --------------------------------------------- canonical Kruskal (checker)
+ * Unique minimum spanning forest under the strict order (w, min, max).
+ * Edges given as undirected triples; ties of identical triples keep the
+ * first. Output: in_forest[i] flags; parent[] receives min-id roots. */
+typedef struct { double w; int64_t lo, hi, idx; } orc_edge;
+
+static int cmp_edge(const void* a, const void* b) {
+  const orc_edge* x = a;
+  const orc_edge* y = b;
+  if (x->w < y->w) return -1;
+  if (x->w > y->w) return 1;
+  if (x->lo != y->lo) return x->lo < y->lo ? -1 : 1;
+  if (x->hi != y->hi) return x->hi < y->hi ? -1 : 1;
+  return x->idx < y->idx ? -1 : (x->idx > y->idx);
+}
+
+static int64_t uf_find(int64_t* p, int64_t x) {
+  while (p[x] != x) { p[x] = p[p[x]]; x = p[x]; }
+  return x;
+}
+/* union by min id: the root of every set is its smallest vertex */
+static int uf_union(int64_t* p, int64_t a, int64_t b) {
+  a = uf_find(p, a); b = uf_find(p, b);
+  if (a == b) return 0;
+  if (a < b) p[b] = a; else p[a] = b;
+  return 1;
+}
+
+int64_t orc_kruskal_canonical(const int64_t* u, const int64_t* v, const double* w, int64_t ne,
+                              int64_t nv, uint8_t* in_forest, int64_t* order_out) {
+  orc_edge* E = malloc(sizeof(orc_edge) * (ne > 0 ? ne : 1));
+  for (int64_t i = 0; i < ne; ++i) {
+    E[i].w = w[i];
+    E[i].lo = u[i] < v[i] ? u[i] : v[i];
+    E[i].hi = u[i] < v[i] ? v[i] : u[i];
+    E[i].idx = i;
+  }
+  qsort(E, ne, sizeof(orc_edge), cmp_edge);
+  int64_t* p = malloc(sizeof(int64_t) * nv);
+  for (int64_t i = 0; i < nv; ++i) p[i] = i;
+  memset(in_forest, 0, ne);
+  int64_t nf = 0;
+  for (int64_t i = 0; i < ne; ++i) {
+    if (E[i].lo == E[i].hi) continue;
+    if (uf_union(p, E[i].lo, E[i].hi)) {
+      in_forest[E[i].idx] = 1;
+      if (order_out) order_out[nf] = E[i].idx;
+      ++nf;
+    }
+  }
+  free(E); free(p);
+  return nf;
+}
+
+/* Components of an undirected edge list, numbered by first occurrence
+ * (the numbering scipy.sparse.csgraph.connected_components returns, used by
+ * eac.py:262). Returns the number of components. */
+int64_t orc_components(const int64_t* u, const int64_t* v, int64_t ne, int64_t nv,
+                       int64_t* labels) {
+  int64_t* p = malloc(sizeof(int64_t) * (nv > 0 ? nv : 1));
+  for (int64_t i = 0; i < nv; ++i) p[i] = i;
+  for (int64_t i = 0; i < ne; ++i) uf_union(p, u[i], v[i]);
+  int64_t* rank = malloc(sizeof(int64_t) * (nv > 0 ? nv : 1));
+  int64_t nc = 0;
+  for (int64_t i = 0; i < nv; ++i) {
+    int64_t r = uf_find(p, i);
+    if (r == i) rank[i] = nc++;
+    labels[i] = rank[r];  /* r <= i, so rank[r] is already set */
+  }
+  free(p); free(rank);
+  return nc;
+}
+
+/* First-occurrence renumbering of arbitrary labels in [0, nvals). */
+int64_t orc_canonical_relabel(const int64_t* in, int64_t n, int64_t nvals, int64_t* out) {
+  int64_t* map = malloc(sizeof(int64_t) * (nvals > 0 ? nvals : 1));
+  for (int64_t i = 0; i < nvals; ++i) map[i] = -1;
+  int64_t c = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (map[in[i]] < 0) map[in[i]] = c++;
+    out[i] = map[in[i]];
+  }
+  free(map);
+  return c;
+}
+
+/* ------------------------------------------------------- graph from edges */
+
+/* build.py:18-45 binaryOriginVertexSearch (integer division of Python 2). */
+static int64_t origin_search(int64_t key, const int64_t* fe, const int64_t* od, int64_t nv) {
+  int64_t imin = 0, imax = nv;
+  while (imin < imax) {
+    int64_t imid = (imax + imin) / 2;
+    int64_t f = fe[imid];
+    if (key < f) imax = imid;
+    else if (key > f + od[imid] - 1) imin = imid + 1;
+    else return imid;
+  }
+  return -1;
+}
+
+/* build.py:49-93 getGraphFromEdges_seq. nod must be zeroed by the caller
+ * (the reference's sl_mst_lifetime_seq passes np.zeros_like, singlelink.py:55).
+ * Returns 0, or -1 when an edge id has no origin row. */
+int64_t orc_graph_from_edges(const int64_t* dest, const double* w, const int64_t* fe,
+                             const int64_t* od, int64_t nv, const int64_t* edges, int64_t nedges,
+                             int64_t* nod, int64_t* nfe, int64_t* ndest, double* nw) {
+  for (int64_t e = 0; e < nedges; ++e) {
+    int64_t ed = edges[e];
+    int64_t ov = dest[ed];
+    int64_t iv = origin_search(ed, fe, od, nv);
+    if (iv == -1) return -1;
+    nod[ov] += 1;
+    nod[iv] += 1;
+  }
+  orc_exprefixsum_i64(nod, nfe, nv, 0);
+  int64_t* top = malloc(sizeof(int64_t) * (nv > 0 ? nv : 1));
+  memcpy(top, nfe, sizeof(int64_t) * nv);
+  for (int64_t e = 0; e < nedges; ++e) {
+    int64_t ed = edges[e];
+    int64_t ov = dest[ed];
+    int64_t iv = origin_search(ed, fe, od, nv);
+    if (iv == -1) { free(top); return -1; }
+    int64_t ip = top[iv], op = top[ov];
+    ndest[ip] = ov;
+    ndest[op] = iv;
+    nw[ip] = w[ed];
+    nw[op] = w[ed];
+    top[iv] += 1;
+    top[ov] += 1;
+  }
+  free(top);
+  return 0;
+}
+
+/* ------------------------------------------------------- lifetime cuts */
+
+typedef struct { double w; int64_t lo, hi; } orc_fedge;
+static int cmp_fedge(const void* a, const void* b) {
+  const orc_fedge* x = a;
+  const orc_fedge* y = b;
+  if (x->w < y->w) return -1;
+  if (x->w > y->w) return 1;
+  if (x->lo != y->lo) return x->lo < y->lo ? -1 : 1;
+  return (x->hi > y->hi) - (x->hi < y->hi);
+}
+
+/* eac.py:366-391 lifetime_n_clusters on ascending weights.
+ * Returns ORC_ERR_VALUE where numpy raises (argmax of an empty array, or
+ * np.min over no close-to-zero lifetime). */
+static int lifetime_n_clusters(const double* sw, int64_t m, int64_t* cont_out, double* th_out,
+                               int64_t* a_out) {
+  if (m < 2) return ORC_ERR_VALUE;
+  int64_t a = 0;
+  double th = sw[1] - sw[0];
+  for (int64_t i = 1; i < m - 1; ++i) {
+    double lt = sw[i + 1] - sw[i];
+    if (lt > th) { th = lt; a = i; }
+  }
+  int64_t above = 0;
+  for (int64_t i = 0; i < m; ++i) above += sw[i] > sw[a];
+  int64_t cont = above == 0 ? 1 : above + 1;
+  /* np.isclose(lt, 0): |lt| <= 1e-8 (atol) + 1e-5 * 0 */
+  int found = 0;
+  double mn = 0.0;
+  for (int64_t i = 0; i < m - 1; ++i) {
+    double lt = sw[i + 1] - sw[i];
+    if (fabs(lt) <= 1e-8) {
+      if (!found || lt < mn) mn = lt;
+      found = 1;
+    }
+  }
+  if (!found) return ORC_ERR_VALUE;
+  if (th < 2 * mn) cont = 1;
+  *cont_out = cont;
+  *th_out = th;
+  *a_out = a;
+  return ORC_OK;
+}
+
+/* eac.py:199-267 sp_sl_lifetime on an undirected graph given as edge
+ * triples (any orientation; each undirected edge once).
+ * max_dist < 0 means None (max weight + 1, eac.py:220-221); otherwise
+ * max_dist + 1 (eac.py:223). n_clusters == 0 uses the lifetime rule.
+ * Writes labels (first-occurrence numbering, scipy's order) and returns the
+ * number of components in *n_comps; *labels_empty = 1 when the reference
+ * returns an empty label array (nc_stable <= 1, eac.py:263-265).
+ * canonical: Kruskal's unstable argsort (_min_spanning_tree.py:91,
+ * eac.py:232) is replaced by the (w, min, max) order; under the lifetime
+ * rule the partition does not depend on it (cuts are all edges with
+ * w > w[a]); with an explicit n_clusters the tied edges at the cut are
+ * taken largest (w, min, max) first. */
+int orc_sp_sl_lifetime(const int64_t* u, const int64_t* v, const double* w, int64_t ne,
+                       int64_t nv, double max_dist, int64_t n_clusters, int64_t* labels,
+                       int64_t* n_comps, int64_t* labels_empty, double* th_out,
+                       int64_t* nc_stable_out) {
+  uint8_t* inf = malloc(ne > 0 ? ne : 1);
+  int64_t* order = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+  double maxw = -INFINITY;
+  for (int64_t i = 0; i < ne; ++i) maxw = w[i] > maxw ? w[i] : maxw;
+  if (max_dist < 0) max_dist = maxw + 1; else max_dist += 1;
+  /* Kruskal drops zero weights through eliminate_zeros
+     (_min_spanning_tree.py:163-175); only positive weights count. */
+  int64_t nf = orc_kruskal_canonical(u, v, w, ne, nv, inf, order);
+  orc_fedge* F = malloc(sizeof(orc_fedge) * (nf > 0 ? nf : 1));
+  int64_t k = 0;
+  for (int64_t i = 0; i < nf; ++i) {
+    int64_t e = order[i];
+    if (w[e] == 0.0) continue;
+    F[k].w = w[e];
+    F[k].lo = u[e] < v[e] ? u[e] : v[e];
+    F[k].hi = u[e] < v[e] ? v[e] : u[e];
+    ++k;
+  }
+  nf = k;
+  qsort(F, nf, sizeof(orc_fedge), cmp_fedge);
+  int64_t n_disc = nv - nf;  /* eac.py:229 mst.shape[0] - mst.nnz */
+  double* sw = malloc(sizeof(double) * (nf > 0 ? nf : 1));
+  for (int64_t i = 0; i < nf; ++i) sw[i] = F[i].w;
+  int64_t nc_stable;
+  double th = 0.0;
+  if (n_clusters == 0) {
+    int64_t cont, a;
+    int rc = lifetime_n_clusters(sw, nf, &cont, &th, &a);
+    if (rc != ORC_OK) { free(inf); free(order); free(F); free(sw); return rc; }
+    if (n_disc > 1) {
+      double disconnect_lifetime = max_dist + 1 - sw[nf - 1];
+      if (th > disconnect_lifetime) cont += n_disc - 1;
+      else cont = n_disc;
+    }
+    nc_stable = cont;
+  } else {
+    nc_stable = n_clusters;
+  }
+  int64_t keep = nf;
+  if (nc_stable > n_disc) {
+    int64_t n_cuts = nc_stable - n_disc;
+    keep = nf - n_cuts;
+    if (keep < 0) keep = 0;
+  }
+  *th_out = th;
+  *nc_stable_out = nc_stable;
+  if (nc_stable > 1) {
+    int64_t* cu = malloc(sizeof(int64_t) * (keep > 0 ? keep : 1));
+    int64_t* cv = malloc(sizeof(int64_t) * (keep > 0 ? keep : 1));
+    for (int64_t i = 0; i < keep; ++i) { cu[i] = F[i].lo; cv[i] = F[i].hi; }
+    *n_comps = orc_components(cu, cv, keep, nv, labels);
+    *labels_empty = 0;
+    free(cu); free(cv);
+  } else {
+    *n_comps = 1;
+    *labels_empty = 1;
+  }
+  free(inf); free(order); free(F); free(sw);
+  return ORC_OK;
+}
+
+/* singlelink.py:14-65 sl_mst_lifetime_seq on the forest of an undirected
+ * graph (edge triples). disconnect_weight < 0 means None (max graph weight,
+ * singlelink.py:16-17). Keeps the a smallest forest edges when the largest
+ * lifetime is >= disconnect_weight - max forest weight (the reference drops
+ * m - a edges, singlelink.py:43-48), then labels the kept forest's
+ * components (first occurrence). canonical: forest edges ordered by
+ * (w, min, max) instead of numpy's unstable argsort (singlelink.py:27).
+ * Returns ORC_ERR_VALUE for fewer than 2 forest edges (argmax of empty). */
+int orc_sl_mst_lifetime(const int64_t* u, const int64_t* v, const double* w, int64_t ne,
+                        int64_t nv, double disconnect_weight, int64_t* labels,
+                        int64_t* n_comps, int64_t* n_kept) {
+  uint8_t* inf = malloc(ne > 0 ? ne : 1);
+  int64_t* order = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+  double maxw = -INFINITY;
+  for (int64_t i = 0; i < ne; ++i) maxw = w[i] > maxw ? w[i] : maxw;
+  if (disconnect_weight < 0) disconnect_weight = maxw;
+  int64_t nf = orc_kruskal_canonical(u, v, w, ne, nv, inf, order);
+  orc_fedge* F = malloc(sizeof(orc_fedge) * (nf > 0 ? nf : 1));
+  for (int64_t i = 0; i < nf; ++i) {
+    int64_t e = order[i];
+    F[i].w = w[e];
+    F[i].lo = u[e] < v[e] ? u[e] : v[e];
+    F[i].hi = u[e] < v[e] ? v[e] : u[e];
+  }
+  qsort(F, nf, sizeof(orc_fedge), cmp_fedge);
+  if (nf < 2) { free(inf); free(order); free(F); return ORC_ERR_VALUE; }
+  int64_t a = 0;
+  double mlt = F[1].w - F[0].w;
+  for (int64_t i = 1; i < nf - 1; ++i) {
+    double lt = F[i + 1].w - F[i].w;
+    if (lt > mlt) { mlt = lt; a = i; }
+  }
+  double lt_threshold = disconnect_weight - F[nf - 1].w;
+  int64_t keep = nf;
+  if (mlt >= lt_threshold) keep = a;  /* n_discarded = (nf-1) - a + 1 */
+  int64_t* cu = malloc(sizeof(int64_t) * (keep > 0 ? keep : 1));
+  int64_t* cv = malloc(sizeof(int64_t) * (keep > 0 ? keep : 1));
+  for (int64_t i = 0; i < keep; ++i) { cu[i] = F[i].lo; cv[i] = F[i].hi; }
+  *n_comps = orc_components(cu, cv, keep, nv, labels);
+  *n_kept = keep;
+  free(cu); free(cv); free(inf); free(order); free(F);
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------- co-association */
+
+/* Co-association counts of a set of partitions, as the condensed sparse
+ * builder of MyML/MyML/EAC/sparse.py accumulates them with no capacity cap
+ * (so nothing is discarded): the first partition processed is the one with
+ * the fewest clusters (sparse.py:154-179, np.argmin -> first minimum); its
+ * clusters write count 1 for every pair j > i of ascending members
+ * (sparse.py:445-465); every other partition increments an existing
+ * (row, col) found by binary search in the sorted row, or inserts it keeping
+ * the row sorted (sparse.py:956-1040 with binary_search_interval
+ * sparse.py:1069-1099).
+ * labels: P x n int64 (row-major), values in [0, k_p). Output: rows of the
+ * upper triangle (col > row) sorted by col, as CSR: indptr[n+1] (int64),
+ * indices, counts. cap_total bounds indices/counts; returns nnz or -1 if
+ * cap_total is too small. */
+int64_t orc_coassoc_condensed(const int64_t* labels, const int64_t* kp, int64_t P, int64_t n,
+                              int64_t* indptr, int64_t* indices, int64_t* counts,
+                              int64_t cap_total) {
+  /* per-row growable arrays */
+  int64_t** rc = calloc(n, sizeof(int64_t*));
+  int64_t** rv = calloc(n, sizeof(int64_t*));
+  int64_t* deg = calloc(n, sizeof(int64_t));
+  int64_t* cap = calloc(n, sizeof(int64_t));
+  int64_t first = 0;
+  for (int64_t p = 1; p < P; ++p) if (kp[p] < kp[first]) first = p;
+  int64_t kmax = 0;
+  for (int64_t p = 0; p < P; ++p) kmax = kp[p] > kmax ? kp[p] : kmax;
+  /* members of each cluster, ascending (partition.py:73-86 order) */
+  int64_t* cnt = malloc(sizeof(int64_t) * (kmax + 1));
+  int64_t* off = malloc(sizeof(int64_t) * (kmax + 1));
+  int64_t* mem = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  for (int64_t step = 0; step < P; ++step) {
+    int64_t p = step == 0 ? first : (step <= first ? step - 1 : step);
+    const int64_t* lab = labels + p * n;
+    memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
+    for (int64_t i = 0; i < n; ++i) cnt[lab[i]]++;
+    off[0] = 0;
+    for (int64_t c = 0; c < kmax; ++c) off[c + 1] = off[c] + cnt[c];
+    memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
+    for (int64_t i = 0; i < n; ++i) mem[off[lab[i]] + cnt[lab[i]]++] = i;
+    for (int64_t c = 0; c < kp[p]; ++c) {
+      const int64_t* S = mem + off[c];
+      int64_t s = off[c + 1] - off[c];
+      for (int64_t a = 0; a < s; ++a) {
+        int64_t r = S[a];
+        for (int64_t b = a + 1; b < s; ++b) {
+          int64_t col = S[b];
+          /* binary search for col in row r */
+          int64_t lo = 0, hi = deg[r];
+          while (lo < hi) {
+            int64_t mid = (lo + hi) / 2;
+            if (rc[r][mid] < col) lo = mid + 1; else hi = mid;
+          }
+          if (lo < deg[r] && rc[r][lo] == col) { rv[r][lo] += 1; continue; }
+          if (deg[r] == cap[r]) {
+            cap[r] = cap[r] ? cap[r] * 2 : 8;
+            rc[r] = realloc(rc[r], sizeof(int64_t) * cap[r]);
+            rv[r] = realloc(rv[r], sizeof(int64_t) * cap[r]);
+          }
+          memmove(rc[r] + lo + 1, rc[r] + lo, sizeof(int64_t) * (deg[r] - lo));
+          memmove(rv[r] + lo + 1, rv[r] + lo, sizeof(int64_t) * (deg[r] - lo));
+          rc[r][lo] = col;
+          rv[r][lo] = 1;
+          deg[r] += 1;
+        }
+      }
+    }
+  }
+  /* sparse.py:336-349 condense_eac_csr: rows packed in order */
+  int64_t nnz = 0;
+  indptr[0] = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    if (nnz + deg[r] > cap_total) { nnz = -1; break; }
+    for (int64_t j = 0; j < deg[r]; ++j) {
+      indices[nnz + j] = rc[r][j];
+      counts[nnz + j] = rv[r][j];
+    }
+    nnz += deg[r];
+    indptr[r + 1] = nnz;
+  }
+  for (int64_t r = 0; r < n; ++r) { free(rc[r]); free(rv[r]); }
+  free(rc); free(rv); free(deg); free(cap); free(cnt); free(off); free(mem);
+  return nnz;
+}
+
+/* ------------------------------------------------------------- K-Means */
+
+/* MyML/MyML/cluster/KMeans.py:453-490 py_label / numba_label as the FP64
+ * oracle: dist = sum over d ascending of (x_d - c_kd)^2 in double, strict <
+ * so the lowest k wins exact ties. data (n, d) and centroids (k, d) are
+ * float32 row-major; dists are returned in double. */
+void orc_kmeans_label_f64(const float* X, const float* Cc, int64_t n, int64_t d, int64_t k,
+                          int32_t* labels, double* dists) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float* x = X + i * d;
+    double best = 0.0;
+    int32_t bl = 0;
+    for (int64_t c = 0; c < k; ++c) {
+      const float* cc = Cc + c * d;
+      double dist = 0.0;
+      for (int64_t j = 0; j < d; ++j) {
+        double diff = (double)x[j] - (double)cc[j];
+        dist += diff * diff;
+      }
+      if (c == 0 || dist < best) { best = dist; bl = (int32_t)c; }
+    }
+    labels[i] = bl;
+    if (dists) dists[i] = best;
+  }
+}
+
+/* MyML/MyML/utils/sorting.py:8-46 arg_k_select (quickselect over argument
+ * indices; Python 2 integer division at :21). out[0:k] = the arguments of
+ * the k smallest values in the order the partitioning leaves them. */
+void orc_arg_k_select(const float* ary, int64_t size, int64_t k, int64_t* out) {
+  int64_t* args = malloc(sizeof(int64_t) * (size > 0 ? size : 1));
+  for (int64_t i = 0; i < size; ++i) args[i] = i;
+  int64_t fro = 0, to = size - 1;
+  while (fro < to) {
+    int64_t r = fro, w = to;
+    int64_t mid_arg = args[(r + w) / 2];
+    float mid = ary[mid_arg];
+    while (r < w) {
+      if (ary[args[r]] >= mid) {
+        int64_t tmp = args[w];
+        args[w] = args[r];
+        args[r] = tmp;
+        w -= 1;
+      } else {
+        r += 1;
+      }
+    }
+    if (ary[args[r]] > mid) r -= 1;
+    if (k <= r) to = r; else fro = r + 1;
+  }
+  for (int64_t i = 0; i < k; ++i) out[i] = args[i];
+  free(args);
+}
+
+/* KMeans.py:709-754 py_recompute_centroids / numba_recompute_centroids:
+ * float32 sums in point order, then division by the count in double
+ * (float32 / int32 in Numba), stored as float32. Empty clusters take, in
+ * ascending cluster order, the points returned by arg_k_select(dists, e)
+ * (the e SMALLEST dists, despite the comment at KMeans.py:727-728).
+ * reseed_mode 0: that quickselect order; 1 (canonical): ascending
+ * (dist, index). dists are the float32 distances of the last labeling. */
+void orc_recompute_centroids(const float* X, const int32_t* labels, const float* dists,
+                             int64_t n, int64_t d, int64_t k, int reseed_mode, float* out) {
+  float* acc = calloc(k * d, sizeof(float));
+  int64_t* cnt = calloc(k, sizeof(int64_t));
+  for (int64_t i = 0; i < n; ++i) cnt[labels[i]]++;
+  int64_t ne = 0;
+  for (int64_t c = 0; c < k; ++c) ne += cnt[c] == 0;
+  int64_t* far = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+  if (ne > 0) {
+    if (reseed_mode == 0) {
+      orc_arg_k_select(dists, n, ne, far);
+    } else {
+      /* canonical: the ne smallest (dist, index) ascending, by selection */
+      uint8_t* used = calloc(n, 1);
+      for (int64_t t = 0; t < ne; ++t) {
+        int64_t bi = -1;
+        for (int64_t i = 0; i < n; ++i) {
+          if (used[i]) continue;
+          if (bi < 0 || dists[i] < dists[bi]) bi = i;
+        }
+        far[t] = bi;
+        used[bi] = 1;
+      }
+      free(used);
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    float* a = acc + labels[i] * d;
+    const float* x = X + i * d;
+    for (int64_t j = 0; j < d; ++j) a[j] = a[j] + x[j];
+  }
+  int64_t t = 0;
+  for (int64_t c = 0; c < k; ++c) {
+    if (cnt[c] != 0) {
+      for (int64_t j = 0; j < d; ++j)
+        out[c * d + j] = (float)((double)acc[c * d + j] / (double)cnt[c]);
+    } else {
+      int64_t src = far[t++];
+      for (int64_t j = 0; j < d; ++j) out[c * d + j] = X[src * d + j];
+    }
+  }
+  free(acc); free(cnt); free(far);
+}
+
+/* KMeans.py:117-189 K_Means.fit with given initial centroids: label, count
+ * the iteration, stop when |sum(dists) - previous| <= tol or
+ * iters >= max_iter, else recompute centroids. The convergence sum is taken
+ * over the float32 dists in double (numpy's float32 pairwise sum is not
+ * restated; with tol=1e-4 both stop exactly when the distances repeat).
+ * labels/dists of the last labeling and the final centroids are returned;
+ * the iteration count is the return value. */
+int64_t orc_kmeans_fit(const float* X, const float* init, int64_t n, int64_t d, int64_t k,
+                       int64_t max_iter, double tol, int reseed_mode, int32_t* labels,
+                       float* dists, float* centroids) {
+  float* C = malloc(sizeof(float) * k * d);
+  float* Cn = malloc(sizeof(float) * k * d);
+  double* dd = malloc(sizeof(double) * (n > 0 ? n : 1));
+  memcpy(C, init, sizeof(float) * k * d);
+  double inertia = INFINITY;
+  int64_t iters = 0;
+  for (;;) {
+    orc_kmeans_label_f64(X, C, n, d, k, labels, dd);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) { dists[i] = (float)dd[i]; s += dists[i]; }
+    iters += 1;
+    int stop = 0;
+    double err = fabs(s - inertia);
+    inertia = s;
+    if (err <= tol) stop = 1;
+    if (iters >= max_iter) stop = 1;
+    if (stop) break;
+    orc_recompute_centroids(X, labels, dists, n, d, k, reseed_mode, Cn);
+    float* t = C; C = Cn; Cn = t;
+  }
+  memcpy(centroids, C, sizeof(float) * k * d);
+  free(C); free(Cn); free(dd);
+  return iters;
+}

@@ -1,0 +1,235 @@
+"""numpy wrappers around liboracle.so (the C restatement) -- TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB = None
+
+_P = C.c_void_p
+_I = C.c_int64
+_D = C.c_double
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboracle.so"
+        if not path.exists():
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(str(path))
+        sig = {
+            "orc_exprefixsum_i64": (_I, [_P, _P, _I, _I]),
+            "orc_exprefixsum_single_i64": (_I, [_P, _I, _I]),
+            "orc_boruvka_minho_seq": (_I, [_P, _P, _P, _P, _I, _I, _P, _P]),
+            "orc_connected_comps_seq": (None, [_P, _P, _P, _P, _I, _I, _P]),
+            "orc_kruskal_canonical": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+            "orc_components": (_I, [_P, _P, _I, _I, _P]),
+            "orc_canonical_relabel": (_I, [_P, _I, _I, _P]),
+            "orc_graph_from_edges": (_I, [_P, _P, _P, _P, _I, _P, _I, _P, _P, _P, _P]),
+            "orc_sp_sl_lifetime": (C.c_int, [_P, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _P]),
+            "orc_sl_mst_lifetime": (C.c_int, [_P, _P, _P, _I, _I, _D, _P, _P, _P]),
+            "orc_coassoc_condensed": (_I, [_P, _P, _I, _I, _P, _P, _P, _I]),
+            "orc_kmeans_label_f64": (None, [_P, _P, _I, _I, _I, _P, _P]),
+            "orc_arg_k_select": (None, [_P, _I, _I, _P]),
+            "orc_recompute_centroids": (None, [_P, _P, _P, _I, _I, _I, C.c_int, _P]),
+            "orc_kmeans_fit": (_I, [_P, _P, _I, _I, _I, _I, _D, C.c_int, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _a(x, dt):
+    return np.ascontiguousarray(x, dtype=dt)
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+# ------------------------------------------------------------------- scans
+def exprefixsum(a, init=0):
+    a = _a(a, np.int64)
+    out = np.empty_like(a)
+    tot = lib().orc_exprefixsum_i64(_p(a), _p(out), a.size, int(init))
+    return out, int(tot)
+
+
+def exprefixsum_single(a, init=0):
+    a = _a(a, np.int64).copy()
+    tot = lib().orc_exprefixsum_single_i64(_p(a), a.size, int(init))
+    return a, int(tot)
+
+
+# ----------------------------------------------------------------- graphs
+def outdegree_from_firstedge(fe, n_edges):
+    """build.py:275-280 / test_boruvka.py:37-42."""
+    fe = np.asarray(fe)
+    od = np.empty_like(fe)
+    od[:-1] = fe[1:] - fe[:-1]
+    od[-1] = n_edges - fe[-1]
+    return od
+
+
+def boruvka_minho_seq(dest, weight, fe, od):
+    dest, fe, od = _a(dest, np.int64), _a(fe, np.int64), _a(od, np.int64)
+    w = _a(weight, np.float64)
+    nv = fe.size
+    mst = np.zeros(max(nv - 1, 1), np.int64)
+    comps = np.zeros(nv, np.int64)
+    nm = lib().orc_boruvka_minho_seq(_p(dest), _p(w), _p(fe), _p(od), nv, dest.size,
+                                     _p(mst), _p(comps))
+    return mst[:max(nv - 1, 0)], int(nm), comps
+
+
+def connected_comps_seq(dest, weight, fe, od):
+    dest, fe, od = _a(dest, np.int64), _a(fe, np.int64), _a(od, np.int64)
+    w = _a(weight, np.float64)
+    lab = np.zeros(fe.size, np.int64)
+    lib().orc_connected_comps_seq(_p(dest), _p(w), _p(fe), _p(od), fe.size, dest.size, _p(lab))
+    return lab
+
+
+def csr_edges(dest, weight, fe, od, symmetric=True):
+    """Undirected edge triples of a CSR: entries with dest > row (symmetric)
+    or every non-loop entry (symmetric=False)."""
+    fe, od, dest = np.asarray(fe, np.int64), np.asarray(od, np.int64), np.asarray(dest, np.int64)
+    rows = np.repeat(np.arange(fe.size, dtype=np.int64), od)
+    pos = np.concatenate([np.arange(f, f + o, dtype=np.int64) for f, o in zip(fe, od)]) \
+        if fe.size and od.sum() else np.zeros(0, np.int64)
+    cols = dest[pos]
+    sel = cols > rows if symmetric else cols != rows
+    return rows[sel], cols[sel], np.asarray(weight, np.float64)[pos][sel], pos[sel]
+
+
+def kruskal_canonical(u, v, w, nv):
+    u, v, w = _a(u, np.int64), _a(v, np.int64), _a(w, np.float64)
+    inf = np.zeros(max(u.size, 1), np.uint8)
+    order = np.zeros(max(u.size, 1), np.int64)
+    nf = lib().orc_kruskal_canonical(_p(u), _p(v), _p(w), u.size, nv, _p(inf), _p(order))
+    return inf[:u.size].astype(bool), order[:nf]
+
+
+def components(u, v, nv):
+    u, v = _a(u, np.int64), _a(v, np.int64)
+    lab = np.zeros(max(nv, 1), np.int64)
+    nc = lib().orc_components(_p(u), _p(v), u.size, nv, _p(lab))
+    return int(nc), lab[:nv]
+
+
+def canonical_relabel(labels):
+    lab = _a(labels, np.int64)
+    nv = int(lab.max()) + 1 if lab.size else 0
+    out = np.zeros(max(lab.size, 1), np.int64)
+    c = lib().orc_canonical_relabel(_p(lab), lab.size, nv, _p(out))
+    return out[:lab.size], int(c)
+
+
+def graph_from_edges(dest, weight, fe, od, edges):
+    dest, fe, od, edges = (_a(dest, np.int64), _a(fe, np.int64), _a(od, np.int64),
+                           _a(edges, np.int64))
+    w = _a(weight, np.float64)
+    nv = fe.size
+    nod = np.zeros(nv, np.int64)
+    nfe = np.zeros(nv, np.int64)
+    ndest = np.zeros(max(2 * edges.size, 1), np.int64)
+    nw = np.zeros(max(2 * edges.size, 1), np.float64)
+    rc = lib().orc_graph_from_edges(_p(dest), _p(w), _p(fe), _p(od), nv, _p(edges), edges.size,
+                                    _p(nod), _p(nfe), _p(ndest), _p(nw))
+    if rc != 0:
+        return None
+    return nod, nfe, ndest[:2 * edges.size], nw[:2 * edges.size]
+
+
+def sp_sl_lifetime(u, v, w, nv, max_dist=None, n_clusters=0):
+    """eac.py:199-267 -> (n_comps, labels) with labels == [] when the
+    reference returns an empty array."""
+    u, v, w = _a(u, np.int64), _a(v, np.int64), _a(w, np.float64)
+    lab = np.zeros(max(nv, 1), np.int64)
+    nc, empty, th, ncs = C.c_int64(), C.c_int64(), C.c_double(), C.c_int64()
+    rc = lib().orc_sp_sl_lifetime(_p(u), _p(v), _p(w), u.size, nv,
+                                  -1.0 if max_dist is None else float(max_dist),
+                                  int(n_clusters), _p(lab), C.byref(nc), C.byref(empty),
+                                  C.byref(th), C.byref(ncs))
+    if rc != 0:
+        raise ValueError("sp_sl_lifetime: the reference raises ValueError here")
+    if empty.value:
+        return 1, np.zeros(0, np.int64)
+    return int(nc.value), lab[:nv]
+
+
+def sl_mst_lifetime(u, v, w, nv, disconnect_weight=None):
+    u, v, w = _a(u, np.int64), _a(v, np.int64), _a(w, np.float64)
+    lab = np.zeros(max(nv, 1), np.int64)
+    nc, kept = C.c_int64(), C.c_int64()
+    rc = lib().orc_sl_mst_lifetime(_p(u), _p(v), _p(w), u.size, nv,
+                                   -1.0 if disconnect_weight is None else float(disconnect_weight),
+                                   _p(lab), C.byref(nc), C.byref(kept))
+    if rc != 0:
+        raise ValueError("sl_mst_lifetime: the reference raises ValueError here")
+    return lab[:nv], int(nc.value), int(kept.value)
+
+
+def coassoc_condensed(labels, kp):
+    """labels: (P, n) ints in [0, k_p). Returns upper-triangle CSR (indptr, indices, counts)."""
+    L = _a(labels, np.int64)
+    kp = _a(kp, np.int64)
+    P, n = L.shape
+    cap = n * (n - 1) // 2 + 1
+    indptr = np.zeros(n + 1, np.int64)
+    ind = np.zeros(cap, np.int64)
+    cnt = np.zeros(cap, np.int64)
+    nnz = lib().orc_coassoc_condensed(_p(L), _p(kp), P, n, _p(indptr), _p(ind), _p(cnt), cap)
+    assert nnz >= 0
+    return indptr, ind[:nnz], cnt[:nnz]
+
+
+# ---------------------------------------------------------------- K-Means
+def kmeans_label_f64(X, Cc):
+    X, Cc = _a(X, np.float32), _a(Cc, np.float32)
+    n, d = X.shape
+    k = Cc.shape[0]
+    lab = np.zeros(n, np.int32)
+    dist = np.zeros(n, np.float64)
+    lib().orc_kmeans_label_f64(_p(X), _p(Cc), n, d, k, _p(lab), _p(dist))
+    return lab, dist
+
+
+def arg_k_select(ary, k):
+    a = _a(ary, np.float32)
+    out = np.zeros(max(k, 1), np.int64)
+    lib().orc_arg_k_select(_p(a), a.size, k, _p(out))
+    return out[:k]
+
+
+def recompute_centroids(X, labels, dists, k, reseed_mode=0):
+    X = _a(X, np.float32)
+    lab = _a(labels, np.int32)
+    dist = _a(dists, np.float32)
+    n, d = X.shape
+    out = np.zeros((k, d), np.float32)
+    lib().orc_recompute_centroids(_p(X), _p(lab), _p(dist), n, d, k, reseed_mode, _p(out))
+    return out
+
+
+def kmeans_fit(X, init, max_iter=3, tol=1e-4, reseed_mode=0):
+    X, init = _a(X, np.float32), _a(init, np.float32)
+    n, d = X.shape
+    k = init.shape[0]
+    lab = np.zeros(n, np.int32)
+    dist = np.zeros(n, np.float32)
+    cen = np.zeros((k, d), np.float32)
+    it = lib().orc_kmeans_fit(_p(X), _p(init), n, d, k, max_iter, tol, reseed_mode, _p(lab),
+                              _p(dist), _p(cen))
+    return lab, dist, cen, int(it)

@@ -1,0 +1,56 @@
+"""The C-ABI library loads and exports every symbol include/eac_hip.h declares
+(no compute calls, so this runs without a GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+from masters_final_amd import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    text = (ROOT / "include" / "eac_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(eac_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "eac_boruvka_mst" in names and "eac_scan_exclusive_i32" in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+        assert isinstance(getattr(lib, name), ctypes._CFuncPtr)
+
+
+def test_ctypes_signatures_cover_header():
+    assert set(_declared()) == set(_lib.SIGNATURES)
+
+
+def test_workspace_queries_and_errors_without_gpu():
+    lib = _lib.load_library()
+    assert lib.eac_abi_version() == 1
+    assert lib.eac_scan_workspace_bytes(10_000_000, 4) > 0
+    assert lib.eac_boruvka_workspace_bytes(1_000_000, 27_000_000) > 27_000_000 * 8
+    # argument errors are reported before any device work
+    rc = lib.eac_boruvka_mst(None, None, 99, None, None, 10, 10, 1, None, None, None, None,
+                             None, 0, None)
+    assert rc == _lib.EAC_ERR_ARG
+    assert b"dtype" in lib.eac_last_error()
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Product calls raise instead of computing on the CPU when no GPU is visible."""
+    import numpy as np
+    import pytest
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from masters_final_amd.graph.mst import boruvka_minho_seq
+    with pytest.raises(_lib.EACUnavailable):
+        boruvka_minho_seq(np.array([1, 0], np.int32), np.array([1, 1], np.int32),
+                          np.array([0, 1], np.int32), np.array([1, 1], np.int32))

@@ -1,0 +1,277 @@
+"""Pins the CPU oracle (oracle/eac_oracle.c) against the reference's golden
+vectors, library routines (scipy) and brute force. CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components, minimum_spanning_tree
+
+from masters_final_amd import synth
+from oracle import oracle as O
+
+
+# ------------------------------------------------------------------ scans
+def test_exprefixsum_closed_form():
+    rng = np.random.default_rng(0)
+    a = rng.integers(0, 10000, 100_000)  # test_utils_scan.py:88-104 pattern
+    out, tot = O.exprefixsum(a, init=7)
+    assert tot == 7 + a.sum()
+    assert out[0] == 7 and np.array_equal(out[1:], 7 + np.cumsum(a)[:-1])
+    single, tot2 = O.exprefixsum_single(a, init=7)
+    assert tot2 == tot and np.array_equal(single, out)
+
+
+@pytest.mark.parametrize("n", [1, 2, 1023, 1024, 1025, 2048, 5000])
+def test_exprefixsum_sizes(n):
+    a = np.arange(n)
+    out, tot = O.exprefixsum(a)
+    assert tot == n * (n - 1) // 2
+    assert np.array_equal(out, np.concatenate([[0], np.cumsum(a)[:-1]]))
+
+
+# ---------------------------------------------------------------- Boruvka
+def test_boruvka_seq_4elt_golden(elt4):
+    """test_boruvka.py:62-99: edge count and total weight equal scipy's."""
+    g = elt4
+    mst, nm, _ = O.boruvka_minho_seq(g["dest"], g["weight"], g["fe"], g["od"])
+    G = sp.csr_matrix((g["weight"], g["dest"], g["indptr"]), shape=(g["n"], g["n"]))
+    ref = minimum_spanning_tree(G)
+    assert nm == ref.nnz == 15605
+    assert int(g["weight"][mst[:nm]].astype(np.int64).sum()) == int(ref.sum()) == 2964801
+
+
+def test_kruskal_canonical_4elt(elt4):
+    g = elt4
+    u, v, w, _ = O.csr_edges(g["dest"], g["weight"], g["fe"], g["od"])
+    inf, order = O.kruskal_canonical(u, v, w, g["n"])
+    assert inf.sum() == 15605 and w[inf].sum() == 2964801
+
+
+def _brute_min_tree(n, u, v, w):
+    """Unique minimum spanning forest under (w, min, max) by exhaustive search."""
+    keys = sorted(range(len(u)), key=lambda i: (w[i], min(u[i], v[i]), max(u[i], v[i])))
+    rank = np.empty(len(u))
+    rank[keys] = np.arange(len(u))
+    pw = np.asarray(w, float) + rank * 1e-6  # strict order, same as (w,min,max)
+    ncomp, _ = connected_components(sp.csr_matrix((np.ones(len(u)), (u, v)), shape=(n, n)),
+                                    directed=False)
+    size = n - ncomp
+    best, best_set = None, None
+    for sub in itertools.combinations(range(len(u)), size):
+        p = list(range(n))
+
+        def f(x):
+            while p[x] != x:
+                x = p[x]
+            return x
+        ok = True
+        for i in sub:
+            a, b = f(u[i]), f(v[i])
+            if a == b:
+                ok = False
+                break
+            p[a] = b
+        if ok:
+            tot = pw[list(sub)].sum()
+            if best is None or tot < best:
+                best, best_set = tot, set(sub)
+    return best_set
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_kruskal_and_boruvka_vs_brute_force(seed):
+    g = synth.random_graph(7, 2.6, seed, wmax=3, n_parts=1 + seed % 2)
+    n = g["n"]
+    fe = g["indptr"][:-1]
+    od = np.diff(g["indptr"]).astype(np.int64)
+    u, v, w, pos = O.csr_edges(g["dest"], g["weight"], fe, od)
+    if len(u) > 14:
+        pytest.skip("too many edges for brute force")
+    want = _brute_min_tree(n, u, v, w)
+    inf, _ = O.kruskal_canonical(u, v, w, n)
+    assert set(np.nonzero(inf)[0]) == want
+    mst, nm, _ = O.boruvka_minho_seq(g["dest"], g["weight"], fe, od)
+    assert np.isclose(g["weight"][mst[:nm]].sum(), w[list(want)].sum())
+    assert nm == len(want)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_boruvka_seq_weight_vs_scipy_random(seed):
+    g = synth.random_graph(400, 6, seed, wmax=5, n_isolated=5, n_parts=3)
+    n = g["n"]
+    fe, od = g["indptr"][:-1], np.diff(g["indptr"])
+    mst, nm, comps = O.boruvka_minho_seq(g["dest"], g["weight"], fe, od)
+    G = sp.csr_matrix((g["weight"], g["dest"], g["indptr"]), shape=(n, n))
+    ref = minimum_spanning_tree(G)
+    assert nm == ref.nnz
+    assert g["weight"][mst[:nm]].sum() == ref.sum()
+    # connected_comps_seq partition (supervertex ids) == scipy's after relabel
+    nc, lab = connected_components(G, directed=False)
+    can, cnt = O.canonical_relabel(comps)
+    assert cnt == nc and np.array_equal(can, lab)
+
+
+def test_boruvka_distinct_weights_edge_set_equals_canonical():
+    """With all weights distinct the reference's tie-break never fires, so its
+    edge set equals the canonical one."""
+    g = synth.random_graph(300, 5, 11, wmax=1)
+    rng = np.random.default_rng(3)
+    fe, od = g["indptr"][:-1], np.diff(g["indptr"])
+    u, v, _, pos = O.csr_edges(g["dest"], g["weight"], fe, od)
+    wu = rng.permutation(len(u)).astype(np.float64) + 1
+    # mirror distinct weights onto both directions
+    n = g["n"]
+    key = {(a, b): x for a, b, x in zip(u, v, wu)}
+    rows = np.repeat(np.arange(n), od)
+    w = np.array([key[(min(r, c), max(r, c))] for r, c in zip(rows, g["dest"])])
+    mst, nm, _ = O.boruvka_minho_seq(g["dest"], w, fe, od)
+    got = {(min(rows[e], g["dest"][e]), max(rows[e], g["dest"][e])) for e in mst[:nm]}
+    inf, _ = O.kruskal_canonical(u, v, wu, n)
+    want = {(a, b) for a, b, f in zip(u, v, inf) if f}
+    assert got == want
+
+
+def test_components_vs_scipy():
+    g = synth.random_graph(1000, 1.2, 5, n_isolated=20, n_parts=4)
+    n = g["n"]
+    fe, od = g["indptr"][:-1], np.diff(g["indptr"])
+    u, v, _, _ = O.csr_edges(g["dest"], g["weight"], fe, od)
+    nc, lab = O.components(u, v, n)
+    G = sp.csr_matrix((g["weight"], g["dest"], g["indptr"]), shape=(n, n))
+    nc2, lab2 = connected_components(G, directed=False)
+    assert nc == nc2 and np.array_equal(lab, lab2)
+
+
+# ------------------------------------------------------- graph from edges
+def test_graph_from_edges_roundtrip(elt4):
+    g = elt4
+    mst, nm, _ = O.boruvka_minho_seq(g["dest"], g["weight"], g["fe"], g["od"])
+    res = O.graph_from_edges(g["dest"], g["weight"], g["fe"], g["od"], mst[:nm])
+    nod, nfe, ndest, nw = res
+    assert nod.sum() == 2 * nm
+    assert np.array_equal(nfe[1:], np.cumsum(nod)[:-1]) and nfe[0] == 0
+    # forest is spanning: one component
+    u = np.repeat(np.arange(g["n"]), nod)
+    nc, _ = O.components(u, ndest, g["n"])
+    assert nc == 1
+    assert nw.sum() == 2 * 2964801
+    # an id outside the graph is not found (build.py:56-58 returns -1)
+    assert O.graph_from_edges(g["dest"], g["weight"], g["fe"], g["od"],
+                              np.array([len(g["dest"]) + 5])) is None
+
+
+# ---------------------------------------------------------- lifetime cuts
+def test_sp_sl_lifetime_4elt_known_answer(elt4):
+    g = elt4
+    u, v, w, _ = O.csr_edges(g["dest"], g["weight"], g["fe"], g["od"])
+    nc, lab = O.sp_sl_lifetime(u, v, w, g["n"])
+    assert nc == 6
+    assert sorted(np.bincount(lab).tolist(), reverse=True) == [15600, 2, 1, 1, 1, 1]
+    # tie-invariant closed form: components of {edges with w <= 768}
+    keep = w <= 768
+    nc2, lab2 = O.components(u[keep], v[keep], g["n"])
+    assert nc2 == 6 and np.array_equal(lab, lab2)
+
+
+def test_sp_sl_lifetime_hand_cases():
+    # path 0-1-2-3 with weights 1,1,5: max gap 4 above w=1 -> cut the w=5 edge
+    u, v, w = np.array([0, 1, 2]), np.array([1, 2, 3]), np.array([1., 1., 5.])
+    nc, lab = O.sp_sl_lifetime(u, v, w, 4)
+    assert nc == 2 and lab.tolist() == [0, 0, 0, 1]
+    # all weights equal -> one cluster -> empty labels (eac.py:263-265)
+    nc, lab = O.sp_sl_lifetime(u, v, np.array([2., 2., 2.]), 4)
+    assert nc == 1 and lab.size == 0
+    # all lifetimes distinct and non-zero -> numpy raises (np.min of empty)
+    with pytest.raises(ValueError):
+        O.sp_sl_lifetime(u, v, np.array([1., 2., 4.]), 4)
+    # two disconnected parts: n_disc=2; max lifetime 4 vs disconnect lifetime
+    u2, v2 = np.array([0, 1, 3, 4]), np.array([1, 2, 4, 5])
+    w2 = np.array([1., 1., 1., 5.])
+    # max_dist=None -> 6; disconnect_lifetime = 6 + 1 - 5 = 2 < 4 -> cont = 2 + 1
+    nc, lab = O.sp_sl_lifetime(u2, v2, w2, 6)
+    assert nc == 3 and lab.tolist() == [0, 0, 0, 1, 1, 2]
+    # max_dist=10 -> 11; disconnect lifetime 11 + 1 - 5 = 7 > 4 -> cont = n_disc
+    nc, lab = O.sp_sl_lifetime(u2, v2, w2, 6, max_dist=10)
+    assert nc == 2 and lab.tolist() == [0, 0, 0, 1, 1, 1]
+    # explicit n_clusters=4 cuts the 2 largest (w,min,max): (5,4,5) and (1,3,4)
+    nc, lab = O.sp_sl_lifetime(u2, v2, w2, 6, n_clusters=4)
+    assert nc == 4 and lab.tolist() == [0, 0, 0, 1, 2, 3]
+
+
+def test_sl_mst_lifetime_hand_cases():
+    # sorted forest weights [1,1,5]: lifetimes [0,4], a=1, max_lt=4 >=
+    # disconnect(5) - 5 = 0 -> keep mst[:1] (singlelink.py:43-48 drops m-a=2)
+    u, v, w = np.array([0, 1, 2]), np.array([1, 2, 3]), np.array([1., 1., 5.])
+    lab, nc, kept = O.sl_mst_lifetime(u, v, w, 4)
+    assert kept == 1 and nc == 3 and lab.tolist() == [0, 0, 1, 2]
+    # disconnect weight 100: 4 < 95 -> no cut, one component
+    lab, nc, kept = O.sl_mst_lifetime(u, v, w, 4, disconnect_weight=100)
+    assert kept == 3 and nc == 1
+    with pytest.raises(ValueError):
+        O.sl_mst_lifetime(np.array([0]), np.array([1]), np.array([1.]), 2)
+
+
+# ---------------------------------------------------------- co-association
+def _dense_coassoc(labels):
+    P, n = labels.shape
+    C = np.zeros((n, n), np.int64)
+    for p in range(P):
+        for c in np.unique(labels[p]):
+            idx = np.nonzero(labels[p] == c)[0]
+            np.add.at(C, (idx[:, None], idx[None, :]), 1)
+    np.fill_diagonal(C, 0)
+    return C
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_coassoc_condensed_vs_dense(seed):
+    rng = np.random.default_rng(seed)
+    n, P = 300, 12
+    kp = rng.integers(3, 20, P)
+    labels = np.stack([rng.integers(0, k, n) for k in kp])
+    indptr, ind, cnt = O.coassoc_condensed(labels, kp)
+    D = np.triu(_dense_coassoc(labels), 1)
+    r, c = np.nonzero(D)
+    assert np.array_equal(ind, c) and np.array_equal(cnt, D[r, c])
+    assert np.array_equal(np.diff(indptr), np.bincount(r, minlength=n))
+
+
+# ---------------------------------------------------------------- K-Means
+def test_kmeans_label_f64_vs_numpy():
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((500, 7)).astype(np.float32)
+    Cc = X[rng.choice(500, 13, replace=False)]
+    lab, dist = O.kmeans_label_f64(X, Cc)
+    D = ((X[:, None, :].astype(np.float64) - Cc[None].astype(np.float64)) ** 2).sum(-1)
+    assert np.array_equal(lab, D.argmin(1))  # argmin: lowest index on ties
+    assert np.allclose(dist, D.min(1), rtol=0, atol=1e-9)
+    # exact ties go to the lowest k (strict <, KMeans.py:479)
+    C2 = np.stack([Cc[0], Cc[0], Cc[1]])
+    lab2, _ = O.kmeans_label_f64(X, C2)
+    assert not np.any(lab2 == 1)
+
+
+def test_arg_k_select_returns_smallest():
+    rng = np.random.default_rng(2)
+    a = rng.random(1000).astype(np.float32)
+    for k in (1, 5, 37):
+        out = O.arg_k_select(a, k)
+        assert set(out.tolist()) == set(np.argsort(a, kind="stable")[:k].tolist())
+
+
+def test_recompute_centroids_closed_form():
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((1000, 5)).astype(np.float32)
+    lab = rng.integers(0, 9, 1000).astype(np.int32)
+    lab[lab == 4] = 3  # cluster 4 empty
+    dists = rng.random(1000).astype(np.float32)
+    out = O.recompute_centroids(X, lab, dists, 9, reseed_mode=1)
+    for c in range(9):
+        idx = np.nonzero(lab == c)[0]
+        if idx.size:
+            s = np.zeros(5, np.float32)
+            for i in idx:  # float32 sequential sum in point order
+                s = (s + X[i]).astype(np.float32)
+            assert np.array_equal(out[c], (s.astype(np.float64) / idx.size).astype(np.float32))
+    assert np.array_equal(out[4], X[np.argmin(dists)])
