@@ -129,6 +129,44 @@ int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
                              int32_t* labels, int64_t* h_ncomp, void* ws,
                              size_t ws_bytes, void* stream);
 
+/* ---- single-link lifetime cut -----------------------------------------
+ * Minimum spanning forest (as eac_boruvka_mst), its edges in ascending
+ * (weight, min, max) order, the largest-lifetime cut, and the components of
+ * the kept forest edges as first-occurrence labels.
+ * rule = EAC_CUT_EAC: MyML/MyML/EAC/eac.py:199-267 sp_sl_lifetime with
+ *   lifetime_n_clusters (eac.py:366-391); max_dist < 0 means None; n_clusters
+ *   = 0 uses the lifetime rule. Forest edges of weight 0 leave the forest
+ *   (Kruskal's eliminate_zeros). When the reference returns an empty label
+ *   array (one cluster), h_out[1] = 1 and labels are not written.
+ * rule = EAC_CUT_SINGLELINK: MyML/MyML/graph/singlelink.py:14-65
+ *   sl_mst_lifetime_seq (and the intended GPU twin :69-153); max_dist is
+ *   disconnect_weight (< 0: None = largest graph weight); the forest keeps
+ *   its a smallest edges when the largest lifetime (first index a) is >=
+ *   disconnect_weight - largest forest weight (the reference drops m - a).
+ * Ties at the cut follow the (weight, min, max) order (the reference uses
+ * numpy's unstable argsort). Returns EAC_ERR_VALUE where the reference raises
+ * ValueError. h_out[6] (host): n_components, labels_empty, nc_stable,
+ * kept forest edges, forest edges, Boruvka rounds.
+ */
+#define EAC_CUT_EAC 0
+#define EAC_CUT_SINGLELINK 1
+size_t eac_sl_lifetime_workspace_bytes(int32_t n, int64_t m);
+int eac_sl_lifetime(const int32_t* dest, const void* weight, int32_t weight_dtype,
+                    const int64_t* first_edge, const int32_t* out_degree, int32_t n,
+                    int64_t m, int32_t symmetric, int32_t rule, double max_dist,
+                    int32_t n_clusters, int32_t* labels, int64_t* h_out, void* ws,
+                    size_t ws_bytes, void* stream);
+
+/* ---- stable LSD radix sort of 64-bit keys (optional 32-bit values) ------
+ * Sorts by bits [begin_bit, end_bit). keys_out/vals_out must not alias the
+ * inputs. No reference counterpart as an API: it replaces the numpy argsorts
+ * of the MST weights (eac.py:232, singlelink.py:27) inside the cuts. */
+size_t eac_sort_workspace_bytes(int64_t n, int32_t key_bytes, int32_t val_bytes);
+int eac_radix_sort_u64(const uint64_t* keys_in, uint64_t* keys_out,
+                       const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
+                       int32_t begin_bit, int32_t end_bit, void* ws, size_t ws_bytes,
+                       void* stream);
+
 /* ---- first-occurrence relabelling --------------------------------------
  * out[v] = rank of labels_in[v] among distinct values ordered by first
  * occurrence. Requires 0 <= labels_in[v] < n_values.

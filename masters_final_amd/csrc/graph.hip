@@ -40,7 +40,11 @@
 #include <climits>
 #include <cstring>
 
+#include <type_traits>
+#include <utility>
+
 #include "prims.hpp"
+#include "sort.hpp"
 
 namespace eac {
 namespace {
@@ -734,19 +738,27 @@ int labels_from_keys(const uint64_t* keys, int64_t nk, int vb, int32_t n, int32_
   return EAC_OK;
 }
 
+struct ForestInfo {
+  int64_t n_f;      // forest edges (keys in B.fkeys, vertex order)
+  int vb;           // vertex-id bits of the key
+  int wbits;        // weight-key bits
+  uint64_t wlo;     // weight word of the lightest entry
+  uint64_t whi;     // weight word of the heaviest entry
+  int rounds;
+};
+
+// Boruvka rounds up to the forest key list (shared by the forest, lifetime
+// cut and batched entry points).
 template <typename W, bool HAS_OD>
-int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-             int32_t n, int64_t m, int symmetric, int64_t* ids, W* wout, int32_t* labels,
-             int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
-  Arena ar(ws, ws_bytes);
-  MstBuffers B = carve_mst(ar, n, m, symmetric);
-  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
+int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
+             hipStream_t s) {
   const int64_t nchunks = (m + kGTile - 1) / kGTile;
   MstState init_st;
   memset(&init_st, 0, sizeof(init_st));
   init_st.range[0] = kInf;
-  // state init through a pinned copy (stream ordered)
   {
+    // state init through a pinned copy (stream ordered)
     void* pin = pinned_scratch(sizeof(MstState));
     if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
     EAC_HIP_CHECK(hipStreamSynchronize(s));
@@ -785,7 +797,7 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                                &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
-  int pending = pr;               // record whose output-list bytes are still unknown
+  int pending = pr;  // record whose output-list bytes are still unknown
   int64_t pending_bytes = round0_bytes;
   int cur = 0;
   int64_t m_cur = 0, nroots = n;
@@ -842,43 +854,344 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_LAUNCH_CHECK();
   int32_t n_f = 0;
   EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
+  fi->n_f = n_f;
+  fi->vb = vb;
+  fi->wbits = wbits;
+  fi->wlo = wlo;
+  fi->whi = whi;
+  fi->rounds = rounds;
+  return EAC_OK;
+}
+
+template <typename W, bool HAS_OD>
+int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, int64_t m, int symmetric, int64_t* ids, W* wout, int32_t* labels,
+             int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  MstBuffers B = carve_mst(ar, n, m, symmetric);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
+  ForestInfo fi;
+  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
+  const int64_t n_f = fi.n_f;
   if (n_f > 0 && (ids || wout)) {
     k_forest_positions<W, HAS_OD><<<grid_for(n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
-        B.fkeys, n_f, dest, weight, fe, od, vb, wlo, ids, wout, &B.st->missing);
+        B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
-  int32_t n_comp = n - n_f;
-  pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n_f + 12 * (int64_t)n, s);
+  int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * n_f + 12 * (int64_t)n, s);
   if (labels) {
-    // B.hook is free now: reuse it as the union-find parent array
     // hook[] and comp[] are free after the rounds: union-find parent / roots
-    EAC_TRY(labels_from_keys(B.fkeys, n_f, vb, n, B.hook, B.comp, B.flag, B.rank, labels,
+    const int64_t ftiles = (n + kGTile - 1) / kGTile;
+    EAC_TRY(labels_from_keys(B.fkeys, n_f, fi.vb, n, B.hook, B.comp, B.flag, B.rank, labels,
                              B.tile_cnt + ftiles, ar, s));
   }
   prof_end(pr, s);
+  MstState hs;
   EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
   prof_flush();
   if (hs.missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
   if (h_out) {
     h_out[0] = n_f;
-    h_out[1] = n_comp;
-    h_out[2] = rounds;
-    h_out[3] = wbits;
+    h_out[1] = n - n_f;
+    h_out[2] = fi.rounds;
+    h_out[3] = fi.wbits;
   }
   return EAC_OK;
 }
 
+// ------------------------------------------------------------ lifetime cuts
+// Numeric weight of a weight word (inverse of word_of).
 template <typename W>
-int mst_dispatch_od(const int32_t* dest, const void* weight, const int64_t* fe,
-                    const int32_t* od, int32_t n, int64_t m, int symmetric, int64_t* ids,
-                    void* wout, int32_t* labels, int64_t* h_out, void* ws, size_t ws_bytes,
-                    hipStream_t s) {
-  if (od)
-    return mst_impl<W, true>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
-                             labels, h_out, ws, ws_bytes, s);
-  return mst_impl<W, false>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
-                            labels, h_out, ws, ws_bytes, s);
+__device__ __forceinline__ double weight_value(uint64_t word) {
+  if (std::is_same<W, float>::value) {
+    uint32_t u = (uint32_t)word;
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    return (double)__uint_as_float(u);
+  } else if (std::is_signed<W>::value) {
+    return (double)(int64_t)(word ^ (1ull << 63));
+  } else {
+    return (double)word;
+  }
 }
+
+struct CutState {
+  unsigned long long max_lt_bits;   // bits of the largest lifetime (>= 0)
+  unsigned long long min_lt_bits;   // bits of the smallest lifetime (>= 0)
+  unsigned long long argmax;        // first index of the largest lifetime
+  unsigned long long zero_w;        // forest edges of weight exactly 0
+  long long keep;                   // forest edges kept (prefix of the sorted order)
+  long long nc_stable;
+  long long err;                    // 1: the reference raises ValueError
+  long long labels_empty;
+  double th;
+};
+
+// Sorted forest weights w[i]; lifetimes w[i+1]-w[i] are >= 0, so their
+// IEEE bit patterns order like the values.
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_lifetime_stats(const uint64_t* __restrict__ sk,
+                                                        int64_t nf, int vb, uint64_t wlo,
+                                                        CutState* cs) {
+  __shared__ unsigned long long s_mx[kGB / kWave], s_mn[kGB / kWave], s_z[kGB / kWave];
+  unsigned long long mx = 0, mn = ~0ull, z = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < nf;
+       i += (int64_t)gridDim.x * kGB) {
+    double wi = weight_value<W>((sk[i] >> (2 * vb)) + wlo);
+    z += wi == 0.0;
+    if (i + 1 < nf) {
+      double wj = weight_value<W>((sk[i + 1] >> (2 * vb)) + wlo);
+      unsigned long long b = (unsigned long long)__double_as_longlong(wj - wi);
+      mx = b > mx ? b : mx;
+      mn = b < mn ? b : mn;
+    }
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    unsigned long long a = __shfl_xor(mx, off, kWave), b = __shfl_xor(mn, off, kWave);
+    mx = a > mx ? a : mx;
+    mn = b < mn ? b : mn;
+    z += __shfl_xor(z, off, kWave);
+  }
+  if (lane_id() == 0) {
+    s_mx[threadIdx.x / kWave] = mx;
+    s_mn[threadIdx.x / kWave] = mn;
+    s_z[threadIdx.x / kWave] = z;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < kGB / kWave; ++i) {
+      mx = s_mx[i] > mx ? s_mx[i] : mx;
+      mn = s_mn[i] < mn ? s_mn[i] : mn;
+      z += s_z[i];
+    }
+    atomicMax(&cs->max_lt_bits, mx);
+    atomicMin(&cs->min_lt_bits, mn);
+    if (z) atomicAdd(&cs->zero_w, z);
+  }
+}
+
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_lifetime_argmax(const uint64_t* __restrict__ sk,
+                                                         int64_t nf, int vb, uint64_t wlo,
+                                                         CutState* cs) {
+  const unsigned long long target = cs->max_lt_bits;
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i + 1 < nf;
+       i += (int64_t)gridDim.x * kGB) {
+    double wi = weight_value<W>((sk[i] >> (2 * vb)) + wlo);
+    double wj = weight_value<W>((sk[i + 1] >> (2 * vb)) + wlo);
+    if ((unsigned long long)__double_as_longlong(wj - wi) == target)
+      atomicMin(&cs->argmax, (unsigned long long)i);
+  }
+}
+
+// Drop forest edges of weight exactly 0 (Kruskal's eliminate_zeros,
+// scipy_numba/.../_min_spanning_tree.py:163-175), keeping the order.
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_mark_nonzero(const uint64_t* __restrict__ sk,
+                                                      int64_t nf, int vb, uint64_t wlo,
+                                                      int32_t* __restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < nf;
+       i += (int64_t)gridDim.x * kGB)
+    flag[i] = weight_value<W>((sk[i] >> (2 * vb)) + wlo) != 0.0;
+}
+__global__ void k_compact_flagged(const uint64_t* __restrict__ in, const int32_t* __restrict__ flag,
+                                  const int32_t* __restrict__ pos, int64_t n,
+                                  uint64_t* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (flag[i]) out[pos[i]] = in[i];
+}
+
+// Single-thread decision of eac.py:220-265 (rule 0) or
+// singlelink.py:16-48 (rule 1) from the lifetime statistics.
+template <typename W>
+__global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_t n, int vb,
+                             uint64_t wlo, uint64_t whi, int rule, double max_dist,
+                             int32_t n_clusters, CutState* cs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  cs->err = 0;
+  cs->labels_empty = 0;
+  const double gmax = weight_value<W>(whi);  // largest graph weight
+  if (nf < 2) {
+    if (rule == 1 || n_clusters == 0) { cs->err = 1; return; }
+  }
+  const int64_t a = (int64_t)cs->argmax;
+  const double th = nf >= 2 ? __longlong_as_double((long long)cs->max_lt_bits) : 0.0;
+  const double wmaxf = nf >= 1 ? weight_value<W>((sk[nf - 1] >> (2 * vb)) + wlo) : 0.0;
+  cs->th = th;
+  if (rule == 1) {
+    // singlelink.py:14-48
+    double disconnect = max_dist < 0 ? gmax : max_dist;
+    double lt_threshold = disconnect - wmaxf;
+    cs->keep = th >= lt_threshold ? a : nf;
+    cs->nc_stable = 0;
+    return;
+  }
+  // eac.py:199-267
+  double md = max_dist < 0 ? gmax + 1 : max_dist + 1;
+  int64_t n_disc = (int64_t)n - nf;
+  int64_t nc_stable;
+  if (n_clusters == 0) {
+    // lifetime_n_clusters, eac.py:366-391
+    double wa = weight_value<W>((sk[a] >> (2 * vb)) + wlo);
+    int64_t lo = a, hi = nf;  // first index with w > w[a]
+    while (lo < hi) {
+      int64_t mid = (lo + hi) >> 1;
+      if (weight_value<W>((sk[mid] >> (2 * vb)) + wlo) > wa) hi = mid; else lo = mid + 1;
+    }
+    int64_t above = nf - lo;
+    int64_t cont = above == 0 ? 1 : above + 1;
+    double mn = __longlong_as_double((long long)cs->min_lt_bits);
+    if (!(mn <= 1e-8)) { cs->err = 1; return; }  // np.min over an empty selection
+    if (th < 2 * mn) cont = 1;
+    if (n_disc > 1) {
+      double disconnect_lifetime = md + 1 - wmaxf;
+      if (th > disconnect_lifetime) cont += n_disc - 1;
+      else cont = n_disc;
+    }
+    nc_stable = cont;
+  } else {
+    nc_stable = n_clusters;
+  }
+  int64_t keep = nf;
+  if (nc_stable > n_disc) {
+    keep = nf - (nc_stable - n_disc);
+    if (keep < 0) keep = 0;
+  }
+  cs->keep = keep;
+  cs->nc_stable = nc_stable;
+  cs->labels_empty = nc_stable > 1 ? 0 : 1;
+}
+
+template <typename W, bool HAS_OD>
+int sl_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+            int32_t n, int64_t m, int symmetric, int rule, double max_dist, int32_t n_clusters,
+            int32_t* labels, int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  MstBuffers B = carve_mst(ar, n, m, symmetric);
+  uint64_t* sk = ar.take<uint64_t>(n);
+  CutState* cs = ar.take<CutState>(1);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "sl_lifetime: workspace too small");
+  ForestInfo fi;
+  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
+  int64_t nf = fi.n_f;
+  const int vb = fi.vb;
+  // canonical (w, min, max) order = ascending key order
+  EAC_TRY((radix_sort<uint64_t, NoValue>(B.fkeys, sk, nullptr, nullptr, nf, 0,
+                                         fi.wbits + 2 * vb, ar, s)));
+  CutState init;
+  memset(&init, 0, sizeof(init));
+  init.min_lt_bits = ~0ull;
+  init.argmax = ~0ull;
+  {
+    void* pin = pinned_scratch(sizeof(CutState));
+    if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+    EAC_HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(pin, &init, sizeof(init));
+    EAC_HIP_CHECK(hipMemcpyAsync(cs, pin, sizeof(CutState), hipMemcpyHostToDevice, s));
+  }
+  const int g = grid_for(nf, kGB, persistent_blocks());
+  if (nf > 0) {
+    k_lifetime_stats<W><<<g, kGB, 0, s>>>(sk, nf, vb, fi.wlo, cs);
+    EAC_LAUNCH_CHECK();
+  }
+  CutState hc;
+  if (rule == 0 && nf > 0) {
+    EAC_TRY(d2h_sync(&hc, cs, sizeof(CutState), s));
+    if (hc.zero_w > 0) {  // rare: zero weights leave the forest, stats again
+      k_mark_nonzero<W><<<g, kGB, 0, s>>>(sk, nf, vb, fi.wlo, B.flag);
+      EAC_LAUNCH_CHECK();
+      int32_t* d_cnt = B.tile_cnt;
+      EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, nf, 0, d_cnt, ar, s)));
+      k_compact_flagged<<<g, kGB, 0, s>>>(sk, B.flag, B.rank, nf, B.fkeys);
+      EAC_LAUNCH_CHECK();
+      nf -= (int64_t)hc.zero_w;
+      EAC_HIP_CHECK(hipMemcpyAsync(sk, B.fkeys, sizeof(uint64_t) * nf, hipMemcpyDeviceToDevice, s));
+      {
+        void* pin = pinned_scratch(sizeof(CutState));
+        EAC_HIP_CHECK(hipStreamSynchronize(s));
+        memcpy(pin, &init, sizeof(init));
+        EAC_HIP_CHECK(hipMemcpyAsync(cs, pin, sizeof(CutState), hipMemcpyHostToDevice, s));
+      }
+      if (nf > 0) {
+        k_lifetime_stats<W><<<g, kGB, 0, s>>>(sk, nf, vb, fi.wlo, cs);
+        EAC_LAUNCH_CHECK();
+      }
+    }
+  }
+  if (nf > 1) {
+    k_lifetime_argmax<W><<<g, kGB, 0, s>>>(sk, nf, vb, fi.wlo, cs);
+    EAC_LAUNCH_CHECK();
+  }
+  k_cut_decide<W><<<1, 64, 0, s>>>(sk, nf, n, vb, fi.wlo, fi.whi, rule, max_dist, n_clusters,
+                                   cs);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY(d2h_sync(&hc, cs, sizeof(CutState), s));
+  if (hc.err)
+    return fail(EAC_ERR_VALUE, rule == 1 ? "sl_mst_lifetime: fewer than 2 forest edges "
+                                           "(numpy argmax of an empty array)"
+                                         : "sp_sl_lifetime: lifetime rule undefined (fewer "
+                                           "than 2 forest edges or no zero lifetime)");
+  int64_t n_comp = 1;
+  const int64_t ftiles = (n + kGTile - 1) / kGTile;
+  if (!hc.labels_empty) {
+    EAC_TRY(labels_from_keys(sk, hc.keep, vb, n, B.hook, B.comp, B.flag, B.rank, labels,
+                             B.tile_cnt + ftiles, ar, s));
+    int32_t nc = 0;
+    EAC_TRY(d2h_sync(&nc, B.tile_cnt + ftiles, sizeof(int32_t), s));
+    n_comp = nc;
+  }
+  prof_flush();
+  if (h_out) {
+    h_out[0] = n_comp;
+    h_out[1] = hc.labels_empty;
+    h_out[2] = hc.nc_stable;
+    h_out[3] = hc.keep;
+    h_out[4] = nf;
+    h_out[5] = fi.rounds;
+  }
+  return EAC_OK;
+}
+
+// Calls F::template run<W, HAS_OD>(...) for the runtime weight dtype.
+template <typename F, typename... A>
+int dispatch_weight(int32_t dt, bool has_od, A&&... a) {
+#define EAC_DISPATCH_OD(T)                                                   \
+  return has_od ? F::template run<T, true>(std::forward<A>(a)...)          \
+                : F::template run<T, false>(std::forward<A>(a)...)
+  switch (dt) {
+    case EAC_W_I32: EAC_DISPATCH_OD(int32_t);
+    case EAC_W_F32: EAC_DISPATCH_OD(float);
+    case EAC_W_U16: EAC_DISPATCH_OD(uint16_t);
+    case EAC_W_U8: EAC_DISPATCH_OD(uint8_t);
+    case EAC_W_I64: EAC_DISPATCH_OD(int64_t);
+    case EAC_W_I16: EAC_DISPATCH_OD(int16_t);
+    case EAC_W_U32: EAC_DISPATCH_OD(uint32_t);
+  }
+#undef EAC_DISPATCH_OD
+  return fail(EAC_ERR_ARG, "unsupported weight dtype %d", dt);
+}
+
+struct MstRun {
+  template <typename W, bool HAS_OD>
+  static int run(const int32_t* dest, const void* weight, const int64_t* fe, const int32_t* od,
+                 int32_t n, int64_t m, int symmetric, int64_t* ids, void* wout, int32_t* labels,
+                 int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+    return mst_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
+                               labels, h_out, ws, ws_bytes, s);
+  }
+};
+
+struct SlRun {
+  template <typename W, bool HAS_OD>
+  static int run(const int32_t* dest, const void* weight, const int64_t* fe, const int32_t* od,
+                 int32_t n, int64_t m, int symmetric, int rule, double max_dist,
+                 int32_t n_clusters, int32_t* labels, int64_t* h_out, void* ws, size_t ws_bytes,
+                 hipStream_t s) {
+    return sl_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, n, m, symmetric, rule, max_dist,
+                              n_clusters, labels, h_out, ws, ws_bytes, s);
+  }
+};
 
 }  // namespace
 
@@ -924,16 +1237,40 @@ extern "C" int eac_boruvka_mst(const int32_t* dest, const void* weight, int32_t 
   if (!ws || ws_bytes < need)
     return fail(EAC_ERR_WORKSPACE, "boruvka: workspace %zu < %zu bytes", ws_bytes, need);
   symmetric = symmetric ? 1 : 0;
-  switch (weight_dtype) {
-    case EAC_W_I32: return mst_dispatch_od<int32_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_F32: return mst_dispatch_od<float>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_U16: return mst_dispatch_od<uint16_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_U8: return mst_dispatch_od<uint8_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_I64: return mst_dispatch_od<int64_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_I16: return mst_dispatch_od<int16_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-    case EAC_W_U32: return mst_dispatch_od<uint32_t>(dest, weight, first_edge, out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels, h_out, ws, ws_bytes, s);
-  }
-  return fail(EAC_ERR_ARG, "boruvka: unsupported weight dtype");
+  return dispatch_weight<MstRun>(weight_dtype, out_degree != nullptr, dest, weight, first_edge,
+                                 out_degree, n, m, symmetric, mst_edge_ids, mst_weight, labels,
+                                 h_out, ws, ws_bytes, s);
+}
+
+extern "C" size_t eac_sl_lifetime_workspace_bytes(int32_t n, int64_t m) {
+  size_t b = eac_boruvka_workspace_bytes(n, m);
+  b += ((size_t)(n < 1 ? 1 : n) * 8 + 255) & ~size_t(255);
+  b += 512 + radix_sort_ws_bytes(n < 1 ? 1 : n, 8, 0);
+  return b;
+}
+
+extern "C" int eac_sl_lifetime(const int32_t* dest, const void* weight, int32_t weight_dtype,
+                               const int64_t* first_edge, const int32_t* out_degree, int32_t n,
+                               int64_t m, int32_t symmetric, int32_t rule, double max_dist,
+                               int32_t n_clusters, int32_t* labels, int64_t* h_out, void* ws,
+                               size_t ws_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 1 || m < 0) return fail(EAC_ERR_ARG, "sl_lifetime: need n >= 1 and m >= 0");
+  if (rule != EAC_CUT_EAC && rule != EAC_CUT_SINGLELINK)
+    return fail(EAC_ERR_ARG, "sl_lifetime: unknown rule %d", rule);
+  if (n_clusters < 0) return fail(EAC_ERR_ARG, "sl_lifetime: negative n_clusters");
+  if (weight_bytes(weight_dtype) == 0)
+    return fail(EAC_ERR_ARG, "sl_lifetime: unsupported weight dtype %d", weight_dtype);
+  if (m == 0)
+    return fail(EAC_ERR_VALUE, "sl_lifetime: graph without edges (the reference raises)");
+  if (!first_edge || !dest || !weight || !labels)
+    return fail(EAC_ERR_ARG, "sl_lifetime: null array");
+  size_t need = eac_sl_lifetime_workspace_bytes(n, m);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "sl_lifetime: workspace %zu < %zu bytes", ws_bytes, need);
+  return dispatch_weight<SlRun>(weight_dtype, out_degree != nullptr, dest, weight, first_edge,
+                                out_degree, n, m, symmetric ? 1 : 0, (int)rule, max_dist,
+                                n_clusters, labels, h_out, ws, ws_bytes, s);
 }
 
 // ------------------------------------------------- connected components

@@ -1,0 +1,112 @@
+"""GPU parity of the lifetime cuts and the radix sort against the oracle."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from masters_final_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+@pytest.mark.parametrize("n,bits", [(1, 64), (1000, 20), (4096, 64), (300_000, 48),
+                                    (1_000_003, 33)])
+def test_radix_sort_u64(torch, n, bits):
+    from masters_final_amd import _lib as L
+    lib = L.load_library()
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 2**63, n, dtype=np.uint64) & np.uint64((1 << bits) - 1 if bits < 64
+                                                                  else 2**64 - 1)
+    keys[: n // 3] = keys[0]  # many duplicates: stability is visible through the values
+    vals = np.arange(n, dtype=np.uint32)
+    dk = torch.from_numpy(keys.view(np.int64)).cuda()
+    dv = torch.from_numpy(vals.view(np.int32)).cuda()
+    ok = torch.empty_like(dk)
+    ov = torch.empty_like(dv)
+    ws = L.workspace(lib.eac_sort_workspace_bytes(n, 8, 4))
+    L.check(lib.eac_radix_sort_u64(L.ptr(dk), L.ptr(ok), L.ptr(dv), L.ptr(ov), n, 0, bits,
+                                   L.ptr(ws), ws.numel(), L.stream_ptr()))
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ok.cpu().numpy().view(np.uint64), keys[order])
+    assert np.array_equal(ov.cpu().numpy().view(np.uint32), vals[order])
+
+
+def _graph_edges(g):
+    fe, od = g["indptr"][:-1], np.diff(g["indptr"])
+    return O.csr_edges(g["dest"], g["weight"], fe, od)
+
+
+@pytest.mark.parametrize("seed,n,deg,wmax,iso,parts", [
+    (0, 200, 3, 4, 2, 1), (1, 2000, 5, 30, 10, 3), (2, 20000, 6, 100, 40, 5),
+    (3, 5000, 2, 7, 200, 10)])
+def test_sl_mst_lifetime_vs_oracle(torch, seed, n, deg, wmax, iso, parts):
+    from masters_final_amd.graph.singlelink import lifetime_cut, sl_mst_lifetime_seq
+    g = synth.random_graph(n, deg, seed, wmax=wmax, n_isolated=iso, n_parts=parts)
+    u, v, w, _ = _graph_edges(g)
+    for dw in (None, float(wmax) * 3):
+        want, nc, kept = O.sl_mst_lifetime(u, v, w, n, disconnect_weight=dw)
+        lab, info = lifetime_cut(g["dest"], g["weight"], g["indptr"], None, rule=1, max_dist=dw)
+        assert info["kept_edges"] == kept and info["n_components"] == nc
+        assert np.array_equal(lab.cpu().numpy(), want)
+    got = sl_mst_lifetime_seq(g["dest"], g["weight"], g["indptr"][:-1],
+                              np.diff(g["indptr"]).astype(np.int32))
+    assert np.array_equal(got, O.sl_mst_lifetime(u, v, w, n)[0])
+
+
+@pytest.mark.parametrize("seed,n,deg,wmax,iso,parts", [
+    (5, 300, 4, 6, 3, 1), (6, 3000, 5, 40, 20, 4), (7, 30000, 8, 100, 0, 1),
+    (8, 8000, 3, 9, 100, 6)])
+def test_sp_sl_lifetime_vs_oracle(torch, seed, n, deg, wmax, iso, parts):
+    from masters_final_amd.EAC.eac import sp_sl_lifetime
+    g = synth.random_graph(n, deg, seed, wmax=wmax, n_isolated=iso, n_parts=parts)
+    u, v, w, _ = _graph_edges(g)
+    # the driver passes the condensed upper triangle
+    up = sp.csr_matrix((w.astype(np.int32), (u, v)), shape=(n, n))
+    up.sort_indices()
+    for md, ncl in ((None, 0), (wmax + 5, 0), (None, 7)):
+        try:
+            want_nc, want = O.sp_sl_lifetime(u, v, w, n, max_dist=md, n_clusters=ncl)
+        except ValueError:
+            with pytest.raises(ValueError):
+                sp_sl_lifetime(up, md, n_clusters=ncl)
+            continue
+        nc, lab = sp_sl_lifetime(up, md, n_clusters=ncl)
+        assert nc == want_nc
+        assert np.array_equal(lab, want)
+
+
+def test_sp_sl_lifetime_4elt_known_answer(torch, elt4):
+    from masters_final_amd.EAC.eac import sp_sl_lifetime
+    g = elt4
+    fe, od = g["fe"], g["od"]
+    u, v, w, _ = O.csr_edges(g["dest"], g["weight"], fe, od)
+    up = sp.csr_matrix((w.astype(np.uint16), (u, v)), shape=(g["n"], g["n"]))
+    nc, lab = sp_sl_lifetime(up)
+    assert nc == 6
+    assert sorted(np.bincount(lab).tolist(), reverse=True) == [15600, 2, 1, 1, 1, 1]
+    assert np.array_equal(lab, O.sp_sl_lifetime(u, v, w, g["n"])[1])
+
+
+def test_sp_sl_lifetime_hand_cases(torch):
+    from masters_final_amd.EAC.eac import sp_sl_lifetime
+
+    def mat(u, v, w, n):
+        return sp.csr_matrix((np.asarray(w, np.int32), (u, v)), shape=(n, n))
+    nc, lab = sp_sl_lifetime(mat([0, 1, 2], [1, 2, 3], [1, 1, 5], 4))
+    assert nc == 2 and lab.tolist() == [0, 0, 0, 1]
+    nc, lab = sp_sl_lifetime(mat([0, 1, 2], [1, 2, 3], [2, 2, 2], 4))
+    assert nc == 1 and lab.size == 0
+    with pytest.raises(ValueError):
+        sp_sl_lifetime(mat([0, 1, 2], [1, 2, 3], [1, 2, 4], 4))
+    m2 = mat([0, 1, 3, 4], [1, 2, 4, 5], [1, 1, 1, 5], 6)
+    assert sp_sl_lifetime(m2)[1].tolist() == [0, 0, 0, 1, 1, 2]
+    assert sp_sl_lifetime(m2, 10)[1].tolist() == [0, 0, 0, 1, 1, 1]
+    assert sp_sl_lifetime(m2, n_clusters=4)[1].tolist() == [0, 0, 0, 1, 2, 3]
