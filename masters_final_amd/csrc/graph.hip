@@ -38,6 +38,7 @@
 //    every root is its component's smallest vertex and first-occurrence
 //    labels follow from one scan of root flags.
 #include <climits>
+#include <cstddef>
 #include <cstring>
 
 #include <type_traits>
@@ -234,6 +235,36 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
   }
 }
 
+// Histogram of weight words over every 8th entry (4096 bins of the range
+// found by k_weight_range), used to pick the light/heavy split threshold.
+// Entries in row gaps may be sampled; that only moves the threshold.
+constexpr int kHistBins = 4096;
+constexpr int kHistStride = 8;
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weight, int64_t m,
+                                                     const unsigned long long* __restrict__ range,
+                                                     unsigned int* __restrict__ hist) {
+  __shared__ unsigned int h[kHistBins];
+  for (int i = threadIdx.x; i < kHistBins; i += kGB) h[i] = 0;
+  __syncthreads();
+  const uint64_t lo = range[0], hi = range[1];
+  if (lo <= hi) {
+    int wb = 0;
+    while (wb < 64 && ((hi - lo) >> wb) != 0) ++wb;
+    const int shift = wb > 12 ? wb - 12 : 0;
+    const int64_t ns = (m + kHistStride - 1) / kHistStride;
+    for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < ns;
+         i += (int64_t)gridDim.x * kGB) {
+      uint64_t w = word_of<W>(weight[i * kHistStride]);
+      if (w < lo || w > hi) continue;
+      atomicAdd(&h[(w - lo) >> shift], 1u);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kHistBins; i += kGB)
+    if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
 // -------------------------------------------------------- Boruvka round 0
 // Streams the CSR: builds the undirected edge list and every vertex's
 // lightest edge key.
@@ -242,9 +273,10 @@ __global__ __launch_bounds__(kGB) void k_round0(
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
     const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int32_t n, int64_t m,
     const int32_t* __restrict__ chunk_row, int64_t nchunks, int vb, uint64_t word_lo,
-    int symmetric, unsigned long long* __restrict__ best, uint64_t* __restrict__ list,
-    unsigned long long* __restrict__ list_count, int64_t list_cap,
-    unsigned int* __restrict__ overflow) {
+    int symmetric, uint64_t light_lim, unsigned long long* __restrict__ best,
+    uint64_t* __restrict__ list, unsigned long long* __restrict__ list_count,
+    uint64_t* __restrict__ heavy, unsigned long long* __restrict__ heavy_count,
+    int64_t list_cap, unsigned int* __restrict__ overflow) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ unsigned long long s_best[kMaxRows];
   __shared__ uint64_t s_out[kGTile];
@@ -256,7 +288,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
       for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) s_best[i] = kInf;
     __syncthreads();
     uint64_t keys[kGI];
-    uint32_t keep = 0;
+    uint32_t keep = 0, light = 0;
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
       keys[j] = kInf;
@@ -271,8 +303,12 @@ __global__ __launch_bounds__(kGB) void k_round0(
         continue;
       }
       if (v == u) continue;  // self loops never join a forest
-      uint64_t key = pack_key(word_of<W>(weight[e]) - word_lo, (uint32_t)u, (uint32_t)v, vb);
+      uint64_t wk = word_of<W>(weight[e]) - word_lo;
+      uint64_t key = pack_key(wk, (uint32_t)u, (uint32_t)v, vb);
       keys[j] = key;
+      if (!symmetric || v > u) keep |= 1u << j;
+      if (wk >= light_lim) continue;  // heavy: listed, not a round-0 candidate
+      light |= 1u << j;
       if (cx.staged) {
         atomicMin(&s_best[ri], (unsigned long long)key);
       } else if (key < best[u]) {
@@ -281,19 +317,26 @@ __global__ __launch_bounds__(kGB) void k_round0(
       if (!symmetric) {
         if (key < best[v]) atomicMin(&best[v], (unsigned long long)key);
       }
-      if (!symmetric || v > u) keep |= 1u << j;
     }
-    uint32_t total;
-    uint32_t pre = block_reserve(__popc(keep), list_count, s_tmp, &s_base, &total);
+    // light edges -> list, heavy edges -> heavy (each one atomic per block)
 #pragma unroll
-    for (int j = 0; j < kGI; ++j)
-      if (keep & (1u << j)) s_out[pre++] = keys[j];
-    __syncthreads();
-    unsigned long long base = s_base;
-    if (base + total > (unsigned long long)list_cap) {
-      if (threadIdx.x == 0) atomicOr(overflow, 1u);
-    } else {
-      for (uint32_t i = threadIdx.x; i < total; i += kGB) list[base + i] = s_out[i];
+    for (int pass = 0; pass < 2; ++pass) {
+      uint32_t sel = pass == 0 ? (keep & light) : (keep & ~light);
+      uint32_t total;
+      uint32_t pre = block_reserve(__popc(sel), pass == 0 ? list_count : heavy_count, s_tmp,
+                                   &s_base, &total);
+#pragma unroll
+      for (int j = 0; j < kGI; ++j)
+        if (sel & (1u << j)) s_out[pre++] = keys[j];
+      __syncthreads();
+      unsigned long long base = s_base;
+      uint64_t* dst = pass == 0 ? list : heavy;
+      if (base + total > (unsigned long long)list_cap) {
+        if (threadIdx.x == 0) atomicOr(overflow, 1u);
+      } else {
+        for (uint32_t i = threadIdx.x; i < total; i += kGB) dst[base + i] = s_out[i];
+      }
+      __syncthreads();
     }
     if (cx.staged) {
       for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) {
@@ -620,15 +663,26 @@ __global__ void k_gather_rank(const int32_t* __restrict__ root,
     out[i] = rank[root[i]];
 }
 
-// First occurrence of each label value (atomicMin with a read-first filter so
-// large clusters settle after their first few waves).
+// First occurrence of each label value. Lanes of a wave hold consecutive
+// indices, so per distinct label in the wave only its lowest lane issues the
+// atomicMin: a giant cluster costs one atomic per wave instead of one per
+// vertex (same-address atomics from all XCDs serialise).
 __global__ void k_first_occ(const int32_t* __restrict__ lab, int32_t n, int32_t nv,
                             int32_t* first, unsigned int* bad) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    int32_t l = lab[i];
-    if (l < 0 || l >= nv) { atomicOr(bad, 1u); continue; }
-    if ((int32_t)i < first[l]) atomicMin(first + l, (int32_t)i);
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i = base + threadIdx.x;
+    int32_t l = i < n ? lab[i] : -1;
+    bool ok = i < n && l >= 0 && l < nv;
+    if (i < n && !ok) atomicOr(bad, 1u);
+    uint64_t pending = __ballot(ok);
+    while (pending) {
+      int leader = __ffsll((unsigned long long)pending) - 1;
+      int32_t ll = __shfl(l, leader, kWave);
+      uint64_t same = __ballot(ok && l == ll) & pending;
+      if (lane_id() == leader && (int32_t)i < first[ll]) atomicMin(first + ll, (int32_t)i);
+      pending &= ~same;
+    }
   }
 }
 __global__ void k_first_flag(const int32_t* __restrict__ lab, const int32_t* __restrict__ first,
@@ -668,12 +722,15 @@ int weight_bytes(int32_t dt) {
 // Device state counters of one MST call.
 struct MstState {
   unsigned long long range[2];
+  unsigned long long heavy_count;
   unsigned long long list_count[2];
   unsigned long long root_count[2];
   unsigned long long hooks;
   unsigned int overflow;
   unsigned int missing;
+  unsigned int hist[4096];  // last: per-round reads stop before it
 };
+constexpr size_t kStateHead = offsetof(MstState, hist);
 
 struct MstBuffers {
   int32_t* chunk_row;
@@ -683,6 +740,7 @@ struct MstBuffers {
   int32_t* hook;
   int32_t* roots[2];
   uint64_t* list[2];
+  uint64_t* heavy;
   int64_t list_cap;
   uint64_t* fkeys;
   int32_t* tile_cnt;
@@ -710,6 +768,7 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   b.list_cap = list_capacity(m, symmetric);
   b.list[0] = ar.take<uint64_t>(b.list_cap);
   b.list[1] = ar.take<uint64_t>(b.list_cap);
+  b.heavy = ar.take<uint64_t>(b.list_cap);
   b.fkeys = ar.take<uint64_t>(n);
   b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 1);
   b.flag = ar.take<int32_t>(n);
@@ -771,6 +830,17 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   k_weight_range<W, HAS_OD><<<gchunks, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
                                                      B.st->range);
   EAC_LAUNCH_CHECK();
+  // Light/heavy split (Filter-Kruskal applied to Boruvka): rounds run first
+  // on the light prefix of the key order, whose forest is exactly the
+  // forest's light part; heavy edges then need one filtering pass against
+  // the light components before the remaining rounds. Only worth it on
+  // graphs with many edges per vertex.
+  const bool try_split = m >= 6 * (int64_t)n && m >= (1 << 16);
+  if (try_split) {
+    k_weight_hist<W><<<grid_for(m / kHistStride + 1, kGB, 1024), kGB, 0, s>>>(
+        weight, m, B.st->range, B.st->hist);
+    EAC_LAUNCH_CHECK();
+  }
   MstState hs;
   EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
   uint64_t wlo = hs.range[0], whi = hs.range[1];
@@ -780,6 +850,22 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   if (wbits + 2 * vb > 64)
     return fail(EAC_ERR_KEY_WIDTH,
                 "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)", wbits, vb);
+  uint64_t light_lim = kInf;
+  if (try_split) {
+    // about 1.5 light undirected edges per vertex: 3n sampled directed entries
+    // for a symmetric CSR, 1.5n for a one-entry-per-edge CSR
+    const double target = (symmetric ? 3.0 : 1.5) * n / kHistStride;
+    const int shift = wbits > 12 ? wbits - 12 : 0;
+    double cum = 0;
+    for (int b = 0; b < kHistBins; ++b) {
+      cum += hs.hist[b];
+      if (cum >= target) {
+        light_lim = (uint64_t)(b + 1) << shift;
+        break;
+      }
+    }
+    if (light_lim != kInf && light_lim > whi - wlo) light_lim = kInf;
+  }
 
   EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
@@ -792,8 +878,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       8 * (int64_t)n;
   int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
-                                               vb, wlo, symmetric, B.best, B.list[0],
-                                               &B.st->list_count[0], B.list_cap,
+                                               vb, wlo, symmetric, light_lim, B.best, B.list[0],
+                                               &B.st->list_count[0], B.heavy,
+                                               &B.st->heavy_count, B.list_cap,
                                                &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
@@ -803,6 +890,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   int64_t m_cur = 0, nroots = n;
   const int32_t* roots_in = nullptr;  // round 0: every vertex
   int rounds = 0;
+  bool heavy_done = false;
   for (;;) {
     ++rounds;
     pr = prof_begin(EAC_PROF_MST_HOOK, 12 * nroots, s);
@@ -816,8 +904,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                                                     &B.st->overflow);
     EAC_LAUNCH_CHECK();
     prof_end(pr, s);
-    EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
-    prof_set_bytes(pending, pending_bytes + 8 * (int64_t)hs.list_count[cur]);
+    EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
+    prof_set_bytes(pending, pending_bytes + 8 * (int64_t)hs.list_count[cur] +
+                                (rounds == 1 ? 8 * (int64_t)hs.heavy_count : 0));
     if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
     if (hs.overflow & 4u)
       return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
@@ -827,8 +916,22 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                "pass symmetric=0");
     m_cur = (int64_t)hs.list_count[cur];
     nroots = (int64_t)hs.root_count[cur];
-    if (nroots == 0 || m_cur == 0 || rounds > 64) break;
-    // next round: filter list[cur] -> list[nxt], recompute minima
+    if (rounds > 64) break;
+    const uint64_t* src = B.list[cur];
+    if (nroots == 0 || m_cur == 0) {
+      if (heavy_done || hs.heavy_count == 0) break;
+      // light forest complete: filter the heavy edges against its components;
+      // every vertex is offered to the next hooking pass (non-representatives
+      // carry no minimum and only mark themselves).
+      heavy_done = true;
+      src = B.heavy;
+      m_cur = (int64_t)hs.heavy_count;
+      roots_in = nullptr;
+      nroots = n;
+    } else {
+      roots_in = B.roots[cur];
+    }
+    // next round: filter src -> list[nxt], recompute minima
     int nxt = cur ^ 1;
     // reset the counters that the next round appends to
     EAC_TRY(fill<uint64_t>((uint64_t*)&B.st->list_count[nxt], 1, 0ull, s));
@@ -836,10 +939,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     pending_bytes = 8 * m_cur + 4 * (int64_t)n;
     pending = prof_begin(EAC_PROF_MST_ROUND, pending_bytes, s);
     k_round<<<grid_for((m_cur + kGTile - 1) / kGTile, 1, persistent_blocks()), kGB, 0, s>>>(
-        B.list[cur], m_cur, B.list[nxt], &B.st->list_count[nxt], B.comp, B.best, vb);
+        src, m_cur, B.list[nxt], &B.st->list_count[nxt], B.comp, B.best, vb);
     EAC_LAUNCH_CHECK();
     prof_end(pending, s);
-    roots_in = B.roots[cur];
     cur = nxt;
   }
   if (rounds > 64) return fail(EAC_ERR_HIP, "boruvka: no convergence after 64 rounds");
@@ -878,16 +980,26 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
         B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
-  int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * n_f + 12 * (int64_t)n, s);
+  int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
   if (labels) {
-    // hook[] and comp[] are free after the rounds: union-find parent / roots
+    // comp[] holds every vertex's final representative: number components
+    // by their smallest vertex (first occurrence)
+    int32_t* first = B.roots[0];
+    int32_t* root = B.hook;
     const int64_t ftiles = (n + kGTile - 1) / kGTile;
-    EAC_TRY(labels_from_keys(B.fkeys, n_f, fi.vb, n, B.hook, B.comp, B.flag, B.rank, labels,
-                             B.tile_cnt + ftiles, ar, s));
+    int g = grid_for(n, 256, 4096);
+    EAC_TRY(fill<int32_t>(first, n, INT32_MAX, s));
+    k_first_occ<<<g, 256, 0, s>>>(B.comp, n, n, first, &B.st->missing);
+    EAC_LAUNCH_CHECK();
+    k_first_flag<<<g, 256, 0, s>>>(B.comp, first, n, n, root, B.flag);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles, ar, s)));
+    k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
+    EAC_LAUNCH_CHECK();
   }
   prof_end(pr, s);
   MstState hs;
-  EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
+  EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
   prof_flush();
   if (hs.missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
   if (h_out) {

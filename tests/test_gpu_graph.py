@@ -207,3 +207,18 @@ def test_canonical_relabel(torch):
     out, cnt = canonical_relabel(lab.astype(np.int32), 500)
     want, wc = O.canonical_relabel(lab)
     assert cnt == wc and np.array_equal(out.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("seed,n,deg,wmax,parts,fdt", [
+    (11, 20000, 16, 100, 1, np.int32), (12, 30000, 20, 3, 4, np.int32),
+    (13, 15000, 24, 1000, 2, np.float32), (14, 40000, 12, 255, 1, np.uint8)])
+def test_boruvka_dense_graphs_light_heavy_split(torch, seed, n, deg, wmax, parts, fdt):
+    """Graphs with >= 6 entries per vertex take the light/heavy split path."""
+    from masters_final_amd.graph.mst import boruvka_mst
+    g = synth.random_graph(n, deg, seed, wmax=wmax, n_parts=parts, dtype=fdt)
+    if fdt == np.float32:
+        g["weight"] = (g["weight"] * 0.37).astype(np.float32)
+    assert g["dest"].size >= 6 * n
+    r = boruvka_mst(g["dest"], g["weight"], g["indptr"], None, want_weights=True,
+                    want_labels=True)
+    _check_forest(g, r)
