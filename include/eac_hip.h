@@ -64,6 +64,8 @@ int eac_abi_version(void);
 #define EAC_PROF_MST_JUMP 3   /* 8 B/vertex read + 8 B/representative reset */
 #define EAC_PROF_MST_FINAL 4  /* forest output + labels */
 #define EAC_PROF_CC 5         /* union-find over the CSR: 4 B/entry + 8 B/vertex */
+#define EAC_PROF_KM_LABEL 6   /* labeling pass; "bytes" = 2 n d sum(k) FLOPs     */
+#define EAC_PROF_KM_RESCORE 7 /* FP64 re-scoring; "bytes" = 3 d k FLOPs/point   */
 int eac_profile_enable(int32_t on);
 int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t* alg_bytes);
 
@@ -166,6 +168,46 @@ int eac_radix_sort_u64(const uint64_t* keys_in, uint64_t* keys_out,
                        const uint32_t* vals_in, uint32_t* vals_out, int64_t n,
                        int32_t begin_bit, int32_t end_bit, void* ws, size_t ws_bytes,
                        void* stream);
+
+/* ---- K-Means labeling and centroid update ------------------------------
+ * X: n x d float32 row-major (the reference transposes to D x N for its
+ * kernel, KMeans.py:321; the row-major layout feeds the MFMA fragments
+ * directly). P partitions share X; partition p has centroids
+ * C[h_koff[p] : h_koff[p+1]] (k_p x d float32, row-major), h_koff a HOST
+ * array of P+1 offsets with h_koff[0] = 0. labels/dists are [P][n].
+ * Modes:
+ *   EAC_KM_F32     plain FP32 direct difference (the plain GPU baseline);
+ *   EAC_KM_F64     FP64 direct difference, ascending dimensions, strict < ->
+ *                  exactly MyML/MyML/cluster/KMeans.py:453-490 numba_label
+ *                  evaluated in FP64 (the oracle);
+ *   EAC_KM_BF16X3  split-BF16 MFMA contraction with fp32 accumulation, a
+ *                  rigorous per-point error bound, FP64 re-scoring of every
+ *                  point whose top-2 margin is within it -> labels and
+ *                  distances equal EAC_KM_F64's (d < 16 runs EAC_KM_F64).
+ * eac_kmeans_label replaces KMeans.py:296-412 _cu_label (kernel
+ * KMeans.py:586-629); h_out[0] = points re-scored in FP64.
+ * eac_kmeans_update replaces KMeans.py:709-754 numba_recompute_centroids
+ * (fp32 sums in point order, / count in double); empty clusters take the
+ * points of smallest (dist, index) in ascending cluster order (the
+ * reference's quickselect, sorting.py:8-46, returns the same set in its own
+ * order); h_out[0] = clusters re-seeded.
+ * eac_kmeans_fit replaces KMeans.py:117-189 K_Means.fit with given initial
+ * centroids (C_inout), batched: each partition stops on its own when
+ * |sum(dists) - previous| <= tol or after max_iter labelings. */
+#define EAC_KM_F32 0
+#define EAC_KM_F64 1
+#define EAC_KM_BF16X3 2
+size_t eac_kmeans_workspace_bytes(int64_t n, int32_t d, int64_t k_total, int32_t P);
+int eac_kmeans_label(const float* X, int64_t n, int32_t d, const float* C,
+                     const int64_t* h_koff, int32_t P, int32_t mode, int32_t* labels,
+                     float* dists, int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
+int eac_kmeans_update(const float* X, int64_t n, int32_t d, const int32_t* labels,
+                      const float* dists, const int64_t* h_koff, int32_t P, float* C_new,
+                      int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
+int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_inout,
+                   const int64_t* h_koff, int32_t P, int32_t max_iter, double tol,
+                   int32_t mode, int32_t* labels, float* dists, int32_t* h_iters,
+                   int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
 
 /* ---- first-occurrence relabelling --------------------------------------
  * out[v] = rank of labels_in[v] among distinct values ordered by first

@@ -50,6 +50,11 @@ SIGNATURES = {
                                   _i32, _p, _p, _p, _sz, _p]),
     "eac_sort_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "eac_radix_sort_u64": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, _p, _sz, _p]),
+    "eac_kmeans_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),
+    "eac_kmeans_label": (C.c_int, [_p, _i64, _i32, _p, _p, _i32, _i32, _p, _p, _p, _p, _sz, _p]),
+    "eac_kmeans_update": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _i32, _p, _p, _p, _sz, _p]),
+    "eac_kmeans_fit": (C.c_int, [_p, _i64, _i32, _p, _p, _i32, _i32, C.c_double, _i32, _p, _p,
+                                 _p, _p, _p, _sz, _p]),
     "eac_relabel_workspace_bytes": (_sz, [_i32, _i32]),
     "eac_canonical_relabel": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _sz, _p]),
 }

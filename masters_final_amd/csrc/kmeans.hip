@@ -1,0 +1,883 @@
+// K-Means labeling and centroid update for gfx950.
+//
+// Reference: MyML/MyML/cluster/KMeans.py:117-189 (K_Means.fit),
+// KMeans.py:453-490 (py_label/numba_label: direct squared differences, strict
+// < so the lowest k wins exact ties), KMeans.py:586-629 (the CUDA labeling
+// kernel) and KMeans.py:709-754 (centroid recomputation: float32 sums in
+// point order, division by the count in double, empty clusters reseeded with
+// the points of smallest distance).
+//
+// Labeling modes (all batched over P partitions that share the data, each
+// with its own centroid range [k_off[p], k_off[p+1]) of one centroid array):
+//  * F32: plain FP32 direct difference, one thread per point, centroid tiles
+//    staged in LDS (the "plain per-call FP32" path).
+//  * F64: FP64 direct difference in ascending dimension order with strict <,
+//    i.e. the oracle's arithmetic; exact by construction.
+//  * BF16X3: the distance ||x||^2 - 2 x.c + ||c||^2 with the contraction on
+//    MFMA. Each operand is split into two bf16 halves (x = xh + xl + ex,
+//    xh = bf16(x), xl = bf16(x - xh)); three 16x16x32 bf16 MFMAs
+//    (xh.ch + xh.cl + xl.ch) accumulate in fp32. The epilogue keeps the two
+//    smallest distances per point. A rigorous per-point bound E on
+//    |approx - FP64 oracle| (split residuals, fp32 accumulation with a
+//    truncation-safe 2^-23 unit, epilogue rounding, oracle rounding) certifies
+//    the argmin when second - best > 2E; every other point is re-scored in
+//    FP64 against all centroids of its partition. Labels therefore equal the
+//    FP64 oracle's; distances are then recomputed in FP64 for every point
+//    (d flops) so they equal the oracle's too.
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "prims.hpp"
+#include "sort.hpp"
+
+namespace eac {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Partition of a global centroid index (k_off ascending, P small).
+__device__ __forceinline__ int part_of(const int64_t* k_off, int P, int64_t c) {
+  int lo = 0, hi = P;
+  while (lo + 1 < hi) {
+    int mid = (lo + hi) >> 1;
+    if (k_off[mid] <= c) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Per-partition maxima of centroid norms, as float bits (non-negative floats
+// order like their bit patterns): [0] ||c||, [1] ||ch||, [2] ||cl||, [3] ||ec||.
+constexpr int kStat = 4;
+
+// Split every centroid into bf16 halves (row length KD, zero padded) and
+// record norms.
+__global__ void k_cent_prep(const float* __restrict__ C, int64_t ktot, int d, int KD,
+                            const int64_t* __restrict__ k_off, int P,
+                            uint16_t* __restrict__ Ch, uint16_t* __restrict__ Cl,
+                            float* __restrict__ ncf, unsigned int* __restrict__ pstat) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ktot;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    double n2 = 0, h2 = 0, l2 = 0, e2 = 0;
+    for (int j = 0; j < KD; ++j) {
+      float x = j < d ? C[c * d + j] : 0.f;
+      uint16_t h = bf16_rne(x);
+      float r = x - bf16_val(h);  // exact
+      uint16_t l = bf16_rne(r);
+      float e = r - bf16_val(l);  // exact
+      Ch[c * KD + j] = h;
+      Cl[c * KD + j] = l;
+      n2 += (double)x * x;
+      h2 += (double)bf16_val(h) * bf16_val(h);
+      l2 += (double)bf16_val(l) * bf16_val(l);
+      e2 += (double)e * e;
+    }
+    ncf[c] = (float)n2;
+    int p = part_of(k_off, P, c);
+    // round norms up so the bound stays rigorous
+    atomicMax(&pstat[p * kStat + 0], __float_as_uint((float)sqrt(n2) * (1.f + 1e-6f)));
+    atomicMax(&pstat[p * kStat + 1], __float_as_uint((float)sqrt(h2) * (1.f + 1e-6f)));
+    atomicMax(&pstat[p * kStat + 2], __float_as_uint((float)sqrt(l2) * (1.f + 1e-6f)));
+    atomicMax(&pstat[p * kStat + 3], __float_as_uint((float)sqrt(e2) * (1.f + 1e-6f)));
+  }
+}
+
+constexpr int kLB = 256;       // threads per labeling block (4 waves)
+constexpr int kRowsW = 32;     // points per wave
+constexpr int kBM = 128;       // points per block
+constexpr int kBN = 64;        // centroids per LDS tile
+
+// Rigorous bound on |approx - oracle| for one point and one partition.
+__device__ __forceinline__ float point_bound(float nx, float xn, float xhn, float xln, float exn,
+                                             const float* st, int KD, int d) {
+  const float u = 5.9604645e-08f;            // 2^-24
+  const float ut = 1.1920929e-07f;           // 2^-23: fp32 accumulation, any rounding
+  const float Cn = st[0], Chn = st[1], Cln = st[2], Cen = st[3];
+  const float nterms = 3.f * KD;
+  const float gam = nterms * ut / (1.f - nterms * ut);
+  float cross = gam * (xhn * (Chn + Cln) + xln * Chn) + xhn * Cen + xln * (Cln + Cen) + exn * Cn;
+  float epi = 6.f * u * (nx + Cn * Cn);
+  float orc = 4.f * d * 1.1102230e-16f * (xn + Cn) * (xn + Cn);
+  return (2.f * cross + epi + orc) * 1.0001f + 1e-30f;
+}
+
+template <int KD>
+__global__ __launch_bounds__(kLB) void k_label_bf16x3(
+    const float* __restrict__ X, int64_t n, int d, const uint16_t* __restrict__ Ch,
+    const uint16_t* __restrict__ Cl, const float* __restrict__ ncf,
+    const int64_t* __restrict__ k_off, int P, const unsigned int* __restrict__ pstat,
+    int32_t* __restrict__ labels, uint64_t* __restrict__ flags,
+    unsigned long long* __restrict__ nflag, int64_t flag_cap) {
+  constexpr int KB = KD / 32;
+  constexpr int LROW = KD + 8;  // padded LDS row (bf16 elements)
+  __shared__ uint16_t s_h[kBN * LROW];
+  __shared__ uint16_t s_l[kBN * LROW];
+  __shared__ float s_nc[kBN];
+  const int t = threadIdx.x;
+  const int w = t / kWave;
+  const int l = lane_id();
+  const int lr = l & 15;  // row (A layout) / column (output layout) inside a 16-block
+  const int lq = l >> 4;  // k-group (A layout) / row quad (output layout)
+  const int64_t p0 = (int64_t)blockIdx.x * kBM + w * kRowsW;
+
+  // A fragments: rows p0 + rb*16 + lr, k = kb*32 + lq*8 .. +8 (hi and lo)
+  bf16x8 Ah[2][KB], Al[2][KB];
+  float nx_a[2], xn_a[2], xhn_a[2], xln_a[2], exn_a[2];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb) {
+    int64_t row = p0 + rb * 16 + lr;
+    float s2 = 0, sh = 0, sl = 0, se = 0;
+    double s2d = 0;
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        int col = kb * 32 + lq * 8 + j;
+        float x = (row < n && col < d) ? X[row * d + col] : 0.f;
+        uint16_t h = bf16_rne(x);
+        float r = x - bf16_val(h);
+        uint16_t lo = bf16_rne(r);
+        float e = r - bf16_val(lo);
+        Ah[rb][kb][j] = __builtin_bit_cast(__bf16, h);
+        Al[rb][kb][j] = __builtin_bit_cast(__bf16, lo);
+        s2d += (double)x * x;
+        sh += bf16_val(h) * bf16_val(h);
+        sl += bf16_val(lo) * bf16_val(lo);
+        se += e * e;
+      }
+    }
+    // the 4 lanes lr, lr+16, lr+32, lr+48 hold one row
+#pragma unroll
+    for (int off = 16; off < 64; off <<= 1) {
+      s2d += __shfl_xor(s2d, off, kWave);
+      sh += __shfl_xor(sh, off, kWave);
+      sl += __shfl_xor(sl, off, kWave);
+      se += __shfl_xor(se, off, kWave);
+    }
+    nx_a[rb] = (float)s2d;
+    xn_a[rb] = (float)sqrt(s2d) * (1.f + 1e-6f);
+    xhn_a[rb] = sqrtf(sh) * (1.f + 1e-5f);
+    xln_a[rb] = sqrtf(sl) * (1.f + 1e-5f);
+    exn_a[rb] = sqrtf(se) * (1.f + 1e-5f);
+  }
+  // squared norms of the 8 output rows this lane covers: row 4*lq + i
+  float nx_o[2][4];
+#pragma unroll
+  for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) nx_o[rb][i] = __shfl(nx_a[rb], lq * 4 + i, kWave);
+
+  for (int p = 0; p < P; ++p) {
+    const int64_t kbeg = k_off[p], kend = k_off[p + 1];
+    float d1[2][4], d2[2][4];
+    int32_t k1[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) { d1[rb][i] = FLT_MAX; d2[rb][i] = FLT_MAX; k1[rb][i] = -1; }
+    for (int64_t c0 = kbeg; c0 < kend; c0 += kBN) {
+      __syncthreads();
+      // stage the tile: kBN rows of KD bf16 (hi and lo), 16-byte vectors
+      constexpr int VPR = KD / 8;  // uint4 per row
+      for (int v = t; v < kBN * VPR; v += kLB) {
+        int r = v / VPR, q = v % VPR;
+        int64_t c = c0 + r;
+        uint4 h = make_uint4(0, 0, 0, 0), lo = make_uint4(0, 0, 0, 0);
+        if (c < kend) {
+          h = reinterpret_cast<const uint4*>(Ch + c * KD)[q];
+          lo = reinterpret_cast<const uint4*>(Cl + c * KD)[q];
+        }
+        *reinterpret_cast<uint4*>(&s_h[r * LROW + q * 8]) = h;
+        *reinterpret_cast<uint4*>(&s_l[r * LROW + q * 8]) = lo;
+      }
+      if (t < kBN) s_nc[t] = (c0 + t < kend) ? ncf[c0 + t] : 0.f;
+      __syncthreads();
+      f32x4 acc[2][4];
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        bf16x8 Bh[4], Bl[4];
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          int r = cb * 16 + lr;
+          Bh[cb] = *reinterpret_cast<const bf16x8*>(&s_h[r * LROW + kb * 32 + lq * 8]);
+          Bl[cb] = *reinterpret_cast<const bf16x8*>(&s_l[r * LROW + kb * 32 + lq * 8]);
+        }
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb) {
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bh[cb],
+                                                                  acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bl[cb],
+                                                                  acc[rb][cb], 0, 0, 0);
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[rb][kb], Bh[cb],
+                                                                  acc[rb][cb], 0, 0, 0);
+          }
+      }
+      // epilogue: lane holds column cb*16 + lr, rows 4*lq + i
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        int64_t c = c0 + cb * 16 + lr;
+        if (c >= kend) continue;
+        float nc = s_nc[cb * 16 + lr];
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            float D = fmaf(-2.f, acc[rb][cb][i], nx_o[rb][i] + nc);
+            if (D < d1[rb][i]) {
+              d2[rb][i] = d1[rb][i];
+              d1[rb][i] = D;
+              k1[rb][i] = (int32_t)(c - kbeg);
+            } else if (D < d2[rb][i]) {
+              d2[rb][i] = D;
+            }
+          }
+      }
+    }
+    // merge the 16 lanes that share rows (same lq)
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+          float od1 = __shfl_xor(d1[rb][i], off, kWave);
+          float od2 = __shfl_xor(d2[rb][i], off, kWave);
+          int32_t ok1 = __shfl_xor(k1[rb][i], off, kWave);
+          float hi1 = fmaxf(d1[rb][i], od1);
+          float nd2 = fminf(fminf(d2[rb][i], od2), hi1);
+          bool take = od1 < d1[rb][i] || (od1 == d1[rb][i] && ok1 >= 0 &&
+                                          (k1[rb][i] < 0 || ok1 < k1[rb][i]));
+          if (take) { d1[rb][i] = od1; k1[rb][i] = ok1; }
+          d2[rb][i] = nd2;
+        }
+      }
+    // bounds in output layout, then write
+    float st[kStat];
+#pragma unroll
+    for (int q = 0; q < kStat; ++q) st[q] = __uint_as_float(pstat[p * kStat + q]);
+    uint64_t mine[8];
+    int nm = 0;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        int src = lq * 4 + i;
+        float xn = __shfl(xn_a[rb], src, kWave);
+        float xhn = __shfl(xhn_a[rb], src, kWave);
+        float xln = __shfl(xln_a[rb], src, kWave);
+        float exn = __shfl(exn_a[rb], src, kWave);
+        int64_t row = p0 + rb * 16 + lq * 4 + i;
+        if (lr == 0 && row < n && k1[rb][i] >= 0) {
+          float E = point_bound(nx_o[rb][i], xn, xhn, xln, exn, st, KD, d);
+          labels[(int64_t)p * n + row] = k1[rb][i];
+          if (!(d2[rb][i] - d1[rb][i] > 2.f * E)) mine[nm++] = ((uint64_t)p << 40) | (uint64_t)row;
+        }
+      }
+    // append flagged (p, row) pairs: one atomic per wave
+    uint32_t cnt = nm;
+    uint32_t incl = wave_incl_scan(cnt);
+    uint32_t tot = __shfl(incl, kWave - 1, kWave);
+    unsigned long long base = 0;
+    if (l == kWave - 1 && tot) base = atomicAdd(nflag, (unsigned long long)tot);
+    base = __shfl(base, kWave - 1, kWave);
+    for (int q = 0; q < nm; ++q) {
+      unsigned long long o = base + incl - cnt + q;
+      if (o < (unsigned long long)flag_cap) flags[o] = mine[q];
+    }
+  }
+}
+
+// FP64 re-scoring of flagged points: one wave per (p, row), lanes stride the
+// partition's centroids; strict < in ascending k, lowest k on ties.
+__global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X, int64_t n,
+                                                     int d, const float* __restrict__ C,
+                                                     const int64_t* __restrict__ k_off,
+                                                     const uint64_t* __restrict__ flags,
+                                                     int64_t nflag, int32_t* __restrict__ labels) {
+  const int l = lane_id();
+  for (int64_t f = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; f < nflag;
+       f += ((int64_t)gridDim.x * blockDim.x) / kWave) {
+    uint64_t fl = flags[f];
+    int p = (int)(fl >> 40);
+    int64_t row = (int64_t)(fl & ((1ull << 40) - 1));
+    const float* x = X + row * d;
+    const int64_t kbeg = k_off[p], kend = k_off[p + 1];
+    double best = DBL_MAX;
+    int64_t bk = -1;
+    for (int64_t c = kbeg + l; c < kend; c += kWave) {
+      const float* cc = C + c * d;
+      double dist = 0.0;
+      for (int j = 0; j < d; ++j) {
+        double diff = (double)x[j] - (double)cc[j];
+        dist += diff * diff;
+      }
+      if (bk < 0 || dist < best) { best = dist; bk = c; }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      double ob = __shfl_xor(best, off, kWave);
+      int64_t ok = __shfl_xor(bk, off, kWave);
+      if (ok >= 0 && (bk < 0 || ob < best || (ob == best && ok < bk))) { best = ob; bk = ok; }
+    }
+    if (l == 0) labels[(int64_t)p * n + row] = (int32_t)(bk - kbeg);
+  }
+}
+
+// FP64 distance of every point to its labelled centroid (oracle arithmetic),
+// stored as float32 like the reference's dists array, and per-partition
+// inertia sums in double.
+__global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X, int64_t n,
+                                                     int d, const float* __restrict__ C,
+                                                     const int64_t* __restrict__ k_off, int P,
+                                                     const int32_t* __restrict__ labels,
+                                                     float* __restrict__ dists,
+                                                     double* __restrict__ inertia) {
+  __shared__ double s_acc[256 / kWave];
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      int64_t c = k_off[p] + labels[(int64_t)p * n + i];
+      const float* x = X + i * d;
+      const float* cc = C + c * d;
+      double dist = 0.0;
+      for (int j = 0; j < d; ++j) {
+        double diff = (double)x[j] - (double)cc[j];
+        dist += diff * diff;
+      }
+      float df = (float)dist;
+      if (dists) dists[(int64_t)p * n + i] = df;
+      acc += (double)df;
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0) s_acc[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0;
+      for (int q = 0; q < 256 / kWave; ++q) s += s_acc[q];
+      if (inertia) atomicAdd(&inertia[p], s);
+    }
+    __syncthreads();
+  }
+}
+
+// Direct-difference labeling, one thread per point, the point in registers
+// (zero padded to DX dimensions) and centroid tiles staged in LDS (zero
+// padded rows, so the padding adds exact zeros). T = float: plain FP32;
+// T = double: the oracle's FP64 arithmetic (ascending dimensions, strict <).
+template <typename T, int DX>
+__global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ X, int64_t n,
+                                                      int d, const float* __restrict__ C,
+                                                      const int64_t* __restrict__ k_off, int P,
+                                                      int32_t* __restrict__ labels,
+                                                      float* __restrict__ dists) {
+  constexpr int kTile = 64;
+  __shared__ float s_c[kTile * DX];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  float xr[DX];
+#pragma unroll
+  for (int j = 0; j < DX; ++j) xr[j] = (valid && j < d) ? X[i * d + j] : 0.f;
+  for (int p = 0; p < P; ++p) {
+    const int64_t kbeg = k_off[p], kend = k_off[p + 1];
+    T best = 0;
+    int32_t bl = -1;
+    for (int64_t c0 = kbeg; c0 < kend; c0 += kTile) {
+      const int rows = (int)((kend - c0) < kTile ? (kend - c0) : kTile);
+      __syncthreads();
+      for (int v = threadIdx.x; v < kTile * DX; v += blockDim.x) {
+        int r = v / DX, j = v % DX;
+        s_c[v] = (r < rows && j < d) ? C[(c0 + r) * d + j] : 0.f;
+      }
+      __syncthreads();
+      if (valid) {
+        for (int r = 0; r < rows; ++r) {
+          T dist = 0;
+#pragma unroll
+          for (int j = 0; j < DX; ++j) {
+            T diff = (T)xr[j] - (T)s_c[r * DX + j];
+            dist += diff * diff;
+          }
+          if (bl < 0 || dist < best) { best = dist; bl = (int32_t)(c0 + r - kbeg); }
+        }
+      }
+    }
+    if (valid) {
+      labels[(int64_t)p * n + i] = bl;
+      if (dists) dists[(int64_t)p * n + i] = (float)best;
+    }
+  }
+}
+
+// Per-partition sums of float distances in double.
+__global__ __launch_bounds__(256) void k_sum_dists(const float* __restrict__ dists, int64_t n,
+                                                   int P, double* __restrict__ inertia) {
+  __shared__ double s_acc[256 / kWave];
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+      acc += (double)dists[(int64_t)p * n + i];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) s_acc[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int q = 0; q < 256 / kWave; ++q) t += s_acc[q];
+      atomicAdd(&inertia[p], t);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+int launch_direct(const float* X, int64_t n, int d, const float* C, const int64_t* k_off, int P,
+                  int32_t* labels, float* dists, hipStream_t s) {
+  int g = grid_for(n, 256);
+  if (d <= 4) k_label_direct<T, 4><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 16) k_label_direct<T, 16><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 32) k_label_direct<T, 32><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 64) k_label_direct<T, 64><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 128) k_label_direct<T, 128><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else return fail(EAC_ERR_UNSUPPORTED, "kmeans direct: d=%d > 128", d);
+  EAC_LAUNCH_CHECK();
+  return EAC_OK;
+}
+
+// ------------------------------------------------------- centroid update
+__global__ void k_label_keys(const int32_t* __restrict__ labels, int64_t n, int P,
+                             const int64_t* __restrict__ k_off, uint32_t* __restrict__ key,
+                             uint32_t* __restrict__ idx, int32_t* __restrict__ count) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)P * n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int p = (int)(t / n);
+    int64_t i = t - (int64_t)p * n;
+    uint32_t c = (uint32_t)(k_off[p] + labels[t]);
+    key[t] = c;
+    idx[t] = (uint32_t)i;
+    atomicAdd(&count[c], 1);
+  }
+}
+
+// new[c][j] = fp32 sequential sum over members in point order, / count in
+// double (KMeans.py:736-745). One thread per (centroid, dimension).
+__global__ void k_centroid_sums(const float* __restrict__ X, int d,
+                                const uint32_t* __restrict__ members,
+                                const int64_t* __restrict__ mstart,
+                                const int32_t* __restrict__ count, int64_t ktot,
+                                float* __restrict__ Cnew) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ktot * d;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t c = t / d;
+    int j = (int)(t - c * d);
+    int32_t cnt = count[c];
+    if (cnt == 0) continue;
+    const uint32_t* m = members + mstart[c];
+    float s = 0.f;
+    for (int32_t q = 0; q < cnt; ++q) s = s + X[(int64_t)m[q] * d + j];
+    Cnew[t] = (float)((double)s / (double)cnt);
+  }
+}
+
+__global__ void k_empty_flags(const int32_t* __restrict__ count, int64_t ktot,
+                              int32_t* __restrict__ flag) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ktot;
+       c += (int64_t)gridDim.x * blockDim.x)
+    flag[c] = count[c] == 0;
+}
+
+// keys (dist bits << 32 | index) of one partition: ascending = (dist, index)
+__global__ void k_dist_keys(const float* __restrict__ dists, int64_t n,
+                            uint64_t* __restrict__ keys) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    keys[i] = ((uint64_t)__float_as_uint(dists[i]) << 32) | (uint64_t)i;
+}
+
+// empty clusters c of one partition (ascending) take the points of the
+// smallest (dist, index), in that order
+__global__ void k_reseed(const float* __restrict__ X, int d, const uint64_t* __restrict__ sorted,
+                         const int32_t* __restrict__ count, const int32_t* __restrict__ erank,
+                         int64_t kbeg, int64_t kend, float* __restrict__ Cnew) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (kend - kbeg) * d;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t c = kbeg + t / d;
+    int j = (int)(t % d);
+    if (count[c] != 0) continue;
+    int32_t r = erank[c] - erank[kbeg];
+    uint32_t src = (uint32_t)(sorted[r] & 0xffffffffu);
+    Cnew[c * d + j] = X[(int64_t)src * d + j];
+  }
+}
+
+int kd_for(int d) {
+  if (d <= 32) return 32;
+  if (d <= 64) return 64;
+  if (d <= 128) return 128;
+  return 0;
+}
+
+int d2h(void* host, const void* dev, size_t bytes, hipStream_t s) {
+  void* pin = pinned_scratch(bytes);
+  if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+  EAC_HIP_CHECK(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  memcpy(host, pin, bytes);
+  return EAC_OK;
+}
+
+int h2d(void* dev, const void* host, size_t bytes, hipStream_t s) {
+  void* pin = pinned_scratch(bytes);
+  if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  memcpy(pin, host, bytes);
+  EAC_HIP_CHECK(hipMemcpyAsync(dev, pin, bytes, hipMemcpyHostToDevice, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));  // pinned buffer is reused
+  return EAC_OK;
+}
+
+struct LabelBufs {
+  int64_t* k_off;
+  uint16_t* Ch;
+  uint16_t* Cl;
+  float* ncf;
+  unsigned int* pstat;
+  uint64_t* flags;
+  unsigned long long* nflag;
+  double* inertia;
+  int64_t flag_cap;
+};
+
+LabelBufs carve_label(Arena& ar, int64_t n, int d, int64_t ktot, int P) {
+  LabelBufs b;
+  int KD = kd_for(d) ? kd_for(d) : 32;
+  b.k_off = ar.take<int64_t>(P + 1);
+  b.Ch = ar.take<uint16_t>((size_t)ktot * KD + 64);
+  b.Cl = ar.take<uint16_t>((size_t)ktot * KD + 64);
+  b.ncf = ar.take<float>(ktot);
+  b.pstat = ar.take<unsigned int>((size_t)P * kStat);
+  b.flag_cap = (int64_t)P * n;
+  b.flags = ar.take<uint64_t>(b.flag_cap);
+  b.nflag = ar.take<unsigned long long>(1);
+  b.inertia = ar.take<double>(P);
+  return b;
+}
+
+}  // namespace
+
+size_t kmeans_label_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
+  Arena ar(nullptr, 0, true);
+  (void)carve_label(ar, n, d, ktot, P);
+  return ar.used + 4096;
+}
+
+// Label all partitions; dists (nullable) receive FP64-exact distances for
+// the F64 / BF16X3 modes and fp32 distances for F32; inertia[p] (device,
+// nullable) their per-partition sums.
+int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t* h_koff,
+                 int P, int mode, int32_t* labels, float* dists, double* inertia_out,
+                 int64_t* n_rescored, Arena& ar, hipStream_t s) {
+  const int64_t ktot = h_koff[P];
+  LabelBufs B = carve_label(ar, n, d, ktot, P);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans: workspace too small");
+  EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1),
+                               hipMemcpyHostToDevice, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_koff may be a pageable stack array
+  if (n_rescored) *n_rescored = 0;
+  if (mode == EAC_KM_F32 || mode == EAC_KM_F64 || (mode == EAC_KM_BF16X3 && d < 16)) {
+    int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
+    if (mode == EAC_KM_F32)
+      EAC_TRY(launch_direct<float>(X, n, d, C, B.k_off, P, labels, dists, s));
+    else
+      EAC_TRY(launch_direct<double>(X, n, d, C, B.k_off, P, labels, dists, s));
+    prof_end(pr, s);
+    if (inertia_out) {
+      if (!dists) return fail(EAC_ERR_ARG, "kmeans: inertia needs dists");
+      EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
+      dim3 grid(grid_for(n, 256, 1024), P < 65535 ? P : 65535);
+      k_sum_dists<<<grid, 256, 0, s>>>(dists, n, P, inertia_out);
+      EAC_LAUNCH_CHECK();
+    }
+    return EAC_OK;
+  }
+  if (mode != EAC_KM_BF16X3) return fail(EAC_ERR_ARG, "kmeans: unknown mode %d", mode);
+  const int KD = kd_for(d);
+  if (!KD) return fail(EAC_ERR_UNSUPPORTED, "kmeans bf16x3: d=%d > 128", d);
+  EAC_TRY(fill<int32_t>((int32_t*)B.pstat, (int64_t)P * kStat, 0, s));
+  EAC_TRY(fill<uint64_t>((uint64_t*)B.nflag, 1, 0ull, s));
+  k_cent_prep<<<grid_for(ktot, 256, 8192), 256, 0, s>>>(C, ktot, d, KD, B.k_off, P, B.Ch, B.Cl,
+                                                        B.ncf, B.pstat);
+  EAC_LAUNCH_CHECK();
+  int g = (int)((n + kBM - 1) / kBM);
+  int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
+  switch (KD) {
+    case 32:
+      k_label_bf16x3<32><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
+                                           labels, B.flags, B.nflag, B.flag_cap);
+      break;
+    case 64:
+      k_label_bf16x3<64><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
+                                           labels, B.flags, B.nflag, B.flag_cap);
+      break;
+    default:
+      k_label_bf16x3<128><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
+                                            labels, B.flags, B.nflag, B.flag_cap);
+      break;
+  }
+  EAC_LAUNCH_CHECK();
+  prof_end(pr, s);
+  unsigned long long nf = 0;
+  EAC_TRY(d2h(&nf, B.nflag, sizeof(nf), s));
+  if ((int64_t)nf > B.flag_cap) return fail(EAC_ERR_HIP, "kmeans: flag list overflow");
+  if (nf > 0) {
+    int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 3 * (int64_t)nf * d * (ktot / P), s);
+    k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
+        X, n, d, C, B.k_off, B.flags, (int64_t)nf, labels);
+    EAC_LAUNCH_CHECK();
+    prof_end(pr2, s);
+  }
+  if (n_rescored) *n_rescored = (int64_t)nf;
+  if (dists || inertia_out) {
+    if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
+    dim3 grid(grid_for(n, 256, 1024), P < 65535 ? P : 65535);
+    k_exact_dists<<<grid, 256, 0, s>>>(X, n, d, C, B.k_off, P, labels, dists, inertia_out);
+    EAC_LAUNCH_CHECK();
+  }
+  return EAC_OK;
+}
+
+size_t kmeans_update_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
+  Arena ar(nullptr, 0, true);
+  int64_t items = (int64_t)P * n;
+  ar.take<int64_t>(P + 1);
+  ar.take<uint32_t>(items);
+  ar.take<uint32_t>(items);
+  ar.take<uint32_t>(items);
+  ar.take<uint32_t>(items);
+  ar.take<int32_t>(ktot + 1);
+  ar.take<int64_t>(ktot + 1);
+  ar.take<int32_t>(ktot + 1);
+  ar.take<int32_t>(ktot + 1);
+  ar.take<uint64_t>(n);
+  ar.take<uint64_t>(n);
+  size_t a = ar.used;
+  size_t sort_a = radix_sort_ws_bytes(items, 4, 4);
+  size_t sort_b = radix_sort_ws_bytes(n, 8, 0);
+  size_t scan_b = scan_ws_bytes(ktot + 1, 8);
+  size_t m = sort_a > sort_b ? sort_a : sort_b;
+  m = m > scan_b ? m : scan_b;
+  return a + m + 4096;
+}
+
+// New centroids for every partition (KMeans.py:709-754); empty clusters take
+// the points of smallest (dist, index) in ascending cluster order.
+int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const float* dists,
+                  const int64_t* h_koff, int P, float* Cnew, int64_t* n_empty, Arena& ar,
+                  hipStream_t s) {
+  const int64_t ktot = h_koff[P];
+  const int64_t items = (int64_t)P * n;
+  if (items > (int64_t)UINT32_MAX || ktot >= (int64_t)UINT32_MAX)
+    return fail(EAC_ERR_UNSUPPORTED, "kmeans update: P*n or k too large");
+  int64_t* k_off = ar.take<int64_t>(P + 1);
+  uint32_t* key = ar.take<uint32_t>(items);
+  uint32_t* idx = ar.take<uint32_t>(items);
+  uint32_t* skey = ar.take<uint32_t>(items);
+  uint32_t* members = ar.take<uint32_t>(items);
+  int32_t* count = ar.take<int32_t>(ktot + 1);
+  int64_t* mstart = ar.take<int64_t>(ktot + 1);
+  int32_t* eflag = ar.take<int32_t>(ktot + 1);
+  int32_t* erank = ar.take<int32_t>(ktot + 1);
+  uint64_t* dk = ar.take<uint64_t>(n);
+  uint64_t* dks = ar.take<uint64_t>(n);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans update: workspace too small");
+  EAC_HIP_CHECK(hipMemcpyAsync(k_off, h_koff, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice,
+                               s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  EAC_TRY(fill<int32_t>(count, ktot + 1, 0, s));
+  k_label_keys<<<grid_for(items, 256, 8192), 256, 0, s>>>(labels, n, P, k_off, key, idx, count);
+  EAC_LAUNCH_CHECK();
+  // stable by centroid: members of each centroid in ascending point order
+  const int bits = bits_for((uint64_t)ktot);
+  size_t mark = ar.used;
+  EAC_TRY((radix_sort<uint32_t, uint32_t>(key, skey, idx, members, items, 0, bits < 1 ? 1 : bits,
+                                          ar, s)));
+  ar.used = mark;
+  EAC_TRY((scan_excl<int32_t, int64_t>(count, mstart, ktot, 0, nullptr, ar, s)));
+  ar.used = mark;
+  k_centroid_sums<<<grid_for(ktot * d, 256, 16384), 256, 0, s>>>(X, d, members, mstart, count,
+                                                                  ktot, Cnew);
+  EAC_LAUNCH_CHECK();
+  // empty clusters
+  k_empty_flags<<<grid_for(ktot, 256, 4096), 256, 0, s>>>(count, ktot, eflag);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(eflag, erank, ktot + 1, 0, nullptr, ar, s)));
+  ar.used = mark;
+  std::vector<int32_t> er(ktot + 1);
+  {
+    // host needs only the per-partition empty counts
+    std::vector<int32_t> tmp(ktot + 1);
+    EAC_TRY(d2h(tmp.data(), erank, sizeof(int32_t) * (ktot + 1), s));
+    er.swap(tmp);
+  }
+  int64_t total_empty = er[ktot];
+  if (n_empty) *n_empty = total_empty;
+  for (int p = 0; p < P && total_empty > 0; ++p) {
+    int64_t e = er[h_koff[p + 1]] - er[h_koff[p]];
+    if (e == 0) continue;
+    if (e > n) return fail(EAC_ERR_ARG, "kmeans update: more empty clusters than points");
+    k_dist_keys<<<grid_for(n, 256, 4096), 256, 0, s>>>(dists + (int64_t)p * n, n, dk);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY((radix_sort<uint64_t, NoValue>(dk, dks, nullptr, nullptr, n, 0, 64, ar, s)));
+    ar.used = mark;
+    k_reseed<<<grid_for((h_koff[p + 1] - h_koff[p]) * d, 256, 4096), 256, 0, s>>>(
+        X, d, dks, count, erank, h_koff[p], h_koff[p + 1], Cnew);
+    EAC_LAUNCH_CHECK();
+  }
+  return EAC_OK;
+}
+
+}  // namespace eac
+
+// ------------------------------------------------------------------ C ABI
+using namespace eac;
+
+extern "C" size_t eac_kmeans_workspace_bytes(int64_t n, int32_t d, int64_t k_total, int32_t P) {
+  if (n < 1) n = 1;
+  if (k_total < 1) k_total = 1;
+  if (P < 1) P = 1;
+  size_t a = kmeans_label_ws_bytes(n, d, k_total, P);
+  size_t b = kmeans_update_ws_bytes(n, d, k_total, P);
+  // fit: labels/dists are caller buffers; two centroid buffers + inertia
+  size_t c = 2 * (((size_t)k_total * d * 4 + 255) & ~size_t(255)) + (size_t)P * 16 + 4096;
+  return (a > b ? a : b) + c;
+}
+
+static int km_check(const float* X, int64_t n, int32_t d, const float* C, const int64_t* h_koff,
+                    int32_t P) {
+  if (n < 1 || d < 1 || P < 1) return fail(EAC_ERR_ARG, "kmeans: need n, d, P >= 1");
+  if (!X || !C || !h_koff) return fail(EAC_ERR_ARG, "kmeans: null array");
+  if (h_koff[0] != 0) return fail(EAC_ERR_ARG, "kmeans: k_off[0] must be 0");
+  for (int p = 0; p < P; ++p)
+    if (h_koff[p + 1] <= h_koff[p]) return fail(EAC_ERR_ARG, "kmeans: partition %d has k < 1", p);
+  if (n >= (1ll << 40) || P >= (1 << 23)) return fail(EAC_ERR_UNSUPPORTED, "kmeans: n or P too large");
+  return EAC_OK;
+}
+
+extern "C" int eac_kmeans_label(const float* X, int64_t n, int32_t d, const float* C,
+                                const int64_t* h_koff, int32_t P, int32_t mode, int32_t* labels,
+                                float* dists, int64_t* h_out, void* ws, size_t ws_bytes,
+                                void* stream) {
+  EAC_TRY(km_check(X, n, d, C, h_koff, P));
+  if (!labels) return fail(EAC_ERR_ARG, "kmeans: null labels");
+  size_t need = eac_kmeans_workspace_bytes(n, d, h_koff[P], P);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "kmeans: workspace %zu < %zu bytes", ws_bytes, need);
+  Arena ar(ws, ws_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  int64_t nres = 0;
+  EAC_TRY(kmeans_label(X, n, d, C, h_koff, P, mode, labels, dists, nullptr, &nres, ar, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  prof_flush();
+  if (h_out) h_out[0] = nres;
+  return EAC_OK;
+}
+
+extern "C" int eac_kmeans_update(const float* X, int64_t n, int32_t d, const int32_t* labels,
+                                 const float* dists, const int64_t* h_koff, int32_t P,
+                                 float* C_new, int64_t* h_out, void* ws, size_t ws_bytes,
+                                 void* stream) {
+  if (!labels || !dists || !C_new) return fail(EAC_ERR_ARG, "kmeans update: null array");
+  EAC_TRY(km_check(X, n, d, C_new, h_koff, P));
+  size_t need = eac_kmeans_workspace_bytes(n, d, h_koff[P], P);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "kmeans update: workspace %zu < %zu bytes", ws_bytes, need);
+  Arena ar(ws, ws_bytes);
+  hipStream_t s = (hipStream_t)stream;
+  int64_t ne = 0;
+  EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, C_new, &ne, ar, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  if (h_out) h_out[0] = ne;
+  return EAC_OK;
+}
+
+// KMeans.py:117-189 per partition: label, count the iteration, stop when
+// |sum(dists) - previous| <= tol or iters >= max_iter, otherwise recompute the
+// centroids. Partitions stop independently; a stopped partition keeps its
+// labels and centroids. labels/dists: [P][n]; C_inout: initial centroids in,
+// final centroids out. h_iters[P] (host): iterations run per partition.
+// h_out (host): [0] points re-scored in FP64 (all iterations), [1] empty
+// clusters re-seeded (all iterations).
+extern "C" int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_inout,
+                              const int64_t* h_koff, int32_t P, int32_t max_iter, double tol,
+                              int32_t mode, int32_t* labels, float* dists, int32_t* h_iters,
+                              int64_t* h_out, void* ws, size_t ws_bytes, void* stream) {
+  EAC_TRY(km_check(X, n, d, C_inout, h_koff, P));
+  if (!labels || !dists) return fail(EAC_ERR_ARG, "kmeans fit: null labels/dists");
+  if (max_iter < 1) return fail(EAC_ERR_ARG, "kmeans fit: max_iter < 1");
+  const int64_t ktot = h_koff[P];
+  size_t need = eac_kmeans_workspace_bytes(n, d, ktot, P);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "kmeans fit: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  float* Cnew = ar.take<float>((size_t)ktot * d);
+  double* inertia = ar.take<double>(P);
+  const size_t mark = ar.used;
+  std::vector<double> prev(P, INFINITY), cur(P);
+  std::vector<int32_t> iters(P, 0);
+  std::vector<char> active(P, 1);
+  int64_t tot_res = 0, tot_empty = 0;
+  for (int it = 1; it <= max_iter; ++it) {
+    // label every partition (stopped ones relabel identically: unchanged
+    // centroids, deterministic kernels)
+    int64_t nres = 0;
+    ar.used = mark;
+    EAC_TRY(kmeans_label(X, n, d, C_inout, h_koff, P, mode, labels, dists, inertia, &nres, ar,
+                         s));
+    tot_res += nres;
+    EAC_TRY(d2h(cur.data(), inertia, sizeof(double) * P, s));
+    bool any = false;
+    std::vector<char> upd(P, 0);
+    for (int p = 0; p < P; ++p) {
+      if (!active[p]) continue;
+      iters[p] = it;
+      double err = std::fabs(cur[p] - prev[p]);
+      prev[p] = cur[p];
+      if (err <= tol || it >= max_iter) { active[p] = 0; continue; }
+      upd[p] = 1;
+      any = true;
+    }
+    if (!any) break;
+    ar.used = mark;
+    int64_t ne = 0;
+    EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, Cnew, &ne, ar, s));
+    tot_empty += ne;
+    // copy updated centroids of partitions still running
+    for (int p = 0; p < P; ++p) {
+      if (!upd[p]) continue;
+      int64_t a = h_koff[p] * d, b = h_koff[p + 1] * d;
+      EAC_HIP_CHECK(hipMemcpyAsync(C_inout + a, Cnew + a, sizeof(float) * (b - a),
+                                   hipMemcpyDeviceToDevice, s));
+    }
+  }
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  prof_flush();
+  if (h_iters) for (int p = 0; p < P; ++p) h_iters[p] = iters[p];
+  if (h_out) { h_out[0] = tot_res; h_out[1] = tot_empty; }
+  return EAC_OK;
+}

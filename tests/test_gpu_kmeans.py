@@ -1,0 +1,126 @@
+"""GPU K-Means labeling / centroid update / fit against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from masters_final_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def _data(n, d, k, seed, spread=2.0):
+    X, _ = synth.gmm_points(n, d, max(k // 2, 1), seed, mean_std=spread)
+    rng = np.random.default_rng(seed + 100)
+    Cc = X[rng.choice(n, k, replace=False)].copy()
+    return X, Cc
+
+
+@pytest.mark.parametrize("n,d,k", [(5000, 16, 37), (4000, 24, 100), (3000, 64, 256),
+                                   (2000, 100, 64), (3000, 2, 75), (777, 33, 5)])
+def test_label_bf16x3_equals_fp64_oracle(torch, n, d, k):
+    from masters_final_amd.cluster.KMeans import label_device
+    X, Cc = _data(n, d, k, seed=d * 7 + k)
+    want, wd = O.kmeans_label_f64(X, Cc)
+    lab, dist, nres = label_device(X, Cc, precision="bf16x3")
+    assert np.array_equal(lab[0].cpu().numpy(), want)
+    assert np.array_equal(dist[0].cpu().numpy(), wd.astype(np.float32))
+    lab64, dist64, _ = label_device(X, Cc, precision="fp64")
+    assert np.array_equal(lab64[0].cpu().numpy(), want)
+    assert np.array_equal(dist64[0].cpu().numpy(), wd.astype(np.float32))
+
+
+def test_label_ties_lowest_index(torch):
+    from masters_final_amd.cluster.KMeans import label_device
+    X, Cc = _data(3000, 32, 40, seed=5)
+    Cd = np.concatenate([Cc[:10], Cc[:10], Cc[10:]])  # exact duplicates: ties
+    want, _ = O.kmeans_label_f64(X, Cd)
+    assert np.all(want < 10) or np.all((want < 10) | (want >= 20))
+    for prec in ("bf16x3", "fp64"):
+        lab, _, nres = label_device(X, Cd, precision=prec)
+        assert np.array_equal(lab[0].cpu().numpy(), want)
+    lab, _, nres = label_device(X, Cd, precision="bf16x3")
+    assert nres > 0  # tied points cannot be certified: they are re-scored
+
+
+def test_label_fp32_plain(torch):
+    from masters_final_amd.cluster.KMeans import label_device
+    X, Cc = _data(4000, 16, 50, seed=9)
+    lab, dist, _ = label_device(X, Cc, precision="fp32")
+    want, wd = O.kmeans_label_f64(X, Cc)
+    agree = np.mean(lab[0].cpu().numpy() == want)
+    assert agree > 0.999
+    assert np.allclose(dist[0].cpu().numpy(), wd, rtol=1e-4, atol=1e-4)
+
+
+def test_label_batched_partitions(torch):
+    from masters_final_amd.cluster.KMeans import label_device
+    rng = np.random.default_rng(3)
+    X, _ = synth.gmm_points(6000, 16, 12, 3, mean_scale=20.0, anisotropic=(0.2, 3.0))
+    ks = [32, 7, 300, 1, 64]
+    C = np.concatenate([X[rng.choice(6000, k, replace=False)] for k in ks])
+    lab, dist, _ = label_device(X, C, k_list=ks, precision="bf16x3")
+    off = np.concatenate([[0], np.cumsum(ks)])
+    for p, k in enumerate(ks):
+        want, wd = O.kmeans_label_f64(X, C[off[p]:off[p + 1]])
+        assert np.array_equal(lab[p].cpu().numpy(), want), p
+        assert np.array_equal(dist[p].cpu().numpy(), wd.astype(np.float32)), p
+
+
+def test_centroid_update_matches_oracle(torch):
+    from masters_final_amd.cluster.KMeans import update_device
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((20000, 12)).astype(np.float32) * 3
+    k = 40
+    lab = rng.integers(0, k, 20000).astype(np.int32)
+    lab[(lab == 7) | (lab == 21)] = 3  # two empty clusters
+    dists = rng.random(20000).astype(np.float32)
+    want = O.recompute_centroids(X, lab, dists, k, reseed_mode=1)
+    got, ne = update_device(X, lab[None], dists[None], [k])
+    assert ne == 2
+    assert np.array_equal(got.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "fp64"])
+def test_fit_matches_oracle(torch, prec):
+    from masters_final_amd.cluster.KMeans import fit_batched
+    X, Cc = _data(8000, 16, 60, seed=11)
+    lab, dist, cen, it = O.kmeans_fit(X, Cc, max_iter=3, reseed_mode=1)
+    r = fit_batched(X, Cc, [60], max_iter=3, precision=prec)
+    assert int(r["iters"][0]) == it
+    assert np.array_equal(r["labels"][0].cpu().numpy(), lab)
+    assert np.array_equal(r["centroids"].cpu().numpy(), cen)
+    assert np.array_equal(r["dists"][0].cpu().numpy(), dist)
+
+
+def test_fit_batched_uneven_k(torch):
+    from masters_final_amd.cluster.KMeans import fit_batched
+    rng = np.random.default_rng(12)
+    X, _ = synth.gmm_points(10000, 16, 32, 12, mean_scale=20.0, anisotropic=(0.2, 3.0))
+    ks = [32, 150, 45, 512, 33]
+    init = np.concatenate([X[rng.choice(10000, k, replace=False)] for k in ks])
+    r = fit_batched(X, init, ks, max_iter=3, precision="bf16x3")
+    off = np.concatenate([[0], np.cumsum(ks)])
+    for p, k in enumerate(ks):
+        lab, dist, cen, it = O.kmeans_fit(X, init[off[p]:off[p + 1]], max_iter=3, reseed_mode=1)
+        assert np.array_equal(r["labels"][p].cpu().numpy(), lab), p
+        assert np.array_equal(r["centroids"][off[p]:off[p + 1]].cpu().numpy(), cen), p
+
+
+def test_k_means_class(torch):
+    from masters_final_amd.cluster.KMeans import K_Means
+    X, y = synth.eac_small(seed=0)
+    init = X[:70].copy()
+    km = K_Means(n_clusters=70, max_iter=3, init=init)
+    km.fit(X)
+    lab, dist, cen, it = O.kmeans_fit(X, init, max_iter=3, reseed_mode=1)
+    assert km.iters_ == it and np.array_equal(km.labels_, lab)
+    with pytest.raises(Warning):
+        K_Means(n_clusters=3).fit(X.astype(np.float64))
