@@ -41,7 +41,18 @@ def main():
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--knn", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        x = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": int(x.numel())}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the workload has no "
               "multi-GPU path in this build", file=sys.stderr)

@@ -66,6 +66,9 @@ int eac_abi_version(void);
 #define EAC_PROF_CC 5         /* union-find over the CSR: 4 B/entry + 8 B/vertex */
 #define EAC_PROF_KM_LABEL 6   /* labeling pass; "bytes" = 2 n d sum(k) FLOPs     */
 #define EAC_PROF_KM_RESCORE 7 /* FP64 re-scoring; "bytes" = 3 d k FLOPs/point   */
+#define EAC_PROF_CO_PLAN 8    /* member lists: 16 B per (partition, point)       */
+#define EAC_PROF_CO_COUNT 9   /* co-association count pass                       */
+#define EAC_PROF_CO_FILL 10   /* co-association fill: 8 B/nnz + 8 B/row written  */
 int eac_profile_enable(int32_t on);
 int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t* alg_bytes);
 
@@ -208,6 +211,45 @@ int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_inout,
                    const int64_t* h_koff, int32_t P, int32_t max_iter, double tol,
                    int32_t mode, int32_t* labels, float* dists, int32_t* h_iters,
                    int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- graph from a list of edge ids -------------------------------------
+ * Replaces MyML/MyML/graph/build.py:49-93 getGraphFromEdges_seq and
+ * build.py:96-151 getGraphFromEdges_gpu: for each id in edges (CSR
+ * positions of (dest, first_edge, out_degree); -1 entries are skipped as in
+ * build.py:170-172) the origin is found by binary search (build.py:18-45),
+ * both directions are inserted, rows keep list order (origin side first).
+ * Outputs: nod[n] int32 degrees, nfe[n] int64 offsets, ndest[2*n_edges],
+ * nweight[2*n_edges] (weight elements of weight_bytes each, nullable);
+ * h_nitems = entries written. EAC_ERR_NOT_FOUND when an id has no origin
+ * (the CPU function returns -1). */
+size_t eac_graph_from_edges_workspace_bytes(int32_t n, int64_t n_edges);
+int eac_graph_from_edges(const int32_t* dest, const void* weight, int32_t weight_bytes,
+                         const int64_t* first_edge, const int32_t* out_degree, int32_t n,
+                         int64_t m, const int64_t* edges, int64_t n_edges, int32_t* nod,
+                         int64_t* nfe, int32_t* ndest, void* nweight, int64_t* h_nitems,
+                         void* ws, size_t ws_bytes, void* stream);
+
+/* ---- sparse co-association (EAC) --------------------------------------
+ * labels: [P][n] int32, labels[p][i] in [0, k_p) or < 0 for "not in any
+ * cluster of partition p"; h_koff (HOST) [P+1] offsets of the per-partition
+ * cluster counts. Replaces MyML/MyML/EAC/sparse.py EAC_CSR.update_ensemble
+ * (:154-195) + _condense (:212-225) + make_diassoc (:295-311), exact (no
+ * max_assocs cap, int64 offsets).
+ * full_rows = 0: condensed upper triangle (col > row, the reference's
+ *   condensed=True default); 1: every co-member j != i (symmetric CSR).
+ * Two passes sharing one workspace (pass the same ws, unchanged):
+ *   eac_coassoc_count -> indptr[n+1] (int64), nnz, largest count;
+ *   eac_coassoc_fill  -> indices[nnz] (columns ascending per row) and
+ *                        data[nnz] = count, or w_offset - count when
+ *                        w_offset != 0 (make_diassoc with max + 1). */
+size_t eac_coassoc_workspace_bytes(int64_t n, int32_t P, int64_t k_total);
+int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                      int32_t P, int32_t full_rows, int64_t* indptr, int64_t* h_nnz,
+                      int32_t* h_max_count, void* ws, size_t ws_bytes, void* stream);
+int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                     int32_t P, int32_t full_rows, const int64_t* indptr, int32_t w_offset,
+                     int32_t* indices, int32_t* data, void* ws, size_t ws_bytes,
+                     void* stream);
 
 /* ---- first-occurrence relabelling --------------------------------------
  * out[v] = rank of labels_in[v] among distinct values ordered by first

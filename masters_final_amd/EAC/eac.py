@@ -41,3 +41,94 @@ def sp_sl_lifetime(mat, max_dist=None, n_clusters=0):
     if labels is None:
         return 1, np.empty(0, dtype=np.int32)
     return n_comps, labels.cpu().numpy()
+
+
+class EAC:
+    """eac.py:40-179 interface. buildMatrix counts the ensemble's
+    co-associations exactly on the GPU (EAC_CSR) and turns them into
+    dissimilarities (max + 1) - count; finalClustering runs the Boruvka forest,
+    the lifetime rule and the cut on the GPU (sp_sl_lifetime).
+    The dense EAC_FULL storage (eac.py:139-144, O(n^2)) and the disk-backed
+    Kruskal (eac.py:270-335) are not built: `sparse=False` uses the same exact
+    sparse matrix."""
+
+    def __init__(self, n_samples, **kwargs):
+        self.n_samples = n_samples
+        self.n_partitions = kwargs.get("n_partitions", 100)
+        self.iters = kwargs.get("iters", 3)
+        self.n_clusters = kwargs.get("n_clusters", "sqrt")
+        self.toFiles = False
+        self.toFiles_folder = None
+        self.condensed = kwargs.get("condensed", True)
+        self.kNN = kwargs.get("kNN", False)
+        self.assoc_dtype = kwargs.get("assoc_dtype", np.uint8)
+        self.sparse = kwargs.get("sparse", False)
+        self.sp_sort = kwargs.get("sparse_sort_mode", "surgical")
+        self.sp_max_assocs = kwargs.get("sparse_max_assocs", None)
+        self.sp_max_assocs_factor = kwargs.get("sparse_max_assocs_factor", 3)
+        self.sp_max_assocs_mode = kwargs.get("sparse_max_assocs_mode", "linear")
+        self.sp_keep_degree = kwargs.get("sparse_keep_degree", False)
+        self.full = not self.sparse and not self.kNN
+        self.coassoc_store = kwargs.get("coassoc_store", False)
+        self.linkage = kwargs.get("linkage", "SL")
+        self.disk_mst = kwargs.get("sl_disk", False)
+
+    def _validate_params(self):
+        pass
+
+    def generateEnsemble(self):
+        pass
+
+    def buildMatrix(self, ensemble):
+        from .sparse import EAC_CSR
+        if self.kNN:
+            raise NotImplementedError("kNN matrix building has not been included in this "
+                                      "version.")
+        if self.sp_max_assocs is None:
+            self.sp_max_assocs = max(max(map(np.size, p)) for p in ensemble)
+            self.sp_max_assocs *= self.sp_max_assocs_factor
+        mode = self.sp_max_assocs_mode if self.condensed else "constant"
+        coassoc = EAC_CSR(self.n_samples, max_assocs=int(self.sp_max_assocs),
+                          condensed=self.condensed, max_assocs_type=mode,
+                          sort_mode=self.sp_sort, dtype=self.assoc_dtype)
+        coassoc.update_ensemble(ensemble)
+        coassoc._condense(keep_degree=self.sp_keep_degree)
+        coassoc.make_diassoc()
+        self.coassoc = coassoc
+
+    def buildMatrixFromLabels(self, labels, k_list):
+        """Device entry: labels [P, n] of the ensemble (no list conversion)."""
+        from .sparse import EAC_CSR
+        coassoc = EAC_CSR(self.n_samples, max_assocs=1, condensed=self.condensed,
+                          max_assocs_type="constant", dtype=self.assoc_dtype)
+        coassoc.update_labels(labels, k_list)
+        coassoc._condense()
+        coassoc.make_diassoc()
+        self.coassoc = coassoc
+
+    def finalClustering(self, n_clusters=0):
+        c = self.coassoc
+        n_fclusts, labels, _ = sp_sl_lifetime_device((c.indptr, c.indices, c.data),
+                                                     self.n_partitions, n_clusters=n_clusters)
+        self.n_fclusts = n_fclusts
+        self.labels = (np.empty(0, dtype=np.int32) if labels is None
+                       else labels.cpu().numpy())
+        return self.labels
+
+
+def eac_pipeline(X, k_list, iters=3, seed=0, precision="bf16x3", init=None, n_clusters=0):
+    """Whole EAC hot path on the GPU: batched K-Means ensemble -> exact
+    co-association -> dissimilarity (max + 1) - count -> Boruvka forest ->
+    lifetime cut (max_dist = number of partitions, eac.py:159-161) -> labels.
+    Returns dict(n_clusters, labels (device tensor or None for one cluster),
+    n_rescored, nnz)."""
+    from ..utils.partition import generate_ensemble_batched
+    from .sparse import coassoc_device
+    labels, fit = generate_ensemble_batched(X, k_list, iters=iters, seed=seed,
+                                            precision=precision, init=init)
+    indptr, indices, data, mx = coassoc_device(labels, k_list, full_rows=False,
+                                               w_offset="max+1")
+    nc, lab, info = sp_sl_lifetime_device((indptr, indices, data), len(k_list),
+                                          n_clusters=n_clusters)
+    return {"n_clusters": nc, "labels": lab, "n_rescored": fit["n_rescored"],
+            "nnz": int(indices.numel()), "ensemble_labels": labels, "max_count": mx}

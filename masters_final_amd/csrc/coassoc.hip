@@ -1,0 +1,382 @@
+// Sparse co-association matrix of an ensemble of partitions (EAC) on gfx950.
+//
+// Reference: MyML/MyML/EAC/sparse.py EAC_CSR (:12-304): for every partition
+// and cluster, every pair of members gets +1 (update_cluster_fp_condensed
+// :445-465 for the first partition, update_cluster_sorted_surgical_condensed
+// :956-1040 with binary_search_interval :1069-1099 for the others); the
+// condensed form keeps the upper triangle (col > row), rows sorted by column;
+// _condense (:212-225, condense_eac_csr :336-349) packs the rows with a
+// sequential offset sum; make_diassoc (:295-311) turns counts into
+// (max + 1) - count. The reference caps each row at max_assocs and silently
+// drops associations beyond it (sparse.py:501-503, :995-997); this build is
+// exact (no cap) and its row offsets are int64.
+//
+// MI355X design:
+//  * Plan: a stable radix sort of (global cluster id, point) turns labels
+//    [P][n] into member lists (ascending point id per cluster) plus each
+//    point's position in its list. A point's later co-members in partition p
+//    are then the suffix of its cluster's list after its own position: no
+//    search and no pair materialisation.
+//  * Count/fill: one workgroup per row (rows handed out by an atomic
+//    counter). Columns are covered in windows of 16384; each window has a
+//    dense u32 counter array in LDS (64 KiB). Per window every partition's
+//    suffix cursor advances over the members inside the window (flattened
+//    over partitions so that lanes stream consecutive members), each member
+//    an LDS atomic increment. The window is then read in column order: pass 1
+//    counts non-zeros (row nnz, max count), pass 2 writes (col, value) after
+//    a block scan, so rows come out sorted without a per-row sort.
+//  * Row offsets: exclusive scan of the per-row nnz into int64 (nnz > 2^31
+//    for the large ensembles).
+#include <cstring>
+#include <vector>
+
+#include "prims.hpp"
+#include "sort.hpp"
+
+namespace eac {
+namespace {
+
+constexpr int kCB = 256;       // threads per row workgroup
+constexpr int kWin = 16384;    // columns per LDS window
+constexpr int kMaxP = 1024;    // partitions staged in LDS
+
+// keys: global cluster id (unassigned points -> ktot, sorted last);
+// values: p * n + i
+__global__ void k_member_keys(const int32_t* __restrict__ labels, int64_t n, int P,
+                              const int64_t* __restrict__ k_off, int64_t ktot,
+                              uint32_t* __restrict__ key, uint32_t* __restrict__ val,
+                              int32_t* __restrict__ csize, unsigned int* __restrict__ bad) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)P * n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int p = (int)(t / n);
+    int32_t l = labels[t];
+    int64_t kp = k_off[p + 1] - k_off[p];
+    uint32_t g;
+    if (l < 0) {
+      g = (uint32_t)ktot;
+    } else if (l >= kp) {
+      atomicOr(bad, 1u);
+      g = (uint32_t)ktot;
+    } else {
+      g = (uint32_t)(k_off[p] + l);
+    }
+    key[t] = g;
+    val[t] = (uint32_t)t;
+    atomicAdd(&csize[g], 1);
+  }
+}
+
+// members[q] = point id; pos[p*n + i] = q (or -1 when unassigned)
+__global__ void k_member_pos(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+                             int64_t total, int64_t n, int64_t ktot,
+                             int32_t* __restrict__ members, int32_t* __restrict__ pos) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t v = sval[q];
+    members[q] = (int32_t)(v % (uint32_t)n);
+    pos[v] = skey[q] == (uint32_t)ktot ? -1 : (int32_t)q;
+  }
+}
+
+struct CoPlan {
+  const int32_t* labels;
+  int64_t n;
+  int P;
+  const int64_t* k_off;   // [P+1] device
+  const int32_t* members;
+  const int32_t* pos;     // [P*n]
+  const int32_t* cstart;  // [ktot+1] start of each global cluster in members
+};
+
+// One row: walk every partition's co-members > i (or all co-members != i for
+// full rows) window by window. PASS 0: per-row nnz and max count. PASS 1:
+// write (col, value) at out_off.
+template <int PASS>
+__global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
+                                                      unsigned long long* __restrict__ row_ctr,
+                                                      int32_t* __restrict__ row_nnz,
+                                                      unsigned int* __restrict__ max_count,
+                                                      const int64_t* __restrict__ indptr,
+                                                      int32_t w_offset,
+                                                      int32_t* __restrict__ out_col,
+                                                      int32_t* __restrict__ out_val) {
+  __shared__ uint32_t s_cnt[kWin];
+  __shared__ int32_t s_cur[kMaxP];   // next member position per partition
+  __shared__ int32_t s_end[kMaxP];   // end of the cluster per partition
+  __shared__ int32_t s_hi[kMaxP];    // end of this window's members
+  __shared__ int32_t s_pre[kMaxP + 1];
+  __shared__ uint32_t s_tmp[kCB / kWave + 1];
+  __shared__ int64_t s_row;
+  __shared__ unsigned int s_max;
+  const int t = threadIdx.x;
+  const int64_t n = pl.n;
+  const int P = pl.P;
+  for (;;) {
+    if (t == 0) s_row = (int64_t)atomicAdd(row_ctr, 1ull);
+    __syncthreads();
+    const int64_t i = s_row;
+    if (i >= n) break;
+    if (t == 0) s_max = 0;
+    for (int p = t; p < P; p += kCB) {
+      int32_t q = pl.pos[(int64_t)p * n + i];
+      if (q < 0) {  // unassigned in partition p
+        s_cur[p] = 0;
+        s_end[p] = 0;
+      } else {
+        int64_t g = pl.k_off[p] + pl.labels[(int64_t)p * n + i];
+        s_cur[p] = full_rows ? pl.cstart[g] : q + 1;
+        s_end[p] = pl.cstart[g + 1];
+      }
+    }
+    int64_t row_base = PASS == 1 ? indptr[i] : 0;
+    int64_t written = 0;
+    uint32_t nnz_local = 0;
+    for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += kWin) {
+      const int64_t c1 = c0 + kWin < n ? c0 + kWin : n;
+      for (int v = t; v < kWin; v += kCB) s_cnt[v] = 0;
+      __syncthreads();
+      // members of this window: [cur, hi) with member < c1 (lists ascending)
+      for (int p = t; p < P; p += kCB) {
+        int32_t lo = s_cur[p], hi = s_end[p];
+        while (lo < hi) {
+          int32_t mid = (lo + hi) >> 1;
+          if (pl.members[mid] < c1) lo = mid + 1; else hi = mid;
+        }
+        s_hi[p] = lo;
+      }
+      __syncthreads();
+      // flattened ranges: prefix of (hi - cur) over partitions
+      if (t < kWave) {
+        int32_t carry = 0;
+        for (int p0 = 0; p0 < P; p0 += kWave) {
+          int p = p0 + t;
+          int32_t len = p < P ? s_hi[p] - s_cur[p] : 0;
+          int32_t inc = wave_incl_scan(len);
+          if (p < P) s_pre[p] = carry + inc - len;
+          carry += __shfl(inc, kWave - 1, kWave);
+        }
+        if (t == 0) s_pre[P] = carry;
+      }
+      __syncthreads();
+      const int32_t tot = s_pre[P];
+      int p = 0;
+      for (int32_t f = t; f < tot; f += kCB) {
+        while (s_pre[p + 1] <= f) ++p;
+        int32_t j = pl.members[s_cur[p] + (f - s_pre[p])];
+        if (j != (int32_t)i) atomicAdd(&s_cnt[j - c0], 1u);
+      }
+      __syncthreads();
+      for (int p2 = t; p2 < P; p2 += kCB) s_cur[p2] = s_hi[p2];
+      // read the window: lane-strided (conflict-free LDS reads)
+      const int width = (int)(c1 - c0);
+      if (PASS == 0) {
+        uint32_t c = 0, mx = 0;
+        for (int v = t; v < width; v += kCB) {
+          uint32_t x = s_cnt[v];
+          c += x != 0;
+          mx = x > mx ? x : mx;
+        }
+        nnz_local += c;
+        if (mx) atomicMax(&s_max, mx);
+      } else {
+        // wave w emits region [w*R, (w+1)*R) in column order
+        const int w = t / kWave, l = lane_id();
+        const int R = ((width + 4 * kWave - 1) / (4 * kWave)) * kWave;
+        const int ra = w * R, rb = (w + 1) * R < width ? (w + 1) * R : width;
+        uint32_t c = 0;
+        for (int v = ra + l; v < rb; v += kWave) c += s_cnt[v] != 0;
+        c = wave_sum(c);
+        if (l == 0) s_tmp[w] = c;
+        __syncthreads();
+        int64_t o = row_base + written;
+        uint32_t wtot = 0;
+        for (int q = 0; q < kCB / kWave; ++q) {
+          if (q < w) o += s_tmp[q];
+          wtot += s_tmp[q];
+        }
+        const uint64_t lt = lanemask_lt();
+        for (int v0 = ra; v0 < rb; v0 += kWave) {
+          int v = v0 + l;
+          uint32_t x = v < rb ? s_cnt[v] : 0u;
+          uint64_t mask = __ballot(x != 0);
+          if (x) {
+            int64_t at = o + __popcll(mask & lt);
+            out_col[at] = (int32_t)(c0 + v);
+            out_val[at] = w_offset ? w_offset - (int32_t)x : (int32_t)x;
+          }
+          o += __popcll(mask);
+        }
+        written += wtot;
+      }
+      __syncthreads();
+    }
+    if (PASS == 0) {
+      uint32_t rt;
+      (void)block_excl_scan<uint32_t>(nnz_local, s_tmp, &rt);
+      if (t == 0) {
+        row_nnz[i] = (int32_t)rt;
+        if (s_max) atomicMax(max_count, s_max);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+struct CoBufs {
+  int64_t* k_off;
+  uint32_t* key;
+  uint32_t* val;
+  uint32_t* skey;
+  uint32_t* sval;
+  int32_t* members;
+  int32_t* pos;
+  int32_t* csize;
+  int32_t* cstart;
+  int32_t* row_nnz;
+  unsigned long long* ctr;
+  unsigned int* misc;  // [0] bad label, [1] max count
+};
+
+CoBufs carve_co(Arena& ar, int64_t n, int P, int64_t ktot) {
+  CoBufs b;
+  int64_t items = (int64_t)P * n;
+  b.k_off = ar.take<int64_t>(P + 1);
+  b.members = ar.take<int32_t>(items);
+  b.pos = ar.take<int32_t>(items);
+  b.csize = ar.take<int32_t>(ktot + 2);
+  b.cstart = ar.take<int32_t>(ktot + 2);
+  b.row_nnz = ar.take<int32_t>(n + 1);
+  b.ctr = ar.take<unsigned long long>(2);
+  b.misc = ar.take<unsigned int>(4);
+  // sort staging last: reused by later passes
+  b.key = ar.take<uint32_t>(items);
+  b.val = ar.take<uint32_t>(items);
+  b.skey = ar.take<uint32_t>(items);
+  b.sval = ar.take<uint32_t>(items);
+  return b;
+}
+
+int d2h(void* host, const void* dev, size_t bytes, hipStream_t s) {
+  void* pin = pinned_scratch(bytes);
+  if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+  EAC_HIP_CHECK(hipMemcpyAsync(pin, dev, bytes, hipMemcpyDeviceToHost, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  memcpy(host, pin, bytes);
+  return EAC_OK;
+}
+
+int co_check(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P) {
+  if (n < 1 || P < 1) return fail(EAC_ERR_ARG, "coassoc: need n, P >= 1");
+  if (P > kMaxP) return fail(EAC_ERR_UNSUPPORTED, "coassoc: more than %d partitions", kMaxP);
+  if (!labels || !h_koff) return fail(EAC_ERR_ARG, "coassoc: null array");
+  if (h_koff[0] != 0) return fail(EAC_ERR_ARG, "coassoc: k_off[0] must be 0");
+  for (int p = 0; p < P; ++p)
+    if (h_koff[p + 1] < h_koff[p]) return fail(EAC_ERR_ARG, "coassoc: k_off not ascending");
+  if ((int64_t)P * n >= (int64_t)INT32_MAX)
+    return fail(EAC_ERR_UNSUPPORTED, "coassoc: P*n too large");
+  return EAC_OK;
+}
+
+}  // namespace
+}  // namespace eac
+
+// ------------------------------------------------------------------ C ABI
+using namespace eac;
+
+extern "C" size_t eac_coassoc_workspace_bytes(int64_t n, int32_t P, int64_t k_total) {
+  if (n < 1) n = 1;
+  if (P < 1) P = 1;
+  Arena ar(nullptr, 0, true);
+  (void)carve_co(ar, n, P, k_total);
+  size_t s1 = radix_sort_ws_bytes((int64_t)P * n, 4, 4);
+  size_t s2 = scan_ws_bytes(k_total + 2, 4);
+  size_t s3 = scan_ws_bytes(n + 1, 8);
+  size_t m = s1 > s2 ? s1 : s2;
+  m = m > s3 ? m : s3;
+  return ar.used + m + 4096;
+}
+
+// Pass 1: builds the member plan in ws (keep ws unchanged for the fill),
+// writes indptr[n+1] (int64) and returns nnz and the largest count.
+extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                 int32_t P, int32_t full_rows, int64_t* indptr, int64_t* h_nnz,
+                                 int32_t* h_max_count, void* ws, size_t ws_bytes, void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (!indptr) return fail(EAC_ERR_ARG, "coassoc: null indptr");
+  const int64_t ktot = h_koff[P];
+  size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  CoBufs B = carve_co(ar, n, P, ktot);
+  const size_t mark = ar.used;
+  const int64_t items = (int64_t)P * n;
+  EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice,
+                               s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  EAC_TRY(fill<int32_t>(B.csize, ktot + 2, 0, s));
+  EAC_TRY(fill<int32_t>((int32_t*)B.misc, 4, 0, s));
+  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
+  int pr = prof_begin(EAC_PROF_CO_PLAN, 16 * items, s);
+  k_member_keys<<<grid_for(items, 256, 8192), 256, 0, s>>>(labels, n, P, B.k_off, ktot, B.key,
+                                                           B.val, B.csize, B.misc);
+  EAC_LAUNCH_CHECK();
+  const int bits = bits_for((uint64_t)ktot);
+  EAC_TRY((radix_sort<uint32_t, uint32_t>(B.key, B.skey, B.val, B.sval, items, 0,
+                                          bits < 1 ? 1 : bits, ar, s)));
+  ar.used = mark;
+  EAC_TRY((scan_excl<int32_t, int32_t>(B.csize, B.cstart, ktot + 2, 0, nullptr, ar, s)));
+  ar.used = mark;
+  k_member_pos<<<grid_for(items, 256, 8192), 256, 0, s>>>(B.skey, B.sval, items, n, ktot,
+                                                          B.members, B.pos);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr, s);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart};
+  int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
+  k_coassoc_rows<0><<<2048, kCB, 0, s>>>(pl, full_rows, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0,
+                                         nullptr, nullptr);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr2, s);
+  EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
+  ar.used = mark;
+  unsigned int misc[4];
+  EAC_TRY(d2h(misc, B.misc, sizeof(misc), s));
+  int64_t nnz = 0;
+  EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
+  prof_flush();
+  if (misc[0]) return fail(EAC_ERR_ARG, "coassoc: label >= k of its partition");
+  if (h_nnz) *h_nnz = nnz;
+  if (h_max_count) *h_max_count = (int32_t)misc[1];
+  return EAC_OK;
+}
+
+// Pass 2 (same ws as pass 1): indices[nnz] (int32 columns, ascending per
+// row) and data[nnz] = count, or w_offset - count when w_offset != 0
+// (make_diassoc, sparse.py:295-311, with w_offset = max + 1).
+extern "C" int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                int32_t P, int32_t full_rows, const int64_t* indptr,
+                                int32_t w_offset, int32_t* indices, int32_t* data, void* ws,
+                                size_t ws_bytes, void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (!indptr || !indices || !data) return fail(EAC_ERR_ARG, "coassoc: null output");
+  const int64_t ktot = h_koff[P];
+  size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  CoBufs B = carve_co(ar, n, P, ktot);
+  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart};
+  int64_t nnz = 0;
+  EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
+  int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
+  k_coassoc_rows<1><<<2048, kCB, 0, s>>>(pl, full_rows, B.ctr, nullptr, nullptr, indptr, w_offset,
+                                         indices, data);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr, s);
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  prof_flush();
+  return EAC_OK;
+}

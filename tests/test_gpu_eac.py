@@ -1,0 +1,135 @@
+"""GPU co-association builder and EAC end-to-end against the oracle."""
+import numpy as np
+import pytest
+
+from masters_final_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+def _dense(labels):
+    P, n = labels.shape
+    C = np.zeros((n, n), np.int64)
+    for p in range(P):
+        for c in np.unique(labels[p]):
+            if c < 0:
+                continue
+            idx = np.nonzero(labels[p] == c)[0]
+            np.add.at(C, (idx[:, None], idx[None, :]), 1)
+    np.fill_diagonal(C, 0)
+    return C
+
+
+@pytest.mark.parametrize("n,P,kmin,kmax,seed", [(500, 10, 3, 20, 0), (3000, 40, 20, 60, 1),
+                                                (20000, 8, 2, 6, 2)])
+def test_coassoc_condensed_vs_oracle(torch, n, P, kmin, kmax, seed):
+    from masters_final_amd.EAC.sparse import coassoc_device
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(kmin, kmax + 1, P)
+    labels = np.stack([rng.integers(0, k, n) for k in ks]).astype(np.int32)
+    indptr, ind, cnt = O.coassoc_condensed(labels, ks) if n <= 3000 else (None, None, None)
+    gi, gc, gd, mx = coassoc_device(labels, ks)
+    gi, gc, gd = gi.cpu().numpy(), gc.cpu().numpy(), gd.cpu().numpy()
+    if indptr is not None:
+        assert np.array_equal(gi, indptr) and np.array_equal(gc, ind) and np.array_equal(gd, cnt)
+    if n <= 3000:
+        D = np.triu(_dense(labels), 1)
+        assert mx == D.max()
+    # rows sorted, columns above the diagonal, counts within [1, P]
+    rows = np.repeat(np.arange(n), np.diff(gi))
+    assert np.all(gc > rows) and np.all((gd >= 1) & (gd <= P))
+    assert np.all(np.diff(gc)[np.diff(rows) == 0] > 0)
+
+
+def test_coassoc_full_rows_and_dissim(torch):
+    from masters_final_amd.EAC.sparse import coassoc_device
+    rng = np.random.default_rng(5)
+    n, P = 1500, 12
+    ks = rng.integers(2, 9, P)
+    labels = np.stack([rng.integers(0, k, n) for k in ks]).astype(np.int32)
+    labels[3, :100] = -1  # samples left out of one partition
+    D = _dense(labels)
+    gi, gc, gd, mx = coassoc_device(labels, ks, full_rows=True, w_offset="max+1")
+    gi, gc, gd = gi.cpu().numpy(), gc.cpu().numpy(), gd.cpu().numpy()
+    r, c = np.nonzero(D)
+    assert np.array_equal(gc, c) and np.array_equal(gi[1:], np.cumsum(np.bincount(r, minlength=n)))
+    assert mx == D.max() and np.array_equal(gd, mx + 1 - D[r, c])
+
+
+def test_eac_csr_class(torch):
+    from masters_final_amd.EAC.sparse import EAC_CSR
+    from masters_final_amd.utils.partition import convertClusterStringToIndex
+    rng = np.random.default_rng(6)
+    n, P = 800, 15
+    ks = rng.integers(3, 12, P)
+    labels = np.stack([rng.integers(0, k, n) for k in ks])
+    ens = [convertClusterStringToIndex(l) for l in labels]
+    # golden vector of the reference's own test (test_partition.py:82-97)
+    out = convertClusterStringToIndex(np.array([1, 1, 1, 3, 2, 4, 2, 4, 3, 3], np.uint32))
+    assert [a.tolist() for a in out] == [[0, 1, 2], [4, 6], [3, 8, 9], [5, 7]]
+    m = EAC_CSR(n, max_assocs=100, condensed=True, max_assocs_type="linear",
+                sort_mode="surgical")
+    m.update_ensemble(ens)
+    m._condense()
+    relab = np.stack([np.searchsorted(np.unique(l), l) for l in labels])
+    ip, ind, cnt = O.coassoc_condensed(relab, [len(e) for e in ens])
+    csr = m.csr
+    assert np.array_equal(csr.indptr, ip) and np.array_equal(csr.indices, ind)
+    assert np.array_equal(csr.data, cnt)
+    m.make_diassoc()
+    assert np.array_equal(m.csr.data, cnt.max() + 1 - cnt)
+
+
+def _oracle_eac(X, init, ks, iters=3):
+    n = X.shape[0]
+    off = np.concatenate([[0], np.cumsum(ks)])
+    labs = []
+    for p, k in enumerate(ks):
+        lab, _, _, _ = O.kmeans_fit(X, init[off[p]:off[p + 1]], max_iter=iters, reseed_mode=1)
+        labs.append(lab)
+    labs = np.stack(labs)
+    ip, ind, cnt = O.coassoc_condensed(labs, ks)
+    w = (cnt.max() + 1 - cnt).astype(np.float64)
+    rows = np.repeat(np.arange(n), np.diff(ip))
+    nc, lab = O.sp_sl_lifetime(rows, ind, w, n, max_dist=len(ks))
+    return labs, nc, lab
+
+
+def test_eac_pipeline_matches_oracle(torch):
+    from masters_final_amd.EAC.eac import eac_pipeline
+    X, _ = synth.gmm_points(2000, 2, 6, 0, mean_scale=10.0)
+    rng = np.random.default_rng(0)
+    ks = rng.integers(10, 21, 20)
+    init = np.concatenate([X[rng.choice(2000, k, replace=False)] for k in ks])
+    labs, nc, lab = _oracle_eac(X, init, ks)
+    for prec in ("bf16x3", "fp32"):
+        r = eac_pipeline(X, ks, iters=3, precision=prec, init=init)
+        if prec == "bf16x3":
+            assert np.array_equal(r["ensemble_labels"].cpu().numpy(), labs)
+            assert r["n_clusters"] == nc
+            got = r["labels"].cpu().numpy() if r["labels"] is not None else np.zeros(0)
+            assert np.array_equal(got, lab)
+
+
+def test_eac_class_end_to_end(torch):
+    from masters_final_amd.EAC.eac import EAC
+    from masters_final_amd.utils.partition import convertClusterStringToIndex
+    X, _ = synth.gmm_points(1500, 2, 5, 3, mean_scale=10.0)
+    rng = np.random.default_rng(3)
+    ks = rng.integers(8, 16, 12)
+    init = np.concatenate([X[rng.choice(1500, k, replace=False)] for k in ks])
+    labs, nc, lab = _oracle_eac(X, init, ks)
+    ens = [convertClusterStringToIndex(l) for l in labs]
+    e = EAC(1500, sparse=True, n_partitions=len(ks))
+    e.buildMatrix(ens)
+    got = e.finalClustering()
+    assert e.n_fclusts == nc and np.array_equal(got, lab)
