@@ -162,6 +162,24 @@ int eac_sl_lifetime(const int32_t* dest, const void* weight, int32_t weight_dtyp
                     int32_t n_clusters, int32_t* labels, int64_t* h_out, void* ws,
                     size_t ws_bytes, void* stream);
 
+/* ---- batched single-link over many independent graphs -----------------
+ * G graphs as one disjoint union: graph g owns vertices [h_goff[g],
+ * h_goff[g+1]) (h_goff HOST, h_goff[0] = 0, h_goff[G] = N) and their rows of
+ * the union CSR (dest holds union ids). Per graph: its own canonical forest,
+ * its own lifetime cut (rule, max_dist, n_clusters as eac_sl_lifetime) and
+ * labels numbered by first occurrence inside the graph (labels[N]).
+ * gout[4*g + {0,1,2,3}] (device int64): components, labels_empty, nc_stable,
+ * kept forest edges (-1 where the reference raises ValueError for that
+ * graph). h_out[4] (host): total components, forest edges, Boruvka rounds,
+ * kept edges. One launch sequence serves the whole batch. */
+size_t eac_sl_lifetime_batched_workspace_bytes(int32_t N, int64_t M, int32_t G);
+int eac_sl_lifetime_batched(const int32_t* dest, const void* weight, int32_t weight_dtype,
+                            const int64_t* first_edge, const int32_t* out_degree, int32_t N,
+                            int64_t M, int32_t symmetric, const int64_t* h_goff, int32_t G,
+                            int32_t rule, double max_dist, int32_t n_clusters, int32_t* labels,
+                            int64_t* gout, int64_t* h_out, void* ws, size_t ws_bytes,
+                            void* stream);
+
 /* ---- stable LSD radix sort of 64-bit keys (optional 32-bit values) ------
  * Sorts by bits [begin_bit, end_bit). keys_out/vals_out must not alias the
  * inputs. No reference counterpart as an API: it replaces the numpy argsorts

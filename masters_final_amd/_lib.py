@@ -48,6 +48,9 @@ SIGNATURES = {
     "eac_sl_lifetime_workspace_bytes": (_sz, [_i32, _i64]),
     "eac_sl_lifetime": (C.c_int, [_p, _p, _i32, _p, _p, _i32, _i64, _i32, _i32, C.c_double,
                                   _i32, _p, _p, _p, _sz, _p]),
+    "eac_sl_lifetime_batched_workspace_bytes": (_sz, [_i32, _i64, _i32]),
+    "eac_sl_lifetime_batched": (C.c_int, [_p, _p, _i32, _p, _p, _i32, _i64, _i32, _p, _i32, _i32,
+                                          C.c_double, _i32, _p, _p, _p, _p, _sz, _p]),
     "eac_sort_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "eac_radix_sort_u64": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, _p, _sz, _p]),
     "eac_kmeans_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),

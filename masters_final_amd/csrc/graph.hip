@@ -1114,34 +1114,35 @@ __global__ void k_compact_flagged(const uint64_t* __restrict__ in, const int32_t
     if (flag[i]) out[pos[i]] = in[i];
 }
 
-// Single-thread decision of eac.py:220-265 (rule 0) or
-// singlelink.py:16-48 (rule 1) from the lifetime statistics.
+// Decision of eac.py:220-265 (rule 0) or singlelink.py:16-48 (rule 1) for
+// one graph from the statistics of its sorted forest weights sk[0:nf].
+struct CutOut {
+  int64_t keep, nc_stable;
+  int err, labels_empty;
+  double th;
+};
 template <typename W>
-__global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_t n, int vb,
-                             uint64_t wlo, uint64_t whi, int rule, double max_dist,
-                             int32_t n_clusters, CutState* cs) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  cs->err = 0;
-  cs->labels_empty = 0;
-  const double gmax = weight_value<W>(whi);  // largest graph weight
-  if (nf < 2) {
-    if (rule == 1 || n_clusters == 0) { cs->err = 1; return; }
-  }
-  const int64_t a = (int64_t)cs->argmax;
-  const double th = nf >= 2 ? __longlong_as_double((long long)cs->max_lt_bits) : 0.0;
+__device__ CutOut cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int64_t n, int vb,
+                             uint64_t wlo, uint64_t gmax_word, int rule, double max_dist,
+                             int32_t n_clusters, unsigned long long max_lt_bits,
+                             unsigned long long min_lt_bits, unsigned long long argmax) {
+  CutOut o{nf, 0, 0, 0, 0.0};
+  const double gmax = weight_value<W>(gmax_word);  // largest graph weight
+  if (nf < 2 && (rule == 1 || n_clusters == 0)) { o.err = 1; return o; }
+  const int64_t a = (int64_t)argmax;
+  const double th = nf >= 2 ? __longlong_as_double((long long)max_lt_bits) : 0.0;
   const double wmaxf = nf >= 1 ? weight_value<W>((sk[nf - 1] >> (2 * vb)) + wlo) : 0.0;
-  cs->th = th;
+  o.th = th;
   if (rule == 1) {
     // singlelink.py:14-48
     double disconnect = max_dist < 0 ? gmax : max_dist;
     double lt_threshold = disconnect - wmaxf;
-    cs->keep = th >= lt_threshold ? a : nf;
-    cs->nc_stable = 0;
-    return;
+    o.keep = th >= lt_threshold ? a : nf;
+    return o;
   }
   // eac.py:199-267
   double md = max_dist < 0 ? gmax + 1 : max_dist + 1;
-  int64_t n_disc = (int64_t)n - nf;
+  int64_t n_disc = n - nf;
   int64_t nc_stable;
   if (n_clusters == 0) {
     // lifetime_n_clusters, eac.py:366-391
@@ -1153,8 +1154,8 @@ __global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_
     }
     int64_t above = nf - lo;
     int64_t cont = above == 0 ? 1 : above + 1;
-    double mn = __longlong_as_double((long long)cs->min_lt_bits);
-    if (!(mn <= 1e-8)) { cs->err = 1; return; }  // np.min over an empty selection
+    double mn = __longlong_as_double((long long)min_lt_bits);
+    if (!(mn <= 1e-8)) { o.err = 1; return o; }  // np.min over an empty selection
     if (th < 2 * mn) cont = 1;
     if (n_disc > 1) {
       double disconnect_lifetime = md + 1 - wmaxf;
@@ -1170,9 +1171,180 @@ __global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_
     keep = nf - (nc_stable - n_disc);
     if (keep < 0) keep = 0;
   }
-  cs->keep = keep;
-  cs->nc_stable = nc_stable;
-  cs->labels_empty = nc_stable > 1 ? 0 : 1;
+  o.keep = keep;
+  o.nc_stable = nc_stable;
+  o.labels_empty = nc_stable > 1 ? 0 : 1;
+  return o;
+}
+
+template <typename W>
+__global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_t n, int vb,
+                             uint64_t wlo, uint64_t whi, int rule, double max_dist,
+                             int32_t n_clusters, CutState* cs) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  CutOut o = cut_decide<W>(sk, nf, n, vb, wlo, whi, rule, max_dist, n_clusters, cs->max_lt_bits,
+                           cs->min_lt_bits, cs->argmax);
+  cs->err = o.err;
+  cs->labels_empty = o.labels_empty;
+  cs->keep = o.keep;
+  cs->nc_stable = o.nc_stable;
+  cs->th = o.th;
+}
+
+// ------------------------------------------------- batched single-link cut
+// Graph of a forest key (by its smaller endpoint) for disjoint-union batches.
+__global__ void k_graph_of_key(const uint64_t* __restrict__ sk, int64_t nf, int vb,
+                               const int64_t* __restrict__ goff, int G,
+                               uint32_t* __restrict__ gid, int32_t* __restrict__ gcount) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a, b;
+    unpack_ends(sk[i], vb, &a, &b);
+    int lo = 0, hi = G;
+    while (lo + 1 < hi) {
+      int mid = (lo + hi) >> 1;
+      if (goff[mid] <= a) lo = mid; else hi = mid;
+    }
+    gid[i] = (uint32_t)lo;
+    atomicAdd(&gcount[lo], 1);
+  }
+}
+
+// Per-graph largest weight word over the graph's CSR entries (rows of a
+// graph are contiguous): one block per graph.
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_graph_wmax(const W* __restrict__ weight,
+                                                    const int64_t* __restrict__ fe,
+                                                    const int32_t* __restrict__ od,
+                                                    const int64_t* __restrict__ goff, int G,
+                                                    unsigned long long* __restrict__ gmax) {
+  __shared__ unsigned long long s_m[kGB / kWave];
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    unsigned long long mx = 0;
+    const int64_t v0 = goff[g], v1 = goff[g + 1];
+    for (int64_t v = v0; v < v1; ++v) {
+      int64_t b = fe[v], e = HAS_OD ? b + od[v] : fe[v + 1];
+      for (int64_t q = b + threadIdx.x; q < e; q += kGB) {
+        unsigned long long x = word_of<W>(weight[q]);
+        mx = x > mx ? x : mx;
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      unsigned long long o = __shfl_xor(mx, off, kWave);
+      mx = o > mx ? o : mx;
+    }
+    if (lane_id() == 0) s_m[threadIdx.x / kWave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < kGB / kWave; ++q) mx = s_m[q] > mx ? s_m[q] : mx;
+      gmax[g] = mx;
+    }
+    __syncthreads();
+  }
+}
+
+// Lifetime statistics and the cut decision of every graph: one block per
+// graph over its segment sk[seg[g] : seg[g+1]] (sorted by key).
+struct GraphCut {
+  long long keep, nc_stable, err, labels_empty;
+};
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_seg_cut(const uint64_t* __restrict__ sk,
+                                                 const int64_t* __restrict__ seg,
+                                                 const int64_t* __restrict__ goff, int G, int vb,
+                                                 uint64_t wlo,
+                                                 const unsigned long long* __restrict__ gmax,
+                                                 int rule, double max_dist, int32_t n_clusters,
+                                                 GraphCut* __restrict__ out) {
+  __shared__ unsigned long long s_a[kGB / kWave], s_b[kGB / kWave];
+  __shared__ unsigned long long s_mx, s_mn, s_arg;
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    const uint64_t* k = sk + seg[g];
+    const int64_t nf = seg[g + 1] - seg[g];
+    unsigned long long mx = 0, mn = ~0ull;
+    for (int64_t i = threadIdx.x; i + 1 < nf; i += kGB) {
+      double wi = weight_value<W>((k[i] >> (2 * vb)) + wlo);
+      double wj = weight_value<W>((k[i + 1] >> (2 * vb)) + wlo);
+      unsigned long long b = (unsigned long long)__double_as_longlong(wj - wi);
+      mx = b > mx ? b : mx;
+      mn = b < mn ? b : mn;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      unsigned long long a = __shfl_xor(mx, off, kWave), b = __shfl_xor(mn, off, kWave);
+      mx = a > mx ? a : mx;
+      mn = b < mn ? b : mn;
+    }
+    if (lane_id() == 0) { s_a[threadIdx.x / kWave] = mx; s_b[threadIdx.x / kWave] = mn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 1; q < kGB / kWave; ++q) {
+        mx = s_a[q] > mx ? s_a[q] : mx;
+        mn = s_b[q] < mn ? s_b[q] : mn;
+      }
+      s_mx = mx;
+      s_mn = mn;
+      s_arg = ~0ull;
+    }
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i + 1 < nf; i += kGB) {
+      double wi = weight_value<W>((k[i] >> (2 * vb)) + wlo);
+      double wj = weight_value<W>((k[i + 1] >> (2 * vb)) + wlo);
+      if ((unsigned long long)__double_as_longlong(wj - wi) == s_mx)
+        atomicMin(&s_arg, (unsigned long long)i);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      CutOut o = cut_decide<W>(k, nf, goff[g + 1] - goff[g], vb, wlo, gmax[g], rule, max_dist,
+                               n_clusters, s_mx, s_mn, s_arg);
+      out[g].keep = o.keep;
+      out[g].nc_stable = o.nc_stable;
+      out[g].err = o.err;
+      out[g].labels_empty = o.labels_empty;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_seg_keep_flags(const uint32_t* __restrict__ gid, int64_t nf,
+                                 const int64_t* __restrict__ seg,
+                                 const GraphCut* __restrict__ cut, int32_t* __restrict__ flag) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t g = gid[i];
+    flag[i] = (i - seg[g]) < cut[g].keep;
+  }
+}
+
+// per-graph labels from union labels numbered by first occurrence (graphs
+// occupy consecutive vertex ranges, so graph g's labels start at the label of
+// its first vertex): base[g] = label of the first vertex, base[G] = total.
+__global__ void k_graph_base(const int32_t* __restrict__ lab, const int64_t* __restrict__ goff,
+                             int G, const int32_t* __restrict__ total,
+                             int32_t* __restrict__ base) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
+    if (g == G) { base[g] = *total; continue; }
+    base[g] = goff[g + 1] > goff[g] ? lab[goff[g]] : -1;
+  }
+}
+__global__ void k_graph_sub(int32_t* __restrict__ lab, const int64_t* __restrict__ goff, int G,
+                            const int32_t* __restrict__ base, const GraphCut* __restrict__ cut,
+                            int64_t* __restrict__ gout) {
+  for (int g = blockIdx.x; g < G; g += gridDim.x) {
+    const int64_t v0 = goff[g], v1 = goff[g + 1];
+    const int32_t b = base[g];
+    for (int64_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) lab[v] -= b;
+    if (threadIdx.x == 0) {
+      int32_t nb = -1;
+      for (int q = g + 1; q <= G; ++q)
+        if (base[q] >= 0) { nb = base[q]; break; }
+      gout[4 * g + 0] = v1 > v0 ? (int64_t)(nb - b) : 0;
+      gout[4 * g + 1] = cut[g].labels_empty;
+      gout[4 * g + 2] = cut[g].nc_stable;
+      gout[4 * g + 3] = cut[g].err ? -1 : cut[g].keep;
+    }
+  }
 }
 
 template <typename W, bool HAS_OD>
@@ -1265,6 +1437,118 @@ int sl_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32
   return EAC_OK;
 }
 
+// Many independent graphs given as one disjoint union (graph g owns the
+// vertex range [goff[g], goff[g+1]) and its rows): one Boruvka run over the
+// union (keys on global ids keep each graph's own (w, min, max) order, so
+// each graph's forest is its own canonical forest), then per-graph sorted
+// forests (global key sort + stable sort by graph), per-graph lifetime cut,
+// and one union-find labelling numbered by first occurrence inside each
+// graph. gout[g] = {components, labels_empty, nc_stable, kept edges or -1
+// where the reference raises ValueError}.
+template <typename W, bool HAS_OD>
+int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+                    int32_t N, int64_t M, int symmetric, const int64_t* h_goff, int G, int rule,
+                    double max_dist, int32_t n_clusters, int32_t* labels, int64_t* gout,
+                    int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  MstBuffers B = carve_mst(ar, N, M, symmetric);
+  uint64_t* sk = ar.take<uint64_t>(N);
+  uint64_t* sk2 = ar.take<uint64_t>(N);
+  uint32_t* gid = ar.take<uint32_t>(N);
+  uint32_t* gid_s = ar.take<uint32_t>(N);
+  int64_t* goff = ar.take<int64_t>(G + 1);
+  int64_t* seg = ar.take<int64_t>(G + 2);
+  int32_t* gcount = ar.take<int32_t>(G + 2);
+  unsigned long long* gmax = ar.take<unsigned long long>(G);
+  GraphCut* cut = ar.take<GraphCut>(G);
+  int32_t* base = ar.take<int32_t>(G + 1);
+  CutState* cs = ar.take<CutState>(1);
+  int32_t* misc = ar.take<int32_t>(8);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "sl_batched: workspace too small");
+  const size_t mark0 = ar.used;
+  ForestInfo fi;
+  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, N, M, symmetric, B, ar, &fi, s)));
+  ar.used = mark0;
+  int64_t nf = fi.n_f;
+  const int vb = fi.vb;
+  EAC_TRY((radix_sort<uint64_t, NoValue>(B.fkeys, sk, nullptr, nullptr, nf, 0,
+                                         fi.wbits + 2 * vb, ar, s)));
+  ar.used = mark0;
+  const int g1 = grid_for(nf, kGB, persistent_blocks());
+  if (rule == 0 && nf > 0) {
+    // zero weights leave the forest (Kruskal's eliminate_zeros)
+    CutState init;
+    memset(&init, 0, sizeof(init));
+    init.min_lt_bits = ~0ull;
+    void* pin = pinned_scratch(sizeof(CutState));
+    if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
+    EAC_HIP_CHECK(hipStreamSynchronize(s));
+    memcpy(pin, &init, sizeof(init));
+    EAC_HIP_CHECK(hipMemcpyAsync(cs, pin, sizeof(CutState), hipMemcpyHostToDevice, s));
+    k_lifetime_stats<W><<<g1, kGB, 0, s>>>(sk, nf, vb, fi.wlo, cs);
+    EAC_LAUNCH_CHECK();
+    CutState hc;
+    EAC_TRY(d2h_sync(&hc, cs, sizeof(CutState), s));
+    if (hc.zero_w > 0) {
+      k_mark_nonzero<W><<<g1, kGB, 0, s>>>(sk, nf, vb, fi.wlo, B.flag);
+      EAC_LAUNCH_CHECK();
+      EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, nf, 0, misc, ar, s)));
+      ar.used = mark0;
+      k_compact_flagged<<<g1, kGB, 0, s>>>(sk, B.flag, B.rank, nf, B.fkeys);
+      EAC_LAUNCH_CHECK();
+      nf -= (int64_t)hc.zero_w;
+      EAC_HIP_CHECK(hipMemcpyAsync(sk, B.fkeys, sizeof(uint64_t) * nf, hipMemcpyDeviceToDevice, s));
+    }
+  }
+  EAC_HIP_CHECK(hipMemcpyAsync(goff, h_goff, sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
+  EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_goff may be pageable
+  EAC_TRY(fill<int32_t>(gcount, G + 2, 0, s));
+  if (nf > 0) {
+    k_graph_of_key<<<g1, kGB, 0, s>>>(sk, nf, vb, goff, G, gid, gcount);
+    EAC_LAUNCH_CHECK();
+    const int gb = bits_for((uint64_t)G);
+    EAC_TRY((radix_sort<uint32_t, uint64_t>(gid, gid_s, sk, sk2, nf, 0, gb < 1 ? 1 : gb, ar, s)));
+    ar.used = mark0;
+  }
+  EAC_TRY((scan_excl<int32_t, int64_t>(gcount, seg, G + 1, 0, nullptr, ar, s)));
+  ar.used = mark0;
+  k_graph_wmax<W, HAS_OD><<<G < 4096 ? G : 4096, kGB, 0, s>>>(weight, fe, od, goff, G, gmax);
+  EAC_LAUNCH_CHECK();
+  k_seg_cut<W><<<G < 4096 ? G : 4096, kGB, 0, s>>>(sk2, seg, goff, G, vb, fi.wlo, gmax, rule,
+                                                  max_dist, n_clusters, cut);
+  EAC_LAUNCH_CHECK();
+  int64_t nk = 0;
+  if (nf > 0) {
+    k_seg_keep_flags<<<g1, kGB, 0, s>>>(gid_s, nf, seg, cut, B.flag);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, nf, 0, misc, ar, s)));
+    ar.used = mark0;
+    k_compact_flagged<<<g1, kGB, 0, s>>>(sk2, B.flag, B.rank, nf, B.fkeys);
+    EAC_LAUNCH_CHECK();
+    int32_t hk = 0;
+    EAC_TRY(d2h_sync(&hk, misc, sizeof(int32_t), s));
+    nk = hk;
+  }
+  const int64_t ftiles = (N + kGTile - 1) / kGTile;
+  int32_t* d_ncomp = B.tile_cnt + ftiles;
+  EAC_TRY(labels_from_keys(B.fkeys, nk, vb, N, B.hook, B.comp, B.flag, B.rank, labels, d_ncomp,
+                           ar, s));
+  k_graph_base<<<grid_for(G + 1, 256), 256, 0, s>>>(labels, goff, G, d_ncomp, base);
+  EAC_LAUNCH_CHECK();
+  k_graph_sub<<<G < 4096 ? G : 4096, 256, 0, s>>>(labels, goff, G, base, cut, gout);
+  EAC_LAUNCH_CHECK();
+  int32_t tot = 0;
+  EAC_TRY(d2h_sync(&tot, d_ncomp, sizeof(int32_t), s));
+  prof_flush();
+  if (h_out) {
+    h_out[0] = tot;
+    h_out[1] = nf;
+    h_out[2] = fi.rounds;
+    h_out[3] = nk;
+  }
+  return EAC_OK;
+}
+
 // Calls F::template run<W, HAS_OD>(...) for the runtime weight dtype.
 template <typename F, typename... A>
 int dispatch_weight(int32_t dt, bool has_od, A&&... a) {
@@ -1291,6 +1575,18 @@ struct MstRun {
                  int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
     return mst_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, n, m, symmetric, ids, (W*)wout,
                                labels, h_out, ws, ws_bytes, s);
+  }
+};
+
+struct SlBatchRun {
+  template <typename W, bool HAS_OD>
+  static int run(const int32_t* dest, const void* weight, const int64_t* fe, const int32_t* od,
+                 int32_t N, int64_t M, int symmetric, const int64_t* h_goff, int G, int rule,
+                 double max_dist, int32_t n_clusters, int32_t* labels, int64_t* gout,
+                 int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+    return sl_batched_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, N, M, symmetric, h_goff, G,
+                                      rule, max_dist, n_clusters, labels, gout, h_out, ws,
+                                      ws_bytes, s);
   }
 };
 
@@ -1495,4 +1791,38 @@ extern "C" int eac_canonical_relabel(const int32_t* labels_in, int32_t n, int32_
   if (hm[0]) return fail(EAC_ERR_ARG, "relabel: label outside [0, n_values)");
   if (h_count) *h_count = (int32_t)hm[1];
   return EAC_OK;
+}
+
+extern "C" size_t eac_sl_lifetime_batched_workspace_bytes(int32_t N, int64_t M, int32_t G) {
+  size_t b = eac_boruvka_workspace_bytes(N, M);
+  size_t n = (size_t)(N < 1 ? 1 : N);
+  b += 4 * ((n * 8 + 255) & ~size_t(255)) + 64 * (size_t)(G + 2) + 8192;
+  size_t r1 = radix_sort_ws_bytes(n, 8, 0), r2 = radix_sort_ws_bytes(n, 4, 8);
+  return b + (r1 > r2 ? r1 : r2);
+}
+
+extern "C" int eac_sl_lifetime_batched(const int32_t* dest, const void* weight,
+                                       int32_t weight_dtype, const int64_t* first_edge,
+                                       const int32_t* out_degree, int32_t N, int64_t M,
+                                       int32_t symmetric, const int64_t* h_goff, int32_t G,
+                                       int32_t rule, double max_dist, int32_t n_clusters,
+                                       int32_t* labels, int64_t* gout, int64_t* h_out, void* ws,
+                                       size_t ws_bytes, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (N < 1 || M < 1 || G < 1) return fail(EAC_ERR_ARG, "sl_batched: need N, M, G >= 1");
+  if (rule != EAC_CUT_EAC && rule != EAC_CUT_SINGLELINK)
+    return fail(EAC_ERR_ARG, "sl_batched: unknown rule %d", rule);
+  if (!dest || !weight || !first_edge || !h_goff || !labels || !gout)
+    return fail(EAC_ERR_ARG, "sl_batched: null array");
+  if (h_goff[0] != 0 || h_goff[G] != N) return fail(EAC_ERR_ARG, "sl_batched: bad graph offsets");
+  for (int g = 0; g < G; ++g)
+    if (h_goff[g + 1] < h_goff[g]) return fail(EAC_ERR_ARG, "sl_batched: offsets not ascending");
+  if (weight_bytes(weight_dtype) == 0) return fail(EAC_ERR_ARG, "sl_batched: bad weight dtype");
+  size_t need = eac_sl_lifetime_batched_workspace_bytes(N, M, G);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "sl_batched: workspace %zu < %zu bytes", ws_bytes, need);
+  return dispatch_weight<SlBatchRun>(weight_dtype, out_degree != nullptr, dest, weight, first_edge,
+                                     out_degree, N, M, symmetric ? 1 : 0, h_goff, (int)G,
+                                     (int)rule, max_dist, n_clusters, labels, gout, h_out, ws,
+                                     ws_bytes, s);
 }

@@ -20,6 +20,8 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
+
 from .. import _lib as L
 from .mst import _graph_to_device
 
@@ -59,3 +61,31 @@ def sl_mst_lifetime_gpu(dest, weight, fe, od, disconnect_weight=None, MAX_TPB=25
 def sl_mst_lifetime_seq(dest, weight, fe, od, disconnect_weight=None):
     """singlelink.py:14-65 signature (host arrays), computed on the GPU."""
     return sl_mst_lifetime_gpu(dest, weight, fe, od, disconnect_weight)
+
+
+def lifetime_cut_batched(dest, weight, indptr, goff, rule=EAC_CUT_EAC, max_dist=None,
+                         n_clusters=0, symmetric=True):
+    """Many independent graphs as one disjoint union (graph g owns vertices
+    goff[g]:goff[g+1] and their rows; dest holds union ids). Returns (labels
+    per graph, numbered by first occurrence inside each graph, as one device
+    int32 tensor [N]; per-graph int64 tensor [G, 4] = components,
+    labels_empty, nc_stable, kept edges (-1 where the reference raises))."""
+    t = L.torch()
+    lib = L.load_library()
+    d_dest, d_w, d_ip, _ = _graph_to_device(dest, weight, indptr, None)
+    N = d_ip.numel() - 1
+    M = d_dest.numel()
+    g = np.ascontiguousarray(np.asarray(goff, dtype=np.int64))
+    G = g.size - 1
+    labels = t.empty(N, dtype=t.int32, device=d_dest.device)
+    gout = t.empty((G, 4), dtype=t.int64, device=d_dest.device)
+    h = (C.c_int64 * 4)()
+    ws = L.workspace(lib.eac_sl_lifetime_batched_workspace_bytes(N, M, G))
+    L.check(lib.eac_sl_lifetime_batched(L.ptr(d_dest), L.ptr(d_w), L.weight_code(d_w.dtype),
+                                        L.ptr(d_ip), None, N, M, 1 if symmetric else 0,
+                                        g.ctypes.data_as(C.c_void_p), G, int(rule),
+                                        -1.0 if max_dist is None else float(max_dist),
+                                        int(n_clusters), L.ptr(labels), L.ptr(gout), h,
+                                        L.ptr(ws), ws.numel(), L.stream_ptr()))
+    return labels, gout, {"components": int(h[0]), "forest_edges": int(h[1]),
+                          "rounds": int(h[2]), "kept_edges": int(h[3])}

@@ -111,3 +111,34 @@ def eac_small(seed=0):
     """n=10,000 points in 2-D, 6-component GMM, means U[-10,10]^2, unit std."""
     X, y = gmm_points(10_000, 2, 6, seed, mean_scale=10.0, std=1.0)
     return X, y
+
+
+def graph_batch(n_graphs, v_range, deg_range, seed, wmax=256, kind="knn", dtype=np.int32):
+    """Independent graphs as one disjoint union CSR: vertex counts log-uniform
+    in v_range, mean degree uniform in deg_range (k-NN with k = 0.75 * target,
+    or Erdos-Renyi for kind="er"), integer weights U[1, wmax]. Returns
+    dict(indptr, dest, weight, goff, n, targets)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(v_range[0]), np.log(v_range[1])
+    nv = np.exp(rng.uniform(lo, hi, n_graphs)).astype(np.int64)
+    deg = rng.uniform(deg_range[0], deg_range[1], n_graphs)
+    parts = []
+    for g in range(n_graphs):
+        sub_seed = int(rng.integers(0, 2**31 - 1))
+        if kind == "knn":
+            k = max(1, int(round(0.75 * deg[g])))
+            gg = knn_graph(n=int(nv[g]), k=min(k, int(nv[g]) - 1), seed=sub_seed, frac_islands=0.0,
+                           n_islands=0, wmin=1, wmax=wmax, dtype=dtype)
+        else:
+            gg = random_graph(int(nv[g]), deg[g], sub_seed, wmax=wmax, dtype=dtype)
+        parts.append(gg)
+    goff = np.zeros(n_graphs + 1, np.int64)
+    np.cumsum([p["n"] for p in parts], out=goff[1:])
+    eoff = np.zeros(n_graphs + 1, np.int64)
+    np.cumsum([p["dest"].size for p in parts], out=eoff[1:])
+    dest = np.concatenate([p["dest"].astype(np.int64) + goff[i] for i, p in enumerate(parts)])
+    weight = np.concatenate([p["weight"] for p in parts])
+    indptr = np.concatenate([p["indptr"][:-1] + eoff[i] for i, p in enumerate(parts)] +
+                            [np.array([eoff[-1]])])
+    return {"indptr": indptr.astype(np.int64), "dest": dest.astype(np.int32),
+            "weight": weight, "goff": goff, "n": int(goff[-1]), "targets": deg}

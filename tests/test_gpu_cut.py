@@ -110,3 +110,32 @@ def test_sp_sl_lifetime_hand_cases(torch):
     assert sp_sl_lifetime(m2)[1].tolist() == [0, 0, 0, 1, 1, 2]
     assert sp_sl_lifetime(m2, 10)[1].tolist() == [0, 0, 0, 1, 1, 1]
     assert sp_sl_lifetime(m2, n_clusters=4)[1].tolist() == [0, 0, 0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("rule,kind", [(0, "er"), (1, "er"), (0, "knn")])
+def test_batched_lifetime_cut_matches_per_graph_oracle(torch, rule, kind):
+    from masters_final_amd.graph.singlelink import lifetime_cut_batched
+    b = synth.graph_batch(12, (50, 3000), (3, 12), seed=7 + rule, wmax=40, kind=kind)
+    labels, gout, info = lifetime_cut_batched(b["dest"], b["weight"], b["indptr"], b["goff"],
+                                              rule=rule)
+    labels = labels.cpu().numpy()
+    gout = gout.cpu().numpy()
+    fe = b["indptr"][:-1]
+    od = np.diff(b["indptr"])
+    u, v, w, _ = O.csr_edges(b["dest"], b["weight"], fe, od)
+    for g in range(len(b["goff"]) - 1):
+        a, z = b["goff"][g], b["goff"][g + 1]
+        sel = (u >= a) & (u < z)
+        try:
+            if rule == 0:
+                nc, want = O.sp_sl_lifetime(u[sel] - a, v[sel] - a, w[sel], z - a)
+            else:
+                want, nc, kept = O.sl_mst_lifetime(u[sel] - a, v[sel] - a, w[sel], z - a)
+        except ValueError:
+            assert gout[g, 3] == -1, g
+            continue
+        if rule == 0 and want.size == 0:
+            assert gout[g, 1] == 1
+            continue
+        assert gout[g, 0] == nc, g
+        assert np.array_equal(labels[a:z], want), g
