@@ -1,21 +1,23 @@
-"""Benchmark of the EAC hot path on one MI355X.
+"""Benchmarks of the EAC hot path on one MI355X.
 
-Workload (default): Boruvka minimum spanning forest + connected components
-of a 1,000,000-vertex symmetrized 3-D k-NN graph (k=20, int32 weights
-uniform in [1,100], 10% of the vertices in 4 separate islands, seed 1).
-One step = one call of the forest + component-label path on the graph,
-inputs already resident in HBM. value = directed CSR entries / step time
+Default workload (the one the headline metric is quoted on): Boruvka minimum
+spanning forest + connected components of a 1,000,000-vertex symmetrized 3-D
+k-NN graph (k=20, int32 weights uniform in [1,100], 10% of the vertices in 4
+separate islands, seed 1). One step = one call of the forest + component-label
+path, inputs resident in HBM; value = directed CSR entries / step time
 (Medges/s). The CPU baseline is the single-threaded C restatement of the
-reference's boruvka_minho_seq, timed on the same graph on this host.
+reference's boruvka_minho_seq on the same graph. The same JSON line carries
+the end-to-end EAC comparison on the 10,000-point workload (batched ensemble +
+BF16 path vs plain per-call FP32 path, exact labels).
 
-Prints one JSON line (see README of the driver contract): metric, value,
-unit, roofline (dominant kernel, HIP events), cpu_baseline.
+Other workloads (--workload): eac, kmeans, coassoc, sl_batched. Each prints
+one JSON line with the same top-level keys.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -25,39 +27,105 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "EAC pts/s + Boruvka MST Medges/s: batched/BF16 vs plain FP32 GPU, exact labels"
 HBM_PEAK_GBS = 8000.0
-PROF_NAMES = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)", 2: "k_hook",
-              3: "k_jump", 4: "forest output + labels"}
+BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)", 2: "k_hook",
+             3: "k_jump", 4: "forest output + labels"}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--knn", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calib", action="store_true",
-                    help="only run a 256 MiB device copy 5 times (counter calibration)")
-    args = ap.parse_args()
-    if args.copy_calib:
-        import torch
-        x = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes": int(x.numel())}))
-        return
-    if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the workload has no "
-              "multi-GPU path in this build", file=sys.stderr)
-        sys.exit(2)
+def read_prof(lib, L, slots):
+    prof = {}
+    for slot in slots:
+        tm, nl, nb = C.c_double(), C.c_int64(), C.c_int64()
+        L.check(lib.eac_profile_read(slot, C.byref(tm), C.byref(nl), C.byref(nb)))
+        prof[slot] = (tm.value, nl.value, nb.value)
+    return prof
 
+
+def base_line(args, value, unit, ms, dtype, config, higher=True):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": config}
+
+
+def timed(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        r = fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = fn()
+    torch.cuda.synchronize()
+    return r, (time.perf_counter() - t0) * 1e3 / steps
+
+
+# ------------------------------------------------------------------ EAC e2e
+def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu=True):
+    """Batched+BF16 EAC vs plain per-call FP32 EAC on the 2-D 6-GMM workload.
+    k ~ U-int[kmin, kmax] inclusive, initial centroids = distinct random points."""
+    import numpy as np
+    import torch
+
+    from masters_final_amd import synth
+    from masters_final_amd.cluster.KMeans import fit_batched
+    from masters_final_amd.EAC.eac import eac_pipeline, sp_sl_lifetime_device
+    from masters_final_amd.EAC.sparse import coassoc_device
+
+    X, _ = synth.gmm_points(n, 2, 6, seed, mean_scale=10.0, std=1.0)
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(kmin, kmax + 1, P)
+    init = np.concatenate([X[rng.choice(n, int(k), replace=False)] for k in ks])
+    dX = torch.from_numpy(X).cuda()
+    dinit = torch.from_numpy(init).cuda()
+    off = np.concatenate([[0], np.cumsum(ks)])
+
+    def batched():
+        return eac_pipeline(dX, ks, iters=iters, precision="bf16x3", init=dinit)
+
+    def plain():
+        labs = []
+        for p in range(P):  # one K-Means call per partition, FP32 labeling
+            r = fit_batched(dX, dinit[off[p]:off[p + 1]], [int(ks[p])], max_iter=iters,
+                            precision="fp32")
+            labs.append(r["labels"])
+        labels = torch.cat(labs)
+        ip, ind, dat, mx = coassoc_device(labels, ks, w_offset="max+1")
+        nc, lab, _ = sp_sl_lifetime_device((ip, ind, dat), P)
+        return {"n_clusters": nc, "labels": lab, "ensemble_labels": labels}
+
+    rb, tb = timed(batched, reps, 1, torch)
+    rp, tp = timed(plain, reps, 1, torch)
+    out = {"n": n, "partitions": P, "k_range": [kmin, kmax], "iters": iters,
+           "batched_bf16_ms": tb, "plain_fp32_ms": tp, "speedup": tp / tb,
+           "batched_pts_per_s": n / (tb * 1e-3), "plain_pts_per_s": n / (tp * 1e-3),
+           "n_clusters": rb["n_clusters"], "nnz": rb["nnz"], "rescored": rb["n_rescored"]}
+    lb = rb["labels"].cpu().numpy() if rb["labels"] is not None else np.zeros(0)
+    lp = rp["labels"].cpu().numpy() if rp["labels"] is not None else np.zeros(0)
+    out["plain_equals_batched"] = bool(np.array_equal(lb, lp))
+    if cpu:
+        from oracle import oracle as O
+        c0 = time.perf_counter()
+        labs = np.stack([O.kmeans_fit(X, init[off[p]:off[p + 1]], max_iter=iters,
+                                      reseed_mode=1)[0] for p in range(P)])
+        ip, ind, cnt = O.coassoc_condensed(labs, ks)
+        w = (cnt.max() + 1 - cnt).astype(np.float64)
+        rows = np.repeat(np.arange(n), np.diff(ip))
+        nc, lab = O.sp_sl_lifetime(rows, ind, w, n, max_dist=P)
+        c1 = time.perf_counter()
+        out["oracle_equal"] = bool(np.array_equal(rb["ensemble_labels"].cpu().numpy(), labs)
+                                   and nc == rb["n_clusters"] and np.array_equal(lb, lab))
+        out["cpu_seconds"] = c1 - c0
+        out["cpu_pts_per_s"] = n / (c1 - c0)
+    return out
+
+
+# --------------------------------------------------------------- workloads
+def run_mst(args):
     import numpy as np
     import torch
 
@@ -76,7 +144,6 @@ def main():
     d_dest = torch.from_numpy(g["dest"]).to(dev)
     d_w = torch.from_numpy(g["weight"]).to(dev)
     d_ip = torch.from_numpy(g["indptr"]).to(dev)
-    torch.cuda.synchronize()
 
     def step():
         return boruvka_mst(d_dest, d_w, d_ip, None, want_ids=True, want_weights=True,
@@ -88,39 +155,26 @@ def main():
     log(f"warmup done: forest {r['n_mst']} edges, {r['n_components']} components, "
         f"{r['rounds']} rounds")
     L.check(lib.eac_profile_enable(1))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    prof = {}
-    import ctypes as C
-    for slot in PROF_NAMES:
-        tm, nl, nb = C.c_double(), C.c_int64(), C.c_int64()
-        L.check(lib.eac_profile_read(slot, C.byref(tm), C.byref(nl), C.byref(nb)))
-        prof[slot] = (tm.value, nl.value, nb.value)
+    r, ms = timed(step, args.steps, 0, torch)
+    prof = read_prof(lib, L, MST_SLOTS)
     L.check(lib.eac_profile_enable(0))
     dom = max(prof, key=lambda s: prof[s][0])
     tm, nl, nb = prof[dom]
     avg_ms = tm / max(nl, 1)
-    bytes_per_launch = nb / max(nl, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    bpl = nb / max(nl, 1)
+    achieved = bpl / (avg_ms * 1e-3) / 1e9
     log("kernel time per step: " + ", ".join(
-        f"{PROF_NAMES[s]} {prof[s][0] / args.steps:.3f} ms" for s in prof))
+        f"{MST_SLOTS[s]} {prof[s][0] / args.steps:.3f} ms" for s in prof))
     total_w = int(r["weights"][: r["n_mst"]].to(torch.int64).sum().item())
     labels_crc = int(torch.sum(r["labels"].to(torch.int64) *
                                torch.arange(n, device=dev, dtype=torch.int64) % 1000003).item())
-
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
-        fe = g["indptr"][:-1]
-        od = np.diff(g["indptr"])
         log("timing the CPU restatement of boruvka_minho_seq (1 thread)")
         c0 = time.perf_counter()
-        mst, nm, _ = O.boruvka_minho_seq(g["dest"], g["weight"], fe, od)
+        mst, nm, _ = O.boruvka_minho_seq(g["dest"], g["weight"], g["indptr"][:-1],
+                                         np.diff(g["indptr"]))
         c1 = time.perf_counter()
         cpu_w = int(g["weight"][mst[:nm]].astype(np.int64).sum())
         if cpu_w != total_w or nm != r["n_mst"]:
@@ -131,34 +185,231 @@ def main():
                          "restatement of mst.py:143-407 boruvka_minho_seq",
                "seconds": c1 - c0}
         log(f"CPU baseline {c1 - c0:.2f}s")
+    out = base_line(args, m / (ms * 1e-3) / 1e6, "Medges/s", ms, "int32",
+                    {"workload": "boruvka_mst_cc_knn", "n_vertices": n, "m_entries": m,
+                     "knn_k": args.knn, "weights": "int32 U[1,100]", "islands": 4, "seed": 1,
+                     "step": "forest ids+weights+first-occurrence component labels"})
+    out["result"] = {"forest_edges": r["n_mst"], "forest_weight": total_w,
+                     "components": r["n_components"], "rounds": r["rounds"],
+                     "labels_checksum": labels_crc}
+    out["roofline"] = {"bound": "hbm", "kernel": MST_SLOTS[dom], "achieved": achieved,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": None, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl}
+    out["kernels_ms_per_step"] = {MST_SLOTS[s]: prof[s][0] / args.steps for s in prof}
+    out["cpu_baseline"] = cpu
+    if not args.no_eac:
+        log("end-to-end EAC: batched+BF16 vs plain per-call FP32 (n=10,000, P=100)")
+        out["eac_small"] = eac_compare(cpu=not args.no_cpu)
+        log(f"EAC speedup {out['eac_small']['speedup']:.2f}x")
+    return out
 
-    out = {
-        "metric": METRIC,
-        "value": m / (ms * 1e-3) / 1e6,
-        "unit": "Medges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "boruvka_mst_cc_knn", "n_vertices": n, "m_entries": m,
-                   "knn_k": args.knn, "weights": "int32 U[1,100]", "islands": 4, "seed": 1,
-                   "step": "forest ids+weights+first-occurrence component labels"},
-        "result": {"forest_edges": r["n_mst"], "forest_weight": total_w,
-                   "components": r["n_components"], "rounds": r["rounds"],
-                   "labels_checksum": labels_crc},
-        "roofline": {"bound": "hbm", "kernel": PROF_NAMES[dom], "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": bytes_per_launch},
-        "kernels_ms_per_step": {PROF_NAMES[s]: prof[s][0] / args.steps for s in prof},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+
+def run_eac(args):
+    import torch
+    r = eac_compare(n=args.n if args.n != 1_000_000 else 10_000, cpu=not args.no_cpu,
+                    reps=max(args.steps, 1))
+    out = base_line(args, r["batched_pts_per_s"], "pts/s", r["batched_bf16_ms"], "f32/f64",
+                    {"workload": "eac_e2e_gmm2d", "n": r["n"], "partitions": r["partitions"],
+                     "k_range": r["k_range"], "iters": r["iters"]})
+    out["result"] = r
+    out["cpu_baseline"] = ({"value": r["cpu_pts_per_s"], "unit": "pts/s", "cores": 1,
+                            "kind": "port", "sample": "whole workload: C restatement of "
+                            "K_Means.fit x P, EAC_CSR build, sp_sl_lifetime"}
+                           if "cpu_pts_per_s" in r else None)
+    return out
+
+
+def run_kmeans(args):
+    import numpy as np
+    import torch
+
+    from masters_final_amd import _lib as L
+    from masters_final_amd import synth
+    from masters_final_amd.cluster.KMeans import label_device
+
+    lib = L.load_library()
+    n = args.n if args.n != 1_000_000 else 4_194_304
+    d, k = 64, 2048
+    log(f"generating GMM n={n} d={d} (2048 components)")
+    X, _ = synth.gmm_points(n, d, 2048, 2, mean_std=2.0, std=1.0)
+    rng = np.random.default_rng(2)
+    Cc = X[rng.choice(n, k, replace=False)].copy()
+    dX = torch.from_numpy(X).cuda()
+    dC = torch.from_numpy(Cc).cuda()
+    res = {}
+    for prec in ("bf16x3", "fp32", "fp64"):
+        L.check(lib.eac_profile_enable(1))
+        (lab, dist, nres), ms = timed(lambda: label_device(dX, dC, precision=prec),
+                                      args.steps, args.warmup, torch)
+        prof = read_prof(lib, L, {6: "label", 7: "rescore"})
+        L.check(lib.eac_profile_enable(0))
+        tm, nl, fl = prof[6]
+        res[prec] = {"ms": ms, "labels": lab, "rescored": nres,
+                     "label_kernel_ms": tm / max(nl, 1),
+                     "label_tflops": (fl / max(nl, 1)) / (tm / max(nl, 1) * 1e-3) / 1e12,
+                     "rescore_ms": prof[7][0] / max(prof[7][1], 1)}
+        log(f"{prec}: {ms:.2f} ms/pass, rescored {nres}")
+    ref = res["fp64"]["labels"]
+    agree = {p: int((res[p]["labels"] != ref).sum().item()) for p in ("bf16x3", "fp32")}
+    from oracle import oracle as O
+    samp = rng.choice(n, 2000, replace=False)
+    c0 = time.perf_counter()
+    want, _ = O.kmeans_label_f64(X[samp], Cc)
+    c1 = time.perf_counter()
+    orc_ok = bool(np.array_equal(res["bf16x3"]["labels"][0].cpu().numpy()[samp], want))
+    b = res["bf16x3"]
+    flops = 2.0 * n * k * d
+    out = base_line(args, flops / (b["ms"] * 1e-3) / 1e12, "TFLOP/s (2nkd)", b["ms"],
+                    "bf16x3->fp32, fp64 rescore",
+                    {"workload": "kmeans_label_gmm", "n": n, "d": d, "k": k, "seed": 2})
+    out["result"] = {"bf16x3_ms": b["ms"], "fp32_ms": res["fp32"]["ms"],
+                     "fp64_ms": res["fp64"]["ms"], "speedup_vs_fp32": res["fp32"]["ms"] / b["ms"],
+                     "rescored_points": b["rescored"], "rescored_frac": b["rescored"] / n,
+                     "mismatch_vs_fp64_gpu": agree, "oracle_sample_equal": orc_ok}
+    out["roofline"] = {"bound": "mfma", "kernel": "k_label_bf16x3",
+                       "achieved": b["label_tflops"], "peak": BF16_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": b["label_tflops"] / BF16_PEAK_TFLOPS,
+                       "traffic": None, "avg_launch_ms": b["label_kernel_ms"],
+                       "note": "algorithmic 2nkd; the split issues 3x that on MFMA"}
+    out["cpu_baseline"] = {"value": 2000 * k * d * 2 / (c1 - c0) / 1e12, "unit": "TFLOP/s",
+                           "cores": 1, "kind": "port",
+                           "sample": "2000 points x 2048 centroids, FP64 oracle labeling"}
+    return out
+
+
+def run_coassoc(args):
+    import numpy as np
+    import torch
+
+    from masters_final_amd import _lib as L
+    from masters_final_amd import synth
+    from masters_final_amd.EAC.sparse import coassoc_device
+    from masters_final_amd.utils.partition import generate_ensemble_batched
+
+    lib = L.load_library()
+    n = args.n if args.n != 1_000_000 else 500_000
+    P = args.partitions
+    log(f"generating anisotropic GMM n={n} d=16, P={P}")
+    X, _ = synth.gmm_points(n, 16, 32, 3, mean_scale=20.0, anisotropic=(0.2, 3.0))
+    rng = np.random.default_rng(3)
+    ks = rng.integers(32, 2049, P)
+    dX = torch.from_numpy(X).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    labels, fit = generate_ensemble_batched(dX, ks, iters=3, seed=3, precision="bf16x3")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    log(f"ensemble {t1 - t0:.2f}s, rescored {fit['n_rescored']}")
+    L.check(lib.eac_profile_enable(1))
+    (ip, ind, dat, mx), ms = timed(lambda: coassoc_device(labels, ks), args.steps, args.warmup,
+                                   torch)
+    prof = read_prof(lib, L, {8: "plan", 9: "count", 10: "fill"})
+    L.check(lib.eac_profile_enable(0))
+    nnz = int(ind.numel())
+    incr = 0
+    lab_h = labels.cpu().numpy()
+    for p in range(P):
+        s = np.bincount(lab_h[p], minlength=int(ks[p])).astype(np.int64)
+        incr += int((s * (s - 1) // 2).sum())
+    out = base_line(args, incr / (ms * 1e-3) / 1e9, "Gpair-increments/s", ms, "int32",
+                    {"workload": "coassoc_batched_ensemble", "n": n, "d": 16, "partitions": P,
+                     "k_range": [32, 2048], "seed": 3})
+    fill_ms = prof[10][0] / max(prof[10][1], 1)
+    fill_bytes = prof[10][2] / max(prof[10][1], 1)
+    out["result"] = {"nnz_condensed": nnz, "max_count": mx, "pair_increments": incr,
+                     "ensemble_seconds": t1 - t0, "rescored": fit["n_rescored"],
+                     "count_ms": prof[9][0] / max(prof[9][1], 1), "fill_ms": fill_ms,
+                     "plan_ms": prof[8][0] / max(prof[8][1], 1)}
+    ach = fill_bytes / (fill_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": "k_coassoc_rows<fill>", "achieved": ach,
+                       "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                       "traffic": None, "avg_launch_ms": fill_ms}
+    out["cpu_baseline"] = None
+    return out
+
+
+def run_sl_batched(args):
+    import numpy as np
+    import torch
+
+    from masters_final_amd import synth
+    from masters_final_amd.graph.singlelink import lifetime_cut, lifetime_cut_batched
+
+    G = args.graphs
+    log(f"generating {G} k-NN graphs")
+    t0 = time.perf_counter()
+    b = synth.graph_batch(G, (1000, args.vmax), (8, 64), seed=4, wmax=256, kind="knn")
+    log(f"batch: {b['n']} vertices, {b['dest'].size} entries, {time.perf_counter() - t0:.1f}s")
+    d_dest = torch.from_numpy(b["dest"]).cuda()
+    d_w = torch.from_numpy(b["weight"]).cuda()
+    d_ip = torch.from_numpy(b["indptr"]).cuda()
+    (lab, gout, info), ms = timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, b["goff"],
+                                                               rule=0), args.steps, args.warmup,
+                                  torch)
+    # plain: one call per graph
+    ips = [torch.from_numpy(b["indptr"][b["goff"][g]:b["goff"][g + 1] + 1] -
+                            b["indptr"][b["goff"][g]]).cuda() for g in range(G)]
+    dests = [d_dest[b["indptr"][b["goff"][g]]:b["indptr"][b["goff"][g + 1]]] - int(b["goff"][g])
+             for g in range(G)]
+    ws_ = [d_w[b["indptr"][b["goff"][g]]:b["indptr"][b["goff"][g + 1]]] for g in range(G)]
+
+    def plain():
+        return [lifetime_cut(dests[g], ws_[g], ips[g], None, rule=0) for g in range(G)]
+
+    rp, mp = timed(plain, 1, 1, torch)
+    same = True
+    lab_h = lab.cpu().numpy()
+    for g in range(G):
+        a, z = b["goff"][g], b["goff"][g + 1]
+        lg = rp[g][0]
+        if lg is None:
+            same &= bool(gout[g, 1].item() == 1)
+        else:
+            same &= bool(np.array_equal(lab_h[a:z], lg.cpu().numpy()))
+    m = int(b["dest"].size)
+    out = base_line(args, m / (ms * 1e-3) / 1e6, "Medges/s", ms, "int32",
+                    {"workload": "single_link_batched_knn_graphs", "graphs": G,
+                     "v_range": [1000, args.vmax], "deg_range": [8, 64], "w_range": [1, 256],
+                     "seed": 4, "n_vertices": b["n"], "m_entries": m})
+    out["result"] = {"batched_ms": ms, "per_graph_calls_ms": mp, "speedup": mp / ms,
+                     "equal_to_per_graph": same, **info}
+    out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mst",
+                    choices=["mst", "eac", "kmeans", "coassoc", "sl_batched"])
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--knn", type=int, default=20)
+    ap.add_argument("--partitions", type=int, default=256)
+    ap.add_argument("--graphs", type=int, default=512)
+    ap.add_argument("--vmax", type=int, default=200_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eac", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 256 MiB device copy 5 times (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        x = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": int(x.numel())}))
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    fn = {"mst": run_mst, "eac": run_eac, "kmeans": run_kmeans, "coassoc": run_coassoc,
+          "sl_batched": run_sl_batched}[args.workload]
+    print(json.dumps(fn(args)), flush=True)
 
 
 if __name__ == "__main__":

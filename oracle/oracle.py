@@ -186,7 +186,11 @@ def coassoc_condensed(labels, kp):
     L = _a(labels, np.int64)
     kp = _a(kp, np.int64)
     P, n = L.shape
-    cap = n * (n - 1) // 2 + 1
+    cap = 1  # upper bound: all within-cluster pairs of all partitions
+    for p in range(P):
+        c = np.bincount(L[p][L[p] >= 0], minlength=1).astype(np.int64)
+        cap += int((c * (c - 1) // 2).sum())
+    cap = min(cap, n * (n - 1) // 2 + 1)
     indptr = np.zeros(n + 1, np.int64)
     ind = np.zeros(cap, np.int64)
     cnt = np.zeros(cap, np.int64)
