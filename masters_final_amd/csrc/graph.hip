@@ -43,6 +43,7 @@
 
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 #include "prims.hpp"
 #include "sort.hpp"
@@ -352,15 +353,49 @@ __global__ __launch_bounds__(kGB) void k_round0(
   }
 }
 
-// ------------------------------------------------------ Boruvka round >= 1
-__global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list_in,
-                                               int64_t m_in, uint64_t* __restrict__ list_out,
-                                               unsigned long long* __restrict__ out_count,
+// ------------------------------------------------------ device round control
+// Rounds are driven from the device so the host enqueues several rounds per
+// synchronisation: a one-thread control kernel reads the counters of the
+// round just finished and writes what the next kernels read (which list,
+// how many edges, which representatives), or clears `active`.
+struct RoundCtl {
+  long long m_in;    // edges the next k_round reads
+  long long nroots;  // representatives the next k_hook reads
+  int src;           // list the next k_round reads: 0/1, or 2 = heavy list
+  int dst;           // list the next k_round writes (0/1)
+  int cur_list;      // list whose count closes the current round
+  int rsrc;          // roots buffer the next k_hook reads; -1 = every vertex
+  int rdst;          // roots buffer the next k_hook appends to
+  int heavy_done;
+  int active;
+  int rounds;
+  int overflow_rounds;
+  int pad;
+};
+struct RoundLog {
+  long long m_in, m_out, nroots;
+};
+
+// Filter the edge list against the current components and atomicMin every
+// cross edge into both components' slots.
+__global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list0,
+                                               const uint64_t* __restrict__ list1,
+                                               const uint64_t* __restrict__ heavy,
+                                               uint64_t* __restrict__ out0,
+                                               uint64_t* __restrict__ out1,
+                                               unsigned long long* __restrict__ counts,
+                                               const RoundCtl* __restrict__ ctl,
                                                const int32_t* __restrict__ comp,
                                                unsigned long long* __restrict__ best, int vb) {
   __shared__ uint64_t s_out[kGTile];
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
   __shared__ unsigned long long s_base;
+  if (!ctl->active) return;
+  const int src = ctl->src, dst = ctl->dst;
+  const uint64_t* list_in = src == 0 ? list0 : (src == 1 ? list1 : heavy);
+  uint64_t* list_out = dst == 0 ? out0 : out1;
+  unsigned long long* out_count = counts + dst;
+  const int64_t m_in = ctl->m_in;
   const int64_t nchunks = (m_in + kGTile - 1) / kGTile;
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int64_t e0 = c * kGTile;
@@ -394,28 +429,30 @@ __global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list
   }
 }
 
-// ------------------------------------------------------------------ hooking
 // For each current representative r: follow its lightest edge to the other
 // component s. If s picked the same edge and r < s, r stays representative
 // (mst.py:282-305: the smaller id keeps no edge); otherwise r hooks under s
 // and the edge joins the forest (recorded in r's slot: every vertex hooks at
 // most once in its lifetime, so slots are written once).
-__global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots_in,
-                                              int64_t nroots,
+__global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots0,
+                                              const int32_t* __restrict__ roots1,
+                                              int32_t* __restrict__ rout0,
+                                              int32_t* __restrict__ rout1,
+                                              unsigned long long* __restrict__ rcounts,
+                                              const RoundCtl* __restrict__ ctl,
                                               const unsigned long long* __restrict__ best,
                                               const int32_t* __restrict__ comp,
                                               int32_t* __restrict__ hook,
-                                              uint64_t* __restrict__ mst_key,
-                                              int32_t* __restrict__ roots_out,
-                                              unsigned long long* __restrict__ roots_out_count,
-                                              unsigned long long* __restrict__ hooks_count,
-                                              int vb) {
+                                              uint64_t* __restrict__ mst_key, int vb) {
   __shared__ int32_t s_out[kGB];
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
   __shared__ unsigned long long s_base;
-  __shared__ uint32_t s_hooks;
-  if (threadIdx.x == 0) s_hooks = 0;
-  __syncthreads();
+  if (!ctl->active) return;
+  const int rsrc = ctl->rsrc, rdst = ctl->rdst;
+  const int32_t* roots_in = rsrc < 0 ? nullptr : (rsrc == 0 ? roots0 : roots1);
+  int32_t* roots_out = rdst == 0 ? rout0 : rout1;
+  unsigned long long* roots_out_count = rcounts + rdst;
+  const int64_t nroots = ctl->nroots;
   for (int64_t base_i = (int64_t)blockIdx.x * kGB; base_i < nroots;
        base_i += (int64_t)gridDim.x * kGB) {
     int64_t i = base_i + threadIdx.x;
@@ -425,7 +462,7 @@ __global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots_
       r = roots_in ? roots_in[i] : (int32_t)i;
       uint64_t k = best[r];
       if (k == kInf) {
-        hook[r] = r;  // component complete: no edge leaves it
+        hook[r] = r;  // complete component, or not a representative
       } else {
         int32_t a, b;
         unpack_ends(k, vb, &a, &b);
@@ -437,7 +474,6 @@ __global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots_
         } else {
           hook[r] = s;
           mst_key[r] = k;
-          atomicAdd(&s_hooks, 1u);
         }
       }
     }
@@ -449,20 +485,17 @@ __global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots_
     for (uint32_t t = threadIdx.x; t < total; t += kGB) roots_out[base + t] = s_out[t];
     __syncthreads();
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && s_hooks) atomicAdd(hooks_count, (unsigned long long)s_hooks);
 }
 
-// ---------------------------------------------------------- pointer jumping
-// Every vertex moves to the representative at the end of its component's
-// hook chain; representatives of this round reset their minimum.
 // A hook chain visits each vertex at most once, so a walk longer than n
 // means the hooks form a cycle, which happens only when a graph passed as
 // symmetric is not: the walk stops and the call reports the error.
 __global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ comp,
                                               int32_t* hook,
                                               unsigned long long* __restrict__ best,
+                                              const RoundCtl* __restrict__ ctl,
                                               unsigned int* __restrict__ overflow) {
+  if (!ctl->active) return;
   for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * kGB) {
     int32_t x = comp[v];
@@ -482,6 +515,52 @@ __global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ c
     }
     if (x == (int32_t)v) best[v] = kInf;
   }
+}
+
+// Close the round: the next round filters the list the last k_round wrote
+// (its count = cross edges left) and hooks the representatives that stayed;
+// once the light edges are exhausted the heavy list is filtered once against
+// the light components with every vertex offered to hooking; then stop.
+__global__ void k_control(RoundCtl* ctl, unsigned long long* counts,
+                          unsigned long long* rcounts, const unsigned long long* heavy_count,
+                          int32_t n, RoundLog* log, int max_rounds) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  RoundCtl c = *ctl;
+  if (!c.active) return;
+  const long long m_out = (long long)counts[c.cur_list];
+  const long long stay = (long long)rcounts[c.rdst];
+  if (c.rounds < max_rounds) log[c.rounds] = RoundLog{c.m_in, m_out, c.nroots};
+  c.rounds += 1;
+  if (stay == 0 || m_out == 0) {
+    if (!c.heavy_done && *heavy_count > 0) {
+      c.heavy_done = 1;
+      c.src = 2;
+      c.m_in = (long long)*heavy_count;
+      c.dst = 0;
+      c.rsrc = -1;
+      c.nroots = n;
+      c.rdst = 0;
+    } else {
+      c.active = 0;
+    }
+  } else {
+    c.src = c.cur_list;
+    c.m_in = m_out;
+    c.dst = 1 - c.cur_list;
+    c.rsrc = c.rdst;
+    c.nroots = stay;
+    c.rdst = 1 - c.rdst;
+  }
+  if (c.rounds >= max_rounds) {
+    c.active = 0;
+    c.overflow_rounds = 1;
+  }
+  if (c.active) {
+    counts[c.dst] = 0;
+    rcounts[c.rdst] = 0;
+    c.cur_list = c.dst;
+  }
+  *ctl = c;
 }
 
 // --------------------------------------------------------- forest output
@@ -728,6 +807,8 @@ struct MstState {
   unsigned long long hooks;
   unsigned int overflow;
   unsigned int missing;
+  RoundCtl ctl;
+  RoundLog log[64];
   unsigned int hist[4096];  // last: per-round reads stop before it
 };
 constexpr size_t kStateHead = offsetof(MstState, hist);
@@ -816,6 +897,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   MstState init_st;
   memset(&init_st, 0, sizeof(init_st));
   init_st.range[0] = kInf;
+  init_st.ctl.rsrc = -1;  // first hooking: every vertex
+  init_st.ctl.nroots = n;
+  init_st.ctl.active = 1;
   {
     // state init through a pinned copy (stream ordered)
     void* pin = pinned_scratch(sizeof(MstState));
@@ -886,27 +970,38 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   prof_end(pr, s);
   int pending = pr;  // record whose output-list bytes are still unknown
   int64_t pending_bytes = round0_bytes;
-  int cur = 0;
-  int64_t m_cur = 0, nroots = n;
-  const int32_t* roots_in = nullptr;  // round 0: every vertex
-  int rounds = 0;
-  bool heavy_done = false;
+  (void)pending;
+  (void)pending_bytes;
+  // Rounds: hook -> jump -> control -> filter, enqueued in batches; the
+  // control kernel stops the sequence on the device.
+  const int gh = grid_for(n, kGB, persistent_blocks());
+  const int gr = grid_for((B.list_cap + kGTile - 1) / kGTile, 1, persistent_blocks());
+  constexpr int kMaxRounds = 64;
+  std::vector<int> rec_hook, rec_jump, rec_round;
+  int enq = 0, batch = 6;
   for (;;) {
-    ++rounds;
-    pr = prof_begin(EAC_PROF_MST_HOOK, 12 * nroots, s);
-    k_hook<<<grid_for(nroots, kGB, persistent_blocks()), kGB, 0, s>>>(
-        roots_in, nroots, B.best, B.comp, B.hook, B.mst_key, B.roots[cur],
-        &B.st->root_count[cur], &B.st->hooks, vb);
-    EAC_LAUNCH_CHECK();
-    prof_end(pr, s);
-    pr = prof_begin(EAC_PROF_MST_JUMP, 8 * (int64_t)n + 8 * nroots, s);
-    k_jump<<<grid_for(n, kGB, persistent_blocks()), kGB, 0, s>>>(n, B.comp, B.hook, B.best,
-                                                                    &B.st->overflow);
-    EAC_LAUNCH_CHECK();
-    prof_end(pr, s);
+    for (int b = 0; b < batch; ++b) {
+      rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
+      k_hook<<<gh, kGB, 0, s>>>(B.roots[0], B.roots[1], B.roots[0], B.roots[1],
+                                B.st->root_count, &B.st->ctl, B.best, B.comp, B.hook, B.mst_key,
+                                vb);
+      EAC_LAUNCH_CHECK();
+      prof_end(rec_hook.back(), s);
+      rec_jump.push_back(prof_begin(EAC_PROF_MST_JUMP, 0, s));
+      k_jump<<<gh, kGB, 0, s>>>(n, B.comp, B.hook, B.best, &B.st->ctl, &B.st->overflow);
+      EAC_LAUNCH_CHECK();
+      prof_end(rec_jump.back(), s);
+      k_control<<<1, 64, 0, s>>>(&B.st->ctl, B.st->list_count, B.st->root_count,
+                                 &B.st->heavy_count, n, B.st->log, kMaxRounds);
+      EAC_LAUNCH_CHECK();
+      rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
+      k_round<<<gr, kGB, 0, s>>>(B.list[0], B.list[1], B.heavy, B.list[0], B.list[1],
+                                 B.st->list_count, &B.st->ctl, B.comp, B.best, vb);
+      EAC_LAUNCH_CHECK();
+      prof_end(rec_round.back(), s);
+    }
+    enq += batch;
     EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
-    prof_set_bytes(pending, pending_bytes + 8 * (int64_t)hs.list_count[cur] +
-                                (rounds == 1 ? 8 * (int64_t)hs.heavy_count : 0));
     if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
     if (hs.overflow & 4u)
       return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
@@ -914,37 +1009,23 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     if (hs.overflow)
       return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
                                "pass symmetric=0");
-    m_cur = (int64_t)hs.list_count[cur];
-    nroots = (int64_t)hs.root_count[cur];
-    if (rounds > 64) break;
-    const uint64_t* src = B.list[cur];
-    if (nroots == 0 || m_cur == 0) {
-      if (heavy_done || hs.heavy_count == 0) break;
-      // light forest complete: filter the heavy edges against its components;
-      // every vertex is offered to the next hooking pass (non-representatives
-      // carry no minimum and only mark themselves).
-      heavy_done = true;
-      src = B.heavy;
-      m_cur = (int64_t)hs.heavy_count;
-      roots_in = nullptr;
-      nroots = n;
-    } else {
-      roots_in = B.roots[cur];
-    }
-    // next round: filter src -> list[nxt], recompute minima
-    int nxt = cur ^ 1;
-    // reset the counters that the next round appends to
-    EAC_TRY(fill<uint64_t>((uint64_t*)&B.st->list_count[nxt], 1, 0ull, s));
-    EAC_TRY(fill<uint64_t>((uint64_t*)&B.st->root_count[nxt], 1, 0ull, s));
-    pending_bytes = 8 * m_cur + 4 * (int64_t)n;
-    pending = prof_begin(EAC_PROF_MST_ROUND, pending_bytes, s);
-    k_round<<<grid_for((m_cur + kGTile - 1) / kGTile, 1, persistent_blocks()), kGB, 0, s>>>(
-        src, m_cur, B.list[nxt], &B.st->list_count[nxt], B.comp, B.best, vb);
-    EAC_LAUNCH_CHECK();
-    prof_end(pending, s);
-    cur = nxt;
+    if (hs.ctl.overflow_rounds)
+      return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
+    if (!hs.ctl.active) break;
+    batch = 4;
   }
-  if (rounds > 64) return fail(EAC_ERR_HIP, "boruvka: no convergence after 64 rounds");
+  const int rounds = hs.ctl.rounds;
+  if (prof_on()) {
+    const int nl = rounds < kMaxRounds ? rounds : kMaxRounds;
+    prof_set_bytes(pr, round0_bytes + 8 * (int64_t)hs.log[0].m_out + 8 * (int64_t)hs.heavy_count);
+    for (int r = 0; r < (int)rec_hook.size(); ++r) {
+      const RoundLog* lh = r < nl ? &hs.log[r] : nullptr;
+      const RoundLog* lr = r + 1 < nl ? &hs.log[r + 1] : nullptr;
+      prof_set_bytes(rec_hook[r], lh ? 12 * lh->nroots : 0);
+      prof_set_bytes(rec_jump[r], lh ? 8 * (int64_t)n + 8 * lh->nroots : 0);
+      prof_set_bytes(rec_round[r], lr ? 8 * lr->m_in + 8 * lr->m_out + 4 * (int64_t)n : 0);
+    }
+  }
 
   // forest keys in vertex order
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
