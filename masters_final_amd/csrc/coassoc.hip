@@ -18,8 +18,8 @@
 //    are then the suffix of its cluster's list after its own position: no
 //    search and no pair materialisation.
 //  * Count/fill: one workgroup per row (rows handed out by an atomic
-//    counter). Columns are covered in windows of 16384; each window has a
-//    dense u32 counter array in LDS (64 KiB). Per window every partition's
+//    counter). Columns are covered in windows of 32768; each window has a
+//    dense array of u16 counters (two per u32 word, 64 KiB of LDS). Per window every partition's
 //    suffix cursor advances over the members inside the window (flattened
 //    over partitions so that lanes stream consecutive members), each member
 //    an LDS atomic increment. The window is then read in column order: pass 1
@@ -37,7 +37,7 @@ namespace eac {
 namespace {
 
 constexpr int kCB = 256;       // threads per row workgroup
-constexpr int kWin = 16384;    // columns per LDS window
+constexpr int kWin = 32768;    // columns per LDS window (two u16 counters per u32 word)
 constexpr int kMaxP = 1024;    // partitions staged in LDS
 
 // keys: global cluster id (unassigned points -> ktot, sorted last);
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
                                                       int32_t w_offset,
                                                       int32_t* __restrict__ out_col,
                                                       int32_t* __restrict__ out_val) {
-  __shared__ uint32_t s_cnt[kWin];
+  __shared__ uint32_t s_cnt[kWin / 2];
   __shared__ int32_t s_cur[kMaxP];   // next member position per partition
   __shared__ int32_t s_end[kMaxP];   // end of the cluster per partition
   __shared__ int32_t s_hi[kMaxP];    // end of this window's members
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
     uint32_t nnz_local = 0;
     for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += kWin) {
       const int64_t c1 = c0 + kWin < n ? c0 + kWin : n;
-      for (int v = t; v < kWin; v += kCB) s_cnt[v] = 0;
+      for (int v = t; v < kWin / 2; v += kCB) s_cnt[v] = 0;
       __syncthreads();
       // members of this window: [cur, hi) with member < c1 (lists ascending)
       for (int p = t; p < P; p += kCB) {
@@ -163,28 +163,37 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
       for (int32_t f = t; f < tot; f += kCB) {
         while (s_pre[p + 1] <= f) ++p;
         int32_t j = pl.members[s_cur[p] + (f - s_pre[p])];
-        if (j != (int32_t)i) atomicAdd(&s_cnt[j - c0], 1u);
+        if (j != (int32_t)i) {
+          const int jj = (int)(j - c0);
+          atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
+        }
       }
       __syncthreads();
       for (int p2 = t; p2 < P; p2 += kCB) s_cur[p2] = s_hi[p2];
-      // read the window: lane-strided (conflict-free LDS reads)
+      // read the window: lane-strided words (two columns per word)
       const int width = (int)(c1 - c0);
+      const int words = (width + 1) >> 1;
       if (PASS == 0) {
         uint32_t c = 0, mx = 0;
-        for (int v = t; v < width; v += kCB) {
+        for (int v = t; v < words; v += kCB) {
           uint32_t x = s_cnt[v];
-          c += x != 0;
-          mx = x > mx ? x : mx;
+          uint32_t lo = x & 0xffffu, hi = x >> 16;
+          c += (lo != 0) + (hi != 0);
+          mx = lo > mx ? lo : mx;
+          mx = hi > mx ? hi : mx;
         }
         nnz_local += c;
         if (mx) atomicMax(&s_max, mx);
       } else {
-        // wave w emits region [w*R, (w+1)*R) in column order
+        // wave w emits words [w*R, (w+1)*R) in column order
         const int w = t / kWave, l = lane_id();
-        const int R = ((width + 4 * kWave - 1) / (4 * kWave)) * kWave;
-        const int ra = w * R, rb = (w + 1) * R < width ? (w + 1) * R : width;
+        const int R = ((words + 4 * kWave - 1) / (4 * kWave)) * kWave;
+        const int ra = w * R, rb = (w + 1) * R < words ? (w + 1) * R : words;
         uint32_t c = 0;
-        for (int v = ra + l; v < rb; v += kWave) c += s_cnt[v] != 0;
+        for (int v = ra + l; v < rb; v += kWave) {
+          uint32_t x = s_cnt[v];
+          c += ((x & 0xffffu) != 0) + ((x >> 16) != 0);
+        }
         c = wave_sum(c);
         if (l == 0) s_tmp[w] = c;
         __syncthreads();
@@ -194,17 +203,23 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
           if (q < w) o += s_tmp[q];
           wtot += s_tmp[q];
         }
-        const uint64_t lt = lanemask_lt();
         for (int v0 = ra; v0 < rb; v0 += kWave) {
           int v = v0 + l;
           uint32_t x = v < rb ? s_cnt[v] : 0u;
-          uint64_t mask = __ballot(x != 0);
-          if (x) {
-            int64_t at = o + __popcll(mask & lt);
-            out_col[at] = (int32_t)(c0 + v);
-            out_val[at] = w_offset ? w_offset - (int32_t)x : (int32_t)x;
+          uint32_t lo = x & 0xffffu, hi = x >> 16;
+          uint32_t cnt = (lo != 0) + (hi != 0);
+          uint32_t incl = wave_incl_scan(cnt);
+          int64_t at = o + (incl - cnt);
+          if (lo) {
+            out_col[at] = (int32_t)(c0 + 2 * v);
+            out_val[at] = w_offset ? w_offset - (int32_t)lo : (int32_t)lo;
+            ++at;
           }
-          o += __popcll(mask);
+          if (hi) {
+            out_col[at] = (int32_t)(c0 + 2 * v + 1);
+            out_val[at] = w_offset ? w_offset - (int32_t)hi : (int32_t)hi;
+          }
+          o += __shfl(incl, kWave - 1, kWave);
         }
         written += wtot;
       }
