@@ -266,6 +266,66 @@ __global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weigh
     if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
+// Weight offset, key width and light/heavy threshold of one call, computed
+// on the device from the range and histogram passes (no host round trip, so
+// the whole call can also be captured in a graph).
+struct MstPlan {
+  unsigned long long wlo;        // weight word of the lightest entry
+  unsigned long long whi;        // weight word of the heaviest entry
+  unsigned long long light_lim;  // key weights below it are light; ~0 = no split
+  int wbits;
+  int err;                       // 1: weight range + ids do not fit 64 bits
+};
+
+// One wave: light threshold = first histogram bin whose cumulative sampled
+// count reaches `target` (about 1.5 light undirected edges per vertex).
+__global__ void k_plan(const unsigned long long* __restrict__ range,
+                       const unsigned int* __restrict__ hist, int vb, int try_split,
+                       double target, MstPlan* __restrict__ plan) {
+  const int l = lane_id();
+  uint64_t wlo = range[0], whi = range[1];
+  if (wlo == kInf) { wlo = 0; whi = 0; }
+  int wbits = 0;
+  while (wbits < 64 && ((whi - wlo) >> wbits) != 0) ++wbits;
+  uint64_t lim = kInf;
+  if (try_split) {
+    const int shift = wbits > 12 ? wbits - 12 : 0;
+    constexpr int per = kHistBins / kWave;
+    double part = 0;
+    for (int q = 0; q < per; ++q) part += hist[l * per + q];
+    double incl = part;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      double o = __shfl_up(incl, off, kWave);
+      if (l >= off) incl += o;
+    }
+    double excl = incl - part;
+    // the lane whose slice crosses the target finds the bin
+    int found = kHistBins;
+    if (excl < target && incl >= target) {
+      double cum = excl;
+      for (int q = 0; q < per; ++q) {
+        cum += hist[l * per + q];
+        if (cum >= target) { found = l * per + q; break; }
+      }
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      int o = __shfl_xor(found, off, kWave);
+      found = o < found ? o : found;
+    }
+    if (found < kHistBins) lim = (uint64_t)(found + 1) << shift;
+    if (lim != kInf && lim > whi - wlo) lim = kInf;
+  }
+  if (l == 0) {
+    plan->wlo = wlo;
+    plan->whi = whi;
+    plan->wbits = wbits;
+    plan->err = wbits + 2 * vb > 64;
+    plan->light_lim = lim;
+  }
+}
+
 // -------------------------------------------------------- Boruvka round 0
 // Streams the CSR: builds the undirected edge list and every vertex's
 // lightest edge key.
@@ -273,8 +333,8 @@ template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_round0(
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
     const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int32_t n, int64_t m,
-    const int32_t* __restrict__ chunk_row, int64_t nchunks, int vb, uint64_t word_lo,
-    int symmetric, uint64_t light_lim, unsigned long long* __restrict__ best,
+    const int32_t* __restrict__ chunk_row, int64_t nchunks, int vb,
+    const MstPlan* __restrict__ plan, int symmetric, unsigned long long* __restrict__ best,
     uint64_t* __restrict__ list, unsigned long long* __restrict__ list_count,
     uint64_t* __restrict__ heavy, unsigned long long* __restrict__ heavy_count,
     int64_t list_cap, unsigned int* __restrict__ overflow) {
@@ -283,6 +343,8 @@ __global__ __launch_bounds__(kGB) void k_round0(
   __shared__ uint64_t s_out[kGTile];
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
   __shared__ unsigned long long s_base;
+  if (plan->err) return;
+  const uint64_t word_lo = plan->wlo, light_lim = plan->light_lim;
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
     if (cx.staged)
@@ -807,6 +869,7 @@ struct MstState {
   unsigned long long hooks;
   unsigned int overflow;
   unsigned int missing;
+  MstPlan plan;
   RoundCtl ctl;
   RoundLog log[64];
   unsigned int hist[4096];  // last: per-round reads stop before it
@@ -879,35 +942,49 @@ int labels_from_keys(const uint64_t* keys, int64_t nk, int vb, int32_t n, int32_
 }
 
 struct ForestInfo {
-  int64_t n_f;      // forest edges (keys in B.fkeys, vertex order)
+  int64_t n_f;      // forest edges (keys in B.fkeys, vertex order); -1 in graph mode
   int vb;           // vertex-id bits of the key
   int wbits;        // weight-key bits
   uint64_t wlo;     // weight word of the lightest entry
   uint64_t whi;     // weight word of the heaviest entry
   int rounds;
+  const int32_t* d_nf;   // device count of forest edges
+  const MstPlan* plan;   // device plan (wlo, whi, wbits)
 };
 
+__global__ void k_state_init(MstState* st, int32_t n) {
+  // every field starts at zero except the ones set below
+  uint32_t* w = reinterpret_cast<uint32_t*>(st);
+  const size_t words = sizeof(MstState) / 4;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+       i += (size_t)gridDim.x * blockDim.x)
+    w[i] = 0u;
+  __syncthreads();
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->range[0] = kInf;
+    st->ctl.rsrc = -1;  // first hooking: every vertex
+    st->ctl.nroots = n;
+    st->ctl.active = 1;
+  }
+}
+
+inline int vbits_for(int32_t n) {
+  int b = bits_for((uint64_t)(n - 1));
+  return b < 1 ? 1 : b;
+}
+
 // Boruvka rounds up to the forest key list (shared by the forest, lifetime
-// cut and batched entry points).
+// cut and batched entry points). graph_mode: no host synchronisation at all
+// (a fixed number of device-controlled rounds, counts left on the device),
+// so the sequence can be captured in a hipGraph.
 template <typename W, bool HAS_OD>
 int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
              int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
-             hipStream_t s) {
+             hipStream_t s, bool graph_mode = false) {
   const int64_t nchunks = (m + kGTile - 1) / kGTile;
-  MstState init_st;
-  memset(&init_st, 0, sizeof(init_st));
-  init_st.range[0] = kInf;
-  init_st.ctl.rsrc = -1;  // first hooking: every vertex
-  init_st.ctl.nroots = n;
-  init_st.ctl.active = 1;
-  {
-    // state init through a pinned copy (stream ordered)
-    void* pin = pinned_scratch(sizeof(MstState));
-    if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
-    EAC_HIP_CHECK(hipStreamSynchronize(s));
-    memcpy(pin, &init_st, sizeof(init_st));
-    EAC_HIP_CHECK(hipMemcpyAsync(B.st, pin, sizeof(MstState), hipMemcpyHostToDevice, s));
-  }
+  const int vb = vbits_for(n);
+  k_state_init<<<1, 1024, 0, s>>>(B.st, n);
+  EAC_LAUNCH_CHECK();
   k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(fe, n, m, nchunks, B.chunk_row);
   EAC_LAUNCH_CHECK();
   const int gchunks = grid_for(nchunks, 1, persistent_blocks());
@@ -918,39 +995,19 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   // on the light prefix of the key order, whose forest is exactly the
   // forest's light part; heavy edges then need one filtering pass against
   // the light components before the remaining rounds. Only worth it on
-  // graphs with many edges per vertex.
+  // graphs with many edges per vertex. The threshold keeps about 1.5 light
+  // undirected edges per vertex: 3n sampled directed entries for a symmetric
+  // CSR, 1.5n for a one-entry-per-edge CSR.
   const bool try_split = m >= 6 * (int64_t)n && m >= (1 << 16);
   if (try_split) {
     k_weight_hist<W><<<grid_for(m / kHistStride + 1, kGB, 1024), kGB, 0, s>>>(
         weight, m, B.st->range, B.st->hist);
     EAC_LAUNCH_CHECK();
   }
-  MstState hs;
-  EAC_TRY(d2h_sync(&hs, B.st, sizeof(MstState), s));
-  uint64_t wlo = hs.range[0], whi = hs.range[1];
-  if (wlo == kInf) { wlo = 0; whi = 0; }  // no entries
-  const int vb = bits_for((uint64_t)(n - 1)) < 1 ? 1 : bits_for((uint64_t)(n - 1));
-  const int wbits = bits_for(whi - wlo);
-  if (wbits + 2 * vb > 64)
-    return fail(EAC_ERR_KEY_WIDTH,
-                "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)", wbits, vb);
-  uint64_t light_lim = kInf;
-  if (try_split) {
-    // about 1.5 light undirected edges per vertex: 3n sampled directed entries
-    // for a symmetric CSR, 1.5n for a one-entry-per-edge CSR
-    const double target = (symmetric ? 3.0 : 1.5) * n / kHistStride;
-    const int shift = wbits > 12 ? wbits - 12 : 0;
-    double cum = 0;
-    for (int b = 0; b < kHistBins; ++b) {
-      cum += hs.hist[b];
-      if (cum >= target) {
-        light_lim = (uint64_t)(b + 1) << shift;
-        break;
-      }
-    }
-    if (light_lim != kInf && light_lim > whi - wlo) light_lim = kInf;
-  }
-
+  const double target = (symmetric ? 3.0 : 1.5) * n / kHistStride;
+  k_plan<<<1, kWave, 0, s>>>(B.st->range, B.st->hist, vb, try_split ? 1 : 0, target,
+                             &B.st->plan);
+  EAC_LAUNCH_CHECK();
   EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
   k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);
@@ -962,23 +1019,23 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       8 * (int64_t)n;
   int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
-                                               vb, wlo, symmetric, light_lim, B.best, B.list[0],
+                                               vb, &B.st->plan, symmetric, B.best, B.list[0],
                                                &B.st->list_count[0], B.heavy,
                                                &B.st->heavy_count, B.list_cap,
                                                &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
-  int pending = pr;  // record whose output-list bytes are still unknown
-  int64_t pending_bytes = round0_bytes;
-  (void)pending;
-  (void)pending_bytes;
   // Rounds: hook -> jump -> control -> filter, enqueued in batches; the
-  // control kernel stops the sequence on the device.
+  // control kernel stops the sequence on the device. Each round at least
+  // halves the active components of a phase, so two phases need at most
+  // 2 * (bits(n) + 2) rounds.
   const int gh = grid_for(n, kGB, persistent_blocks());
   const int gr = grid_for((B.list_cap + kGTile - 1) / kGTile, 1, persistent_blocks());
   constexpr int kMaxRounds = 64;
+  const int fixed_rounds = 2 * (vb + 2) < kMaxRounds ? 2 * (vb + 2) : kMaxRounds;
   std::vector<int> rec_hook, rec_jump, rec_round;
-  int enq = 0, batch = 6;
+  int enq = 0, batch = graph_mode ? fixed_rounds : 6;
+  MstState hs;
   for (;;) {
     for (int b = 0; b < batch; ++b) {
       rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
@@ -1001,7 +1058,12 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       prof_end(rec_round.back(), s);
     }
     enq += batch;
+    if (graph_mode) break;
     EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
+    if (hs.plan.err)
+      return fail(EAC_ERR_KEY_WIDTH,
+                  "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
+                  hs.plan.wbits, vb);
     if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
     if (hs.overflow & 4u)
       return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
@@ -1014,8 +1076,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     if (!hs.ctl.active) break;
     batch = 4;
   }
-  const int rounds = hs.ctl.rounds;
-  if (prof_on()) {
+  if (!graph_mode && prof_on()) {
+    const int rounds = hs.ctl.rounds;
     const int nl = rounds < kMaxRounds ? rounds : kMaxRounds;
     prof_set_bytes(pr, round0_bytes + 8 * (int64_t)hs.log[0].m_out + 8 * (int64_t)hs.heavy_count);
     for (int r = 0; r < (int)rec_hook.size(); ++r) {
@@ -1027,22 +1089,33 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     }
   }
 
-  // forest keys in vertex order
+  // forest keys in vertex order (slots past the count stay ~0)
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
   int32_t* d_nf = B.tile_cnt + ftiles;
+  EAC_TRY(fill<uint64_t>(B.fkeys, n, kInf, s));
   k_count_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt);
   EAC_LAUNCH_CHECK();
   EAC_TRY((scan_excl<int32_t, int32_t>(B.tile_cnt, B.tile_cnt, ftiles, 0, d_nf, ar, s)));
   k_write_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, B.fkeys);
   EAC_LAUNCH_CHECK();
+  fi->vb = vb;
+  fi->d_nf = d_nf;
+  fi->plan = &B.st->plan;
+  if (graph_mode) {
+    fi->n_f = -1;
+    fi->wbits = 64 - 2 * vb;  // upper bound: sort passes must be host-known
+    fi->wlo = 0;
+    fi->whi = 0;
+    fi->rounds = fixed_rounds;
+    return EAC_OK;
+  }
   int32_t n_f = 0;
   EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
   fi->n_f = n_f;
-  fi->vb = vb;
-  fi->wbits = wbits;
-  fi->wlo = wlo;
-  fi->whi = whi;
-  fi->rounds = rounds;
+  fi->wbits = hs.plan.wbits;
+  fi->wlo = hs.plan.wlo;
+  fi->whi = hs.plan.whi;
+  fi->rounds = hs.ctl.rounds;
   return EAC_OK;
 }
 
