@@ -180,6 +180,35 @@ int eac_sl_lifetime_batched(const int32_t* dest, const void* weight, int32_t wei
                             int64_t* gout, int64_t* h_out, void* ws, size_t ws_bytes,
                             void* stream);
 
+/* ---- the batched cut captured once into a hipGraph ----------------------
+ * No reference counterpart as an API: the reference runs its GPU MST one
+ * graph per host loop iteration (eac.py:199-267 via boruvka_gpu,
+ * graph/mst.py:24-176); this is the same batched cut with every host
+ * decision moved onto the device, recorded once and replayed per batch.
+ * The graph is bound to the buffers given here (dest, weight, first_edge,
+ * out_degree, goff, labels, gout, summary, ws): refill them in place (same
+ * N, M, G) and call _launch for each batch. goff is a DEVICE array of G+1
+ * offsets. summary (device int64[5]) receives: total components, forest
+ * edges, Boruvka rounds, kept edges, error flags (1 edge list overflow,
+ * 2 dest ids outside [0, N), 4 graph passed as symmetric is not, 8 key
+ * wider than 64 bits, 16 rounds not converged within max_rounds, 32 bad
+ * goff); results are valid only when the flags are 0. max_rounds <= 0
+ * takes the bound 2 * (bits(N-1) + 2) that always suffices; a smaller value
+ * shortens replays and sets flag 16 when a batch needs more. Launches on
+ * one graph must not overlap (they share ws). */
+typedef struct eac_sl_graph eac_sl_graph;
+int eac_sl_lifetime_batched_graph_create(const int32_t* dest, const void* weight,
+                                         int32_t weight_dtype, const int64_t* first_edge,
+                                         const int32_t* out_degree, int32_t N, int64_t M,
+                                         int32_t symmetric, const int64_t* goff, int32_t G,
+                                         int32_t rule, double max_dist, int32_t n_clusters,
+                                         int32_t max_rounds, int32_t* labels, int64_t* gout,
+                                         int64_t* summary, void* ws, size_t ws_bytes,
+                                         eac_sl_graph** out);
+int eac_sl_lifetime_batched_graph_launch(eac_sl_graph* graph, void* stream);
+int eac_sl_lifetime_batched_graph_rounds(const eac_sl_graph* graph);
+int eac_sl_lifetime_batched_graph_destroy(eac_sl_graph* graph);
+
 /* ---- stable LSD radix sort of 64-bit keys (optional 32-bit values) ------
  * Sorts by bits [begin_bit, end_bit). keys_out/vals_out must not alias the
  * inputs. No reference counterpart as an API: it replaces the numpy argsorts

@@ -71,6 +71,7 @@ struct ProfRec {
   int64_t bytes;
 };
 bool prof_on();
+void prof_suspend(bool on);  // no records while on (hipGraph capture)
 int prof_begin(int slot, int64_t bytes, hipStream_t s);  // returns record id or -1
 void prof_end(int rec, hipStream_t s);
 void prof_set_bytes(int rec, int64_t bytes);

@@ -743,8 +743,15 @@ __global__ void k_iota(int32_t* p, int32_t n) {
     p[i] = (int32_t)i;
 }
 
+// Unites the endpoints of keys[0 : nk], or of keys[0 : min(nk, *d_cnt)]
+// when the count lives on the device.
 __global__ __launch_bounds__(kGB) void k_unite_keys(const uint64_t* __restrict__ keys,
-                                                    int64_t nk, int vb, int32_t* P) {
+                                                    int64_t nk, const int32_t* __restrict__ d_cnt,
+                                                    int vb, int32_t* P) {
+  if (d_cnt) {
+    const int64_t c = *d_cnt;
+    nk = c < nk ? c : nk;
+  }
   for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < nk;
        i += (int64_t)gridDim.x * kGB) {
     int32_t a, b;
@@ -924,11 +931,13 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
 // union-find labels over a key list, numbered by first occurrence
 int labels_from_keys(const uint64_t* keys, int64_t nk, int vb, int32_t n, int32_t* parent,
                      int32_t* root, int32_t* flag, int32_t* rank, int32_t* labels,
-                     int32_t* d_ncomp, Arena& ar, hipStream_t s) {
+                     int32_t* d_ncomp, Arena& ar, hipStream_t s,
+                     const int32_t* d_nk = nullptr) {
   k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n);
   EAC_LAUNCH_CHECK();
   if (nk > 0) {
-    k_unite_keys<<<grid_for(nk, kGB, persistent_blocks()), kGB, 0, s>>>(keys, nk, vb, parent);
+    k_unite_keys<<<grid_for(nk, kGB, persistent_blocks()), kGB, 0, s>>>(keys, nk, d_nk, vb,
+                                                                        parent);
     EAC_LAUNCH_CHECK();
   }
   k_uf_flatten<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n, root, flag);
@@ -973,14 +982,24 @@ inline int vbits_for(int32_t n) {
   return b < 1 ? 1 : b;
 }
 
+// Rounds that always suffice: every round at least halves the components
+// still active in a phase (light edges, then heavy), so each phase ends
+// within bits(n) + 2 rounds.
+inline int mst_round_bound(int32_t n) {
+  const int r = 2 * (vbits_for(n) + 2);
+  return r < 64 ? r : 64;
+}
+
 // Boruvka rounds up to the forest key list (shared by the forest, lifetime
-// cut and batched entry points). graph_mode: no host synchronisation at all
-// (a fixed number of device-controlled rounds, counts left on the device),
-// so the sequence can be captured in a hipGraph.
+// cut and batched entry points). graph_rounds > 0 selects graph mode: no
+// host synchronisation at all (that many device-controlled rounds are
+// enqueued, rounds after convergence exit at once, counts stay on the
+// device), so the sequence can be captured in a hipGraph.
 template <typename W, bool HAS_OD>
 int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
              int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
-             hipStream_t s, bool graph_mode = false) {
+             hipStream_t s, int graph_rounds = 0) {
+  const bool graph_mode = graph_rounds > 0;
   const int64_t nchunks = (m + kGTile - 1) / kGTile;
   const int vb = vbits_for(n);
   k_state_init<<<1, 1024, 0, s>>>(B.st, n);
@@ -1032,7 +1051,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   const int gh = grid_for(n, kGB, persistent_blocks());
   const int gr = grid_for((B.list_cap + kGTile - 1) / kGTile, 1, persistent_blocks());
   constexpr int kMaxRounds = 64;
-  const int fixed_rounds = 2 * (vb + 2) < kMaxRounds ? 2 * (vb + 2) : kMaxRounds;
+  const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
   std::vector<int> rec_hook, rec_jump, rec_round;
   int enq = 0, batch = graph_mode ? fixed_rounds : 6;
   MstState hs;
@@ -1260,6 +1279,18 @@ __global__ __launch_bounds__(kGB) void k_mark_nonzero(const uint64_t* __restrict
        i += (int64_t)gridDim.x * kGB)
     flag[i] = weight_value<W>((sk[i] >> (2 * vb)) + wlo) != 0.0;
 }
+// Batched form of the same rule: zero-weight forest keys become kInf
+// (absent) in place, so no count leaves the device.
+template <typename W>
+__global__ __launch_bounds__(kGB) void k_drop_zero(uint64_t* __restrict__ keys, int64_t n, int vb,
+                                                   const MstPlan* __restrict__ plan) {
+  const uint64_t wlo = plan->wlo;
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kGB) {
+    const uint64_t k = keys[i];
+    if (k != kInf && weight_value<W>((k >> (2 * vb)) + wlo) == 0.0) keys[i] = kInf;
+  }
+}
 __global__ void k_compact_flagged(const uint64_t* __restrict__ in, const int32_t* __restrict__ flag,
                                   const int32_t* __restrict__ pos, int64_t n,
                                   uint64_t* __restrict__ out) {
@@ -1346,14 +1377,20 @@ __global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_
 }
 
 // ------------------------------------------------- batched single-link cut
-// Graph of a forest key (by its smaller endpoint) for disjoint-union batches.
+// Graph of a forest key (by its smaller endpoint) for disjoint-union batches;
+// absent keys (kInf) get the sentinel id G and are not counted.
 __global__ void k_graph_of_key(const uint64_t* __restrict__ sk, int64_t nf, int vb,
                                const int64_t* __restrict__ goff, int G,
                                uint32_t* __restrict__ gid, int32_t* __restrict__ gcount) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
        i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = sk[i];
+    if (key == kInf) {  // dropped key (zero weight) or padding: sentinel graph G
+      gid[i] = (uint32_t)G;
+      continue;
+    }
     int32_t a, b;
-    unpack_ends(sk[i], vb, &a, &b);
+    unpack_ends(key, vb, &a, &b);
     int lo = 0, hi = G;
     while (lo + 1 < hi) {
       int mid = (lo + hi) >> 1;
@@ -1376,11 +1413,19 @@ __global__ __launch_bounds__(kGB) void k_graph_wmax(const W* __restrict__ weight
   for (int g = blockIdx.x; g < G; g += gridDim.x) {
     unsigned long long mx = 0;
     const int64_t v0 = goff[g], v1 = goff[g + 1];
-    for (int64_t v = v0; v < v1; ++v) {
-      int64_t b = fe[v], e = HAS_OD ? b + od[v] : fe[v + 1];
-      for (int64_t q = b + threadIdx.x; q < e; q += kGB) {
+    if (!HAS_OD) {  // the graph's rows are one contiguous entry range
+      const int64_t e = v1 > v0 ? fe[v1] : 0;
+      for (int64_t q = (v1 > v0 ? fe[v0] : 0) + threadIdx.x; q < e; q += kGB) {
         unsigned long long x = word_of<W>(weight[q]);
         mx = x > mx ? x : mx;
+      }
+    } else {
+      for (int64_t v = v0; v < v1; ++v) {
+        int64_t b = fe[v], e = b + od[v];
+        for (int64_t q = b + threadIdx.x; q < e; q += kGB) {
+          unsigned long long x = word_of<W>(weight[q]);
+          mx = x > mx ? x : mx;
+        }
       }
     }
 #pragma unroll
@@ -1407,10 +1452,11 @@ template <typename W>
 __global__ __launch_bounds__(kGB) void k_seg_cut(const uint64_t* __restrict__ sk,
                                                  const int64_t* __restrict__ seg,
                                                  const int64_t* __restrict__ goff, int G, int vb,
-                                                 uint64_t wlo,
+                                                 const MstPlan* __restrict__ plan,
                                                  const unsigned long long* __restrict__ gmax,
                                                  int rule, double max_dist, int32_t n_clusters,
                                                  GraphCut* __restrict__ out) {
+  const uint64_t wlo = plan->wlo;
   __shared__ unsigned long long s_a[kGB / kWave], s_b[kGB / kWave];
   __shared__ unsigned long long s_mx, s_mn, s_arg;
   for (int g = blockIdx.x; g < G; g += gridDim.x) {
@@ -1461,13 +1507,13 @@ __global__ __launch_bounds__(kGB) void k_seg_cut(const uint64_t* __restrict__ sk
   }
 }
 
-__global__ void k_seg_keep_flags(const uint32_t* __restrict__ gid, int64_t nf,
+__global__ void k_seg_keep_flags(const uint32_t* __restrict__ gid, int64_t nf, int G,
                                  const int64_t* __restrict__ seg,
                                  const GraphCut* __restrict__ cut, int32_t* __restrict__ flag) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint32_t g = gid[i];
-    flag[i] = (i - seg[g]) < cut[g].keep;
+    flag[i] = g < (uint32_t)G && (i - seg[g]) < cut[g].keep;
   }
 }
 
@@ -1599,63 +1645,126 @@ int sl_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32
 // and one union-find labelling numbered by first occurrence inside each
 // graph. gout[g] = {components, labels_empty, nc_stable, kept edges or -1
 // where the reference raises ValueError}.
+// Graph offsets as the batched kernels read them: each clamped to [0, N]
+// (so no kernel indexes outside the union); bad[0] = 1 when the input was
+// not 0 = goff[0] <= ... <= goff[G] = N.
+__global__ void k_goff_prep(const int64_t* __restrict__ in, int G, int32_t N,
+                            int64_t* __restrict__ out, int32_t* __restrict__ bad) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
+    const int64_t v = in[g];
+    out[g] = v < 0 ? 0 : (v > N ? N : v);
+    const bool ok = (g > 0 || v == 0) && (g < G || v == N) && (g == 0 || in[g - 1] <= v);
+    if (!ok) *bad = 1;
+  }
+}
+
+// Run summary of a batched cut, written on the device: {components,
+// forest edges (nonzero weight under rule 0), Boruvka rounds, kept edges,
+// error flags}. Flags: 1 edge list overflow, 2 dest ids outside [0, N),
+// 4 graph passed as symmetric is not, 8 key wider than 64 bits,
+// 16 rounds not converged, 32 bad graph offsets.
+__global__ void k_batch_summary(const MstState* __restrict__ st,
+                                const int32_t* __restrict__ d_ncomp,
+                                const int64_t* __restrict__ seg_total,
+                                const int32_t* __restrict__ misc, int64_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  out[0] = *d_ncomp;
+  out[1] = *seg_total;
+  out[2] = st->ctl.rounds;
+  out[3] = misc[0];
+  int64_t flags = st->overflow & 7u;
+  if (misc[1]) flags |= 32;
+  if (st->plan.err) flags |= 8;
+  if (st->ctl.active || st->ctl.overflow_rounds) flags |= 16;
+  out[4] = flags;
+}
+
+// Error message of summary flags (0 when none).
+int summary_error(int64_t flags) {
+  if (flags & 32) return fail(EAC_ERR_ARG, "sl_batched: graph offsets not 0 <= ... <= N");
+  if (flags & 8) return fail(EAC_ERR_KEY_WIDTH, "boruvka: weight range and vertex ids need > 64 key bits");
+  if (flags & 2) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+  if (flags & 4)
+    return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric (pass symmetric=0)");
+  if (flags & 1)
+    return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
+                             "pass symmetric=0");
+  if (flags & 16) return fail(EAC_ERR_HIP, "boruvka: rounds did not converge");
+  return EAC_OK;
+}
+
+// Batched single-link over a disjoint union of G graphs (graph g owns the
+// vertex range [goff[g], goff[g+1]) and its rows): one Boruvka run over the
+// union (keys on global ids keep each graph's own (w, min, max) order, so
+// each graph's forest is its own canonical forest), then per-graph sorted
+// forests (global key sort + stable sort by graph), per-graph lifetime cut,
+// and one union-find labelling numbered by first occurrence inside each
+// graph. gout[g] = {components, labels_empty, nc_stable, kept edges or -1
+// where the reference raises ValueError}; summary (device int64[5]) as
+// k_batch_summary writes it.
+//
+// graph_mode: the sequence never synchronises with the host (fixed number
+// of Boruvka rounds, every count kept on the device, absent keys as kInf in
+// fixed-size buffers), so it can be captured into a hipGraph; d_goff must
+// then be a device array and h_out is not written.
 template <typename W, bool HAS_OD>
 int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-                    int32_t N, int64_t M, int symmetric, const int64_t* h_goff, int G, int rule,
-                    double max_dist, int32_t n_clusters, int32_t* labels, int64_t* gout,
-                    int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+                    int32_t N, int64_t M, int symmetric, const int64_t* h_goff,
+                    const int64_t* d_goff, int G, int rule, double max_dist, int32_t n_clusters,
+                    int32_t* labels, int64_t* gout, int64_t* summary, int64_t* h_out, void* ws,
+                    size_t ws_bytes, hipStream_t s, int graph_rounds = 0) {
+  const bool graph_mode = graph_rounds > 0;
   Arena ar(ws, ws_bytes);
   MstBuffers B = carve_mst(ar, N, M, symmetric);
   uint64_t* sk = ar.take<uint64_t>(N);
   uint64_t* sk2 = ar.take<uint64_t>(N);
   uint32_t* gid = ar.take<uint32_t>(N);
   uint32_t* gid_s = ar.take<uint32_t>(N);
-  int64_t* goff = ar.take<int64_t>(G + 1);
+  int64_t* goff_ws = ar.take<int64_t>(G + 1);
   int64_t* seg = ar.take<int64_t>(G + 2);
   int32_t* gcount = ar.take<int32_t>(G + 2);
   unsigned long long* gmax = ar.take<unsigned long long>(G);
   GraphCut* cut = ar.take<GraphCut>(G);
   int32_t* base = ar.take<int32_t>(G + 1);
-  CutState* cs = ar.take<CutState>(1);
+  int64_t* sum_ws = ar.take<int64_t>(8);
   int32_t* misc = ar.take<int32_t>(8);
   if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "sl_batched: workspace too small");
+  if (!summary) summary = sum_ws;
   const size_t mark0 = ar.used;
-  ForestInfo fi;
-  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, N, M, symmetric, B, ar, &fi, s)));
-  ar.used = mark0;
-  int64_t nf = fi.n_f;
-  const int vb = fi.vb;
-  EAC_TRY((radix_sort<uint64_t, NoValue>(B.fkeys, sk, nullptr, nullptr, nf, 0,
-                                         fi.wbits + 2 * vb, ar, s)));
-  ar.used = mark0;
-  const int g1 = grid_for(nf, kGB, persistent_blocks());
-  if (rule == 0 && nf > 0) {
-    // zero weights leave the forest (Kruskal's eliminate_zeros)
-    CutState init;
-    memset(&init, 0, sizeof(init));
-    init.min_lt_bits = ~0ull;
-    void* pin = pinned_scratch(sizeof(CutState));
-    if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
-    EAC_HIP_CHECK(hipStreamSynchronize(s));
-    memcpy(pin, &init, sizeof(init));
-    EAC_HIP_CHECK(hipMemcpyAsync(cs, pin, sizeof(CutState), hipMemcpyHostToDevice, s));
-    k_lifetime_stats<W><<<g1, kGB, 0, s>>>(sk, nf, vb, fi.wlo, cs);
+  EAC_TRY(fill<int32_t>(misc, 2, 0, s));
+  if (d_goff) {
+    k_goff_prep<<<grid_for(G + 1, 256), 256, 0, s>>>(d_goff, G, N, goff_ws, misc + 1);
     EAC_LAUNCH_CHECK();
-    CutState hc;
-    EAC_TRY(d2h_sync(&hc, cs, sizeof(CutState), s));
-    if (hc.zero_w > 0) {
-      k_mark_nonzero<W><<<g1, kGB, 0, s>>>(sk, nf, vb, fi.wlo, B.flag);
-      EAC_LAUNCH_CHECK();
-      EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, nf, 0, misc, ar, s)));
-      ar.used = mark0;
-      k_compact_flagged<<<g1, kGB, 0, s>>>(sk, B.flag, B.rank, nf, B.fkeys);
-      EAC_LAUNCH_CHECK();
-      nf -= (int64_t)hc.zero_w;
-      EAC_HIP_CHECK(hipMemcpyAsync(sk, B.fkeys, sizeof(uint64_t) * nf, hipMemcpyDeviceToDevice, s));
-    }
+  } else {
+    if (graph_mode) return fail(EAC_ERR_ARG, "sl_batched: graph mode needs device goff");
+    EAC_HIP_CHECK(hipMemcpyAsync(goff_ws, h_goff, sizeof(int64_t) * (G + 1),
+                                 hipMemcpyHostToDevice, s));
+    EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_goff may be pageable
   }
-  EAC_HIP_CHECK(hipMemcpyAsync(goff, h_goff, sizeof(int64_t) * (G + 1), hipMemcpyHostToDevice, s));
-  EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_goff may be pageable
+  const int64_t* goff = goff_ws;
+  ForestInfo fi;
+  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, N, M, symmetric, B, ar, &fi, s,
+                               graph_rounds)));
+  ar.used = mark0;
+  const int vb = fi.vb;
+  // graph mode sorts all N slots of the forest buffer (kInf past the count)
+  // over the widest key the weight type allows
+  int64_t nf = graph_mode ? N : fi.n_f;
+  int key_bits = fi.wbits + 2 * vb;
+  if (graph_mode) {
+    key_bits = 8 * (int)sizeof(W) + 2 * vb;
+    if (key_bits > 64) key_bits = 64;
+  }
+  const int g1 = grid_for(nf, kGB, persistent_blocks());
+  // zero weights leave the forest under rule 0 (Kruskal's eliminate_zeros);
+  // the absent keys sort after every present one (a present key never has
+  // all key bits set since its endpoints differ)
+  if (rule == 0 && nf > 0) {
+    k_drop_zero<W><<<g1, kGB, 0, s>>>(B.fkeys, nf, vb, fi.plan);
+    EAC_LAUNCH_CHECK();
+  }
+  EAC_TRY((radix_sort<uint64_t, NoValue>(B.fkeys, sk, nullptr, nullptr, nf, 0, key_bits, ar, s)));
+  ar.used = mark0;
   EAC_TRY(fill<int32_t>(gcount, G + 2, 0, s));
   if (nf > 0) {
     k_graph_of_key<<<g1, kGB, 0, s>>>(sk, nf, vb, goff, G, gid, gcount);
@@ -1664,41 +1773,43 @@ int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, con
     EAC_TRY((radix_sort<uint32_t, uint64_t>(gid, gid_s, sk, sk2, nf, 0, gb < 1 ? 1 : gb, ar, s)));
     ar.used = mark0;
   }
+  // seg[G] = present forest keys; they lead sk2, grouped by graph
   EAC_TRY((scan_excl<int32_t, int64_t>(gcount, seg, G + 1, 0, nullptr, ar, s)));
   ar.used = mark0;
   k_graph_wmax<W, HAS_OD><<<G < 4096 ? G : 4096, kGB, 0, s>>>(weight, fe, od, goff, G, gmax);
   EAC_LAUNCH_CHECK();
-  k_seg_cut<W><<<G < 4096 ? G : 4096, kGB, 0, s>>>(sk2, seg, goff, G, vb, fi.wlo, gmax, rule,
+  k_seg_cut<W><<<G < 4096 ? G : 4096, kGB, 0, s>>>(sk2, seg, goff, G, vb, fi.plan, gmax, rule,
                                                   max_dist, n_clusters, cut);
   EAC_LAUNCH_CHECK();
-  int64_t nk = 0;
+  EAC_TRY(fill<int32_t>(misc, 1, 0, s));
   if (nf > 0) {
-    k_seg_keep_flags<<<g1, kGB, 0, s>>>(gid_s, nf, seg, cut, B.flag);
+    k_seg_keep_flags<<<g1, kGB, 0, s>>>(gid_s, nf, G, seg, cut, B.flag);
     EAC_LAUNCH_CHECK();
     EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, nf, 0, misc, ar, s)));
     ar.used = mark0;
     k_compact_flagged<<<g1, kGB, 0, s>>>(sk2, B.flag, B.rank, nf, B.fkeys);
     EAC_LAUNCH_CHECK();
-    int32_t hk = 0;
-    EAC_TRY(d2h_sync(&hk, misc, sizeof(int32_t), s));
-    nk = hk;
   }
   const int64_t ftiles = (N + kGTile - 1) / kGTile;
   int32_t* d_ncomp = B.tile_cnt + ftiles;
-  EAC_TRY(labels_from_keys(B.fkeys, nk, vb, N, B.hook, B.comp, B.flag, B.rank, labels, d_ncomp,
-                           ar, s));
+  EAC_TRY(labels_from_keys(B.fkeys, nf, vb, N, B.hook, B.comp, B.flag, B.rank, labels, d_ncomp,
+                           ar, s, misc));
   k_graph_base<<<grid_for(G + 1, 256), 256, 0, s>>>(labels, goff, G, d_ncomp, base);
   EAC_LAUNCH_CHECK();
   k_graph_sub<<<G < 4096 ? G : 4096, 256, 0, s>>>(labels, goff, G, base, cut, gout);
   EAC_LAUNCH_CHECK();
-  int32_t tot = 0;
-  EAC_TRY(d2h_sync(&tot, d_ncomp, sizeof(int32_t), s));
+  k_batch_summary<<<1, 64, 0, s>>>(B.st, d_ncomp, seg + G, misc, summary);
+  EAC_LAUNCH_CHECK();
+  if (graph_mode) return EAC_OK;
+  int64_t hs[5];
+  EAC_TRY(d2h_sync(hs, summary, sizeof(hs), s));
   prof_flush();
+  EAC_TRY(summary_error(hs[4]));
   if (h_out) {
-    h_out[0] = tot;
-    h_out[1] = nf;
+    h_out[0] = hs[0];
+    h_out[1] = hs[1];
     h_out[2] = fi.rounds;
-    h_out[3] = nk;
+    h_out[3] = hs[3];
   }
   return EAC_OK;
 }
@@ -1738,9 +1849,21 @@ struct SlBatchRun {
                  int32_t N, int64_t M, int symmetric, const int64_t* h_goff, int G, int rule,
                  double max_dist, int32_t n_clusters, int32_t* labels, int64_t* gout,
                  int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
-    return sl_batched_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, N, M, symmetric, h_goff, G,
-                                      rule, max_dist, n_clusters, labels, gout, h_out, ws,
-                                      ws_bytes, s);
+    return sl_batched_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, N, M, symmetric, h_goff,
+                                      nullptr, G, rule, max_dist, n_clusters, labels, gout,
+                                      nullptr, h_out, ws, ws_bytes, s);
+  }
+};
+
+struct SlBatchGraphRun {
+  template <typename W, bool HAS_OD>
+  static int run(const int32_t* dest, const void* weight, const int64_t* fe, const int32_t* od,
+                 int32_t N, int64_t M, int symmetric, const int64_t* d_goff, int G, int rule,
+                 double max_dist, int32_t n_clusters, int32_t* labels, int64_t* gout,
+                 int64_t* summary, int rounds, void* ws, size_t ws_bytes, hipStream_t s) {
+    return sl_batched_impl<W, HAS_OD>(dest, (const W*)weight, fe, od, N, M, symmetric, nullptr,
+                                      d_goff, G, rule, max_dist, n_clusters, labels, gout,
+                                      summary, nullptr, ws, ws_bytes, s, rounds);
   }
 };
 
@@ -1979,4 +2102,91 @@ extern "C" int eac_sl_lifetime_batched(const int32_t* dest, const void* weight,
                                      out_degree, N, M, symmetric ? 1 : 0, h_goff, (int)G,
                                      (int)rule, max_dist, n_clusters, labels, gout, h_out, ws,
                                      ws_bytes, s);
+}
+
+// ------------------------------------------- batched cut as one hipGraph
+struct eac_sl_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int device = 0;
+  int rounds = 0;
+};
+
+extern "C" int eac_sl_lifetime_batched_graph_create(
+    const int32_t* dest, const void* weight, int32_t weight_dtype, const int64_t* first_edge,
+    const int32_t* out_degree, int32_t N, int64_t M, int32_t symmetric, const int64_t* goff,
+    int32_t G, int32_t rule, double max_dist, int32_t n_clusters, int32_t max_rounds,
+    int32_t* labels, int64_t* gout, int64_t* summary, void* ws, size_t ws_bytes,
+    eac_sl_graph** out) {
+  using namespace eac;
+  if (!out) return fail(EAC_ERR_ARG, "sl_graph: null output handle");
+  *out = nullptr;
+  if (N < 1 || M < 1 || G < 1) return fail(EAC_ERR_ARG, "sl_graph: need N, M, G >= 1");
+  if (rule != EAC_CUT_EAC && rule != EAC_CUT_SINGLELINK)
+    return fail(EAC_ERR_ARG, "sl_graph: unknown rule %d", rule);
+  if (!dest || !weight || !first_edge || !goff || !labels || !gout || !summary)
+    return fail(EAC_ERR_ARG, "sl_graph: null array");
+  if (weight_bytes(weight_dtype) == 0) return fail(EAC_ERR_ARG, "sl_graph: bad weight dtype");
+  size_t need = eac_sl_lifetime_batched_workspace_bytes(N, M, G);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "sl_graph: workspace %zu < %zu bytes", ws_bytes, need);
+  const int bound = mst_round_bound(N);
+  const int rounds = max_rounds <= 0 || max_rounds > bound ? bound : max_rounds;
+  eac_sl_graph* g = new eac_sl_graph();
+  g->rounds = rounds;
+  EAC_HIP_CHECK(hipGetDevice(&g->device));
+  // captured on a stream of the library's own: nothing runs now
+  hipStream_t cs = nullptr;
+  if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+    delete g;
+    return fail(EAC_ERR_HIP, "sl_graph: stream creation failed");
+  }
+  prof_suspend(true);
+  int rc = EAC_OK;
+  if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(EAC_ERR_HIP, "sl_graph: begin capture failed");
+  } else {
+    rc = dispatch_weight<SlBatchGraphRun>(weight_dtype, out_degree != nullptr, dest, weight,
+                                          first_edge, out_degree, N, M, symmetric ? 1 : 0, goff,
+                                          (int)G, (int)rule, max_dist, n_clusters, labels, gout,
+                                          summary, rounds, ws, ws_bytes, cs);
+    hipError_t e = hipStreamEndCapture(cs, &g->graph);
+    if (rc == EAC_OK && e != hipSuccess)
+      rc = fail(EAC_ERR_HIP, "sl_graph: end capture: %s", hipGetErrorString(e));
+  }
+  prof_suspend(false);
+  if (rc == EAC_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = fail(EAC_ERR_HIP, "sl_graph: instantiation failed");
+  (void)hipStreamDestroy(cs);
+  if (rc != EAC_OK) {
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return EAC_OK;
+}
+
+extern "C" int eac_sl_lifetime_batched_graph_launch(eac_sl_graph* g, void* stream) {
+  using namespace eac;
+  if (!g || !g->exec) return fail(EAC_ERR_ARG, "sl_graph: null graph");
+  int dev = -1;
+  EAC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev != g->device)
+    return fail(EAC_ERR_ARG, "sl_graph: captured on device %d, current is %d", g->device, dev);
+  EAC_HIP_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return EAC_OK;
+}
+
+extern "C" int eac_sl_lifetime_batched_graph_rounds(const eac_sl_graph* g) {
+  return g ? g->rounds : 0;
+}
+
+extern "C" int eac_sl_lifetime_batched_graph_destroy(eac_sl_graph* g) {
+  if (!g) return EAC_OK;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+  return EAC_OK;
 }

@@ -52,11 +52,14 @@ struct ProfSlot {
   int64_t bytes = 0;
 };
 static bool g_prof = false;
+static thread_local bool g_prof_suspended = false;  // set while a hipGraph is captured
 static ProfSlot g_slots[EAC_PROF_SLOTS];
 static std::vector<ProfRec> g_recs;
 static std::vector<hipEvent_t> g_pool;
 
-bool prof_on() { return g_prof; }
+bool prof_on() { return g_prof && !g_prof_suspended; }
+
+void prof_suspend(bool on) { g_prof_suspended = on; }
 
 static hipEvent_t ev_get() {
   if (!g_pool.empty()) {
@@ -70,7 +73,7 @@ static hipEvent_t ev_get() {
 }
 
 int prof_begin(int slot, int64_t bytes, hipStream_t s) {
-  if (!g_prof || slot < 0 || slot >= EAC_PROF_SLOTS) return -1;
+  if (!prof_on() || slot < 0 || slot >= EAC_PROF_SLOTS) return -1;
   ProfRec r{slot, ev_get(), ev_get(), bytes};
   if (!r.a || !r.b) return -1;
   (void)hipEventRecord(r.a, s);

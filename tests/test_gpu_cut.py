@@ -139,3 +139,49 @@ def test_batched_lifetime_cut_matches_per_graph_oracle(torch, rule, kind):
             continue
         assert gout[g, 0] == nc, g
         assert np.array_equal(labels[a:z], want), g
+
+
+@pytest.mark.parametrize("rule", [0, 1])
+def test_batched_graph_replays_match_eager(torch, rule):
+    """hipGraph replays of the batched cut: three batches of different sizes
+    through one recording, eager work and host reads between replays, every
+    replay equal to the eager batched call on the same batch."""
+    from masters_final_amd.graph.singlelink import LifetimeCutGraph, lifetime_cut_batched
+    batches = [synth.graph_batch(10, (40, 2500), (3, 10), seed=31 + rule, wmax=30, kind="knn"),
+               synth.graph_batch(6, (30, 900), (2, 8), seed=41 + rule, wmax=5, kind="er"),
+               synth.graph_batch(14, (20, 1500), (3, 12), seed=51 + rule, wmax=200, kind="knn")]
+    batches[1]["weight"][::7] = 0  # zero weights leave the forest under rule 0
+    n_cap = max(b["n"] for b in batches) + 5
+    m_cap = max(b["dest"].size for b in batches) + 11
+    g_cap = max(len(b["goff"]) - 1 for b in batches) + 2
+    cg = LifetimeCutGraph(n_cap, m_cap, g_cap, torch.int32, rule=rule)
+    for rep in range(2):
+        for i, b in enumerate(batches):
+            lab, gout, info = cg.run(b["dest"], b["weight"], b["indptr"], b["goff"])
+            lab_h, gout_h = lab.cpu().numpy().copy(), gout.cpu().numpy().copy()
+            want_lab, want_gout, want_info = lifetime_cut_batched(
+                b["dest"], b["weight"], b["indptr"], b["goff"], rule=rule)
+            assert np.array_equal(gout_h, want_gout.cpu().numpy()), (rep, i)
+            assert np.array_equal(lab_h, want_lab.cpu().numpy()), (rep, i)
+            assert info["kept_edges"] == want_info["kept_edges"]
+            assert info["forest_edges"] == want_info["forest_edges"]
+    cg.close()
+
+
+def test_batched_graph_tuned_rounds_fall_back(torch):
+    """A recording with too few rounds is detected on the device and the
+    batch is run again at the bound; tuning records the observed rounds."""
+    from masters_final_amd.graph.singlelink import LifetimeCutGraph, lifetime_cut_batched
+    b = synth.graph_batch(8, (200, 3000), (4, 10), seed=77, wmax=50, kind="er")
+    cg = LifetimeCutGraph(b["n"] + 1, b["dest"].size + 1, 10, torch.int32, rule=0, max_rounds=2)
+    assert cg.rounds == 2
+    lab, gout, info = cg.run(b["dest"], b["weight"], b["indptr"], b["goff"])
+    assert cg.rounds == cg.bound_rounds
+    want_lab, want_gout, _ = lifetime_cut_batched(b["dest"], b["weight"], b["indptr"], b["goff"])
+    assert np.array_equal(lab.cpu().numpy(), want_lab.cpu().numpy())
+    tuned = LifetimeCutGraph(b["n"] + 1, b["dest"].size + 1, 10, torch.int32, rule=0, tune=True)
+    lab, gout, info = tuned.run(b["dest"], b["weight"], b["indptr"], b["goff"])
+    assert tuned.rounds == min(info["rounds"] + tuned.margin, tuned.bound_rounds)
+    lab, gout, info = tuned.run(b["dest"], b["weight"], b["indptr"], b["goff"])
+    assert np.array_equal(lab.cpu().numpy(), want_lab.cpu().numpy())
+    assert np.array_equal(gout.cpu().numpy(), want_gout.cpu().numpy())
