@@ -150,13 +150,17 @@ def test_batched_graph_replays_match_eager(torch, rule):
     batches = [synth.graph_batch(10, (40, 2500), (3, 10), seed=31 + rule, wmax=30, kind="knn"),
                synth.graph_batch(6, (30, 900), (2, 8), seed=41 + rule, wmax=5, kind="er"),
                synth.graph_batch(14, (20, 1500), (3, 12), seed=51 + rule, wmax=200, kind="knn")]
-    batches[1]["weight"][::7] = 0  # zero weights leave the forest under rule 0
+    # zero weights (symmetric: chosen per undirected edge) leave the forest under rule 0
+    b1 = batches[1]
+    rows = np.repeat(np.arange(b1["n"]), np.diff(b1["indptr"]))
+    b1["weight"][(rows + b1["dest"]) % 7 == 0] = 0
     n_cap = max(b["n"] for b in batches) + 5
     m_cap = max(b["dest"].size for b in batches) + 11
     g_cap = max(len(b["goff"]) - 1 for b in batches) + 2
     cg = LifetimeCutGraph(n_cap, m_cap, g_cap, torch.int32, rule=rule)
     for rep in range(2):
         for i, b in enumerate(batches):
+            print(f"replay {rep} batch {i}: N={b['n']} M={b['dest'].size}", flush=True)
             lab, gout, info = cg.run(b["dest"], b["weight"], b["indptr"], b["goff"])
             lab_h, gout_h = lab.cpu().numpy().copy(), gout.cpu().numpy().copy()
             want_lab, want_gout, want_info = lifetime_cut_batched(
