@@ -406,16 +406,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eac", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
-                    help="only run a 256 MiB device copy 5 times (counter calibration)")
+                    help="only run a 256 MiB elementwise copy kernel 5 times (counter "
+                         "calibration: 256 MiB read + 256 MiB written per launch)")
     args = ap.parse_args()
     if args.copy_calib:
         import torch
-        x = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         for _ in range(5):
-            y.copy_(x)
+            torch.add(x, 1.0, out=y)  # one elementwise kernel per launch
         torch.cuda.synchronize()
-        print(json.dumps({"copy_calib_bytes": int(x.numel())}))
+        print(json.dumps({"copy_calib_bytes": int(x.numel()) * 4}))
         return
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "

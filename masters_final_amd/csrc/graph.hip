@@ -201,19 +201,30 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ uint64_t s_mm[2 * (kGB / kWave)];
   uint64_t lo = kInf, hi = 0;
-  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kGI; ++j) {
-      int64_t e = cx.e0 + j * kGB + threadIdx.x;
-      if (e < cx.e1 && row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end) >= 0) {
-        uint64_t w = word_of<W>(weight[e]);
-        lo = w < lo ? w : lo;
-        hi = w > hi ? w : hi;
-      }
+  if (!HAS_OD) {
+    // rows tile [fe[0], fe[n]): one flat pass over that range (within [0, m))
+    const int64_t b = fe[0] > 0 ? fe[0] : 0, e = fe[n] < m ? fe[n] : m;
+    for (int64_t q = b + (int64_t)blockIdx.x * kGB + threadIdx.x; q < e;
+         q += (int64_t)gridDim.x * kGB) {
+      uint64_t w = word_of<W>(weight[q]);
+      lo = w < lo ? w : lo;
+      hi = w > hi ? w : hi;
     }
-    __syncthreads();
+  } else {
+    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+      ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kGI; ++j) {
+        int64_t e = cx.e0 + j * kGB + threadIdx.x;
+        if (e < cx.e1 && row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end) >= 0) {
+          uint64_t w = word_of<W>(weight[e]);
+          lo = w < lo ? w : lo;
+          hi = w > hi ? w : hi;
+        }
+      }
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -549,44 +560,13 @@ __global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots0
   }
 }
 
-// A hook chain visits each vertex at most once, so a walk longer than n
-// means the hooks form a cycle, which happens only when a graph passed as
-// symmetric is not: the walk stops and the call reports the error.
-__global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ comp,
-                                              int32_t* hook,
-                                              unsigned long long* __restrict__ best,
-                                              const RoundCtl* __restrict__ ctl,
-                                              unsigned int* __restrict__ overflow) {
-  if (!ctl->active) return;
-  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kGB) {
-    int32_t x = comp[v];
-    int32_t y = x;
-    for (int32_t steps = 0;; ++steps) {
-      int32_t h = hook[y];
-      if (h == y) break;
-      if (steps > n) {
-        atomicOr(overflow, 4u);
-        break;
-      }
-      y = h;
-    }
-    if (y != x) {
-      comp[v] = y;
-      hook[x] = y;  // shortcut: any ancestor is a valid value
-    }
-    if (x == (int32_t)v) best[v] = kInf;
-  }
-}
-
 // Close the round: the next round filters the list the last k_round wrote
 // (its count = cross edges left) and hooks the representatives that stayed;
 // once the light edges are exhausted the heavy list is filtered once against
 // the light components with every vertex offered to hooking; then stop.
-__global__ void k_control(RoundCtl* ctl, unsigned long long* counts,
-                          unsigned long long* rcounts, const unsigned long long* heavy_count,
-                          int32_t n, RoundLog* log, int max_rounds) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void control_step(RoundCtl* ctl, unsigned long long* counts,
+                             unsigned long long* rcounts, const unsigned long long* heavy_count,
+                             int32_t n, RoundLog* log, int max_rounds) {
   RoundCtl c = *ctl;
   if (!c.active) return;
   const long long m_out = (long long)counts[c.cur_list];
@@ -624,6 +604,57 @@ __global__ void k_control(RoundCtl* ctl, unsigned long long* counts,
   }
   *ctl = c;
 }
+
+// A hook chain visits each vertex at most once, so a walk longer than n
+// means the hooks form a cycle, which happens only when a graph passed as
+// symmetric is not: the walk stops and the call reports the error.
+//
+// The last block to finish also closes the round (control_step): every
+// other block has read ctl and finished its vertices by then, and the
+// counters it reads were written by earlier kernels.
+__global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ comp,
+                                              int32_t* hook,
+                                              unsigned long long* __restrict__ best,
+                                              RoundCtl* ctl, unsigned int* __restrict__ overflow,
+                                              unsigned long long* counts,
+                                              unsigned long long* rcounts,
+                                              const unsigned long long* heavy_count,
+                                              RoundLog* log, int max_rounds,
+                                              unsigned int* done) {
+  __shared__ int s_last;
+  if (!ctl->active) return;
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
+    int32_t x = comp[v];
+    int32_t y = x;
+    for (int32_t steps = 0;; ++steps) {
+      int32_t h = hook[y];
+      if (h == y) break;
+      if (steps > n) {
+        atomicOr(overflow, 4u);
+        break;
+      }
+      y = h;
+    }
+    if (y != x) {
+      comp[v] = y;
+      hook[x] = y;  // shortcut: any ancestor is a valid value
+    }
+    if (x == (int32_t)v) best[v] = kInf;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    s_last = atomicAdd(done, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    *done = 0u;
+    control_step(ctl, counts, rcounts, heavy_count, n, log, max_rounds);
+  }
+}
+
 
 // --------------------------------------------------------- forest output
 // Tile-wise compaction of non-sentinel keys in index order (deterministic):
@@ -666,18 +697,49 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
     if (v[j] != kInf) out[o++] = v[j];
 }
 
+// Forest keys grouped by their smaller endpoint (counting sort on a >> kFpShift),
+// so that neighbouring lanes of the position search read neighbouring rows.
+constexpr int kFpShift = 4;
+__global__ __launch_bounds__(kGB) void k_fp_count(const uint64_t* __restrict__ fkeys,
+                                                  int64_t n_f, int vb,
+                                                  int32_t* __restrict__ bcnt) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
+       i += (int64_t)gridDim.x * kGB) {
+    int32_t a, b;
+    unpack_ends(fkeys[i], vb, &a, &b);
+    atomicAdd(&bcnt[a >> kFpShift], 1);
+  }
+}
+__global__ __launch_bounds__(kGB) void k_fp_scatter(const uint64_t* __restrict__ fkeys,
+                                                    int64_t n_f, int vb,
+                                                    int32_t* __restrict__ cursor,
+                                                    uint64_t* __restrict__ skeys,
+                                                    int32_t* __restrict__ sidx) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
+       i += (int64_t)gridDim.x * kGB) {
+    const uint64_t k = fkeys[i];
+    int32_t a, b;
+    unpack_ends(k, vb, &a, &b);
+    const int32_t slot = atomicAdd(&cursor[a >> kFpShift], 1);
+    skeys[slot] = k;
+    sidx[slot] = (int32_t)i;
+  }
+}
+
 // CSR position (in the smaller endpoint's row) and weight of each forest
-// edge. Short rows are scanned; long rows are binary-searched by column and
-// verified, falling back to a scan when the row is not sorted.
+// edge; results go to slot sidx[i] (or i). Short rows are scanned; long rows
+// are binary-searched by column and verified, falling back to a scan when
+// the row is not sorted.
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
-    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ dest,
-    const W* __restrict__ weight, const int64_t* __restrict__ fe,
-    const int32_t* __restrict__ od, int vb, uint64_t word_lo, int64_t* __restrict__ ids,
-    W* __restrict__ wout, unsigned int* __restrict__ missing) {
+    const uint64_t* __restrict__ fkeys, const int32_t* __restrict__ sidx, int64_t n_f,
+    const int32_t* __restrict__ dest, const W* __restrict__ weight,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, uint64_t word_lo,
+    int64_t* __restrict__ ids, W* __restrict__ wout, unsigned int* __restrict__ missing) {
   for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
        i += (int64_t)gridDim.x * kGB) {
     uint64_t k = fkeys[i];
+    const int64_t o = sidx ? sidx[i] : i;
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
     uint64_t wk = k >> (2 * vb);
@@ -701,8 +763,8 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
       atomicOr(missing, 1u);
       pos = 0;
     }
-    if (ids) ids[i] = pos;
-    if (wout) wout[i] = weight[pos];
+    if (ids) ids[o] = pos;
+    if (wout) wout[o] = weight[pos];
   }
 }
 
@@ -815,22 +877,52 @@ __global__ void k_gather_rank(const int32_t* __restrict__ root,
 // indices, so per distinct label in the wave only its lowest lane issues the
 // atomicMin: a giant cluster costs one atomic per wave instead of one per
 // vertex (same-address atomics from all XCDs serialise).
-__global__ void k_first_occ(const int32_t* __restrict__ lab, int32_t n, int32_t nv,
-                            int32_t* first, unsigned int* bad) {
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    int64_t i = base + threadIdx.x;
-    int32_t l = i < n ? lab[i] : -1;
-    bool ok = i < n && l >= 0 && l < nv;
-    if (i < n && !ok) atomicOr(bad, 1u);
-    uint64_t pending = __ballot(ok);
-    while (pending) {
-      int leader = __ffsll((unsigned long long)pending) - 1;
-      int32_t ll = __shfl(l, leader, kWave);
-      uint64_t same = __ballot(ok && l == ll) & pending;
-      if (lane_id() == leader && (int32_t)i < first[ll]) atomicMin(first + ll, (int32_t)i);
-      pending &= ~same;
+// first[l] = smallest i with lab[i] == l. Lanes of a wave with the same
+// label elect their lowest lane (the smallest index); leaders merge into a
+// per-block LDS table of (label, min index), flushed with one global
+// atomicMin per distinct label of the tile, so a label held by most
+// vertices costs one global atomic per tile instead of one per wave.
+constexpr int kFoTile = 2048, kFoSlots = 256;
+__global__ __launch_bounds__(256) void k_first_occ(const int32_t* __restrict__ lab, int32_t n,
+                                                   int32_t nv, int32_t* first,
+                                                   unsigned int* bad) {
+  __shared__ int32_t s_key[kFoSlots], s_min[kFoSlots];
+  for (int64_t t0 = (int64_t)blockIdx.x * kFoTile; t0 < n; t0 += (int64_t)gridDim.x * kFoTile) {
+    for (int i = threadIdx.x; i < kFoSlots; i += blockDim.x) {
+      s_key[i] = -1;
+      s_min[i] = INT32_MAX;
     }
+    __syncthreads();
+    for (int j = 0; j < kFoTile; j += blockDim.x) {
+      const int64_t i = t0 + j + threadIdx.x;
+      const int32_t l = i < n ? lab[i] : -1;
+      const bool ok = i < n && l >= 0 && l < nv;
+      if (i < n && !ok) atomicOr(bad, 1u);
+      uint64_t pending = __ballot(ok);
+      while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int32_t ll = __shfl(l, leader, kWave);
+        pending &= ~__ballot(ok && l == ll);
+        if (lane_id() == leader) {
+          uint32_t h = ((uint32_t)ll * 2654435761u) >> 24;  // 8 bits: kFoSlots
+          bool done = false;
+          for (int p = 0; p < 8 && !done; ++p, h = (h + 1) & (kFoSlots - 1)) {
+            const int32_t k = atomicCAS(&s_key[h], -1, ll);
+            if (k == -1 || k == ll) {
+              atomicMin(&s_min[h], (int32_t)i);
+              done = true;
+            }
+          }
+          if (!done) atomicMin(first + ll, (int32_t)i);
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < kFoSlots; q += blockDim.x) {
+      const int32_t k = s_key[q];
+      if (k >= 0) atomicMin(first + k, s_min[q]);
+    }
+    __syncthreads();
   }
 }
 __global__ void k_first_flag(const int32_t* __restrict__ lab, const int32_t* __restrict__ first,
@@ -876,6 +968,8 @@ struct MstState {
   unsigned long long hooks;
   unsigned int overflow;
   unsigned int missing;
+  unsigned int jump_done;  // blocks of the current k_jump that finished
+  unsigned int pad0;
   MstPlan plan;
   RoundCtl ctl;
   RoundLog log[64];
@@ -1044,8 +1138,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                                &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
-  // Rounds: hook -> jump -> control -> filter, enqueued in batches; the
-  // control kernel stops the sequence on the device. Each round at least
+  // Rounds: hook -> jump (whose last block closes the round) -> filter,
+  // enqueued in batches; the round control stops the sequence on the device. Each round at least
   // halves the active components of a phase, so two phases need at most
   // 2 * (bits(n) + 2) rounds.
   const int gh = grid_for(n, kGB, persistent_blocks());
@@ -1064,12 +1158,11 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       EAC_LAUNCH_CHECK();
       prof_end(rec_hook.back(), s);
       rec_jump.push_back(prof_begin(EAC_PROF_MST_JUMP, 0, s));
-      k_jump<<<gh, kGB, 0, s>>>(n, B.comp, B.hook, B.best, &B.st->ctl, &B.st->overflow);
+      k_jump<<<gh, kGB, 0, s>>>(n, B.comp, B.hook, B.best, &B.st->ctl, &B.st->overflow,
+                                B.st->list_count, B.st->root_count, &B.st->heavy_count,
+                                B.st->log, kMaxRounds, &B.st->jump_done);
       EAC_LAUNCH_CHECK();
       prof_end(rec_jump.back(), s);
-      k_control<<<1, 64, 0, s>>>(&B.st->ctl, B.st->list_count, B.st->root_count,
-                                 &B.st->heavy_count, n, B.st->log, kMaxRounds);
-      EAC_LAUNCH_CHECK();
       rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
       k_round<<<gr, kGB, 0, s>>>(B.list[0], B.list[1], B.heavy, B.list[0], B.list[1],
                                  B.st->list_count, &B.st->ctl, B.comp, B.best, vb);
@@ -1149,8 +1242,20 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
   const int64_t n_f = fi.n_f;
   if (n_f > 0 && (ids || wout)) {
-    k_forest_positions<W, HAS_OD><<<grid_for(n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
-        B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
+    // group the forest keys by row block, then look their positions up
+    const int64_t nb = ((int64_t)n >> kFpShift) + 1;
+    int32_t* bcnt = B.rank;        // nb <= n for n >= 2 (a forest edge needs 2 vertices)
+    int32_t* sidx = B.roots[1];
+    uint64_t* skeys = B.mst_key;   // free once the forest keys are compacted
+    const int gf = grid_for(n_f, kGB, persistent_blocks());
+    EAC_TRY(fill<int32_t>(bcnt, nb, 0, s));
+    k_fp_count<<<gf, kGB, 0, s>>>(B.fkeys, n_f, fi.vb, bcnt);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY((scan_excl<int32_t, int32_t>(bcnt, bcnt, nb, 0, nullptr, ar, s)));
+    k_fp_scatter<<<gf, kGB, 0, s>>>(B.fkeys, n_f, fi.vb, bcnt, skeys, sidx);
+    EAC_LAUNCH_CHECK();
+    k_forest_positions<W, HAS_OD><<<gf, kGB, 0, s>>>(skeys, sidx, n_f, dest, weight, fe, od,
+                                                     fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
   int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
@@ -1162,7 +1267,7 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     const int64_t ftiles = (n + kGTile - 1) / kGTile;
     int g = grid_for(n, 256, 4096);
     EAC_TRY(fill<int32_t>(first, n, INT32_MAX, s));
-    k_first_occ<<<g, 256, 0, s>>>(B.comp, n, n, first, &B.st->missing);
+    k_first_occ<<<grid_for(n, kFoTile, 4096), 256, 0, s>>>(B.comp, n, n, first, &B.st->missing);
     EAC_LAUNCH_CHECK();
     k_first_flag<<<g, 256, 0, s>>>(B.comp, first, n, n, root, B.flag);
     EAC_LAUNCH_CHECK();
@@ -2056,7 +2161,7 @@ extern "C" int eac_canonical_relabel(const int32_t* labels_in, int32_t n, int32_
   EAC_TRY(fill<int32_t>(first, n_values, INT32_MAX, s));
   EAC_TRY(fill<int32_t>((int32_t*)misc, 4, 0, s));
   int g = grid_for(n, 256, 4096);
-  k_first_occ<<<g, 256, 0, s>>>(labels_in, n, n_values, first, misc);
+  k_first_occ<<<grid_for(n, kFoTile, 4096), 256, 0, s>>>(labels_in, n, n_values, first, misc);
   EAC_LAUNCH_CHECK();
   k_first_flag<<<g, 256, 0, s>>>(labels_in, first, n, n_values, root, flag);
   EAC_LAUNCH_CHECK();
