@@ -175,21 +175,13 @@ __device__ __forceinline__ int32_t row_of(const ChunkCtx& cx, int64_t e, const i
   return r - cx.r_lo;
 }
 
-// Reserve `cnt` output slots per thread with one atomic per block. Returns
-// this thread's offset inside the block's run; *s_base holds the run start.
-__device__ __forceinline__ uint32_t block_reserve(uint32_t cnt, unsigned long long* counter,
-                                                  uint32_t* s_tmp,
-                                                  unsigned long long* s_base,
-                                                  uint32_t* total_out) {
-  uint32_t total;
-  uint32_t pre = block_excl_scan<uint32_t>(cnt, s_tmp, &total);
-  if (threadIdx.x == 0) *s_base = total ? atomicAdd(counter, (unsigned long long)total) : 0ull;
-  __syncthreads();
-  *total_out = total;
-  return pre;
-}
-
 // ----------------------------------------------------- weight range pass
+// Device-scope atomics on one address serialise (tens of ns each), so every
+// per-block result below lands in one of kSlots slots (blockIdx mod kSlots)
+// and consumers fold the slots.
+constexpr int kSlots = 64;
+static_assert(kSlots == kWave, "slot folds use one wave");
+
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weight,
                                                       const int64_t* __restrict__ fe,
@@ -197,7 +189,8 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
                                                       int32_t n, int64_t m,
                                                       const int32_t* __restrict__ chunk_row,
                                                       int64_t nchunks,
-                                                      unsigned long long* range) {
+                                                      unsigned long long* range_lo,
+                                                      unsigned long long* range_hi) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ uint64_t s_mm[2 * (kGB / kWave)];
   uint64_t lo = kInf, hi = 0;
@@ -241,25 +234,48 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
       hi = s_mm[2 * i + 1] > hi ? s_mm[2 * i + 1] : hi;
     }
     if (lo != kInf) {
-      atomicMin(&range[0], (unsigned long long)lo);
-      atomicMax(&range[1], (unsigned long long)hi);
+      const int slot = blockIdx.x & (kSlots - 1);
+      atomicMin(&range_lo[slot], (unsigned long long)lo);
+      atomicMax(&range_hi[slot], (unsigned long long)hi);
     }
   }
 }
 
-// Histogram of weight words over every 8th entry (4096 bins of the range
-// found by k_weight_range), used to pick the light/heavy split threshold.
-// Entries in row gaps may be sampled; that only moves the threshold.
+// Folds the range slots with one wave: (kInf, 0) when there is no entry.
+__device__ __forceinline__ void fold_range(const unsigned long long* range_lo,
+                                           const unsigned long long* range_hi, uint64_t* lo,
+                                           uint64_t* hi) {
+  uint64_t l = range_lo[lane_id()], h = range_hi[lane_id()];
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    uint64_t a = __shfl_xor(l, off, kWave), b = __shfl_xor(h, off, kWave);
+    l = a < l ? a : l;
+    h = b > h ? b : h;
+  }
+  *lo = l;
+  *hi = h;
+}
+
+// Histogram of weight words over every 8th entry (4096 bins of the weight
+// range), used to pick the light/heavy split threshold. Entries in row gaps
+// may be sampled; that only moves the threshold.
 constexpr int kHistBins = 4096;
 constexpr int kHistStride = 8;
 template <typename W>
 __global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weight, int64_t m,
-                                                     const unsigned long long* __restrict__ range,
+                                                     const unsigned long long* __restrict__ range_lo,
+                                                     const unsigned long long* __restrict__ range_hi,
                                                      unsigned int* __restrict__ hist) {
   __shared__ unsigned int h[kHistBins];
+  __shared__ uint64_t s_lo, s_hi;
   for (int i = threadIdx.x; i < kHistBins; i += kGB) h[i] = 0;
+  if (threadIdx.x < kWave) {
+    uint64_t l, hh;
+    fold_range(range_lo, range_hi, &l, &hh);
+    if (threadIdx.x == 0) { s_lo = l; s_hi = hh; }
+  }
   __syncthreads();
-  const uint64_t lo = range[0], hi = range[1];
+  const uint64_t lo = s_lo, hi = s_hi;
   if (lo <= hi) {
     int wb = 0;
     while (wb < 64 && ((hi - lo) >> wb) != 0) ++wb;
@@ -278,23 +294,21 @@ __global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weigh
 }
 
 // Weight offset, key width and light/heavy threshold of one call, computed
-// on the device from the range and histogram passes (no host round trip, so
-// the whole call can also be captured in a graph).
+// on the device so that no host round trip sits between the passes.
 struct MstPlan {
   unsigned long long wlo;        // weight word of the lightest entry
   unsigned long long whi;        // weight word of the heaviest entry
-  unsigned long long light_lim;  // key weights below it are light; ~0 = no split
-  int wbits;
-  int err;                       // 1: weight range + ids do not fit 64 bits
+  unsigned long long light_lim;  // keys with (word - wlo) >= light_lim are heavy
+  int wbits;                     // bits of whi - wlo
+  int err;                       // 1: wbits + 2 * vb > 64
 };
-
-// One wave: light threshold = first histogram bin whose cumulative sampled
-// count reaches `target` (about 1.5 light undirected edges per vertex).
-__global__ void k_plan(const unsigned long long* __restrict__ range,
+__global__ void k_plan(const unsigned long long* __restrict__ range_lo,
+                       const unsigned long long* __restrict__ range_hi,
                        const unsigned int* __restrict__ hist, int vb, int try_split,
                        double target, MstPlan* __restrict__ plan) {
   const int l = lane_id();
-  uint64_t wlo = range[0], whi = range[1];
+  uint64_t wlo, whi;
+  fold_range(range_lo, range_hi, &wlo, &whi);
   if (wlo == kInf) { wlo = 0; whi = 0; }
   int wbits = 0;
   while (wbits < 64 && ((whi - wlo) >> wbits) != 0) ++wbits;
@@ -338,31 +352,42 @@ __global__ void k_plan(const unsigned long long* __restrict__ range,
 }
 
 // -------------------------------------------------------- Boruvka round 0
-// Streams the CSR: builds the undirected edge list and every vertex's
-// lightest edge key.
+// Edge lists are chunk-local: chunk c of the CSR (kGTile entries) owns slots
+// [c * kGTile, c * kGTile + count[c]) of each list, so no pass ever needs a
+// shared output counter; later passes filter each chunk in place.
+//
+// Round 0 streams the CSR once: every entry becomes a key; light keys are
+// round-0 candidates (segmented min per row in LDS) and go to the light
+// list, heavy keys go to the heavy list. A symmetric CSR lists each edge
+// once (dest > row) and takes each row's minimum from the row itself.
+constexpr int kLogSlots = 16;
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_round0(
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
     const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int32_t n, int64_t m,
     const int32_t* __restrict__ chunk_row, int64_t nchunks, int vb,
     const MstPlan* __restrict__ plan, int symmetric, unsigned long long* __restrict__ best,
-    uint64_t* __restrict__ list, unsigned long long* __restrict__ list_count,
-    uint64_t* __restrict__ heavy, unsigned long long* __restrict__ heavy_count,
-    int64_t list_cap, unsigned int* __restrict__ overflow) {
+    uint64_t* __restrict__ light, uint64_t* __restrict__ heavy,
+    int32_t* __restrict__ cnt_light, int32_t* __restrict__ cnt_heavy,
+    unsigned long long* __restrict__ tot, unsigned int* __restrict__ overflow) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ unsigned long long s_best[kMaxRows];
   __shared__ uint64_t s_out[kGTile];
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
-  __shared__ unsigned long long s_base;
-  if (plan->err) return;
+  const bool err = plan->err != 0;
   const uint64_t word_lo = plan->wlo, light_lim = plan->light_lim;
+  unsigned long long sum_l = 0, sum_h = 0;
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    if (err) {  // key too wide: nothing is listed, the host reports the error
+      if (threadIdx.x == 0) { cnt_light[c] = 0; cnt_heavy[c] = 0; }
+      continue;
+    }
     ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
     if (cx.staged)
       for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) s_best[i] = kInf;
     __syncthreads();
     uint64_t keys[kGI];
-    uint32_t keep = 0, light = 0;
+    uint32_t keep = 0, light_m = 0;
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
       keys[j] = kInf;
@@ -382,7 +407,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
       keys[j] = key;
       if (!symmetric || v > u) keep |= 1u << j;
       if (wk >= light_lim) continue;  // heavy: listed, not a round-0 candidate
-      light |= 1u << j;
+      light_m |= 1u << j;
       if (cx.staged) {
         atomicMin(&s_best[ri], (unsigned long long)key);
       } else if (key < best[u]) {
@@ -392,25 +417,31 @@ __global__ __launch_bounds__(kGB) void k_round0(
         if (key < best[v]) atomicMin(&best[v], (unsigned long long)key);
       }
     }
-    // light edges -> list, heavy edges -> heavy (each one atomic per block)
+    // light keys to s_out[0, TL), heavy keys to s_out[TL, TL + TH): one scan
+    // of the two per-thread counts packed in 16-bit halves (each <= kGTile)
+    const uint32_t nl = __popc(keep & light_m), nh = __popc(keep & ~light_m);
+    uint32_t total;
+    const uint32_t pre = block_excl_scan<uint32_t>(nl | (nh << 16), s_tmp, &total);
+    const uint32_t TL = total & 0xffffu, TH = total >> 16;
+    uint32_t pl = pre & 0xffffu, ph = TL + (pre >> 16);
 #pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      uint32_t sel = pass == 0 ? (keep & light) : (keep & ~light);
-      uint32_t total;
-      uint32_t pre = block_reserve(__popc(sel), pass == 0 ? list_count : heavy_count, s_tmp,
-                                   &s_base, &total);
-#pragma unroll
-      for (int j = 0; j < kGI; ++j)
-        if (sel & (1u << j)) s_out[pre++] = keys[j];
-      __syncthreads();
-      unsigned long long base = s_base;
-      uint64_t* dst = pass == 0 ? list : heavy;
-      if (base + total > (unsigned long long)list_cap) {
-        if (threadIdx.x == 0) atomicOr(overflow, 1u);
-      } else {
-        for (uint32_t i = threadIdx.x; i < total; i += kGB) dst[base + i] = s_out[i];
-      }
-      __syncthreads();
+    for (int j = 0; j < kGI; ++j) {
+      if (!(keep & (1u << j))) continue;
+      if (light_m & (1u << j)) s_out[pl++] = keys[j];
+      else s_out[ph++] = keys[j];
+    }
+    __syncthreads();
+    uint64_t* lc = light + c * kGTile;
+    uint64_t* hc = heavy + c * kGTile;
+    for (uint32_t i = threadIdx.x; i < TL + TH; i += kGB) {
+      if (i < TL) lc[i] = s_out[i];
+      else hc[i - TL] = s_out[i];
+    }
+    if (threadIdx.x == 0) {
+      cnt_light[c] = (int32_t)TL;
+      cnt_heavy[c] = (int32_t)TH;
+      sum_l += TL;
+      sum_h += TH;
     }
     if (cx.staged) {
       for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) {
@@ -424,60 +455,74 @@ __global__ __launch_bounds__(kGB) void k_round0(
     }
     __syncthreads();
   }
+  if (threadIdx.x == 0 && (sum_l | sum_h)) {
+    const int slot = blockIdx.x & (kLogSlots - 1);
+    atomicAdd(&tot[slot], sum_l);
+    atomicAdd(&tot[kLogSlots + slot], sum_h);
+  }
 }
 
 // ------------------------------------------------------ device round control
-// Rounds are driven from the device so the host enqueues several rounds per
-// synchronisation: a one-thread control kernel reads the counters of the
-// round just finished and writes what the next kernels read (which list,
-// how many edges, which representatives), or clears `active`.
+// A round is hook(r) -> jump(r) -> filter(r). The control state of round r
+// lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
+// from it and from had_edge[r] (some representative had a candidate edge in
+// hook(r)), and block 0 stores it in ctl[(r + 1) & 1]. Phase 0 filters the
+// light list; when a hook finds no candidate the heavy list is filtered
+// (phase 1); when that happens in phase 1 the forest is complete.
+constexpr int kMaxRounds = 64;
 struct RoundCtl {
-  long long m_in;    // edges the next k_round reads
-  long long nroots;  // representatives the next k_hook reads
-  int src;           // list the next k_round reads: 0/1, or 2 = heavy list
-  int dst;           // list the next k_round writes (0/1)
-  int cur_list;      // list whose count closes the current round
-  int rsrc;          // roots buffer the next k_hook reads; -1 = every vertex
-  int rdst;          // roots buffer the next k_hook appends to
-  int heavy_done;
+  int phase;            // 0: light list, 1: heavy list
   int active;
-  int rounds;
-  int overflow_rounds;
-  int pad;
-};
-struct RoundLog {
-  long long m_in, m_out, nroots;
+  int rounds;           // rounds started
+  int overflow_rounds;  // stopped at the round limit
 };
 
-// Filter the edge list against the current components and atomicMin every
-// cross edge into both components' slots.
-__global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list0,
-                                               const uint64_t* __restrict__ list1,
-                                               const uint64_t* __restrict__ heavy,
-                                               uint64_t* __restrict__ out0,
-                                               uint64_t* __restrict__ out1,
-                                               unsigned long long* __restrict__ counts,
-                                               const RoundCtl* __restrict__ ctl,
+__device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had,
+                                             int max_rounds) {
+  if (!c.active) return c;
+  if (!had) {
+    if (c.phase == 0) c.phase = 1;
+    else c.active = 0;
+  }
+  c.rounds = r + 1;
+  if (c.active && r + 1 >= max_rounds) {
+    c.active = 0;
+    c.overflow_rounds = 1;
+  }
+  return c;
+}
+
+// Filter one list in place against the current components and atomicMin
+// every cross edge into both components' slots.
+__global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
+                                               uint64_t* __restrict__ heavy,
+                                               int32_t* __restrict__ cnt_light,
+                                               int32_t* __restrict__ cnt_heavy,
+                                               int64_t nchunks, RoundCtl* ctl,
+                                               const unsigned int* __restrict__ had_edge,
                                                const int32_t* __restrict__ comp,
-                                               unsigned long long* __restrict__ best, int vb) {
+                                               unsigned long long* __restrict__ best, int vb,
+                                               int max_rounds,
+                                               unsigned long long* __restrict__ rlog) {
   __shared__ uint64_t s_out[kGTile];
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
-  __shared__ unsigned long long s_base;
-  if (!ctl->active) return;
-  const int src = ctl->src, dst = ctl->dst;
-  const uint64_t* list_in = src == 0 ? list0 : (src == 1 ? list1 : heavy);
-  uint64_t* list_out = dst == 0 ? out0 : out1;
-  unsigned long long* out_count = counts + dst;
-  const int64_t m_in = ctl->m_in;
-  const int64_t nchunks = (m_in + kGTile - 1) / kGTile;
-  for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const int64_t e0 = c * kGTile;
+  const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
+  if (!c.active) return;
+  uint64_t* list = c.phase ? heavy : light;
+  int32_t* cnt = c.phase ? cnt_heavy : cnt_light;
+  unsigned long long sum_in = 0, sum_out = 0;
+  for (int64_t cc = blockIdx.x; cc < nchunks; cc += gridDim.x) {
+    int32_t n_in = cnt[cc];
+    n_in = n_in < 0 ? 0 : (n_in > kGTile ? kGTile : n_in);
+    if (n_in == 0) continue;  // uniform over the block
+    uint64_t* lc = list + cc * kGTile;
     uint64_t keys[kGI];
     uint32_t keep = 0;
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
-      int64_t e = e0 + j * kGB + threadIdx.x;
-      keys[j] = e < m_in ? list_in[e] : kInf;
+      const int e = j * kGB + threadIdx.x;
+      keys[j] = e < n_in ? lc[e] : kInf;
     }
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
@@ -491,138 +536,68 @@ __global__ __launch_bounds__(kGB) void k_round(const uint64_t* __restrict__ list
       if (keys[j] < best[rb]) atomicMin(&best[rb], (unsigned long long)keys[j]);
     }
     uint32_t total;
-    uint32_t pre = block_reserve(__popc(keep), out_count, s_tmp, &s_base, &total);
+    uint32_t pre = block_excl_scan<uint32_t>(__popc(keep), s_tmp, &total);
 #pragma unroll
     for (int j = 0; j < kGI; ++j)
       if (keep & (1u << j)) s_out[pre++] = keys[j];
+    __syncthreads();  // every key of the chunk is in registers or s_out
+    for (uint32_t i = threadIdx.x; i < total; i += kGB) lc[i] = s_out[i];
+    if (threadIdx.x == 0) {
+      cnt[cc] = (int32_t)total;
+      sum_in += (unsigned long long)n_in;
+      sum_out += total;
+    }
     __syncthreads();
-    unsigned long long base = s_base;
-    for (uint32_t i = threadIdx.x; i < total; i += kGB) list_out[base + i] = s_out[i];
-    __syncthreads();
+  }
+  if (threadIdx.x == 0 && sum_in && r < kMaxRounds) {
+    const int slot = blockIdx.x & (kLogSlots - 1);
+    atomicAdd(&rlog[(2 * r) * kLogSlots + slot], sum_in);
+    atomicAdd(&rlog[(2 * r + 1) * kLogSlots + slot], sum_out);
   }
 }
 
-// For each current representative r: follow its lightest edge to the other
-// component s. If s picked the same edge and r < s, r stays representative
-// (mst.py:282-305: the smaller id keeps no edge); otherwise r hooks under s
-// and the edge joins the forest (recorded in r's slot: every vertex hooks at
-// most once in its lifetime, so slots are written once).
-__global__ __launch_bounds__(kGB) void k_hook(const int32_t* __restrict__ roots0,
-                                              const int32_t* __restrict__ roots1,
-                                              int32_t* __restrict__ rout0,
-                                              int32_t* __restrict__ rout1,
-                                              unsigned long long* __restrict__ rcounts,
-                                              const RoundCtl* __restrict__ ctl,
+// For each current representative r (the vertices whose slot holds a
+// candidate; every other slot is kInf): follow its lightest edge to the
+// other component s. If s picked the same edge and r < s, r stays
+// representative (mst.py:282-305: the smaller id keeps no edge); otherwise
+// r hooks under s and the edge joins the forest (recorded in r's slot:
+// every vertex hooks at most once in its lifetime, so slots are written
+// once). A representative's hook[] entry is itself until it hooks.
+__global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
                                               const unsigned long long* __restrict__ best,
                                               const int32_t* __restrict__ comp,
                                               int32_t* __restrict__ hook,
-                                              uint64_t* __restrict__ mst_key, int vb) {
-  __shared__ int32_t s_out[kGB];
-  __shared__ uint32_t s_tmp[kGB / kWave + 1];
-  __shared__ unsigned long long s_base;
-  if (!ctl->active) return;
-  const int rsrc = ctl->rsrc, rdst = ctl->rdst;
-  const int32_t* roots_in = rsrc < 0 ? nullptr : (rsrc == 0 ? roots0 : roots1);
-  int32_t* roots_out = rdst == 0 ? rout0 : rout1;
-  unsigned long long* roots_out_count = rcounts + rdst;
-  const int64_t nroots = ctl->nroots;
-  for (int64_t base_i = (int64_t)blockIdx.x * kGB; base_i < nroots;
-       base_i += (int64_t)gridDim.x * kGB) {
-    int64_t i = base_i + threadIdx.x;
-    uint32_t stay = 0;
-    int32_t r = -1;
-    if (i < nroots) {
-      r = roots_in ? roots_in[i] : (int32_t)i;
-      uint64_t k = best[r];
-      if (k == kInf) {
-        hook[r] = r;  // complete component, or not a representative
-      } else {
-        int32_t a, b;
-        unpack_ends(k, vb, &a, &b);
-        int32_t ra = comp[a], rb = comp[b];
-        int32_t s = (ra == r) ? rb : ra;
-        if (best[s] == k && r < s) {
-          hook[r] = r;
-          stay = 1;
-        } else {
-          hook[r] = s;
-          mst_key[r] = k;
-        }
-      }
-    }
-    uint32_t total;
-    uint32_t pre = block_reserve(stay, roots_out_count, s_tmp, &s_base, &total);
-    if (stay) s_out[pre] = r;
-    __syncthreads();
-    unsigned long long base = s_base;
-    for (uint32_t t = threadIdx.x; t < total; t += kGB) roots_out[base + t] = s_out[t];
-    __syncthreads();
+                                              uint64_t* __restrict__ mst_key,
+                                              unsigned int* __restrict__ had_edge, int vb) {
+  if (!ctl[r & 1].active) return;
+  bool any = false;
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
+    const uint64_t k = best[v];
+    if (k == kInf) continue;
+    any = true;
+    int32_t a, b;
+    unpack_ends(k, vb, &a, &b);
+    const int32_t ra = comp[a], rb = comp[b];
+    const int32_t s = (ra == (int32_t)v) ? rb : ra;
+    if (best[s] == k && (int32_t)v < s) continue;  // stays
+    hook[v] = s;
+    mst_key[v] = k;
   }
-}
-
-// Close the round: the next round filters the list the last k_round wrote
-// (its count = cross edges left) and hooks the representatives that stayed;
-// once the light edges are exhausted the heavy list is filtered once against
-// the light components with every vertex offered to hooking; then stop.
-__device__ void control_step(RoundCtl* ctl, unsigned long long* counts,
-                             unsigned long long* rcounts, const unsigned long long* heavy_count,
-                             int32_t n, RoundLog* log, int max_rounds) {
-  RoundCtl c = *ctl;
-  if (!c.active) return;
-  const long long m_out = (long long)counts[c.cur_list];
-  const long long stay = (long long)rcounts[c.rdst];
-  if (c.rounds < max_rounds) log[c.rounds] = RoundLog{c.m_in, m_out, c.nroots};
-  c.rounds += 1;
-  if (stay == 0 || m_out == 0) {
-    if (!c.heavy_done && *heavy_count > 0) {
-      c.heavy_done = 1;
-      c.src = 2;
-      c.m_in = (long long)*heavy_count;
-      c.dst = 0;
-      c.rsrc = -1;
-      c.nroots = n;
-      c.rdst = 0;
-    } else {
-      c.active = 0;
-    }
-  } else {
-    c.src = c.cur_list;
-    c.m_in = m_out;
-    c.dst = 1 - c.cur_list;
-    c.rsrc = c.rdst;
-    c.nroots = stay;
-    c.rdst = 1 - c.rdst;
-  }
-  if (c.rounds >= max_rounds) {
-    c.active = 0;
-    c.overflow_rounds = 1;
-  }
-  if (c.active) {
-    counts[c.dst] = 0;
-    rcounts[c.rdst] = 0;
-    c.cur_list = c.dst;
-  }
-  *ctl = c;
+  if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
 }
 
 // A hook chain visits each vertex at most once, so a walk longer than n
 // means the hooks form a cycle, which happens only when a graph passed as
 // symmetric is not: the walk stops and the call reports the error.
-//
-// The last block to finish also closes the round (control_step): every
-// other block has read ctl and finished its vertices by then, and the
-// counters it reads were written by earlier kernels.
-__global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ comp,
+__global__ __launch_bounds__(kGB) void k_jump(int r, int32_t n, int32_t* __restrict__ comp,
                                               int32_t* hook,
                                               unsigned long long* __restrict__ best,
-                                              RoundCtl* ctl, unsigned int* __restrict__ overflow,
-                                              unsigned long long* counts,
-                                              unsigned long long* rcounts,
-                                              const unsigned long long* heavy_count,
-                                              RoundLog* log, int max_rounds,
-                                              unsigned int* done) {
-  __shared__ int s_last;
-  if (!ctl->active) return;
+                                              const RoundCtl* __restrict__ ctl,
+                                              const unsigned int* __restrict__ had_edge,
+                                              unsigned int* __restrict__ overflow) {
+  // no candidate in hook(r): nothing hooked and every slot is already kInf
+  if (!ctl[r & 1].active || !had_edge[r]) return;
   for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * kGB) {
     int32_t x = comp[v];
@@ -642,19 +617,7 @@ __global__ __launch_bounds__(kGB) void k_jump(int32_t n, int32_t* __restrict__ c
     }
     if (x == (int32_t)v) best[v] = kInf;
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    s_last = atomicAdd(done, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    *done = 0u;
-    control_step(ctl, counts, rcounts, heavy_count, n, log, max_rounds);
-  }
 }
-
 
 // --------------------------------------------------------- forest output
 // Tile-wise compaction of non-sentinel keys in index order (deterministic):
@@ -697,56 +660,35 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
     if (v[j] != kInf) out[o++] = v[j];
 }
 
-// Forest keys grouped by their smaller endpoint (counting sort on a >> kFpShift),
-// so that neighbouring lanes of the position search read neighbouring rows.
-constexpr int kFpShift = 4;
-__global__ __launch_bounds__(kGB) void k_fp_count(const uint64_t* __restrict__ fkeys,
-                                                  int64_t n_f, int vb,
-                                                  int32_t* __restrict__ bcnt) {
-  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
-       i += (int64_t)gridDim.x * kGB) {
-    int32_t a, b;
-    unpack_ends(fkeys[i], vb, &a, &b);
-    atomicAdd(&bcnt[a >> kFpShift], 1);
-  }
-}
-__global__ __launch_bounds__(kGB) void k_fp_scatter(const uint64_t* __restrict__ fkeys,
-                                                    int64_t n_f, int vb,
-                                                    int32_t* __restrict__ cursor,
-                                                    uint64_t* __restrict__ skeys,
-                                                    int32_t* __restrict__ sidx) {
-  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
-       i += (int64_t)gridDim.x * kGB) {
-    const uint64_t k = fkeys[i];
-    int32_t a, b;
-    unpack_ends(k, vb, &a, &b);
-    const int32_t slot = atomicAdd(&cursor[a >> kFpShift], 1);
-    skeys[slot] = k;
-    sidx[slot] = (int32_t)i;
-  }
-}
-
 // CSR position (in the smaller endpoint's row) and weight of each forest
-// edge; results go to slot sidx[i] (or i). Short rows are scanned; long rows
-// are binary-searched by column and verified, falling back to a scan when
-// the row is not sorted.
+// edge. Rows of up to 32 entries are read with 32 independent loads issued
+// together (one memory latency per row instead of one per entry); longer
+// rows are binary-searched by column and verified, falling back to a scan
+// when the row is not sorted.
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
-    const uint64_t* __restrict__ fkeys, const int32_t* __restrict__ sidx, int64_t n_f,
-    const int32_t* __restrict__ dest, const W* __restrict__ weight,
-    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, uint64_t word_lo,
-    int64_t* __restrict__ ids, W* __restrict__ wout, unsigned int* __restrict__ missing) {
+    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ dest,
+    const W* __restrict__ weight, const int64_t* __restrict__ fe,
+    const int32_t* __restrict__ od, int vb, uint64_t word_lo, int64_t* __restrict__ ids,
+    W* __restrict__ wout, unsigned int* __restrict__ missing) {
+  constexpr int kShort = 32;
   for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
        i += (int64_t)gridDim.x * kGB) {
     uint64_t k = fkeys[i];
-    const int64_t o = sidx ? sidx[i] : i;
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
     uint64_t wk = k >> (2 * vb);
     int64_t beg = fe[a];
     int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
     int64_t pos = -1;
-    if (end - beg > 32) {
+    if (end - beg <= kShort) {
+      int32_t d[kShort];
+#pragma unroll
+      for (int j = 0; j < kShort; ++j) d[j] = beg + j < end ? dest[beg + j] : -1;
+#pragma unroll
+      for (int j = 0; j < kShort; ++j)
+        if (pos < 0 && d[j] == b && word_of<W>(weight[beg + j]) - word_lo == wk) pos = beg + j;
+    } else {
       int64_t lo = beg, hi = end;
       while (lo < hi) {
         int64_t mid = lo + ((hi - lo) >> 1);
@@ -754,17 +696,17 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
       }
       for (int64_t p = lo; p < end && dest[p] == b; ++p)
         if (word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
-    }
-    if (pos < 0) {
-      for (int64_t p = beg; p < end; ++p)
-        if (dest[p] == b && word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
+      if (pos < 0) {
+        for (int64_t p = beg; p < end; ++p)
+          if (dest[p] == b && word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
+      }
     }
     if (pos < 0) {
       atomicOr(missing, 1u);
       pos = 0;
     }
-    if (ids) ids[o] = pos;
-    if (wout) wout[o] = weight[pos];
+    if (ids) ids[i] = pos;
+    if (wout) wout[i] = weight[pos];
   }
 }
 
@@ -961,21 +903,24 @@ int weight_bytes(int32_t dt) {
 
 // Device state counters of one MST call.
 struct MstState {
-  unsigned long long range[2];
-  unsigned long long heavy_count;
-  unsigned long long list_count[2];
-  unsigned long long root_count[2];
-  unsigned long long hooks;
+  unsigned long long range_lo[kSlots];  // weight-word range, spread over slots
+  unsigned long long range_hi[kSlots];
+  unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
+  unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
+  unsigned int had_edge[kMaxRounds + 1];
   unsigned int overflow;
   unsigned int missing;
-  unsigned int jump_done;  // blocks of the current k_jump that finished
-  unsigned int pad0;
   MstPlan plan;
-  RoundCtl ctl;
-  RoundLog log[64];
-  unsigned int hist[4096];  // last: per-round reads stop before it
+  RoundCtl ctl[2];
+  unsigned int hist[kHistBins];  // last: per-round reads stop before it
 };
 constexpr size_t kStateHead = offsetof(MstState, hist);
+
+inline unsigned long long fold_slots(const unsigned long long* v) {
+  unsigned long long t = 0;
+  for (int i = 0; i < kLogSlots; ++i) t += v[i];
+  return t;
+}
 
 struct MstBuffers {
   int32_t* chunk_row;
@@ -983,10 +928,12 @@ struct MstBuffers {
   uint64_t* mst_key;
   int32_t* comp;
   int32_t* hook;
-  int32_t* roots[2];
-  uint64_t* list[2];
+  int32_t* roots[2];     // scratch int32[n] pairs used after the rounds
+  int64_t nchunks;       // CSR chunks = list chunks
+  uint64_t* light;       // chunk-local lists: chunk c owns [c * kGTile, (c+1) * kGTile)
   uint64_t* heavy;
-  int64_t list_cap;
+  int32_t* cnt_light;    // keys held by each chunk
+  int32_t* cnt_heavy;
   uint64_t* fkeys;
   int32_t* tile_cnt;
   int32_t* flag;
@@ -994,15 +941,10 @@ struct MstBuffers {
   MstState* st;
 };
 
-int64_t list_capacity(int64_t m, int symmetric) {
-  // For a symmetric CSR at most half of the entries have dest > row, plus
-  // slack for inputs that are not exactly symmetric (overflow is detected).
-  return symmetric ? m / 2 + 4096 : m;
-}
-
 MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
+  (void)symmetric;
   MstBuffers b;
-  int64_t nchunks = (m + kGTile - 1) / kGTile;
+  const int64_t nchunks = (m + kGTile - 1) / kGTile;
   b.chunk_row = ar.take<int32_t>(nchunks + 1);
   b.best = ar.take<unsigned long long>(n);
   b.mst_key = ar.take<uint64_t>(n);
@@ -1010,10 +952,11 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   b.hook = ar.take<int32_t>(n);
   b.roots[0] = ar.take<int32_t>(n);
   b.roots[1] = ar.take<int32_t>(n);
-  b.list_cap = list_capacity(m, symmetric);
-  b.list[0] = ar.take<uint64_t>(b.list_cap);
-  b.list[1] = ar.take<uint64_t>(b.list_cap);
-  b.heavy = ar.take<uint64_t>(b.list_cap);
+  b.nchunks = nchunks;
+  b.light = ar.take<uint64_t>(nchunks * kGTile);
+  b.heavy = ar.take<uint64_t>(nchunks * kGTile);
+  b.cnt_light = ar.take<int32_t>(nchunks);
+  b.cnt_heavy = ar.take<int32_t>(nchunks);
   b.fkeys = ar.take<uint64_t>(n);
   b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 1);
   b.flag = ar.take<int32_t>(n);
@@ -1051,11 +994,12 @@ struct ForestInfo {
   uint64_t wlo;     // weight word of the lightest entry
   uint64_t whi;     // weight word of the heaviest entry
   int rounds;
+  int enq;               // rounds enqueued: the final control state is ctl[enq & 1]
   const int32_t* d_nf;   // device count of forest edges
   const MstPlan* plan;   // device plan (wlo, whi, wbits)
 };
 
-__global__ void k_state_init(MstState* st, int32_t n) {
+__global__ void k_state_init(MstState* st) {
   // every field starts at zero except the ones set below
   uint32_t* w = reinterpret_cast<uint32_t*>(st);
   const size_t words = sizeof(MstState) / 4;
@@ -1063,11 +1007,9 @@ __global__ void k_state_init(MstState* st, int32_t n) {
        i += (size_t)gridDim.x * blockDim.x)
     w[i] = 0u;
   __syncthreads();
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    st->range[0] = kInf;
-    st->ctl.rsrc = -1;  // first hooking: every vertex
-    st->ctl.nroots = n;
-    st->ctl.active = 1;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < kSlots; i += blockDim.x) st->range_lo[i] = kInf;
+    if (threadIdx.x == 0) st->ctl[0].active = 1;
   }
 }
 
@@ -1077,11 +1019,12 @@ inline int vbits_for(int32_t n) {
 }
 
 // Rounds that always suffice: every round at least halves the components
-// still active in a phase (light edges, then heavy), so each phase ends
-// within bits(n) + 2 rounds.
+// still active in a phase (light edges, then heavy), and each phase ends
+// with one round whose hook finds no candidate, so each phase ends within
+// bits(n) + 2 rounds.
 inline int mst_round_bound(int32_t n) {
   const int r = 2 * (vbits_for(n) + 2);
-  return r < 64 ? r : 64;
+  return r < kMaxRounds ? r : kMaxRounds;
 }
 
 // Boruvka rounds up to the forest key list (shared by the forest, lifetime
@@ -1094,15 +1037,15 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
              int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
              hipStream_t s, int graph_rounds = 0) {
   const bool graph_mode = graph_rounds > 0;
-  const int64_t nchunks = (m + kGTile - 1) / kGTile;
+  const int64_t nchunks = B.nchunks;
   const int vb = vbits_for(n);
-  k_state_init<<<1, 1024, 0, s>>>(B.st, n);
+  k_state_init<<<1, 1024, 0, s>>>(B.st);
   EAC_LAUNCH_CHECK();
   k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(fe, n, m, nchunks, B.chunk_row);
   EAC_LAUNCH_CHECK();
   const int gchunks = grid_for(nchunks, 1, persistent_blocks());
   k_weight_range<W, HAS_OD><<<gchunks, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
-                                                     B.st->range);
+                                                     B.st->range_lo, B.st->range_hi);
   EAC_LAUNCH_CHECK();
   // Light/heavy split (Filter-Kruskal applied to Boruvka): rounds run first
   // on the light prefix of the key order, whose forest is exactly the
@@ -1114,16 +1057,18 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   const bool try_split = m >= 6 * (int64_t)n && m >= (1 << 16);
   if (try_split) {
     k_weight_hist<W><<<grid_for(m / kHistStride + 1, kGB, 1024), kGB, 0, s>>>(
-        weight, m, B.st->range, B.st->hist);
+        weight, m, B.st->range_lo, B.st->range_hi, B.st->hist);
     EAC_LAUNCH_CHECK();
   }
   const double target = (symmetric ? 3.0 : 1.5) * n / kHistStride;
-  k_plan<<<1, kWave, 0, s>>>(B.st->range, B.st->hist, vb, try_split ? 1 : 0, target,
-                             &B.st->plan);
+  k_plan<<<1, kWave, 0, s>>>(B.st->range_lo, B.st->range_hi, B.st->hist, vb, try_split ? 1 : 0,
+                             target, &B.st->plan);
   EAC_LAUNCH_CHECK();
   EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
   k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);
+  EAC_LAUNCH_CHECK();
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.hook, n);
   EAC_LAUNCH_CHECK();
 
   // round 0 from the CSR
@@ -1132,44 +1077,39 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       8 * (int64_t)n;
   int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
-                                               vb, &B.st->plan, symmetric, B.best, B.list[0],
-                                               &B.st->list_count[0], B.heavy,
-                                               &B.st->heavy_count, B.list_cap,
-                                               &B.st->overflow);
+                                               vb, &B.st->plan, symmetric, B.best, B.light,
+                                               B.heavy, B.cnt_light, B.cnt_heavy,
+                                               &B.st->tot0[0][0], &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
-  // Rounds: hook -> jump (whose last block closes the round) -> filter,
-  // enqueued in batches; the round control stops the sequence on the device. Each round at least
-  // halves the active components of a phase, so two phases need at most
-  // 2 * (bits(n) + 2) rounds.
+  // Rounds, enqueued in batches; the round control stops the sequence on
+  // the device (later kernels of a finished run exit at once).
   const int gh = grid_for(n, kGB, persistent_blocks());
-  const int gr = grid_for((B.list_cap + kGTile - 1) / kGTile, 1, persistent_blocks());
-  constexpr int kMaxRounds = 64;
+  const int gr = grid_for(nchunks, 1, persistent_blocks());
   const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
   std::vector<int> rec_hook, rec_jump, rec_round;
   int enq = 0, batch = graph_mode ? fixed_rounds : 6;
   MstState hs;
   for (;;) {
-    for (int b = 0; b < batch; ++b) {
+    for (int b = 0; b < batch && enq < kMaxRounds; ++b, ++enq) {
+      const int r = enq;
       rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
-      k_hook<<<gh, kGB, 0, s>>>(B.roots[0], B.roots[1], B.roots[0], B.roots[1],
-                                B.st->root_count, &B.st->ctl, B.best, B.comp, B.hook, B.mst_key,
-                                vb);
+      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best, B.comp, B.hook, B.mst_key,
+                                B.st->had_edge, vb);
       EAC_LAUNCH_CHECK();
       prof_end(rec_hook.back(), s);
       rec_jump.push_back(prof_begin(EAC_PROF_MST_JUMP, 0, s));
-      k_jump<<<gh, kGB, 0, s>>>(n, B.comp, B.hook, B.best, &B.st->ctl, &B.st->overflow,
-                                B.st->list_count, B.st->root_count, &B.st->heavy_count,
-                                B.st->log, kMaxRounds, &B.st->jump_done);
+      k_jump<<<gh, kGB, 0, s>>>(r, n, B.comp, B.hook, B.best, B.st->ctl, B.st->had_edge,
+                                &B.st->overflow);
       EAC_LAUNCH_CHECK();
       prof_end(rec_jump.back(), s);
       rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
-      k_round<<<gr, kGB, 0, s>>>(B.list[0], B.list[1], B.heavy, B.list[0], B.list[1],
-                                 B.st->list_count, &B.st->ctl, B.comp, B.best, vb);
+      k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, nchunks,
+                                 B.st->ctl, B.st->had_edge, B.comp, B.best, vb, kMaxRounds,
+                                 &B.st->rlog[0][0][0]);
       EAC_LAUNCH_CHECK();
       prof_end(rec_round.back(), s);
     }
-    enq += batch;
     if (graph_mode) break;
     EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
     if (hs.plan.err)
@@ -1180,24 +1120,22 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     if (hs.overflow & 4u)
       return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
                                "(pass symmetric=0)");
-    if (hs.overflow)
-      return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
-                               "pass symmetric=0");
-    if (hs.ctl.overflow_rounds)
+    const RoundCtl& c = hs.ctl[enq & 1];
+    if (c.overflow_rounds || (c.active && enq >= kMaxRounds))
       return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
-    if (!hs.ctl.active) break;
+    if (!c.active) break;
     batch = 4;
   }
   if (!graph_mode && prof_on()) {
-    const int rounds = hs.ctl.rounds;
-    const int nl = rounds < kMaxRounds ? rounds : kMaxRounds;
-    prof_set_bytes(pr, round0_bytes + 8 * (int64_t)hs.log[0].m_out + 8 * (int64_t)hs.heavy_count);
+    const unsigned long long l0 = fold_slots(hs.tot0[0]), h0 = fold_slots(hs.tot0[1]);
+    prof_set_bytes(pr, round0_bytes + 8 * (int64_t)(l0 + h0));
     for (int r = 0; r < (int)rec_hook.size(); ++r) {
-      const RoundLog* lh = r < nl ? &hs.log[r] : nullptr;
-      const RoundLog* lr = r + 1 < nl ? &hs.log[r + 1] : nullptr;
-      prof_set_bytes(rec_hook[r], lh ? 12 * lh->nroots : 0);
-      prof_set_bytes(rec_jump[r], lh ? 8 * (int64_t)n + 8 * lh->nroots : 0);
-      prof_set_bytes(rec_round[r], lr ? 8 * lr->m_in + 8 * lr->m_out + 4 * (int64_t)n : 0);
+      const bool ran = r < hs.ctl[enq & 1].rounds;
+      const int64_t in = ran ? (int64_t)fold_slots(hs.rlog[r][0]) : 0;
+      const int64_t out = ran ? (int64_t)fold_slots(hs.rlog[r][1]) : 0;
+      prof_set_bytes(rec_hook[r], ran ? 8 * (int64_t)n : 0);
+      prof_set_bytes(rec_jump[r], ran && hs.had_edge[r] ? 12 * (int64_t)n : 0);
+      prof_set_bytes(rec_round[r], ran ? 8 * in + 8 * out + 4 * nchunks : 0);
     }
   }
 
@@ -1213,6 +1151,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   fi->vb = vb;
   fi->d_nf = d_nf;
   fi->plan = &B.st->plan;
+  fi->enq = enq;
   if (graph_mode) {
     fi->n_f = -1;
     fi->wbits = 64 - 2 * vb;  // upper bound: sort passes must be host-known
@@ -1227,7 +1166,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   fi->wbits = hs.plan.wbits;
   fi->wlo = hs.plan.wlo;
   fi->whi = hs.plan.whi;
-  fi->rounds = hs.ctl.rounds;
+  fi->rounds = hs.ctl[enq & 1].rounds;
   return EAC_OK;
 }
 
@@ -1242,20 +1181,8 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
   const int64_t n_f = fi.n_f;
   if (n_f > 0 && (ids || wout)) {
-    // group the forest keys by row block, then look their positions up
-    const int64_t nb = ((int64_t)n >> kFpShift) + 1;
-    int32_t* bcnt = B.rank;        // nb <= n for n >= 2 (a forest edge needs 2 vertices)
-    int32_t* sidx = B.roots[1];
-    uint64_t* skeys = B.mst_key;   // free once the forest keys are compacted
-    const int gf = grid_for(n_f, kGB, persistent_blocks());
-    EAC_TRY(fill<int32_t>(bcnt, nb, 0, s));
-    k_fp_count<<<gf, kGB, 0, s>>>(B.fkeys, n_f, fi.vb, bcnt);
-    EAC_LAUNCH_CHECK();
-    EAC_TRY((scan_excl<int32_t, int32_t>(bcnt, bcnt, nb, 0, nullptr, ar, s)));
-    k_fp_scatter<<<gf, kGB, 0, s>>>(B.fkeys, n_f, fi.vb, bcnt, skeys, sidx);
-    EAC_LAUNCH_CHECK();
-    k_forest_positions<W, HAS_OD><<<gf, kGB, 0, s>>>(skeys, sidx, n_f, dest, weight, fe, od,
-                                                     fi.vb, fi.wlo, ids, wout, &B.st->missing);
+    k_forest_positions<W, HAS_OD><<<grid_for(n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
+        B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
   int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
@@ -1768,19 +1695,20 @@ __global__ void k_goff_prep(const int64_t* __restrict__ in, int G, int32_t N,
 // error flags}. Flags: 1 edge list overflow, 2 dest ids outside [0, N),
 // 4 graph passed as symmetric is not, 8 key wider than 64 bits,
 // 16 rounds not converged, 32 bad graph offsets.
-__global__ void k_batch_summary(const MstState* __restrict__ st,
+__global__ void k_batch_summary(const MstState* __restrict__ st, int enq,
                                 const int32_t* __restrict__ d_ncomp,
                                 const int64_t* __restrict__ seg_total,
                                 const int32_t* __restrict__ misc, int64_t* __restrict__ out) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   out[0] = *d_ncomp;
   out[1] = *seg_total;
-  out[2] = st->ctl.rounds;
+  const RoundCtl c = st->ctl[enq & 1];
+  out[2] = c.rounds;
   out[3] = misc[0];
   int64_t flags = st->overflow & 7u;
   if (misc[1]) flags |= 32;
   if (st->plan.err) flags |= 8;
-  if (st->ctl.active || st->ctl.overflow_rounds) flags |= 16;
+  if (c.active || c.overflow_rounds) flags |= 16;
   out[4] = flags;
 }
 
@@ -1903,7 +1831,7 @@ int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, con
   EAC_LAUNCH_CHECK();
   k_graph_sub<<<G < 4096 ? G : 4096, 256, 0, s>>>(labels, goff, G, base, cut, gout);
   EAC_LAUNCH_CHECK();
-  k_batch_summary<<<1, 64, 0, s>>>(B.st, d_ncomp, seg + G, misc, summary);
+  k_batch_summary<<<1, 64, 0, s>>>(B.st, fi.enq, d_ncomp, seg + G, misc, summary);
   EAC_LAUNCH_CHECK();
   if (graph_mode) return EAC_OK;
   int64_t hs[5];
