@@ -29,8 +29,8 @@ METRIC = "EAC pts/s + Boruvka MST Medges/s: batched/BF16 vs plain FP32 GPU, exac
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
-MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)", 2: "k_hook",
-             3: "k_jump", 4: "forest output + labels"}
+MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)",
+             2: "k_hook (union-find links)", 4: "forest output + labels"}
 
 
 def log(msg):

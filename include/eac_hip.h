@@ -60,8 +60,8 @@ int eac_abi_version(void);
 #define EAC_PROF_SLOTS 16
 #define EAC_PROF_MST_ROUND0 0 /* CSR stream: 8 B/entry + 16 B/vertex + 8 B/edge out */
 #define EAC_PROF_MST_ROUND 1  /* list pass: 8 B/edge in + 8 B/edge out + 4 B/vertex */
-#define EAC_PROF_MST_HOOK 2   /* 12 B per representative */
-#define EAC_PROF_MST_JUMP 3   /* 8 B/vertex read + 8 B/representative reset */
+#define EAC_PROF_MST_HOOK 2   /* slot scan + links: 8 B per vertex */
+#define EAC_PROF_MST_JUMP 3   /* unused (links and root lookups need no jump pass) */
 #define EAC_PROF_MST_FINAL 4  /* forest output + labels */
 #define EAC_PROF_CC 5         /* union-find over the CSR: 4 B/entry + 8 B/vertex */
 #define EAC_PROF_KM_LABEL 6   /* labeling pass; "bytes" = 2 n d sum(k) FLOPs     */
@@ -190,7 +190,7 @@ int eac_sl_lifetime_batched(const int32_t* dest, const void* weight, int32_t wei
  * N, M, G) and call _launch for each batch. goff is a DEVICE array of G+1
  * offsets. summary (device int64[5]) receives: total components, forest
  * edges, Boruvka rounds, kept edges, error flags (1 edge list overflow,
- * 2 dest ids outside [0, N), 4 graph passed as symmetric is not, 8 key
+ * 2 dest ids outside [0, N), 4 (unused), 8 key
  * wider than 64 bits, 16 rounds not converged within max_rounds, 32 bad
  * goff); results are valid only when the flags are 0. max_rounds <= 0
  * takes the bound 2 * (bits(N-1) + 2) that always suffices; a smaller value

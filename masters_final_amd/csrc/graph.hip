@@ -256,7 +256,7 @@ __device__ __forceinline__ void fold_range(const unsigned long long* range_lo,
   *hi = h;
 }
 
-// Histogram of weight words over every 8th entry (4096 bins of the weight
+// Histogram of weight words over one entry in 8 (4096 bins of the weight
 // range), used to pick the light/heavy split threshold. Entries in row gaps
 // may be sampled; that only moves the threshold.
 constexpr int kHistBins = 4096;
@@ -280,10 +280,13 @@ __global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weigh
     int wb = 0;
     while (wb < 64 && ((hi - lo) >> wb) != 0) ++wb;
     const int shift = wb > 12 ? wb - 12 : 0;
-    const int64_t ns = (m + kHistStride - 1) / kHistStride;
+    // sample the first 256 entries of every 2048 (coalesced, 1 in 8)
+    const int64_t ns = ((m + 2047) / 2048) * 256;
     for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < ns;
          i += (int64_t)gridDim.x * kGB) {
-      uint64_t w = word_of<W>(weight[i * kHistStride]);
+      const int64_t e = (i >> 8) * 2048 + (i & 255);
+      if (e >= m) continue;
+      uint64_t w = word_of<W>(weight[e]);
       if (w < lo || w > hi) continue;
       atomicAdd(&h[(w - lo) >> shift], 1u);
     }
@@ -455,11 +458,71 @@ __global__ __launch_bounds__(kGB) void k_round0(
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0 && (sum_l | sum_h)) {
+  if (tot && threadIdx.x == 0 && (sum_l | sum_h)) {  // profiling only
     const int slot = blockIdx.x & (kLogSlots - 1);
     atomicAdd(&tot[slot], sum_l);
     atomicAdd(&tot[kLogSlots + slot], sum_h);
   }
+}
+
+// ------------------------------------------------------------- union-find
+// Parent array with parent[x] <= x (larger roots hook under smaller ones),
+// so every walk ends at the smallest vertex of the component.
+__device__ __forceinline__ int32_t uf_ld(int32_t* p, int32_t i) {
+  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void uf_st(int32_t* p, int32_t i, int32_t v) {
+  __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ int32_t uf_find(int32_t* P, int32_t x) {
+  for (;;) {
+    int32_t p = uf_ld(P, x);
+    if (p == x) return x;
+    int32_t gp = uf_ld(P, p);
+    if (gp == p) return p;
+    uf_st(P, x, gp);  // path halving
+    x = gp;
+  }
+}
+__device__ void uf_unite(int32_t* P, int32_t a, int32_t b) {
+  a = uf_find(P, a);
+  b = uf_find(P, b);
+  while (a != b) {
+    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    int32_t old = atomicCAS(P + hi, hi, lo);
+    if (old == hi) return;
+    a = uf_find(P, old);
+    b = uf_find(P, lo);
+  }
+}
+
+// Links the components of a and b; returns the root that was hooked (the
+// larger one), or -1 when they were already one component. A root is
+// hooked at most once in its lifetime.
+__device__ int32_t uf_link(int32_t* P, int32_t a, int32_t b) {
+  a = uf_find(P, a);
+  b = uf_find(P, b);
+  while (a != b) {
+    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    int32_t old = atomicCAS(P + hi, hi, lo);
+    if (old == hi) return hi;
+    a = uf_find(P, old);
+    b = uf_find(P, lo);
+  }
+  return -1;
+}
+// Root lookup in a kernel where no link happens: plain loads (a stale value
+// is still an ancestor) and path splitting with plain stores (any ancestor
+// is a valid parent).
+__device__ __forceinline__ int32_t find_fast(int32_t* P, int32_t x) {
+  int32_t p = P[x];
+  while (p != x) {
+    const int32_t gp = P[p];
+    if (gp != p) P[x] = gp;
+    x = p;
+    p = gp;
+  }
+  return x;
 }
 
 // ------------------------------------------------------ device round control
@@ -493,80 +556,91 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 }
 
 // Filter one list in place against the current components and atomicMin
-// every cross edge into both components' slots.
+// every cross edge into both components' slots. One wavefront per chunk
+// (4 x 64 keys in flight per step); survivors are written back in order at
+// the front of the chunk (a write never passes the read position).
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
                                                int32_t* __restrict__ cnt_heavy,
                                                int64_t nchunks, RoundCtl* ctl,
                                                const unsigned int* __restrict__ had_edge,
-                                               const int32_t* __restrict__ comp,
-                                               unsigned long long* __restrict__ best, int vb,
-                                               int max_rounds,
+                                               int32_t* P, unsigned long long* __restrict__ best,
+                                               int vb, int max_rounds,
                                                unsigned long long* __restrict__ rlog) {
-  __shared__ uint64_t s_out[kGTile];
-  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  __shared__ unsigned long long s_io[2 * (kGB / kWave)];
   const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
   uint64_t* list = c.phase ? heavy : light;
   int32_t* cnt = c.phase ? cnt_heavy : cnt_light;
+  const int lane = lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
   unsigned long long sum_in = 0, sum_out = 0;
-  for (int64_t cc = blockIdx.x; cc < nchunks; cc += gridDim.x) {
+  for (int64_t cc = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave; cc < nchunks;
+       cc += nwaves) {
     int32_t n_in = cnt[cc];
     n_in = n_in < 0 ? 0 : (n_in > kGTile ? kGTile : n_in);
-    if (n_in == 0) continue;  // uniform over the block
+    if (n_in == 0) continue;  // uniform over the wave
     uint64_t* lc = list + cc * kGTile;
-    uint64_t keys[kGI];
-    uint32_t keep = 0;
+    int32_t out = 0;
+    for (int base = 0; base < n_in; base += 4 * kWave) {
+      uint64_t k[4];
 #pragma unroll
-    for (int j = 0; j < kGI; ++j) {
-      const int e = j * kGB + threadIdx.x;
-      keys[j] = e < n_in ? lc[e] : kInf;
+      for (int q = 0; q < 4; ++q) {
+        const int e = base + q * kWave + lane;
+        k[q] = e < n_in ? lc[e] : kInf;
+      }
+      bool keep[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        keep[q] = false;
+        if (k[q] == kInf) continue;
+        int32_t a, b;
+        unpack_ends(k[q], vb, &a, &b);
+        const int32_t ra = find_fast(P, a), rb = find_fast(P, b);
+        if (ra == rb) continue;
+        keep[q] = true;
+        if (k[q] < best[ra]) atomicMin(&best[ra], (unsigned long long)k[q]);
+        if (k[q] < best[rb]) atomicMin(&best[rb], (unsigned long long)k[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint64_t m = __ballot(keep[q]);
+        if (keep[q]) lc[out + __popcll(m & below)] = k[q];
+        out += __popcll(m);
+      }
     }
-#pragma unroll
-    for (int j = 0; j < kGI; ++j) {
-      if (keys[j] == kInf) continue;
-      int32_t a, b;
-      unpack_ends(keys[j], vb, &a, &b);
-      int32_t ra = comp[a], rb = comp[b];
-      if (ra == rb) continue;
-      keep |= 1u << j;
-      if (keys[j] < best[ra]) atomicMin(&best[ra], (unsigned long long)keys[j]);
-      if (keys[j] < best[rb]) atomicMin(&best[rb], (unsigned long long)keys[j]);
-    }
-    uint32_t total;
-    uint32_t pre = block_excl_scan<uint32_t>(__popc(keep), s_tmp, &total);
-#pragma unroll
-    for (int j = 0; j < kGI; ++j)
-      if (keep & (1u << j)) s_out[pre++] = keys[j];
-    __syncthreads();  // every key of the chunk is in registers or s_out
-    for (uint32_t i = threadIdx.x; i < total; i += kGB) lc[i] = s_out[i];
-    if (threadIdx.x == 0) {
-      cnt[cc] = (int32_t)total;
+    if (lane == 0) {
+      cnt[cc] = out;
       sum_in += (unsigned long long)n_in;
-      sum_out += total;
+      sum_out += (unsigned long long)out;
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0 && sum_in && r < kMaxRounds) {
-    const int slot = blockIdx.x & (kLogSlots - 1);
-    atomicAdd(&rlog[(2 * r) * kLogSlots + slot], sum_in);
-    atomicAdd(&rlog[(2 * r + 1) * kLogSlots + slot], sum_out);
+  if (rlog && r < kMaxRounds) {  // profiling only: per-round traffic
+    const int w = threadIdx.x / kWave;
+    if (lane == 0) { s_io[2 * w] = sum_in; s_io[2 * w + 1] = sum_out; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int i = 1; i < kGB / kWave; ++i) { sum_in += s_io[2 * i]; sum_out += s_io[2 * i + 1]; }
+      const int slot = blockIdx.x & (kLogSlots - 1);
+      if (sum_in) {
+        atomicAdd(&rlog[(2 * r) * kLogSlots + slot], sum_in);
+        atomicAdd(&rlog[(2 * r + 1) * kLogSlots + slot], sum_out);
+      }
+    }
   }
 }
 
-// For each current representative r (the vertices whose slot holds a
-// candidate; every other slot is kInf): follow its lightest edge to the
-// other component s. If s picked the same edge and r < s, r stays
-// representative (mst.py:282-305: the smaller id keeps no edge); otherwise
-// r hooks under s and the edge joins the forest (recorded in r's slot:
-// every vertex hooks at most once in its lifetime, so slots are written
-// once). A representative's hook[] entry is itself until it hooks.
+// Every component slot that holds a candidate key (the component's
+// lightest edge to another component) links the edge's two components;
+// each successful link records the edge at the root it hooked, so an edge
+// that is the lightest of both its components (mst.py:282-305) is recorded
+// once. The lightest edges of one round form a forest, so apart from such
+// shared edges every candidate links. Slots are cleared for the next round.
 __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
-                                              const unsigned long long* __restrict__ best,
-                                              const int32_t* __restrict__ comp,
-                                              int32_t* __restrict__ hook,
+                                              unsigned long long* __restrict__ best, int32_t* P,
                                               uint64_t* __restrict__ mst_key,
                                               unsigned int* __restrict__ had_edge, int vb) {
   if (!ctl[r & 1].active) return;
@@ -576,47 +650,13 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
     const uint64_t k = best[v];
     if (k == kInf) continue;
     any = true;
+    best[v] = kInf;
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
-    const int32_t ra = comp[a], rb = comp[b];
-    const int32_t s = (ra == (int32_t)v) ? rb : ra;
-    if (best[s] == k && (int32_t)v < s) continue;  // stays
-    hook[v] = s;
-    mst_key[v] = k;
+    const int32_t h = uf_link(P, a, b);
+    if (h >= 0) mst_key[h] = k;
   }
   if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
-}
-
-// A hook chain visits each vertex at most once, so a walk longer than n
-// means the hooks form a cycle, which happens only when a graph passed as
-// symmetric is not: the walk stops and the call reports the error.
-__global__ __launch_bounds__(kGB) void k_jump(int r, int32_t n, int32_t* __restrict__ comp,
-                                              int32_t* hook,
-                                              unsigned long long* __restrict__ best,
-                                              const RoundCtl* __restrict__ ctl,
-                                              const unsigned int* __restrict__ had_edge,
-                                              unsigned int* __restrict__ overflow) {
-  // no candidate in hook(r): nothing hooked and every slot is already kInf
-  if (!ctl[r & 1].active || !had_edge[r]) return;
-  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kGB) {
-    int32_t x = comp[v];
-    int32_t y = x;
-    for (int32_t steps = 0;; ++steps) {
-      int32_t h = hook[y];
-      if (h == y) break;
-      if (steps > n) {
-        atomicOr(overflow, 4u);
-        break;
-      }
-      y = h;
-    }
-    if (y != x) {
-      comp[v] = y;
-      hook[x] = y;  // shortcut: any ancestor is a valid value
-    }
-    if (x == (int32_t)v) best[v] = kInf;
-  }
 }
 
 // --------------------------------------------------------- forest output
@@ -661,86 +701,58 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
 }
 
 // CSR position (in the smaller endpoint's row) and weight of each forest
-// edge. Rows of up to 32 entries are read with 32 independent loads issued
-// together (one memory latency per row instead of one per entry); longer
-// rows are binary-searched by column and verified, falling back to a scan
-// when the row is not sorted.
+// edge: 8 lanes per edge scan the row 32 entries at a time (each load
+// instruction reads 8 consecutive entries of 8 rows), keeping the first
+// entry with the edge's column and weight.
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
     const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ dest,
     const W* __restrict__ weight, const int64_t* __restrict__ fe,
     const int32_t* __restrict__ od, int vb, uint64_t word_lo, int64_t* __restrict__ ids,
     W* __restrict__ wout, unsigned int* __restrict__ missing) {
-  constexpr int kShort = 32;
-  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n_f;
-       i += (int64_t)gridDim.x * kGB) {
-    uint64_t k = fkeys[i];
+  constexpr int kG = 8;  // lanes per forest edge
+  constexpr int64_t kNone = INT64_MAX;
+  const int sub = threadIdx.x & (kG - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kGB / kG);
+  for (int64_t i = ((int64_t)blockIdx.x * kGB + threadIdx.x) / kG; i < n_f; i += stride) {
+    const uint64_t k = fkeys[i];
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
-    uint64_t wk = k >> (2 * vb);
-    int64_t beg = fe[a];
-    int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
-    int64_t pos = -1;
-    if (end - beg <= kShort) {
-      int32_t d[kShort];
+    const uint64_t wk = k >> (2 * vb);
+    const int64_t beg = fe[a];
+    const int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
+    int64_t pos = kNone;
+    for (int64_t p0 = beg; p0 < end; p0 += 4 * kG) {  // trip count uniform over the group
+      int32_t d[4];
 #pragma unroll
-      for (int j = 0; j < kShort; ++j) d[j] = beg + j < end ? dest[beg + j] : -1;
+      for (int q = 0; q < 4; ++q) {
+        const int64_t e = p0 + q * kG + sub;
+        d[q] = e < end ? dest[e] : -1;
+      }
 #pragma unroll
-      for (int j = 0; j < kShort; ++j)
-        if (pos < 0 && d[j] == b && word_of<W>(weight[beg + j]) - word_lo == wk) pos = beg + j;
-    } else {
-      int64_t lo = beg, hi = end;
-      while (lo < hi) {
-        int64_t mid = lo + ((hi - lo) >> 1);
-        if (dest[mid] < b) lo = mid + 1; else hi = mid;
+      for (int q = 0; q < 4; ++q) {
+        const int64_t e = p0 + q * kG + sub;
+        if (d[q] == b && e < pos && word_of<W>(weight[e]) - word_lo == wk) pos = e;
       }
-      for (int64_t p = lo; p < end && dest[p] == b; ++p)
-        if (word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
-      if (pos < 0) {
-        for (int64_t p = beg; p < end; ++p)
-          if (dest[p] == b && word_of<W>(weight[p]) - word_lo == wk) { pos = p; break; }
+#pragma unroll
+      for (int off = kG / 2; off > 0; off >>= 1) {
+        const int64_t o = __shfl_xor(pos, off, kG);
+        pos = o < pos ? o : pos;
       }
+      if (pos != kNone) break;
     }
-    if (pos < 0) {
-      atomicOr(missing, 1u);
-      pos = 0;
+    if (sub == 0) {
+      if (pos == kNone) {
+        atomicOr(missing, 1u);
+        pos = 0;
+      }
+      if (ids) ids[i] = pos;
+      if (wout) wout[i] = weight[pos];
     }
-    if (ids) ids[i] = pos;
-    if (wout) wout[i] = weight[pos];
   }
 }
 
 // ------------------------------------------------------------- union-find
-__device__ __forceinline__ int32_t uf_ld(int32_t* p, int32_t i) {
-  return __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void uf_st(int32_t* p, int32_t i, int32_t v) {
-  __hip_atomic_store(p + i, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// parent[x] <= x always holds, so every walk terminates at the smallest
-// vertex of the component.
-__device__ int32_t uf_find(int32_t* P, int32_t x) {
-  for (;;) {
-    int32_t p = uf_ld(P, x);
-    if (p == x) return x;
-    int32_t gp = uf_ld(P, p);
-    if (gp == p) return p;
-    uf_st(P, x, gp);  // path halving
-    x = gp;
-  }
-}
-__device__ void uf_unite(int32_t* P, int32_t a, int32_t b) {
-  a = uf_find(P, a);
-  b = uf_find(P, b);
-  while (a != b) {
-    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    int32_t old = atomicCAS(P + hi, hi, lo);
-    if (old == hi) return;
-    a = uf_find(P, old);
-    b = uf_find(P, lo);
-  }
-}
-
 __global__ void k_iota(int32_t* p, int32_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -903,18 +915,19 @@ int weight_bytes(int32_t dt) {
 
 // Device state counters of one MST call.
 struct MstState {
-  unsigned long long range_lo[kSlots];  // weight-word range, spread over slots
-  unsigned long long range_hi[kSlots];
-  unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
-  unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
-  unsigned int had_edge[kMaxRounds + 1];
   unsigned int overflow;
   unsigned int missing;
   MstPlan plan;
   RoundCtl ctl[2];
+  unsigned int had_edge[kMaxRounds + 1];
+  unsigned long long range_lo[kSlots];  // weight-word range, spread over slots
+  unsigned long long range_hi[kSlots];
+  unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
+  unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
   unsigned int hist[kHistBins];  // last: per-round reads stop before it
 };
 constexpr size_t kStateHead = offsetof(MstState, hist);
+constexpr size_t kCtlHead = offsetof(MstState, range_lo);  // what the round loop reads
 
 inline unsigned long long fold_slots(const unsigned long long* v) {
   unsigned long long t = 0;
@@ -1066,9 +1079,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_LAUNCH_CHECK();
   EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
-  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);
-  EAC_LAUNCH_CHECK();
-  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.hook, n);
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);  // union-find parents
   EAC_LAUNCH_CHECK();
 
   // round 0 from the CSR
@@ -1079,7 +1090,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
                                                vb, &B.st->plan, symmetric, B.best, B.light,
                                                B.heavy, B.cnt_light, B.cnt_heavy,
-                                               &B.st->tot0[0][0], &B.st->overflow);
+                                               prof_on() ? &B.st->tot0[0][0] : nullptr,
+                                               &B.st->overflow);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
   // Rounds, enqueued in batches; the round control stops the sequence on
@@ -1087,39 +1099,31 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   const int gh = grid_for(n, kGB, persistent_blocks());
   const int gr = grid_for(nchunks, 1, persistent_blocks());
   const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
-  std::vector<int> rec_hook, rec_jump, rec_round;
+  std::vector<int> rec_hook, rec_round;
   int enq = 0, batch = graph_mode ? fixed_rounds : 6;
   MstState hs;
   for (;;) {
     for (int b = 0; b < batch && enq < kMaxRounds; ++b, ++enq) {
       const int r = enq;
       rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
-      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best, B.comp, B.hook, B.mst_key,
-                                B.st->had_edge, vb);
+      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best, B.comp, B.mst_key, B.st->had_edge,
+                                vb);
       EAC_LAUNCH_CHECK();
       prof_end(rec_hook.back(), s);
-      rec_jump.push_back(prof_begin(EAC_PROF_MST_JUMP, 0, s));
-      k_jump<<<gh, kGB, 0, s>>>(r, n, B.comp, B.hook, B.best, B.st->ctl, B.st->had_edge,
-                                &B.st->overflow);
-      EAC_LAUNCH_CHECK();
-      prof_end(rec_jump.back(), s);
       rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
       k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, nchunks,
                                  B.st->ctl, B.st->had_edge, B.comp, B.best, vb, kMaxRounds,
-                                 &B.st->rlog[0][0][0]);
+                                 prof_on() ? &B.st->rlog[0][0][0] : nullptr);
       EAC_LAUNCH_CHECK();
       prof_end(rec_round.back(), s);
     }
     if (graph_mode) break;
-    EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
+    EAC_TRY(d2h_sync(&hs, B.st, prof_on() ? kStateHead : kCtlHead, s));
     if (hs.plan.err)
       return fail(EAC_ERR_KEY_WIDTH,
                   "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
                   hs.plan.wbits, vb);
     if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
-    if (hs.overflow & 4u)
-      return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
-                               "(pass symmetric=0)");
     const RoundCtl& c = hs.ctl[enq & 1];
     if (c.overflow_rounds || (c.active && enq >= kMaxRounds))
       return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
@@ -1134,7 +1138,6 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       const int64_t in = ran ? (int64_t)fold_slots(hs.rlog[r][0]) : 0;
       const int64_t out = ran ? (int64_t)fold_slots(hs.rlog[r][1]) : 0;
       prof_set_bytes(rec_hook[r], ran ? 8 * (int64_t)n : 0);
-      prof_set_bytes(rec_jump[r], ran && hs.had_edge[r] ? 12 * (int64_t)n : 0);
       prof_set_bytes(rec_round[r], ran ? 8 * in + 8 * out + 4 * nchunks : 0);
     }
   }
@@ -1181,32 +1184,28 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
   const int64_t n_f = fi.n_f;
   if (n_f > 0 && (ids || wout)) {
-    k_forest_positions<W, HAS_OD><<<grid_for(n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
+    k_forest_positions<W, HAS_OD><<<grid_for(8 * n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
         B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
   int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
   if (labels) {
-    // comp[] holds every vertex's final representative: number components
-    // by their smallest vertex (first occurrence)
-    int32_t* first = B.roots[0];
+    // the union-find root of a component is its smallest vertex, so the
+    // roots in vertex order number the components by first occurrence
     int32_t* root = B.hook;
     const int64_t ftiles = (n + kGTile - 1) / kGTile;
     int g = grid_for(n, 256, 4096);
-    EAC_TRY(fill<int32_t>(first, n, INT32_MAX, s));
-    k_first_occ<<<grid_for(n, kFoTile, 4096), 256, 0, s>>>(B.comp, n, n, first, &B.st->missing);
-    EAC_LAUNCH_CHECK();
-    k_first_flag<<<g, 256, 0, s>>>(B.comp, first, n, n, root, B.flag);
+    k_uf_flatten<<<g, 256, 0, s>>>(B.comp, n, root, B.flag);
     EAC_LAUNCH_CHECK();
     EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles, ar, s)));
     k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
     EAC_LAUNCH_CHECK();
   }
   prof_end(pr, s);
-  MstState hs;
-  EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
+  unsigned int missing = 0;
+  EAC_TRY(d2h_sync(&missing, &B.st->missing, sizeof(missing), s));
   prof_flush();
-  if (hs.missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
+  if (missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
   if (h_out) {
     h_out[0] = n_f;
     h_out[1] = n - n_f;
@@ -1693,7 +1692,7 @@ __global__ void k_goff_prep(const int64_t* __restrict__ in, int G, int32_t N,
 // Run summary of a batched cut, written on the device: {components,
 // forest edges (nonzero weight under rule 0), Boruvka rounds, kept edges,
 // error flags}. Flags: 1 edge list overflow, 2 dest ids outside [0, N),
-// 4 graph passed as symmetric is not, 8 key wider than 64 bits,
+// 4 unused, 8 key wider than 64 bits,
 // 16 rounds not converged, 32 bad graph offsets.
 __global__ void k_batch_summary(const MstState* __restrict__ st, int enq,
                                 const int32_t* __restrict__ d_ncomp,
@@ -1705,7 +1704,7 @@ __global__ void k_batch_summary(const MstState* __restrict__ st, int enq,
   const RoundCtl c = st->ctl[enq & 1];
   out[2] = c.rounds;
   out[3] = misc[0];
-  int64_t flags = st->overflow & 7u;
+  int64_t flags = st->overflow & 3u;
   if (misc[1]) flags |= 32;
   if (st->plan.err) flags |= 8;
   if (c.active || c.overflow_rounds) flags |= 16;
@@ -1717,8 +1716,6 @@ int summary_error(int64_t flags) {
   if (flags & 32) return fail(EAC_ERR_ARG, "sl_batched: graph offsets not 0 <= ... <= N");
   if (flags & 8) return fail(EAC_ERR_KEY_WIDTH, "boruvka: weight range and vertex ids need > 64 key bits");
   if (flags & 2) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
-  if (flags & 4)
-    return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric (pass symmetric=0)");
   if (flags & 1)
     return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
                              "pass symmetric=0");
