@@ -364,6 +364,7 @@ __global__ void k_plan(const unsigned long long* __restrict__ range_lo,
 // list, heavy keys go to the heavy list. A symmetric CSR lists each edge
 // once (dest > row) and takes each row's minimum from the row itself.
 constexpr int kLogSlots = 16;
+constexpr int kScatterMax = 64;  // longer rows are located by binary search
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_round0(
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
     unsigned long long* __restrict__ tot, unsigned int* __restrict__ overflow) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ unsigned long long s_best[kMaxRows];
-  __shared__ uint64_t s_out[kGTile];
+  __shared__ int16_t s_row[kGTile];  // staged row of each entry, -1: gap or long row
   __shared__ uint32_t s_tmp[kGB / kWave + 1];
   const bool err = plan->err != 0;
   const uint64_t word_lo = plan->wlo, light_lim = plan->light_lim;
@@ -385,28 +386,53 @@ __global__ __launch_bounds__(kGB) void k_round0(
       if (threadIdx.x == 0) { cnt_light[c] = 0; cnt_heavy[c] = 0; }
       continue;
     }
+    const int64_t e0 = c * kGTile;
+    const int64_t e1 = e0 + kGTile < m ? e0 + kGTile : m;
+    // the entries first: their loads overlap the row staging
+    int32_t dv[kGI];
+    W wv[kGI];
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      const int64_t e = e0 + j * kGB + threadIdx.x;
+      dv[j] = e < e1 ? dest[e] : -1;
+      wv[j] = e < e1 ? weight[e] : W(0);
+    }
     ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
-    if (cx.staged)
-      for (int i = threadIdx.x; i < cx.r_cnt; i += blockDim.x) s_best[i] = kInf;
+    if (cx.staged) {
+      for (int i = threadIdx.x; i < kGTile; i += kGB) s_row[i] = -1;
+      for (int i = threadIdx.x; i < cx.r_cnt; i += kGB) s_best[i] = kInf;
+    }
     __syncthreads();
+    if (cx.staged) {
+      // each short row marks its entries (stage_chunk gave row i to this thread)
+      const int32_t clen = (int32_t)(e1 - e0);
+      for (int i = threadIdx.x; i < cx.r_cnt; i += kGB) {
+        const int32_t rb = s_beg[i] > 0 ? s_beg[i] : 0;
+        const int32_t re = s_end[i] < clen ? s_end[i] : clen;
+        if (re - rb <= kScatterMax)
+          for (int32_t q = rb; q < re; ++q) s_row[q] = (int16_t)i;
+      }
+      __syncthreads();
+    }
     uint64_t keys[kGI];
     uint32_t keep = 0, light_m = 0;
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
       keys[j] = kInf;
-      int64_t e = cx.e0 + j * kGB + threadIdx.x;
-      if (e >= cx.e1) continue;
-      int32_t ri = row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end);
+      const int64_t e = e0 + j * kGB + threadIdx.x;
+      if (e >= e1) continue;
+      int32_t ri = cx.staged ? (int32_t)s_row[e - e0] : -1;
+      if (ri < 0) ri = row_of<HAS_OD>(cx, e, fe, od, s_beg, s_end);
       if (ri < 0) continue;
-      int32_t u = cx.r_lo + ri;
-      int32_t v = dest[e];
+      const int32_t u = cx.r_lo + ri;
+      const int32_t v = dv[j];
       if (v < 0 || v >= n) {  // invalid id: reported, never dereferenced
         atomicOr(overflow, 2u);
         continue;
       }
       if (v == u) continue;  // self loops never join a forest
-      uint64_t wk = word_of<W>(weight[e]) - word_lo;
-      uint64_t key = pack_key(wk, (uint32_t)u, (uint32_t)v, vb);
+      const uint64_t wk = word_of<W>(wv[j]) - word_lo;
+      const uint64_t key = pack_key(wk, (uint32_t)u, (uint32_t)v, vb);
       keys[j] = key;
       if (!symmetric || v > u) keep |= 1u << j;
       if (wk >= light_lim) continue;  // heavy: listed, not a round-0 candidate
@@ -420,27 +446,22 @@ __global__ __launch_bounds__(kGB) void k_round0(
         if (key < best[v]) atomicMin(&best[v], (unsigned long long)key);
       }
     }
-    // light keys to s_out[0, TL), heavy keys to s_out[TL, TL + TH): one scan
-    // of the two per-thread counts packed in 16-bit halves (each <= kGTile)
+    // light keys to the light chunk, heavy keys to the heavy chunk, in entry
+    // order per thread: one scan of the two counts packed in 16-bit halves
     const uint32_t nl = __popc(keep & light_m), nh = __popc(keep & ~light_m);
     uint32_t total;
     const uint32_t pre = block_excl_scan<uint32_t>(nl | (nh << 16), s_tmp, &total);
-    const uint32_t TL = total & 0xffffu, TH = total >> 16;
-    uint32_t pl = pre & 0xffffu, ph = TL + (pre >> 16);
+    uint32_t pl = pre & 0xffffu, ph = pre >> 16;
+    uint64_t* lc = light + c * kGTile;
+    uint64_t* hc = heavy + c * kGTile;
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
       if (!(keep & (1u << j))) continue;
-      if (light_m & (1u << j)) s_out[pl++] = keys[j];
-      else s_out[ph++] = keys[j];
-    }
-    __syncthreads();
-    uint64_t* lc = light + c * kGTile;
-    uint64_t* hc = heavy + c * kGTile;
-    for (uint32_t i = threadIdx.x; i < TL + TH; i += kGB) {
-      if (i < TL) lc[i] = s_out[i];
-      else hc[i - TL] = s_out[i];
+      if (light_m & (1u << j)) lc[pl++] = keys[j];
+      else hc[ph++] = keys[j];
     }
     if (threadIdx.x == 0) {
+      const uint32_t TL = total & 0xffffu, TH = total >> 16;
       cnt_light[c] = (int32_t)TL;
       cnt_heavy[c] = (int32_t)TH;
       sum_l += TL;
@@ -451,7 +472,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
         unsigned long long b = s_best[i];
         if (b == kInf) continue;
         int32_t r = cx.r_lo + i;
-        bool inside = s_beg[i] >= 0 && s_end[i] <= (int32_t)(cx.e1 - cx.e0);
+        bool inside = s_beg[i] >= 0 && s_end[i] <= (int32_t)(e1 - e0);
         if (inside && symmetric) best[r] = b;
         else atomicMin(&best[r], b);
       }
@@ -752,7 +773,7 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
   }
 }
 
-// ------------------------------------------------------------- union-find
+// ----------------------------------------------------- union-find kernels
 __global__ void k_iota(int32_t* p, int32_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
