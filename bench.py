@@ -329,6 +329,11 @@ def run_coassoc(args):
 
 
 def run_sl_batched(args):
+    """512 independent symmetric CSR graphs (vertex counts log-uniform in
+    [1,000, vmax], mean degree U[8, 64], int32 weights U[1, 256]) as one
+    disjoint union, generated on the GPU; one batched lifetime cut (eac.py
+    rule) per step, as an eager launch sequence and as one hipGraph replay,
+    against one call per graph."""
     import numpy as np
     import torch
 
@@ -337,57 +342,90 @@ def run_sl_batched(args):
                                                     lifetime_cut_batched)
 
     G = args.graphs
-    log(f"generating {G} k-NN graphs")
+    log(f"generating {G} random graphs on the GPU")
     t0 = time.perf_counter()
-    b = synth.graph_batch(G, (1000, args.vmax), (8, 64), seed=4, wmax=256, kind="knn")
-    log(f"batch: {b['n']} vertices, {b['dest'].size} entries, {time.perf_counter() - t0:.1f}s")
-    d_dest = torch.from_numpy(b["dest"]).cuda()
-    d_w = torch.from_numpy(b["weight"]).cuda()
-    d_ip = torch.from_numpy(b["indptr"]).cuda()
-    (lab, gout, info), ms = timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, b["goff"],
-                                                               rule=0), args.steps, args.warmup,
-                                  torch)
-    # plain: one call per graph
-    ips = [torch.from_numpy(b["indptr"][b["goff"][g]:b["goff"][g + 1] + 1] -
-                            b["indptr"][b["goff"][g]]).cuda() for g in range(G)]
-    dests = [d_dest[b["indptr"][b["goff"][g]]:b["indptr"][b["goff"][g + 1]]] - int(b["goff"][g])
-             for g in range(G)]
-    ws_ = [d_w[b["indptr"][b["goff"][g]]:b["indptr"][b["goff"][g + 1]]] for g in range(G)]
-
-    def plain():
-        return [lifetime_cut(dests[g], ws_[g], ips[g], None, rule=0) for g in range(G)]
-
-    rp, mp = timed(plain, 1, 1, torch)
-    # the same batched cut recorded as one hipGraph (round count tuned on the
-    # first replay), inputs resident in the recorded buffers: replay only
-    cg = LifetimeCutGraph(b["n"] + 1, int(b["dest"].size) + 1, G + 1, torch.int32, rule=0,
-                          tune=True)
-    cg.run(d_dest, d_w, d_ip, b["goff"])
-    cg.load(d_dest, d_w, d_ip, b["goff"])
+    b = synth.graph_batch_device(G, (1000, args.vmax), (8, 64), seed=4, wmax=256)
+    torch.cuda.synchronize()
+    d_dest, d_w, d_ip, goff = b["dest"], b["weight"], b["indptr"], b["goff"]
+    N, M = b["n"], int(d_dest.numel())
+    log(f"batch: {N} vertices, {M} entries, {time.perf_counter() - t0:.1f}s")
+    (lab, gout, info), ms = timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, goff, rule=0),
+                                  args.steps, args.warmup, torch)
+    log(f"batched eager {ms:.2f} ms")
+    # the same cut recorded as one hipGraph (rounds tuned on the first
+    # replay); the batch is loaded into the recorded buffers before timing
+    cg = LifetimeCutGraph(N + 1, M + 1, G + 1, torch.int32, rule=0, tune=True)
+    cg.run(d_dest, d_w, d_ip, goff)
+    cg.load(d_dest, d_w, d_ip, goff)
     _, mg = timed(cg.launch, args.steps, args.warmup, torch)
     s_g = cg.summary[:5].cpu().numpy()
-    same_graph = (int(s_g[4]) == 0 and
-                  bool(torch.equal(cg.labels[:b["n"]], lab)) and
+    same_graph = (int(s_g[4]) == 0 and bool(torch.equal(cg.labels[:N], lab)) and
                   bool(torch.equal(cg.gout[:G], gout)))
+    log(f"graph replay {mg:.2f} ms ({cg.rounds} recorded rounds), equal: {same_graph}")
+    # plain: one call per graph
+    ip_h = d_ip.cpu().numpy()
+    parts = []
+    for g in range(G):
+        a, z = int(goff[g]), int(goff[g + 1])
+        e0, e1 = int(ip_h[a]), int(ip_h[z])
+        parts.append((d_dest[e0:e1] - a, d_w[e0:e1], d_ip[a:z + 1] - e0))
+
+    def plain():
+        return [lifetime_cut(dd, ww, ii, None, rule=0) for dd, ww, ii in parts]
+
+    rp, mp = timed(plain, 1, 1, torch)
+    log(f"per-graph calls {mp:.2f} ms")
     same = True
     lab_h = lab.cpu().numpy()
+    gout_h = gout.cpu().numpy()
     for g in range(G):
-        a, z = b["goff"][g], b["goff"][g + 1]
+        a, z = goff[g], goff[g + 1]
         lg = rp[g][0]
         if lg is None:
-            same &= bool(gout[g, 1].item() == 1)
+            same &= bool(gout_h[g, 1] == 1)
         else:
             same &= bool(np.array_equal(lab_h[a:z], lg.cpu().numpy()))
-    m = int(b["dest"].size)
-    out = base_line(args, m / (ms * 1e-3) / 1e6, "Medges/s", ms, "int32",
-                    {"workload": "single_link_batched_knn_graphs", "graphs": G,
+    cpu = None
+    if not args.no_cpu:
+        # per-graph CPU reference (C restatement of eac.py:199-267 over scipy's
+        # Kruskal order) on the first graphs, about 10M entries
+        from oracle import oracle as O
+        dest_h = d_dest.cpu().numpy()
+        w_h = d_w.cpu().numpy()
+        done_e, secs, ok, ng = 0, 0.0, True, 0
+        for g in range(G):
+            a, z = int(goff[g]), int(goff[g + 1])
+            e0, e1 = int(ip_h[a]), int(ip_h[z])
+            fe = ip_h[a:z] - e0
+            od = np.diff(ip_h[a:z + 1])
+            c0 = time.perf_counter()
+            u, v, w, _ = O.csr_edges(dest_h[e0:e1] - a, w_h[e0:e1], fe, od)
+            try:
+                nc, want = O.sp_sl_lifetime(u, v, w, z - a)
+                ok &= bool(want.size == 0 and gout_h[g, 1] == 1) or \
+                    bool(np.array_equal(lab_h[a:z], want))
+            except ValueError:
+                ok &= bool(gout_h[g, 3] == -1)
+            secs += time.perf_counter() - c0
+            done_e += e1 - e0
+            ng += 1
+            if done_e >= 10_000_000:
+                break
+        cpu = {"value": done_e / secs / 1e6, "unit": "Medges/s", "cores": 1, "kind": "port",
+               "sample": f"first {ng} graphs ({done_e} entries), C restatement of the "
+                         "per-graph eac.py:199-267 sp_sl_lifetime (Kruskal order)",
+               "seconds": secs, "equal_to_gpu": ok}
+        log(f"CPU sample {ng} graphs {secs:.2f}s, equal {ok}")
+    out = base_line(args, M / (mg * 1e-3) / 1e6, "Medges/s", mg, "int32",
+                    {"workload": "single_link_batched_random_graphs", "graphs": G,
                      "v_range": [1000, args.vmax], "deg_range": [8, 64], "w_range": [1, 256],
-                     "seed": 4, "n_vertices": b["n"], "m_entries": m})
-    out["result"] = {"batched_ms": ms, "per_graph_calls_ms": mp, "speedup": mp / ms,
-                     "equal_to_per_graph": same, "graph_replay_ms": mg,
-                     "graph_recorded_rounds": cg.rounds, "graph_equal_to_batched": same_graph,
-                     "graph_speedup_vs_per_graph": mp / mg, **info}
-    out["cpu_baseline"] = None
+                     "seed": 4, "n_vertices": N, "m_entries": M,
+                     "step": "hipGraph replay of the batched lifetime cut (eac.py rule)"})
+    out["result"] = {"graph_replay_ms": mg, "batched_eager_ms": ms, "per_graph_calls_ms": mp,
+                     "speedup_vs_per_graph": mp / mg, "graph_recorded_rounds": cg.rounds,
+                     "graph_equal_to_batched": same_graph, "batched_equal_to_per_graph": same,
+                     **info}
+    out["cpu_baseline"] = cpu
     return out
 
 

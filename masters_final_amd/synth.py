@@ -142,3 +142,57 @@ def graph_batch(n_graphs, v_range, deg_range, seed, wmax=256, kind="knn", dtype=
                             [np.array([eoff[-1]])])
     return {"indptr": indptr.astype(np.int64), "dest": dest.astype(np.int32),
             "weight": weight, "goff": goff, "n": int(goff[-1]), "targets": deg}
+
+
+def graph_batch_device(n_graphs, v_range, deg_range, seed, wmax=256, device="cuda"):
+    """Large disjoint-union batches generated on the GPU (torch): graph g has
+    a vertex count log-uniform in v_range and n_g * deg_g / 2 undirected
+    Erdos-Renyi edges (deg_g uniform in deg_range; self loops and repeats
+    dropped), int32 weights U[1, wmax] per undirected edge, stored in both
+    rows of a symmetric CSR sorted by (row, column). Sizes come from numpy
+    PCG64(seed), edges from a torch CUDA generator seeded with `seed`.
+    Returns dict(indptr int64, dest int32, weight int32 (device tensors),
+    goff (host int64), n, targets)."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(v_range[0]), np.log(v_range[1])
+    nv = np.exp(rng.uniform(lo, hi, n_graphs)).astype(np.int64)
+    deg = rng.uniform(deg_range[0], deg_range[1], n_graphs)
+    me = np.maximum((nv * deg / 2).astype(np.int64), 1)
+    goff = np.zeros(n_graphs + 1, np.int64)
+    np.cumsum(nv, out=goff[1:])
+    N = int(goff[-1])
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    dev = torch.device(device)
+    g_of = torch.repeat_interleave(torch.arange(n_graphs, device=dev),
+                                   torch.from_numpy(me).to(dev))
+    base = torch.from_numpy(goff[:-1]).to(dev)[g_of]
+    size = torch.from_numpy(nv).to(dev)[g_of]
+    u = base + (torch.rand(g_of.numel(), generator=gen, device=dev, dtype=torch.float64)
+                * size).to(torch.int64).clamp_max(size - 1)
+    v = base + (torch.rand(g_of.numel(), generator=gen, device=dev, dtype=torch.float64)
+                * size).to(torch.int64).clamp_max(size - 1)
+    del g_of, base, size
+    keep = u != v
+    a = torch.minimum(u, v)[keep]
+    b = torch.maximum(u, v)[keep]
+    del u, v, keep
+    key = torch.unique(a * N + b)  # sorted
+    del a, b
+    a, b = key // N, key % N
+    del key
+    w = torch.randint(1, wmax + 1, (a.numel(),), generator=gen, device=dev, dtype=torch.int32)
+    rows = torch.cat([a, b])
+    cols = torch.cat([b, a])
+    ww = torch.cat([w, w])
+    del a, b, w
+    order = torch.sort(rows * N + cols).indices
+    rows, cols, ww = rows[order], cols[order], ww[order]
+    del order
+    indptr = torch.zeros(N + 1, dtype=torch.int64, device=dev)
+    indptr[1:] = torch.cumsum(torch.bincount(rows, minlength=N), 0)
+    del rows
+    return {"indptr": indptr, "dest": cols.to(torch.int32), "weight": ww, "goff": goff,
+            "n": N, "targets": deg}

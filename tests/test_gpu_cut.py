@@ -189,3 +189,41 @@ def test_batched_graph_tuned_rounds_fall_back(torch):
     lab, gout, info = tuned.run(b["dest"], b["weight"], b["indptr"], b["goff"])
     assert np.array_equal(lab.cpu().numpy(), want_lab.cpu().numpy())
     assert np.array_equal(gout.cpu().numpy(), want_gout.cpu().numpy())
+
+
+def test_device_batch_generator_and_batched_cut(torch):
+    """The GPU batch generator of the large batched workload gives symmetric
+    CSR graphs (rows sorted, weights mirrored, no self loops); the batched
+    cut over it equals the per-graph oracle."""
+    from masters_final_amd.graph.singlelink import lifetime_cut_batched
+    b = synth.graph_batch_device(9, (50, 4000), (4, 20), seed=5, wmax=256)
+    ip = b["indptr"].cpu().numpy()
+    dest = b["dest"].cpu().numpy().astype(np.int64)
+    w = b["weight"].cpu().numpy()
+    N = b["n"]
+    rows = np.repeat(np.arange(N), np.diff(ip))
+    assert np.all(rows != dest)
+    key = rows * N + dest
+    assert np.all(np.diff(key) > 0)  # sorted by (row, column), no repeats
+    mirror = np.searchsorted(key, dest * N + rows)
+    assert np.array_equal(key[mirror], dest * N + rows)
+    assert np.array_equal(w[mirror], w)
+    goff = b["goff"]
+    gid = np.searchsorted(goff, rows, side="right") - 1
+    assert np.all((dest >= goff[gid]) & (dest < goff[gid + 1]))
+    labels, gout, _ = lifetime_cut_batched(b["dest"], b["weight"], b["indptr"], goff, rule=0)
+    labels, gout = labels.cpu().numpy(), gout.cpu().numpy()
+    u, v, ww, _ = O.csr_edges(dest, w, ip[:-1], np.diff(ip))
+    for g in range(len(goff) - 1):
+        a, z = goff[g], goff[g + 1]
+        sel = (u >= a) & (u < z)
+        try:
+            nc, want = O.sp_sl_lifetime(u[sel] - a, v[sel] - a, ww[sel], z - a)
+        except ValueError:
+            assert gout[g, 3] == -1
+            continue
+        if want.size == 0:
+            assert gout[g, 1] == 1
+            continue
+        assert gout[g, 0] == nc
+        assert np.array_equal(labels[a:z], want), g
