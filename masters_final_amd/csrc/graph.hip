@@ -517,31 +517,39 @@ __device__ void uf_unite(int32_t* P, int32_t a, int32_t b) {
   }
 }
 
-// Links the components of a and b; returns the root that was hooked (the
-// larger one), or -1 when they were already one component. A root is
-// hooked at most once in its lifetime.
-__device__ int32_t uf_link(int32_t* P, int32_t a, int32_t b) {
-  a = uf_find(P, a);
-  b = uf_find(P, b);
-  while (a != b) {
-    int32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    int32_t old = atomicCAS(P + hi, hi, lo);
-    if (old == hi) return hi;
-    a = uf_find(P, old);
-    b = uf_find(P, lo);
+// Root lookups of the Borůvka rounds. Hooks only ever point a root at
+// another current root along the round's lightest-edge choices, which
+// contain no cycle for a symmetric input; a walk longer than n therefore
+// means a graph passed as symmetric is not, and the walk stops with flag 4.
+__device__ int32_t find_live(int32_t* P, int32_t x, int32_t n, unsigned int* overflow) {
+  for (int32_t steps = 0;; ++steps) {
+    int32_t p = uf_ld(P, x);
+    if (p == x) return x;
+    int32_t gp = uf_ld(P, p);
+    if (gp == p) return p;
+    uf_st(P, x, gp);  // path halving
+    x = gp;
+    if (steps > n) {
+      atomicOr(overflow, 4u);
+      return x;
+    }
   }
-  return -1;
 }
-// Root lookup in a kernel where no link happens: plain loads (a stale value
+// Root lookup in a kernel where no hook happens: plain loads (a stale value
 // is still an ancestor) and path splitting with plain stores (any ancestor
 // is a valid parent).
-__device__ __forceinline__ int32_t find_fast(int32_t* P, int32_t x) {
+__device__ __forceinline__ int32_t find_fast(int32_t* P, int32_t x, int32_t n,
+                                             unsigned int* overflow) {
   int32_t p = P[x];
-  while (p != x) {
+  for (int32_t steps = 0; p != x; ++steps) {
     const int32_t gp = P[p];
     if (gp != p) P[x] = gp;
     x = p;
     p = gp;
+    if (steps > n) {
+      atomicOr(overflow, 4u);
+      break;
+    }
   }
   return x;
 }
@@ -577,7 +585,8 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 }
 
 // Filter one list in place against the current components and atomicMin
-// every cross edge into both components' slots. One wavefront per chunk
+// every cross edge into both roots' slots of the next round (`best` is that
+// round's slot array). One wavefront per chunk
 // (4 x 64 keys in flight per step); survivors are written back in order at
 // the front of the chunk (a write never passes the read position).
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
@@ -587,8 +596,9 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                int64_t nchunks, RoundCtl* ctl,
                                                const unsigned int* __restrict__ had_edge,
                                                int32_t* P, unsigned long long* __restrict__ best,
-                                               int vb, int max_rounds,
-                                               unsigned long long* __restrict__ rlog) {
+                                               int32_t n, int vb, int max_rounds,
+                                               unsigned long long* __restrict__ rlog,
+                                               unsigned int* __restrict__ overflow) {
   __shared__ unsigned long long s_io[2 * (kGB / kWave)];
   const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
@@ -620,7 +630,7 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
         if (k[q] == kInf) continue;
         int32_t a, b;
         unpack_ends(k[q], vb, &a, &b);
-        const int32_t ra = find_fast(P, a), rb = find_fast(P, b);
+        const int32_t ra = find_fast(P, a, n, overflow), rb = find_fast(P, b, n, overflow);
         if (ra == rb) continue;
         keep[q] = true;
         if (k[q] < best[ra]) atomicMin(&best[ra], (unsigned long long)k[q]);
@@ -654,28 +664,37 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   }
 }
 
-// Every component slot that holds a candidate key (the component's
-// lightest edge to another component) links the edge's two components;
-// each successful link records the edge at the root it hooked, so an edge
-// that is the lightest of both its components (mst.py:282-305) is recorded
-// once. The lightest edges of one round form a forest, so apart from such
-// shared edges every candidate links. Slots are cleared for the next round.
+// Every root v whose slot holds a candidate k (its component's lightest edge
+// to another component) follows k to the other side's current root s. If s
+// picked the same edge and v < s, v stays a root (mst.py:282-305: the
+// smaller id keeps no edge); otherwise v hooks under s and k joins the
+// forest, recorded in v's slot (a root hooks at most once in its lifetime,
+// so forest slots are written once and their order does not depend on
+// timing). Slots of round r are read-only here (the next round's slots are
+// cleared instead), so the mirror test sees every choice of the round.
+// If s's side already hooked under v (the mirror case with v < s), v stays.
 __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
-                                              unsigned long long* __restrict__ best, int32_t* P,
-                                              uint64_t* __restrict__ mst_key,
-                                              unsigned int* __restrict__ had_edge, int vb) {
+                                              const unsigned long long* __restrict__ best,
+                                              unsigned long long* __restrict__ best_next,
+                                              int32_t* P, uint64_t* __restrict__ mst_key,
+                                              unsigned int* __restrict__ had_edge, int vb,
+                                              unsigned int* __restrict__ overflow) {
   if (!ctl[r & 1].active) return;
   bool any = false;
   for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * kGB) {
+    best_next[v] = kInf;
     const uint64_t k = best[v];
     if (k == kInf) continue;
     any = true;
-    best[v] = kInf;
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
-    const int32_t h = uf_link(P, a, b);
-    if (h >= 0) mst_key[h] = k;
+    const int32_t ra = find_live(P, a, n, overflow), rb = find_live(P, b, n, overflow);
+    const int32_t s = (ra == (int32_t)v) ? rb : ra;
+    if (s == (int32_t)v) continue;
+    if (best[s] == k && (int32_t)v < s) continue;  // mirror pick: v stays
+    uf_st(P, (int32_t)v, s);
+    mst_key[v] = k;
   }
   if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
 }
@@ -840,6 +859,15 @@ __global__ void k_uf_flatten(int32_t* P, int32_t n, int32_t* __restrict__ root,
   }
 }
 
+// Roots of the Borůvka parents (which need not satisfy parent <= child):
+// bounded walks, flag 4 on a cycle.
+__global__ void k_flatten_mst(int32_t* P, int32_t n, int32_t* __restrict__ root,
+                              unsigned int* __restrict__ overflow) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    root[i] = find_live(P, (int32_t)i, n, overflow);
+}
+
 __global__ void k_gather_rank(const int32_t* __restrict__ root,
                               const int32_t* __restrict__ rank, int32_t n,
                               int32_t* __restrict__ out) {
@@ -958,7 +986,7 @@ inline unsigned long long fold_slots(const unsigned long long* v) {
 
 struct MstBuffers {
   int32_t* chunk_row;
-  unsigned long long* best;
+  unsigned long long* best[2];  // candidate slots, by round parity
   uint64_t* mst_key;
   int32_t* comp;
   int32_t* hook;
@@ -980,7 +1008,8 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   MstBuffers b;
   const int64_t nchunks = (m + kGTile - 1) / kGTile;
   b.chunk_row = ar.take<int32_t>(nchunks + 1);
-  b.best = ar.take<unsigned long long>(n);
+  b.best[0] = ar.take<unsigned long long>(n);
+  b.best[1] = ar.take<unsigned long long>(n);
   b.mst_key = ar.take<uint64_t>(n);
   b.comp = ar.take<int32_t>(n);
   b.hook = ar.take<int32_t>(n);
@@ -1098,7 +1127,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   k_plan<<<1, kWave, 0, s>>>(B.st->range_lo, B.st->range_hi, B.st->hist, vb, try_split ? 1 : 0,
                              target, &B.st->plan);
   EAC_LAUNCH_CHECK();
-  EAC_TRY(fill<uint64_t>((uint64_t*)B.best, n, kInf, s));
+  EAC_TRY(fill<uint64_t>((uint64_t*)B.best[0], n, kInf, s));  // hook(0) clears best[1]
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
   k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);  // union-find parents
   EAC_LAUNCH_CHECK();
@@ -1109,7 +1138,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       8 * (int64_t)n;
   int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
-                                               vb, &B.st->plan, symmetric, B.best, B.light,
+                                               vb, &B.st->plan, symmetric, B.best[0], B.light,
                                                B.heavy, B.cnt_light, B.cnt_heavy,
                                                prof_on() ? &B.st->tot0[0][0] : nullptr,
                                                &B.st->overflow);
@@ -1127,14 +1156,15 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     for (int b = 0; b < batch && enq < kMaxRounds; ++b, ++enq) {
       const int r = enq;
       rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
-      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best, B.comp, B.mst_key, B.st->had_edge,
-                                vb);
+      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
+                                B.mst_key, B.st->had_edge, vb, &B.st->overflow);
       EAC_LAUNCH_CHECK();
       prof_end(rec_hook.back(), s);
       rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
       k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, nchunks,
-                                 B.st->ctl, B.st->had_edge, B.comp, B.best, vb, kMaxRounds,
-                                 prof_on() ? &B.st->rlog[0][0][0] : nullptr);
+                                 B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
+                                 kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
+                                 &B.st->overflow);
       EAC_LAUNCH_CHECK();
       prof_end(rec_round.back(), s);
     }
@@ -1145,6 +1175,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                   "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
                   hs.plan.wbits, vb);
     if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+    if (hs.overflow & 4u)
+      return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
+                               "(pass symmetric=0)");
     const RoundCtl& c = hs.ctl[enq & 1];
     if (c.overflow_rounds || (c.active && enq >= kMaxRounds))
       return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
@@ -1211,12 +1244,19 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   }
   int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
   if (labels) {
-    // the union-find root of a component is its smallest vertex, so the
-    // roots in vertex order number the components by first occurrence
+    // flatten the parents to roots, then number components by their
+    // smallest vertex (first occurrence)
+    int32_t* rootv = B.roots[1];
+    int32_t* first = B.roots[0];
     int32_t* root = B.hook;
     const int64_t ftiles = (n + kGTile - 1) / kGTile;
     int g = grid_for(n, 256, 4096);
-    k_uf_flatten<<<g, 256, 0, s>>>(B.comp, n, root, B.flag);
+    k_flatten_mst<<<g, 256, 0, s>>>(B.comp, n, rootv, &B.st->overflow);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY(fill<int32_t>(first, n, INT32_MAX, s));
+    k_first_occ<<<grid_for(n, kFoTile, 4096), 256, 0, s>>>(rootv, n, n, first, &B.st->missing);
+    EAC_LAUNCH_CHECK();
+    k_first_flag<<<g, 256, 0, s>>>(rootv, first, n, n, root, B.flag);
     EAC_LAUNCH_CHECK();
     EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles, ar, s)));
     k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
@@ -1713,7 +1753,7 @@ __global__ void k_goff_prep(const int64_t* __restrict__ in, int G, int32_t N,
 // Run summary of a batched cut, written on the device: {components,
 // forest edges (nonzero weight under rule 0), Boruvka rounds, kept edges,
 // error flags}. Flags: 1 edge list overflow, 2 dest ids outside [0, N),
-// 4 unused, 8 key wider than 64 bits,
+// 4 graph passed as symmetric is not, 8 key wider than 64 bits,
 // 16 rounds not converged, 32 bad graph offsets.
 __global__ void k_batch_summary(const MstState* __restrict__ st, int enq,
                                 const int32_t* __restrict__ d_ncomp,
@@ -1725,7 +1765,7 @@ __global__ void k_batch_summary(const MstState* __restrict__ st, int enq,
   const RoundCtl c = st->ctl[enq & 1];
   out[2] = c.rounds;
   out[3] = misc[0];
-  int64_t flags = st->overflow & 3u;
+  int64_t flags = st->overflow & 7u;
   if (misc[1]) flags |= 32;
   if (st->plan.err) flags |= 8;
   if (c.active || c.overflow_rounds) flags |= 16;
@@ -1737,6 +1777,8 @@ int summary_error(int64_t flags) {
   if (flags & 32) return fail(EAC_ERR_ARG, "sl_batched: graph offsets not 0 <= ... <= N");
   if (flags & 8) return fail(EAC_ERR_KEY_WIDTH, "boruvka: weight range and vertex ids need > 64 key bits");
   if (flags & 2) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+  if (flags & 4)
+    return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric (pass symmetric=0)");
   if (flags & 1)
     return fail(EAC_ERR_ARG, "boruvka: more edges with dest > row than a symmetric CSR has; "
                              "pass symmetric=0");
