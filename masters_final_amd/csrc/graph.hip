@@ -586,9 +586,9 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 
 // Filter one list in place against the current components and atomicMin
 // every cross edge into both roots' slots of the next round (`best` is that
-// round's slot array). One wavefront per chunk
-// (4 x 64 keys in flight per step); survivors are written back in order at
-// the front of the chunk (a write never passes the read position).
+// round's slot array). A wavefront filters one chunk at a time (4 x 64 keys
+// in flight per step); survivors are written back in order at the front of
+// the chunk (a write never passes the read position).
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
@@ -607,46 +607,57 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   int32_t* cnt = c.phase ? cnt_heavy : cnt_light;
   const int lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
+  // each wavefront owns a contiguous run of chunks and reads their counts
+  // 64 at a time, so empty chunks (most of them in late rounds) cost nothing
   const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
+  const int64_t wid = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave;
+  const int64_t per = (nchunks + nwaves - 1) / nwaves;
+  const int64_t c_begin = wid * per;
+  const int64_t c_end = c_begin + per < nchunks ? c_begin + per : nchunks;
   unsigned long long sum_in = 0, sum_out = 0;
-  for (int64_t cc = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave; cc < nchunks;
-       cc += nwaves) {
-    int32_t n_in = cnt[cc];
-    n_in = n_in < 0 ? 0 : (n_in > kGTile ? kGTile : n_in);
-    if (n_in == 0) continue;  // uniform over the wave
-    uint64_t* lc = list + cc * kGTile;
-    int32_t out = 0;
-    for (int base = 0; base < n_in; base += 4 * kWave) {
-      uint64_t k[4];
+  for (int64_t g0 = c_begin; g0 < c_end; g0 += kWave) {
+    int32_t myn = g0 + lane < c_end ? cnt[g0 + lane] : 0;
+    myn = myn < 0 ? 0 : (myn > kGTile ? kGTile : myn);
+    uint64_t pending = __ballot(myn > 0);
+    while (pending) {
+      const int l = __ffsll((unsigned long long)pending) - 1;
+      pending &= pending - 1;
+      const int64_t cc = g0 + l;
+      const int32_t n_in = __shfl(myn, l, kWave);
+      uint64_t* lc = list + cc * kGTile;
+      int32_t out = 0;
+      for (int base = 0; base < n_in; base += 4 * kWave) {
+        uint64_t k[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int e = base + q * kWave + lane;
-        k[q] = e < n_in ? lc[e] : kInf;
-      }
-      bool keep[4];
+        for (int q = 0; q < 4; ++q) {
+          const int e = base + q * kWave + lane;
+          k[q] = e < n_in ? lc[e] : kInf;
+        }
+        bool keep[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        keep[q] = false;
-        if (k[q] == kInf) continue;
-        int32_t a, b;
-        unpack_ends(k[q], vb, &a, &b);
-        const int32_t ra = find_fast(P, a, n, overflow), rb = find_fast(P, b, n, overflow);
-        if (ra == rb) continue;
-        keep[q] = true;
-        if (k[q] < best[ra]) atomicMin(&best[ra], (unsigned long long)k[q]);
-        if (k[q] < best[rb]) atomicMin(&best[rb], (unsigned long long)k[q]);
-      }
+        for (int q = 0; q < 4; ++q) {
+          keep[q] = false;
+          if (k[q] == kInf) continue;
+          int32_t a, b;
+          unpack_ends(k[q], vb, &a, &b);
+          const int32_t ra = find_fast(P, a, n, overflow), rb = find_fast(P, b, n, overflow);
+          if (ra == rb) continue;
+          keep[q] = true;
+          if (k[q] < best[ra]) atomicMin(&best[ra], (unsigned long long)k[q]);
+          if (k[q] < best[rb]) atomicMin(&best[rb], (unsigned long long)k[q]);
+        }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint64_t m = __ballot(keep[q]);
-        if (keep[q]) lc[out + __popcll(m & below)] = k[q];
-        out += __popcll(m);
+        for (int q = 0; q < 4; ++q) {
+          const uint64_t m = __ballot(keep[q]);
+          if (keep[q]) lc[out + __popcll(m & below)] = k[q];
+          out += __popcll(m);
+        }
       }
-    }
-    if (lane == 0) {
-      cnt[cc] = out;
-      sum_in += (unsigned long long)n_in;
-      sum_out += (unsigned long long)out;
+      if (lane == 0) {
+        cnt[cc] = out;
+        sum_in += (unsigned long long)n_in;
+        sum_out += (unsigned long long)out;
+      }
     }
   }
   if (rlog && r < kMaxRounds) {  // profiling only: per-round traffic
