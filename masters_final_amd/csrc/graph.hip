@@ -394,8 +394,8 @@ __global__ __launch_bounds__(kGB) void k_round0(
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
       const int64_t e = e0 + j * kGB + threadIdx.x;
-      dv[j] = e < e1 ? dest[e] : -1;
-      wv[j] = e < e1 ? weight[e] : W(0);
+      dv[j] = e < e1 ? __builtin_nontemporal_load(dest + e) : -1;  // streamed once
+      wv[j] = e < e1 ? __builtin_nontemporal_load(weight + e) : W(0);
     }
     ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
     if (cx.staged) {
@@ -589,6 +589,25 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 // round's slot array). A wavefront filters one chunk at a time (4 x 64 keys
 // in flight per step); survivors are written back in order at the front of
 // the chunk (a write never passes the read position).
+// Per-block table of (root, lightest key) pairs: cross edges of one block
+// that share a root meet in LDS, and each distinct root costs one global
+// atomicMin per block (a component touching most edges -- the giant one --
+// would otherwise take one serialised atomic per edge).
+constexpr int kMinSlots = 2048;
+__device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* s_min,
+                                           int32_t root, uint64_t key,
+                                           unsigned long long* best) {
+  uint32_t h = ((uint32_t)root * 2654435761u) >> 21;  // 11 bits: kMinSlots
+  for (int p = 0; p < 8; ++p, h = (h + 1) & (kMinSlots - 1)) {
+    const int32_t k = atomicCAS(&s_root[h], -1, root);
+    if (k == -1 || k == root) {
+      atomicMin(&s_min[h], (unsigned long long)key);
+      return;
+    }
+  }
+  atomicMin(&best[root], (unsigned long long)key);  // table crowded: direct
+}
+
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
@@ -599,10 +618,17 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                int32_t n, int vb, int max_rounds,
                                                unsigned long long* __restrict__ rlog,
                                                unsigned int* __restrict__ overflow) {
+  __shared__ int32_t s_root[kMinSlots];
+  __shared__ unsigned long long s_min[kMinSlots];
   __shared__ unsigned long long s_io[2 * (kGB / kWave)];
   const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
+  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
+    s_root[i] = -1;
+    s_min[i] = kInf;
+  }
+  __syncthreads();
   uint64_t* list = c.phase ? heavy : light;
   int32_t* cnt = c.phase ? cnt_heavy : cnt_light;
   const int lane = lane_id();
@@ -631,7 +657,7 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int e = base + q * kWave + lane;
-          k[q] = e < n_in ? lc[e] : kInf;
+          k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
         }
         bool keep[4];
 #pragma unroll
@@ -643,8 +669,8 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
           const int32_t ra = find_fast(P, a, n, overflow), rb = find_fast(P, b, n, overflow);
           if (ra == rb) continue;
           keep[q] = true;
-          if (k[q] < best[ra]) atomicMin(&best[ra], (unsigned long long)k[q]);
-          if (k[q] < best[rb]) atomicMin(&best[rb], (unsigned long long)k[q]);
+          min_insert(s_root, s_min, ra, k[q], best);
+          min_insert(s_root, s_min, rb, k[q], best);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -659,6 +685,15 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
         sum_out += (unsigned long long)out;
       }
     }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
+    const int32_t root = s_root[i];
+    if (root < 0) continue;
+    const unsigned long long key = s_min[i];
+    // a fresh read (past this XCD's L2) skips most losing atomics
+    if (key < __hip_atomic_load(&best[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMin(&best[root], key);
   }
   if (rlog && r < kMaxRounds) {  // profiling only: per-round traffic
     const int w = threadIdx.x / kWave;
