@@ -381,6 +381,21 @@ __global__ __launch_bounds__(kGB) void k_round0(
   const bool err = plan->err != 0;
   const uint64_t word_lo = plan->wlo, light_lim = plan->light_lim;
   unsigned long long sum_l = 0, sum_h = 0;
+  // a chunk's entries are loaded one iteration ahead (software pipelining:
+  // their latency overlaps the previous chunk's scan, writes and row staging)
+  int32_t dv[kGI];
+  W wv[kGI];
+  auto load_entries = [&](int64_t cc) {
+    const int64_t f0 = cc * kGTile;
+    const int64_t f1 = f0 + kGTile < m ? f0 + kGTile : m;
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      const int64_t e = f0 + j * kGB + threadIdx.x;
+      dv[j] = e < f1 ? __builtin_nontemporal_load(dest + e) : -1;  // streamed once
+      wv[j] = e < f1 ? __builtin_nontemporal_load(weight + e) : W(0);
+    }
+  };
+  if (!err && blockIdx.x < nchunks) load_entries(blockIdx.x);
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
     if (err) {  // key too wide: nothing is listed, the host reports the error
       if (threadIdx.x == 0) { cnt_light[c] = 0; cnt_heavy[c] = 0; }
@@ -388,15 +403,6 @@ __global__ __launch_bounds__(kGB) void k_round0(
     }
     const int64_t e0 = c * kGTile;
     const int64_t e1 = e0 + kGTile < m ? e0 + kGTile : m;
-    // the entries first: their loads overlap the row staging
-    int32_t dv[kGI];
-    W wv[kGI];
-#pragma unroll
-    for (int j = 0; j < kGI; ++j) {
-      const int64_t e = e0 + j * kGB + threadIdx.x;
-      dv[j] = e < e1 ? __builtin_nontemporal_load(dest + e) : -1;  // streamed once
-      wv[j] = e < e1 ? __builtin_nontemporal_load(weight + e) : W(0);
-    }
     ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
     if (cx.staged) {
       for (int i = threadIdx.x; i < kGTile; i += kGB) s_row[i] = -1;
@@ -446,6 +452,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
         if (key < best[v]) atomicMin(&best[v], (unsigned long long)key);
       }
     }
+    if (c + gridDim.x < nchunks) load_entries(c + gridDim.x);
     // light keys to the light chunk, heavy keys to the heavy chunk, in entry
     // order per thread: one scan of the two counts packed in 16-bit halves
     const uint32_t nl = __popc(keep & light_m), nh = __popc(keep & ~light_m);
@@ -535,23 +542,61 @@ __device__ int32_t find_live(int32_t* P, int32_t x, int32_t n, unsigned int* ove
     }
   }
 }
-// Root lookup in a kernel where no hook happens: plain loads (a stale value
-// is still an ancestor) and path splitting with plain stores (any ancestor
-// is a valid parent).
-__device__ __forceinline__ int32_t find_fast(int32_t* P, int32_t x, int32_t n,
-                                             unsigned int* overflow) {
-  int32_t p = P[x];
-  for (int32_t steps = 0; p != x; ++steps) {
-    const int32_t gp = P[p];
-    if (gp != p) P[x] = gp;
-    x = p;
-    p = gp;
+// Roots of two vertices with both walks in flight together, in a kernel
+// where hooks happen (fresh loads; a start vertex is then pointed at the
+// root it found, which is an ancestor whatever hooks follow).
+__device__ __forceinline__ void roots_live2(int32_t* P, int32_t* a, int32_t* b, int32_t n,
+                                            unsigned int* overflow) {
+  int32_t x = *a, y = *b;
+  for (int32_t steps = 0;; ++steps) {
+    const int32_t px = uf_ld(P, x), py = uf_ld(P, y);
+    if (px == x && py == y) break;
+    x = px;
+    y = py;
     if (steps > n) {
       atomicOr(overflow, 4u);
       break;
     }
   }
-  return x;
+  if (x != *a && uf_ld(P, *a) != x) uf_st(P, *a, x);
+  if (y != *b && uf_ld(P, *b) != y) uf_st(P, *b, y);
+  *a = x;
+  *b = y;
+}
+
+// Roots of N vertices at once, in a kernel where no hook happens: the N
+// walks advance one level per step, so their loads are in flight together
+// (a chain of N sequential walks would pay N times the memory latency).
+// Each start vertex is then pointed at its root (plain store, any ancestor
+// is a valid parent).
+template <int N>
+__device__ __forceinline__ void roots_fast(int32_t* P, int32_t (&x)[N], int32_t n,
+                                           unsigned int* overflow) {
+  int32_t cur[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) cur[i] = x[i];
+  for (int32_t steps = 0;; ++steps) {
+    int32_t p[N];
+    bool moved = false;
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = cur[i] >= 0 ? P[cur[i]] : cur[i];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      if (p[i] != cur[i]) {
+        cur[i] = p[i];
+        moved = true;
+      }
+    if (!moved) break;
+    if (steps > n) {
+      atomicOr(overflow, 4u);
+      break;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if (x[i] >= 0 && cur[i] != x[i] && P[x[i]] != cur[i]) P[x[i]] = cur[i];
+    x[i] = cur[i];
+  }
 }
 
 // ------------------------------------------------------ device round control
@@ -659,18 +704,24 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
           const int e = base + q * kWave + lane;
           k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
         }
+        int32_t x[8];  // endpoints of the 4 keys (-1: no key)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (k[q] == kInf) {
+            x[2 * q] = -1;
+            x[2 * q + 1] = -1;
+          } else {
+            unpack_ends(k[q], vb, &x[2 * q], &x[2 * q + 1]);
+          }
+        }
+        roots_fast<8>(P, x, n, overflow);
         bool keep[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          keep[q] = false;
-          if (k[q] == kInf) continue;
-          int32_t a, b;
-          unpack_ends(k[q], vb, &a, &b);
-          const int32_t ra = find_fast(P, a, n, overflow), rb = find_fast(P, b, n, overflow);
-          if (ra == rb) continue;
-          keep[q] = true;
-          min_insert(s_root, s_min, ra, k[q], best);
-          min_insert(s_root, s_min, rb, k[q], best);
+          keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
+          if (!keep[q]) continue;
+          min_insert(s_root, s_min, x[2 * q], k[q], best);
+          min_insert(s_root, s_min, x[2 * q + 1], k[q], best);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -733,9 +784,9 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
     const uint64_t k = best[v];
     if (k == kInf) continue;
     any = true;
-    int32_t a, b;
-    unpack_ends(k, vb, &a, &b);
-    const int32_t ra = find_live(P, a, n, overflow), rb = find_live(P, b, n, overflow);
+    int32_t ra, rb;
+    unpack_ends(k, vb, &ra, &rb);
+    roots_live2(P, &ra, &rb, n, overflow);
     const int32_t s = (ra == (int32_t)v) ? rb : ra;
     if (s == (int32_t)v) continue;
     if (best[s] == k && (int32_t)v < s) continue;  // mirror pick: v stays
