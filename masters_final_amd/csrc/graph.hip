@@ -43,6 +43,8 @@
 
 #include <type_traits>
 #include <utility>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "prims.hpp"
@@ -638,11 +640,11 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 // that share a root meet in LDS, and each distinct root costs one global
 // atomicMin per block (a component touching most edges -- the giant one --
 // would otherwise take one serialised atomic per edge).
-constexpr int kMinSlots = 2048;
+constexpr int kMinSlots = 1024;
 __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* s_min,
                                            int32_t root, uint64_t key,
                                            unsigned long long* best) {
-  uint32_t h = ((uint32_t)root * 2654435761u) >> 21;  // 11 bits: kMinSlots
+  uint32_t h = ((uint32_t)root * 2654435761u) >> 22;  // 10 bits: kMinSlots
   for (int p = 0; p < 8; ++p, h = (h + 1) & (kMinSlots - 1)) {
     const int32_t k = atomicCAS(&s_root[h], -1, root);
     if (k == -1 || k == root) {
@@ -738,12 +740,16 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
+#pragma unroll
+  for (int j = 0; j < kMinSlots / kGB; ++j) {
+    const int i = j * kGB + threadIdx.x;
     const int32_t root = s_root[i];
     if (root < 0) continue;
     const unsigned long long key = s_min[i];
-    // a fresh read (past this XCD's L2) skips most losing atomics
-    if (key < __hip_atomic_load(&best[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    // a cached value is an upper bound of the current one, a fresh read
+    // (past this XCD's L2) is exact: both skip losing atomics
+    if (key < best[root] &&
+        key < __hip_atomic_load(&best[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       atomicMin(&best[root], key);
   }
   if (rlog && r < kMaxRounds) {  // profiling only: per-round traffic
@@ -1040,6 +1046,32 @@ __global__ void k_first_flag(const int32_t* __restrict__ lab, const int32_t* __r
 // ---------------------------------------------------------------- helpers
 inline int persistent_blocks() { return 256 * 8; }
 
+// Grid of a persistent (grid-stride) kernel: every block resident at once,
+// i.e. compute units x the blocks per unit that the kernel's registers and
+// LDS allow (the runtime's occupancy query), cached per kernel. A larger
+// grid runs in several block waves, the last one mostly idle.
+template <typename K>
+int resident_grid(K kernel, int block) {
+  static std::mutex mu;
+  static std::unordered_map<const void*, int> cache;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  int dev = 0, cus = 256, per = 1;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess ||
+      per < 1)
+    per = 1;
+  const int g = cus * per;
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = g;
+  return g;
+}
+
 int d2h_sync(void* host, const void* dev, size_t bytes, hipStream_t s) {
   void* pin = pinned_scratch(bytes);
   if (!pin) return fail(EAC_ERR_HIP, "pinned host allocation failed");
@@ -1203,8 +1235,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_LAUNCH_CHECK();
   k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(fe, n, m, nchunks, B.chunk_row);
   EAC_LAUNCH_CHECK();
-  const int gchunks = grid_for(nchunks, 1, persistent_blocks());
-  k_weight_range<W, HAS_OD><<<gchunks, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
+  const int gchunks = grid_for(nchunks, 1, resident_grid(k_round0<W, HAS_OD>, kGB));
+  const int grange = grid_for(nchunks, 1, resident_grid(k_weight_range<W, HAS_OD>, kGB));
+  k_weight_range<W, HAS_OD><<<grange, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
                                                      B.st->range_lo, B.st->range_hi);
   EAC_LAUNCH_CHECK();
   // Light/heavy split (Filter-Kruskal applied to Boruvka): rounds run first
@@ -1243,8 +1276,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   prof_end(pr, s);
   // Rounds, enqueued in batches; the round control stops the sequence on
   // the device (later kernels of a finished run exit at once).
-  const int gh = grid_for(n, kGB, persistent_blocks());
-  const int gr = grid_for(nchunks, 1, persistent_blocks());
+  const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
+  const int gr = grid_for(nchunks, 1, resident_grid(k_round, kGB));
   const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
   std::vector<int> rec_hook, rec_round;
   int enq = 0, batch = graph_mode ? fixed_rounds : 6;
@@ -1335,7 +1368,9 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
   const int64_t n_f = fi.n_f;
   if (n_f > 0 && (ids || wout)) {
-    k_forest_positions<W, HAS_OD><<<grid_for(8 * n_f, kGB, persistent_blocks()), kGB, 0, s>>>(
+    k_forest_positions<W, HAS_OD><<<grid_for(8 * n_f, kGB,
+                                             resident_grid(k_forest_positions<W, HAS_OD>, kGB)),
+                                    kGB, 0, s>>>(
         B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
