@@ -199,8 +199,28 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
   if (!HAS_OD) {
     // rows tile [fe[0], fe[n]): one flat pass over that range (within [0, m))
     const int64_t b = fe[0] > 0 ? fe[0] : 0, e = fe[n] < m ? fe[n] : m;
-    for (int64_t q = b + (int64_t)blockIdx.x * kGB + threadIdx.x; q < e;
-         q += (int64_t)gridDim.x * kGB) {
+    const int64_t tid = (int64_t)blockIdx.x * kGB + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * kGB;
+    int64_t s0 = b;
+    if (sizeof(W) == 4 && (reinterpret_cast<uintptr_t>(weight + b) & 15) == 0 && e > b) {
+      // 16-byte loads over the aligned body
+      const int64_t n4 = (e - b) / 4;
+      const uint4* w4 = reinterpret_cast<const uint4*>(weight + b);
+      for (int64_t q = tid; q < n4; q += stride) {
+        const uint4 v = w4[q];
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          W x;
+          memcpy(&x, &u[t], sizeof(W));
+          const uint64_t w = word_of<W>(x);
+          lo = w < lo ? w : lo;
+          hi = w > hi ? w : hi;
+        }
+      }
+      s0 = b + 4 * n4;
+    }
+    for (int64_t q = s0 + tid; q < e; q += stride) {
       uint64_t w = word_of<W>(weight[q]);
       lo = w < lo ? w : lo;
       hi = w > hi ? w : hi;
@@ -258,45 +278,7 @@ __device__ __forceinline__ void fold_range(const unsigned long long* range_lo,
   *hi = h;
 }
 
-// Histogram of weight words over one entry in 8 (4096 bins of the weight
-// range), used to pick the light/heavy split threshold. Entries in row gaps
-// may be sampled; that only moves the threshold.
 constexpr int kHistBins = 4096;
-constexpr int kHistStride = 8;
-template <typename W>
-__global__ __launch_bounds__(kGB) void k_weight_hist(const W* __restrict__ weight, int64_t m,
-                                                     const unsigned long long* __restrict__ range_lo,
-                                                     const unsigned long long* __restrict__ range_hi,
-                                                     unsigned int* __restrict__ hist) {
-  __shared__ unsigned int h[kHistBins];
-  __shared__ uint64_t s_lo, s_hi;
-  for (int i = threadIdx.x; i < kHistBins; i += kGB) h[i] = 0;
-  if (threadIdx.x < kWave) {
-    uint64_t l, hh;
-    fold_range(range_lo, range_hi, &l, &hh);
-    if (threadIdx.x == 0) { s_lo = l; s_hi = hh; }
-  }
-  __syncthreads();
-  const uint64_t lo = s_lo, hi = s_hi;
-  if (lo <= hi) {
-    int wb = 0;
-    while (wb < 64 && ((hi - lo) >> wb) != 0) ++wb;
-    const int shift = wb > 12 ? wb - 12 : 0;
-    // sample the first 256 entries of every 2048 (coalesced, 1 in 8)
-    const int64_t ns = ((m + 2047) / 2048) * 256;
-    for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < ns;
-         i += (int64_t)gridDim.x * kGB) {
-      const int64_t e = (i >> 8) * 2048 + (i & 255);
-      if (e >= m) continue;
-      uint64_t w = word_of<W>(weight[e]);
-      if (w < lo || w > hi) continue;
-      atomicAdd(&h[(w - lo) >> shift], 1u);
-    }
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kHistBins; i += kGB)
-    if (h[i]) atomicAdd(&hist[i], h[i]);
-}
 
 // Weight offset, key width and light/heavy threshold of one call, computed
 // on the device so that no host round trip sits between the passes.
@@ -307,47 +289,66 @@ struct MstPlan {
   int wbits;                     // bits of whi - wlo
   int err;                       // 1: wbits + 2 * vb > 64
 };
-__global__ void k_plan(const unsigned long long* __restrict__ range_lo,
-                       const unsigned long long* __restrict__ range_hi,
-                       const unsigned int* __restrict__ hist, int vb, int try_split,
-                       double target, MstPlan* __restrict__ plan) {
-  const int l = lane_id();
-  uint64_t wlo, whi;
-  fold_range(range_lo, range_hi, &wlo, &whi);
+// One block: folds the exact weight range, then (try_split) estimates the
+// weight quantile `frac` from kPlanSamples evenly spaced entries with a
+// 4096-bin histogram of the range. The threshold only steers the split;
+// any value gives the same forest.
+constexpr int kPlanThreads = 256;
+constexpr int kPlanSamples = 8192;
+template <typename W>
+__global__ __launch_bounds__(kPlanThreads) void k_plan(
+    const W* __restrict__ weight, int64_t m, const unsigned long long* __restrict__ range_lo,
+    const unsigned long long* __restrict__ range_hi, int vb, int try_split, double frac,
+    MstPlan* __restrict__ plan) {
+  __shared__ unsigned int h[kHistBins];
+  __shared__ unsigned int s_tmp[kPlanThreads / kWave + 1];
+  __shared__ uint64_t s_lo, s_hi;
+  __shared__ int s_found;
+  if (threadIdx.x < kWave) {
+    uint64_t l, hh;
+    fold_range(range_lo, range_hi, &l, &hh);
+    if (threadIdx.x == 0) {
+      s_lo = l;
+      s_hi = hh;
+      s_found = kHistBins;
+    }
+  }
+  for (int i = threadIdx.x; i < kHistBins; i += kPlanThreads) h[i] = 0;
+  __syncthreads();
+  uint64_t wlo = s_lo, whi = s_hi;
   if (wlo == kInf) { wlo = 0; whi = 0; }
   int wbits = 0;
   while (wbits < 64 && ((whi - wlo) >> wbits) != 0) ++wbits;
   uint64_t lim = kInf;
-  if (try_split) {
+  if (try_split && m > 0) {  // uniform over the block
     const int shift = wbits > 12 ? wbits - 12 : 0;
-    constexpr int per = kHistBins / kWave;
-    double part = 0;
-    for (int q = 0; q < per; ++q) part += hist[l * per + q];
-    double incl = part;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-      double o = __shfl_up(incl, off, kWave);
-      if (l >= off) incl += o;
+    for (int i = threadIdx.x; i < kPlanSamples; i += kPlanThreads) {
+      const int64_t e = (int64_t)((double)i * (double)m / kPlanSamples);
+      const uint64_t w = word_of<W>(weight[e < m ? e : m - 1]);
+      if (w >= wlo && w <= whi) atomicAdd(&h[(w - wlo) >> shift], 1u);
     }
-    double excl = incl - part;
-    // the lane whose slice crosses the target finds the bin
-    int found = kHistBins;
-    if (excl < target && incl >= target) {
-      double cum = excl;
+    __syncthreads();
+    constexpr int per = kHistBins / kPlanThreads;
+    unsigned int part = 0;
+    for (int q = 0; q < per; ++q) part += h[threadIdx.x * per + q];
+    unsigned int total;
+    const unsigned int excl = block_excl_scan<unsigned int>(part, s_tmp, &total);
+    const double target = frac * kPlanSamples;
+    if ((double)excl < target && (double)(excl + part) >= target) {
+      unsigned int cum = excl;
       for (int q = 0; q < per; ++q) {
-        cum += hist[l * per + q];
-        if (cum >= target) { found = l * per + q; break; }
+        cum += h[threadIdx.x * per + q];
+        if ((double)cum >= target) {
+          atomicMin(&s_found, threadIdx.x * per + q);
+          break;
+        }
       }
     }
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-      int o = __shfl_xor(found, off, kWave);
-      found = o < found ? o : found;
-    }
-    if (found < kHistBins) lim = (uint64_t)(found + 1) << shift;
+    __syncthreads();
+    if (s_found < kHistBins) lim = (uint64_t)(s_found + 1) << shift;
     if (lim != kInf && lim > whi - wlo) lim = kInf;
   }
-  if (l == 0) {
+  if (threadIdx.x == 0) {
     plan->wlo = wlo;
     plan->whi = whi;
     plan->wbits = wbits;
@@ -1102,9 +1103,8 @@ struct MstState {
   unsigned long long range_hi[kSlots];
   unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
   unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
-  unsigned int hist[kHistBins];  // last: per-round reads stop before it
 };
-constexpr size_t kStateHead = offsetof(MstState, hist);
+constexpr size_t kStateHead = sizeof(MstState);
 constexpr size_t kCtlHead = offsetof(MstState, range_lo);  // what the round loop reads
 
 inline unsigned long long fold_slots(const unsigned long long* v) {
@@ -1245,17 +1245,12 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   // forest's light part; heavy edges then need one filtering pass against
   // the light components before the remaining rounds. Only worth it on
   // graphs with many edges per vertex. The threshold keeps about 1.5 light
-  // undirected edges per vertex: 3n sampled directed entries for a symmetric
-  // CSR, 1.5n for a one-entry-per-edge CSR.
+  // undirected edges per vertex: a fraction 3n/m of the entries of a
+  // symmetric CSR, 1.5n/m of a one-entry-per-edge CSR.
   const bool try_split = m >= 6 * (int64_t)n && m >= (1 << 16);
-  if (try_split) {
-    k_weight_hist<W><<<grid_for(m / kHistStride + 1, kGB, 1024), kGB, 0, s>>>(
-        weight, m, B.st->range_lo, B.st->range_hi, B.st->hist);
-    EAC_LAUNCH_CHECK();
-  }
-  const double target = (symmetric ? 3.0 : 1.5) * n / kHistStride;
-  k_plan<<<1, kWave, 0, s>>>(B.st->range_lo, B.st->range_hi, B.st->hist, vb, try_split ? 1 : 0,
-                             target, &B.st->plan);
+  const double frac = (symmetric ? 3.0 : 1.5) * n / (double)m;
+  k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb,
+                                       try_split ? 1 : 0, frac, &B.st->plan);
   EAC_LAUNCH_CHECK();
   EAC_TRY(fill<uint64_t>((uint64_t*)B.best[0], n, kInf, s));  // hook(0) clears best[1]
   EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
