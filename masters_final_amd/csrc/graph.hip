@@ -803,6 +803,37 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
   if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
 }
 
+// After the links of round r: every vertex points straight at its root, so
+// the filter's root lookups are one load deep (no link happens here; plain
+// loads see ancestors, path halving keeps every pointer an ancestor).
+__global__ __launch_bounds__(kGB) void k_compress(int r, int32_t n,
+                                                  const RoundCtl* __restrict__ ctl,
+                                                  const unsigned int* __restrict__ had_edge,
+                                                  int32_t* P, unsigned int* __restrict__ overflow) {
+  if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked this round
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
+    const int32_t x0 = (int32_t)v;
+    int32_t x = x0, p = P[x0];
+    if (p == x0) continue;
+    int32_t root;
+    for (int32_t steps = 0;; ++steps) {
+      const int32_t gp = P[p];
+      if (gp == p) { root = p; break; }
+      P[x] = gp;  // path halving
+      x = gp;
+      p = P[x];
+      if (p == x) { root = x; break; }
+      if (steps > n) {
+        atomicOr(overflow, 4u);
+        root = x;
+        break;
+      }
+    }
+    if (P[x0] != root) P[x0] = root;
+  }
+}
+
 // --------------------------------------------------------- forest output
 // Tile-wise compaction of non-sentinel keys in index order (deterministic):
 // pass 1 counts per tile, a scan turns counts into offsets, pass 2 writes.
@@ -847,51 +878,104 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
 // CSR position (in the smaller endpoint's row) and weight of each forest
 // edge: 8 lanes per edge scan the row 32 entries at a time (each load
 // instruction reads 8 consecutive entries of 8 rows), keeping the first
-// entry with the edge's column and weight.
+// entry with the edge's column and weight. A lane group works on 4 edges at
+// once so that their loads are in flight together. n_f is bounded by
+// *d_nf when given (the count left on the device).
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
-    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ dest,
-    const W* __restrict__ weight, const int64_t* __restrict__ fe,
-    const int32_t* __restrict__ od, int vb, uint64_t word_lo, int64_t* __restrict__ ids,
-    W* __restrict__ wout, unsigned int* __restrict__ missing) {
+    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
+    const int32_t* __restrict__ dest, const W* __restrict__ weight,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, uint64_t word_lo,
+    int64_t* __restrict__ ids, W* __restrict__ wout, unsigned int* __restrict__ missing) {
   constexpr int kG = 8;  // lanes per forest edge
+  constexpr int kU = 4;  // edges per lane group in flight
   constexpr int64_t kNone = INT64_MAX;
+  if (d_nf) {
+    const int64_t c = *d_nf;
+    n_f = c < n_f ? c : n_f;
+  }
   const int sub = threadIdx.x & (kG - 1);
-  const int64_t stride = (int64_t)gridDim.x * (kGB / kG);
-  for (int64_t i = ((int64_t)blockIdx.x * kGB + threadIdx.x) / kG; i < n_f; i += stride) {
-    const uint64_t k = fkeys[i];
-    int32_t a, b;
-    unpack_ends(k, vb, &a, &b);
-    const uint64_t wk = k >> (2 * vb);
-    const int64_t beg = fe[a];
-    const int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
-    int64_t pos = kNone;
-    for (int64_t p0 = beg; p0 < end; p0 += 4 * kG) {  // trip count uniform over the group
-      int32_t d[4];
+  const int64_t groups = (int64_t)gridDim.x * (kGB / kG);
+  for (int64_t g = ((int64_t)blockIdx.x * kGB + threadIdx.x) / kG; g < n_f; g += groups * kU) {
+    int32_t bcol[kU];
+    uint64_t wk[kU];
+    int64_t beg[kU], end[kU], pos[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int64_t i = g + u * groups;
+      pos[u] = kNone;
+      beg[u] = 0;
+      end[u] = 0;
+      bcol[u] = -2;
+      wk[u] = 0;
+      if (i < n_f) {
+        const uint64_t k = fkeys[i];
+        int32_t a, b;
+        unpack_ends(k, vb, &a, &b);
+        bcol[u] = b;
+        wk[u] = k >> (2 * vb);
+        beg[u] = fe[a];
+        end[u] = HAS_OD ? beg[u] + od[a] : fe[a + 1];
+      }
+    }
+    // first 32 entries of every row, all loads issued together
+    int32_t d[kU][4];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int64_t e = p0 + q * kG + sub;
-        d[q] = e < end ? dest[e] : -1;
+        const int64_t e = beg[u] + q * kG + sub;
+        d[u][q] = e < end[u] ? dest[e] : -1;
       }
 #pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int64_t e = p0 + q * kG + sub;
-        if (d[q] == b && e < pos && word_of<W>(weight[e]) - word_lo == wk) pos = e;
+        const int64_t e = beg[u] + q * kG + sub;
+        if (d[u][q] == bcol[u] && e < pos[u] && word_of<W>(weight[e]) - word_lo == wk[u])
+          pos[u] = e;
       }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
 #pragma unroll
       for (int off = kG / 2; off > 0; off >>= 1) {
-        const int64_t o = __shfl_xor(pos, off, kG);
-        pos = o < pos ? o : pos;
+        const int64_t o = __shfl_xor(pos[u], off, kG);
+        pos[u] = o < pos[u] ? o : pos[u];
       }
-      if (pos != kNone) break;
+      // longer rows: further windows (trip count uniform over the group)
+      for (int64_t p0 = beg[u] + 4 * kG; pos[u] == kNone && p0 < end[u]; p0 += 4 * kG) {
+        int32_t dd[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t e = p0 + q * kG + sub;
+          dd[q] = e < end[u] ? dest[e] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t e = p0 + q * kG + sub;
+          if (dd[q] == bcol[u] && e < pos[u] && word_of<W>(weight[e]) - word_lo == wk[u])
+            pos[u] = e;
+        }
+#pragma unroll
+        for (int off = kG / 2; off > 0; off >>= 1) {
+          const int64_t o = __shfl_xor(pos[u], off, kG);
+          pos[u] = o < pos[u] ? o : pos[u];
+        }
+      }
     }
     if (sub == 0) {
-      if (pos == kNone) {
-        atomicOr(missing, 1u);
-        pos = 0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int64_t i = g + u * groups;
+        if (i >= n_f) continue;
+        int64_t p = pos[u];
+        if (p == kNone) {
+          atomicOr(missing, 1u);
+          p = 0;
+        }
+        if (ids) ids[i] = p;
+        if (wout) wout[i] = weight[p];
       }
-      if (ids) ids[i] = pos;
-      if (wout) wout[i] = weight[pos];
     }
   }
 }
@@ -1227,7 +1311,7 @@ inline int mst_round_bound(int32_t n) {
 template <typename W, bool HAS_OD>
 int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
              int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
-             hipStream_t s, int graph_rounds = 0) {
+             hipStream_t s, int graph_rounds = 0, bool read_nf = true) {
   const bool graph_mode = graph_rounds > 0;
   const int64_t nchunks = B.nchunks;
   const int vb = vbits_for(n);
@@ -1275,7 +1359,9 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   const int gr = grid_for(nchunks, 1, resident_grid(k_round, kGB));
   const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
   std::vector<int> rec_hook, rec_round;
-  int enq = 0, batch = graph_mode ? fixed_rounds : 6;
+  // first batch ~ the rounds a graph of this size usually needs (idle
+  // rounds cost a few microseconds, a host read tens), then 4 at a time
+  int enq = 0, batch = graph_mode ? fixed_rounds : (vb * 3 / 4 > 6 ? vb * 3 / 4 : 6);
   MstState hs;
   for (;;) {
     for (int b = 0; b < batch && enq < kMaxRounds; ++b, ++enq) {
@@ -1285,6 +1371,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                 B.mst_key, B.st->had_edge, vb, &B.st->overflow);
       EAC_LAUNCH_CHECK();
       prof_end(rec_hook.back(), s);
+      k_compress<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
+      EAC_LAUNCH_CHECK();
       rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
       k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, nchunks,
                                  B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
@@ -1342,14 +1430,20 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     fi->rounds = fixed_rounds;
     return EAC_OK;
   }
-  int32_t n_f = 0;
-  EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
+  int32_t n_f = -1;  // left on the device unless the caller needs it now
+  if (read_nf) EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
   fi->n_f = n_f;
   fi->wbits = hs.plan.wbits;
   fi->wlo = hs.plan.wlo;
   fi->whi = hs.plan.whi;
   fi->rounds = hs.ctl[enq & 1].rounds;
   return EAC_OK;
+}
+
+__global__ void k_pack_tail(const int32_t* __restrict__ d_nf,
+                            const unsigned int* __restrict__ missing, int32_t* __restrict__ out) {
+  out[0] = *d_nf;
+  out[1] = (int32_t)*missing;
 }
 
 template <typename W, bool HAS_OD>
@@ -1360,13 +1454,15 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   MstBuffers B = carve_mst(ar, n, m, symmetric);
   if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
   ForestInfo fi;
-  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s)));
-  const int64_t n_f = fi.n_f;
-  if (n_f > 0 && (ids || wout)) {
-    k_forest_positions<W, HAS_OD><<<grid_for(8 * n_f, kGB,
+  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s, 0,
+                               false)));
+  // the forest count stays on the device: positions run over the bound n-1
+  if (n > 1 && (ids || wout)) {
+    k_forest_positions<W, HAS_OD><<<grid_for(8 * (int64_t)(n - 1), kGB,
                                              resident_grid(k_forest_positions<W, HAS_OD>, kGB)),
                                     kGB, 0, s>>>(
-        B.fkeys, n_f, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout, &B.st->missing);
+        B.fkeys, n - 1, fi.d_nf, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout,
+        &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
   int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
@@ -1390,10 +1486,14 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     EAC_LAUNCH_CHECK();
   }
   prof_end(pr, s);
-  unsigned int missing = 0;
-  EAC_TRY(d2h_sync(&missing, &B.st->missing, sizeof(missing), s));
+  // one read at the end: the forest count and the lookup flag
+  k_pack_tail<<<1, 1, 0, s>>>(fi.d_nf, &B.st->missing, B.rank);
+  EAC_LAUNCH_CHECK();
+  int32_t tail[2] = {0, 0};
+  EAC_TRY(d2h_sync(tail, B.rank, sizeof(tail), s));
   prof_flush();
-  if (missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
+  if (tail[1]) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
+  const int64_t n_f = tail[0];
   if (h_out) {
     h_out[0] = n_f;
     h_out[1] = n - n_f;
