@@ -1234,7 +1234,7 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   b.cnt_light = ar.take<int32_t>(nchunks);
   b.cnt_heavy = ar.take<int32_t>(nchunks);
   b.fkeys = ar.take<uint64_t>(n);
-  b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 1);
+  b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 2);  // + forest count, + labels count
   b.flag = ar.take<int32_t>(n);
   b.rank = ar.take<int32_t>(n);
   b.st = ar.take<MstState>(1);
@@ -1481,7 +1481,8 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     EAC_LAUNCH_CHECK();
     k_first_flag<<<g, 256, 0, s>>>(rootv, first, n, n, root, B.flag);
     EAC_LAUNCH_CHECK();
-    EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles, ar, s)));
+    // component count in the slot after the forest count (which is still needed)
+    EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles + 1, ar, s)));
     k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
     EAC_LAUNCH_CHECK();
   }
