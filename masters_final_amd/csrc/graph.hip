@@ -567,41 +567,6 @@ __device__ __forceinline__ void roots_live2(int32_t* P, int32_t* a, int32_t* b, 
   *b = y;
 }
 
-// Roots of N vertices at once, in a kernel where no hook happens: the N
-// walks advance one level per step, so their loads are in flight together
-// (a chain of N sequential walks would pay N times the memory latency).
-// Each start vertex is then pointed at its root (plain store, any ancestor
-// is a valid parent).
-template <int N>
-__device__ __forceinline__ void roots_fast(int32_t* P, int32_t (&x)[N], int32_t n,
-                                           unsigned int* overflow) {
-  int32_t cur[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) cur[i] = x[i];
-  for (int32_t steps = 0;; ++steps) {
-    int32_t p[N];
-    bool moved = false;
-#pragma unroll
-    for (int i = 0; i < N; ++i) p[i] = cur[i] >= 0 ? P[cur[i]] : cur[i];
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-      if (p[i] != cur[i]) {
-        cur[i] = p[i];
-        moved = true;
-      }
-    if (!moved) break;
-    if (steps > n) {
-      atomicOr(overflow, 4u);
-      break;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if (x[i] >= 0 && cur[i] != x[i] && P[x[i]] != cur[i]) P[x[i]] = cur[i];
-    x[i] = cur[i];
-  }
-}
-
 // ------------------------------------------------------ device round control
 // A round is hook(r) -> jump(r) -> filter(r). The control state of round r
 // lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
@@ -656,6 +621,31 @@ __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* 
   atomicMin(&best[root], (unsigned long long)key);  // table crowded: direct
 }
 
+// min_insert from a wavefront: lanes holding the same root (the giant
+// component's, typically) are first reduced to one lane, for up to two
+// distinct roots per call, so a hot root costs one LDS atomic per wave
+// instead of one per lane.
+__device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long long* s_min,
+                                                int32_t root, uint64_t key, bool active,
+                                                unsigned long long* best) {
+  uint64_t pend = __ballot(active);
+  for (int it = 0; it < 2 && pend; ++it) {
+    const int leader = __ffsll((unsigned long long)pend) - 1;
+    const int32_t R = __shfl(root, leader, kWave);
+    const bool mine = active && root == R;
+    uint64_t v = mine ? key : kInf;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const uint64_t o = __shfl_xor(v, off, kWave);
+      v = o < v ? o : v;
+    }
+    if (lane_id() == leader) min_insert(s_root, s_min, R, v, best);
+    pend &= ~__ballot(mine);
+    if (mine) active = false;
+  }
+  if (active) min_insert(s_root, s_min, root, key, best);
+}
+
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
@@ -707,25 +697,21 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
           const int e = base + q * kWave + lane;
           k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
         }
-        int32_t x[8];  // endpoints of the 4 keys (-1: no key)
+        int32_t x[8];  // roots of the endpoints of the 4 keys (parents are flat)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          if (k[q] == kInf) {
-            x[2 * q] = -1;
-            x[2 * q + 1] = -1;
-          } else {
-            unpack_ends(k[q], vb, &x[2 * q], &x[2 * q + 1]);
-          }
+          int32_t a = 0, b = 0;
+          if (k[q] != kInf) unpack_ends(k[q], vb, &a, &b);
+          x[2 * q] = a;
+          x[2 * q + 1] = b;
         }
-        roots_fast<8>(P, x, n, overflow);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
         bool keep[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
-          if (!keep[q]) continue;
-          min_insert(s_root, s_min, x[2 * q], k[q], best);
-          min_insert(s_root, s_min, x[2 * q + 1], k[q], best);
-        }
+        for (int q = 0; q < 4; ++q) keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], keep[i / 2], best);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const uint64_t m = __ballot(keep[q]);
@@ -803,34 +789,31 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
   if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
 }
 
-// After the links of round r: every vertex points straight at its root, so
-// the filter's root lookups are one load deep (no link happens here; plain
-// loads see ancestors, path halving keeps every pointer an ancestor).
+// After the links of round r every vertex is pointed straight at its root:
+// each thread walks its own vertex's chain (plain loads; no link happens
+// here and parents only ever move up) and writes only that vertex, so when
+// the kernel ends the parents are exactly flat and the filter finds a root
+// with one load.
 __global__ __launch_bounds__(kGB) void k_compress(int r, int32_t n,
                                                   const RoundCtl* __restrict__ ctl,
                                                   const unsigned int* __restrict__ had_edge,
                                                   int32_t* P, unsigned int* __restrict__ overflow) {
-  if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked this round
+  if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked: still flat
   for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * kGB) {
     const int32_t x0 = (int32_t)v;
-    int32_t x = x0, p = P[x0];
-    if (p == x0) continue;
-    int32_t root;
+    int32_t x = P[x0];
+    if (x == x0) continue;
     for (int32_t steps = 0;; ++steps) {
-      const int32_t gp = P[p];
-      if (gp == p) { root = p; break; }
-      P[x] = gp;  // path halving
-      x = gp;
-      p = P[x];
-      if (p == x) { root = x; break; }
+      const int32_t p = P[x];
+      if (p == x) break;
+      x = p;
       if (steps > n) {
         atomicOr(overflow, 4u);
-        root = x;
         break;
       }
     }
-    if (P[x0] != root) P[x0] = root;
+    P[x0] = x;
   }
 }
 
@@ -878,8 +861,8 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
 // CSR position (in the smaller endpoint's row) and weight of each forest
 // edge: 8 lanes per edge scan the row 32 entries at a time (each load
 // instruction reads 8 consecutive entries of 8 rows), keeping the first
-// entry with the edge's column and weight. A lane group works on 4 edges at
-// once so that their loads are in flight together. n_f is bounded by
+// entry with the edge's column and weight (kU edges per lane group in
+// flight; more costs occupancy for no gain). n_f is bounded by
 // *d_nf when given (the count left on the device).
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
@@ -888,7 +871,7 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
     const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, uint64_t word_lo,
     int64_t* __restrict__ ids, W* __restrict__ wout, unsigned int* __restrict__ missing) {
   constexpr int kG = 8;  // lanes per forest edge
-  constexpr int kU = 4;  // edges per lane group in flight
+  constexpr int kU = 1;  // edges per lane group in flight
   constexpr int64_t kNone = INT64_MAX;
   if (d_nf) {
     const int64_t c = *d_nf;
