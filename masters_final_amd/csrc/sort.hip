@@ -149,6 +149,7 @@ int radix_sort(const K* kin, K* kout, const V* vin, V* vout, int64_t n, int begi
   // ping-pong so that the last pass writes kout/vout
   const K* src_k = kin;
   const V* src_v = vin;
+  const size_t mark = ar.used;  // each pass's scan reuses the same scratch
   for (int p = 0; p < passes; ++p) {
     int shift = begin_bit + 8 * p;
     bool to_out = ((passes - 1 - p) % 2) == 0;
@@ -158,6 +159,7 @@ int radix_sort(const K* kin, K* kout, const V* vin, V* vout, int64_t n, int begi
     EAC_LAUNCH_CHECK();
     EAC_TRY((scan_excl<int32_t, int32_t>(table, table, (int64_t)ntiles * kRadix, 0, nullptr, ar,
                                          s)));
+    ar.used = mark;
     k_scatter<K, V, HAS_V><<<ntiles, kRB, 0, s>>>(src_k, src_v, dk, dv, n, shift, ntiles, table);
     EAC_LAUNCH_CHECK();
     src_k = dk;
