@@ -33,6 +33,18 @@ MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)",
              2: "k_hook (union-find links)", 4: "forest output + labels"}
 
 
+def profile_traffic(kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (scaled by a copy-kernel calibration) and committed under
+    profiles/; None when no summary covers it."""
+    import re
+    path = Path(__file__).resolve().parent / "profiles" / "boruvka_mst_knn1m_hbm_traffic.md"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch %s: (\d+)" % re.escape(kernel), path.read_text())
+    return int(m.group(1)) if m else None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -195,9 +207,12 @@ def run_mst(args):
     out["result"] = {"forest_edges": r["n_mst"], "forest_weight": total_w,
                      "components": r["n_components"], "rounds": r["rounds"],
                      "labels_checksum": labels_crc}
+    kname = {0: "k_round0", 1: "k_round"}.get(dom)
     out["roofline"] = {"bound": "hbm", "kernel": MST_SLOTS[dom], "achieved": achieved,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                       "traffic": None, "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl}
+                       "traffic": profile_traffic(kname) if kname else None,
+                       "traffic_source": "profiles/boruvka_mst_knn1m_hbm_traffic.md",
+                       "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl}
     out["kernels_ms_per_step"] = {MST_SLOTS[s]: prof[s][0] / args.steps for s in prof}
     out["cpu_baseline"] = cpu
     if not args.no_eac:

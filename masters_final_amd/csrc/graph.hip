@@ -1388,7 +1388,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
       const int64_t in = ran ? (int64_t)fold_slots(hs.rlog[r][0]) : 0;
       const int64_t out = ran ? (int64_t)fold_slots(hs.rlog[r][1]) : 0;
       prof_set_bytes(rec_hook[r], ran ? 8 * (int64_t)n : 0);
-      prof_set_bytes(rec_round[r], ran ? 8 * in + 8 * out + 4 * nchunks : 0);
+      // key 8 B + two 4 B root lookups per key read, 8 B per key kept, counts
+      prof_set_bytes(rec_round[r], ran ? 16 * in + 8 * out + 4 * nchunks : 0);
     }
   }
 
