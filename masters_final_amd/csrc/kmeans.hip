@@ -304,6 +304,37 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
 
 // FP64 re-scoring of flagged points: one wave per (p, row), lanes stride the
 // partition's centroids; strict < in ascending k, lowest k on ties.
+// acc + diff^2 with separately rounded multiply and add in FP64 (the
+// oracle's arithmetic: no fused multiply-add), plain expression otherwise.
+__device__ __forceinline__ double sq_acc(double acc, double diff) {
+  return __dadd_rn(acc, __dmul_rn(diff, diff));
+}
+__device__ __forceinline__ float sq_acc(float acc, float diff) { return acc + diff * diff; }
+
+// Squared distance of two float rows in FP64, accumulated in ascending
+// dimension order (the oracle's arithmetic); 16-byte loads when both rows
+// are 16-byte aligned and d is a multiple of 4.
+__device__ __forceinline__ double dist2_f64(const float* __restrict__ x,
+                                           const float* __restrict__ c, int d, bool vec4) {
+  double acc = 0.0;
+  if (vec4) {
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* c4 = reinterpret_cast<const float4*>(c);
+    for (int j = 0; j < d / 4; ++j) {
+      const float4 xv = x4[j], cv = c4[j];
+      acc = sq_acc(acc, (double)xv.x - (double)cv.x);
+      acc = sq_acc(acc, (double)xv.y - (double)cv.y);
+      acc = sq_acc(acc, (double)xv.z - (double)cv.z);
+      acc = sq_acc(acc, (double)xv.w - (double)cv.w);
+    }
+  } else {
+    for (int j = 0; j < d; ++j) {
+      acc = sq_acc(acc, (double)x[j] - (double)c[j]);
+    }
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X, int64_t n,
                                                      int d, const float* __restrict__ C,
                                                      const int64_t* __restrict__ k_off,
@@ -319,13 +350,10 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
     const int64_t kbeg = k_off[p], kend = k_off[p + 1];
     double best = DBL_MAX;
     int64_t bk = -1;
+    const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
+                                      reinterpret_cast<uintptr_t>(C)) & 15) == 0;
     for (int64_t c = kbeg + l; c < kend; c += kWave) {
-      const float* cc = C + c * d;
-      double dist = 0.0;
-      for (int j = 0; j < d; ++j) {
-        double diff = (double)x[j] - (double)cc[j];
-        dist += diff * diff;
-      }
+      const double dist = dist2_f64(x, C + c * d, d, v4);
       if (bk < 0 || dist < best) { best = dist; bk = c; }
     }
 #pragma unroll
@@ -348,18 +376,14 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      float* __restrict__ dists,
                                                      double* __restrict__ inertia) {
   __shared__ double s_acc[256 / kWave];
+  const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
+                                    reinterpret_cast<uintptr_t>(C)) & 15) == 0;
   for (int p = blockIdx.y; p < P; p += gridDim.y) {
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
       int64_t c = k_off[p] + labels[(int64_t)p * n + i];
-      const float* x = X + i * d;
-      const float* cc = C + c * d;
-      double dist = 0.0;
-      for (int j = 0; j < d; ++j) {
-        double diff = (double)x[j] - (double)cc[j];
-        dist += diff * diff;
-      }
+      const double dist = dist2_f64(X + i * d, C + c * d, d, v4);
       float df = (float)dist;
       if (dists) dists[(int64_t)p * n + i] = df;
       acc += (double)df;
@@ -410,8 +434,7 @@ __global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ 
           T dist = 0;
 #pragma unroll
           for (int j = 0; j < DX; ++j) {
-            T diff = (T)xr[j] - (T)s_c[r * DX + j];
-            dist += diff * diff;
+            dist = sq_acc(dist, (T)xr[j] - (T)s_c[r * DX + j]);
           }
           if (bl < 0 || dist < best) { best = dist; bl = (int32_t)(c0 + r - kbeg); }
         }
