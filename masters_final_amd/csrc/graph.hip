@@ -868,8 +868,10 @@ template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_forest_positions(
     const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
-    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, uint64_t word_lo,
-    int64_t* __restrict__ ids, W* __restrict__ wout, unsigned int* __restrict__ missing) {
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb,
+    const MstPlan* __restrict__ plan, int64_t* __restrict__ ids, W* __restrict__ wout,
+    unsigned int* __restrict__ missing) {
+  const uint64_t word_lo = plan->wlo;
   constexpr int kG = 8;  // lanes per forest edge
   constexpr int kU = 1;  // edges per lane group in flight
   constexpr int64_t kNone = INT64_MAX;
@@ -1286,16 +1288,29 @@ inline int mst_round_bound(int32_t n) {
   return r < kMaxRounds ? r : kMaxRounds;
 }
 
-// Boruvka rounds up to the forest key list (shared by the forest, lifetime
-// cut and batched entry points). graph_rounds > 0 selects graph mode: no
-// host synchronisation at all (that many device-controlled rounds are
-// enqueued, rounds after convergence exit at once, counts stay on the
-// device), so the sequence can be captured in a hipGraph.
+// best[0] = ~0, mst_key = ~0, comp = identity (union-find parents).
+__global__ void k_vertex_init(int32_t n, unsigned long long* __restrict__ best,
+                              uint64_t* __restrict__ mst_key, int32_t* __restrict__ comp) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    best[v] = kInf;
+    mst_key[v] = kInf;
+    comp[v] = (int32_t)v;
+  }
+}
+
+// Host-side records of one Borůvka call for the library profiler.
+struct MstRecs {
+  int pr0 = -1;                // round-0 record
+  int64_t round0_bytes = 0;
+  std::vector<int> hook, round;
+};
+
+// Set-up and round 0: state, chunk rows, weight range and plan, round-0
+// candidates and the chunk-local light/heavy lists. Nothing is read back.
 template <typename W, bool HAS_OD>
-int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-             int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
-             hipStream_t s, int graph_rounds = 0, bool read_nf = true) {
-  const bool graph_mode = graph_rounds > 0;
+int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+              int32_t n, int64_t m, int symmetric, MstBuffers& B, MstRecs& rec, hipStream_t s) {
   const int64_t nchunks = B.nchunks;
   const int vb = vbits_for(n);
   k_state_init<<<1, 1024, 0, s>>>(B.st);
@@ -1305,7 +1320,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   const int gchunks = grid_for(nchunks, 1, resident_grid(k_round0<W, HAS_OD>, kGB));
   const int grange = grid_for(nchunks, 1, resident_grid(k_weight_range<W, HAS_OD>, kGB));
   k_weight_range<W, HAS_OD><<<grange, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
-                                                     B.st->range_lo, B.st->range_hi);
+                                                    B.st->range_lo, B.st->range_hi);
   EAC_LAUNCH_CHECK();
   // Light/heavy split (Filter-Kruskal applied to Boruvka): rounds run first
   // on the light prefix of the key order, whose forest is exactly the
@@ -1319,89 +1334,129 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb,
                                        try_split ? 1 : 0, frac, &B.st->plan);
   EAC_LAUNCH_CHECK();
-  EAC_TRY(fill<uint64_t>((uint64_t*)B.best[0], n, kInf, s));  // hook(0) clears best[1]
-  EAC_TRY(fill<uint64_t>(B.mst_key, n, kInf, s));
-  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(B.comp, n);  // union-find parents
+  k_vertex_init<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, B.best[0], B.mst_key, B.comp);
   EAC_LAUNCH_CHECK();
-
-  // round 0 from the CSR
-  const int64_t round0_bytes =
-      (int64_t)(4 + sizeof(W)) * m + 8 * ((int64_t)n + 1) + (HAS_OD ? 4 * (int64_t)n : 0) +
-      8 * (int64_t)n;
-  int pr = prof_begin(EAC_PROF_MST_ROUND0, round0_bytes, s);
+  rec.round0_bytes = (int64_t)(4 + sizeof(W)) * m + 8 * ((int64_t)n + 1) +
+                     (HAS_OD ? 4 * (int64_t)n : 0) + 8 * (int64_t)n;
+  rec.pr0 = prof_begin(EAC_PROF_MST_ROUND0, rec.round0_bytes, s);
   k_round0<W, HAS_OD><<<gchunks, kGB, 0, s>>>(dest, weight, fe, od, n, m, B.chunk_row, nchunks,
                                                vb, &B.st->plan, symmetric, B.best[0], B.light,
                                                B.heavy, B.cnt_light, B.cnt_heavy,
                                                prof_on() ? &B.st->tot0[0][0] : nullptr,
                                                &B.st->overflow);
   EAC_LAUNCH_CHECK();
-  prof_end(pr, s);
-  // Rounds, enqueued in batches; the round control stops the sequence on
-  // the device (later kernels of a finished run exit at once).
-  const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
-  const int gr = grid_for(nchunks, 1, resident_grid(k_round, kGB));
-  const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
-  std::vector<int> rec_hook, rec_round;
-  // first batch ~ the rounds a graph of this size usually needs (idle
-  // rounds cost a few microseconds, a host read tens), then 4 at a time
-  int enq = 0, batch = graph_mode ? fixed_rounds : (vb * 3 / 4 > 6 ? vb * 3 / 4 : 6);
-  MstState hs;
-  for (;;) {
-    for (int b = 0; b < batch && enq < kMaxRounds; ++b, ++enq) {
-      const int r = enq;
-      rec_hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
-      k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
-                                B.mst_key, B.st->had_edge, vb, &B.st->overflow);
-      EAC_LAUNCH_CHECK();
-      prof_end(rec_hook.back(), s);
-      k_compress<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
-      EAC_LAUNCH_CHECK();
-      rec_round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
-      k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, nchunks,
-                                 B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
-                                 kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
-                                 &B.st->overflow);
-      EAC_LAUNCH_CHECK();
-      prof_end(rec_round.back(), s);
-    }
-    if (graph_mode) break;
-    EAC_TRY(d2h_sync(&hs, B.st, prof_on() ? kStateHead : kCtlHead, s));
-    if (hs.plan.err)
-      return fail(EAC_ERR_KEY_WIDTH,
-                  "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
-                  hs.plan.wbits, vb);
-    if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
-    if (hs.overflow & 4u)
-      return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
-                               "(pass symmetric=0)");
-    const RoundCtl& c = hs.ctl[enq & 1];
-    if (c.overflow_rounds || (c.active && enq >= kMaxRounds))
-      return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
-    if (!c.active) break;
-    batch = 4;
-  }
-  if (!graph_mode && prof_on()) {
-    const unsigned long long l0 = fold_slots(hs.tot0[0]), h0 = fold_slots(hs.tot0[1]);
-    prof_set_bytes(pr, round0_bytes + 8 * (int64_t)(l0 + h0));
-    for (int r = 0; r < (int)rec_hook.size(); ++r) {
-      const bool ran = r < hs.ctl[enq & 1].rounds;
-      const int64_t in = ran ? (int64_t)fold_slots(hs.rlog[r][0]) : 0;
-      const int64_t out = ran ? (int64_t)fold_slots(hs.rlog[r][1]) : 0;
-      prof_set_bytes(rec_hook[r], ran ? 8 * (int64_t)n : 0);
-      // key 8 B + two 4 B root lookups per key read, 8 B per key kept, counts
-      prof_set_bytes(rec_round[r], ran ? 16 * in + 8 * out + 4 * nchunks : 0);
-    }
-  }
+  prof_end(rec.pr0, s);
+  return EAC_OK;
+}
 
-  // forest keys in vertex order (slots past the count stay ~0)
+// Enqueues rounds [r0, r0 + count) (capped at kMaxRounds); returns the new
+// number of enqueued rounds. Rounds after convergence exit at once.
+int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStream_t s) {
+  const int vb = vbits_for(n);
+  const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
+  const int gc = grid_for(n, kGB, resident_grid(k_compress, kGB));
+  const int gr = grid_for(B.nchunks, 1, resident_grid(k_round, kGB));
+  int r = r0;
+  for (; r < r0 + count && r < kMaxRounds; ++r) {
+    rec.hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
+    k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
+                              B.mst_key, B.st->had_edge, vb, &B.st->overflow);
+    EAC_LAUNCH_CHECK();
+    prof_end(rec.hook.back(), s);
+    k_compress<<<gc, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
+    EAC_LAUNCH_CHECK();
+    rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
+    k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
+                               B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
+                               kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
+                               &B.st->overflow);
+    EAC_LAUNCH_CHECK();
+    prof_end(rec.round.back(), s);
+  }
+  return r;
+}
+
+// Errors recorded on the device, and whether rounds are still running.
+int mst_check(const MstState& hs, int enq, int vb, bool* active) {
+  if (hs.plan.err)
+    return fail(EAC_ERR_KEY_WIDTH,
+                "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
+                hs.plan.wbits, vb);
+  if (hs.overflow & 2u) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+  if (hs.overflow & 4u)
+    return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
+                             "(pass symmetric=0)");
+  const RoundCtl& c = hs.ctl[enq & 1];
+  if (c.overflow_rounds || (c.active && enq >= kMaxRounds))
+    return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
+  *active = c.active != 0;
+  return EAC_OK;
+}
+
+// Algorithmic bytes of the profiled launches, from the device counters.
+void mst_prof_bytes(const MstState& hs, int enq, int32_t n, int64_t nchunks, MstRecs& rec) {
+  if (!prof_on()) return;
+  const unsigned long long l0 = fold_slots(hs.tot0[0]), h0 = fold_slots(hs.tot0[1]);
+  prof_set_bytes(rec.pr0, rec.round0_bytes + 8 * (int64_t)(l0 + h0));
+  for (int r = 0; r < (int)rec.hook.size(); ++r) {
+    const bool ran = r < hs.ctl[enq & 1].rounds;
+    const int64_t in = ran ? (int64_t)fold_slots(hs.rlog[r][0]) : 0;
+    const int64_t out = ran ? (int64_t)fold_slots(hs.rlog[r][1]) : 0;
+    prof_set_bytes(rec.hook[r], ran ? 8 * (int64_t)n : 0);
+    // key 8 B + two 4 B root lookups per key read, 8 B per key kept, counts
+    prof_set_bytes(rec.round[r], ran ? 16 * in + 8 * out + 4 * nchunks : 0);
+  }
+}
+
+// Forest keys in vertex order (slots past the count stay ~0); the count
+// goes to *d_nf on the device.
+int mst_forest(MstBuffers& B, int32_t n, int32_t* d_nf, Arena& ar, hipStream_t s) {
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
-  int32_t* d_nf = B.tile_cnt + ftiles;
   EAC_TRY(fill<uint64_t>(B.fkeys, n, kInf, s));
   k_count_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt);
   EAC_LAUNCH_CHECK();
   EAC_TRY((scan_excl<int32_t, int32_t>(B.tile_cnt, B.tile_cnt, ftiles, 0, d_nf, ar, s)));
   k_write_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, B.fkeys);
   EAC_LAUNCH_CHECK();
+  return EAC_OK;
+}
+
+inline int first_batch(int32_t n) {
+  // about the rounds a graph of this size needs (idle rounds cost a few
+  // microseconds, a host read tens)
+  const int vb = vbits_for(n);
+  return vb * 3 / 4 > 6 ? vb * 3 / 4 : 6;
+}
+
+// Boruvka rounds up to the forest key list (shared by the lifetime cut and
+// batched entry points). graph_rounds > 0 selects graph mode: no host
+// synchronisation at all (that many device-controlled rounds are enqueued,
+// rounds after convergence exit at once, counts stay on the device), so the
+// sequence can be captured in a hipGraph.
+template <typename W, bool HAS_OD>
+int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, int64_t m, int symmetric, MstBuffers& B, Arena& ar, ForestInfo* fi,
+             hipStream_t s, int graph_rounds = 0) {
+  const bool graph_mode = graph_rounds > 0;
+  const int vb = vbits_for(n);
+  MstRecs rec;
+  EAC_TRY((mst_begin<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, rec, s)));
+  const int fixed_rounds = graph_rounds < kMaxRounds ? graph_rounds : kMaxRounds;
+  int enq = 0, batch = graph_mode ? fixed_rounds : first_batch(n);
+  MstState hs;
+  for (;;) {
+    enq = mst_rounds(B, n, enq, batch, rec, s);
+    if (graph_mode) break;
+    EAC_TRY(d2h_sync(&hs, B.st, prof_on() ? kStateHead : kCtlHead, s));
+    bool active = false;
+    EAC_TRY(mst_check(hs, enq, vb, &active));
+    if (!active) break;
+    batch = 4;
+  }
+  if (!graph_mode) mst_prof_bytes(hs, enq, n, B.nchunks, rec);
+  const int64_t ftiles = (n + kGTile - 1) / kGTile;
+  int32_t* d_nf = B.tile_cnt + ftiles;
+  EAC_TRY(mst_forest(B, n, d_nf, ar, s));
   fi->vb = vb;
   fi->d_nf = d_nf;
   fi->plan = &B.st->plan;
@@ -1414,8 +1469,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     fi->rounds = fixed_rounds;
     return EAC_OK;
   }
-  int32_t n_f = -1;  // left on the device unless the caller needs it now
-  if (read_nf) EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
+  int32_t n_f = 0;
+  EAC_TRY(d2h_sync(&n_f, d_nf, sizeof(int32_t), s));
   fi->n_f = n_f;
   fi->wbits = hs.plan.wbits;
   fi->wlo = hs.plan.wlo;
@@ -1424,32 +1479,39 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   return EAC_OK;
 }
 
-__global__ void k_pack_tail(const int32_t* __restrict__ d_nf,
-                            const unsigned int* __restrict__ missing, int32_t* __restrict__ out) {
-  out[0] = *d_nf;
-  out[1] = (int32_t)*missing;
+// Everything a speculative Borůvka call needs to read back, packed by one
+// kernel: forest count, lookup flag, error flags, weight bits, control.
+struct MstTail {
+  int32_t n_f, missing, overflow, plan_err, wbits, active, overflow_rounds, rounds;
+};
+__global__ void k_pack_tail(const MstState* __restrict__ st, int enq,
+                            const int32_t* __restrict__ d_nf, MstTail* __restrict__ out) {
+  const RoundCtl c = st->ctl[enq & 1];
+  out->n_f = *d_nf;
+  out->missing = (int32_t)st->missing;
+  out->overflow = (int32_t)st->overflow;
+  out->plan_err = st->plan.err;
+  out->wbits = st->plan.wbits;
+  out->active = c.active;
+  out->overflow_rounds = c.overflow_rounds;
+  out->rounds = c.rounds;
 }
 
+// Forest positions and weights plus first-occurrence component labels,
+// enqueued after the forest keys (nothing is read back).
 template <typename W, bool HAS_OD>
-int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-             int32_t n, int64_t m, int symmetric, int64_t* ids, W* wout, int32_t* labels,
-             int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
-  Arena ar(ws, ws_bytes);
-  MstBuffers B = carve_mst(ar, n, m, symmetric);
-  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
-  ForestInfo fi;
-  EAC_TRY((mst_core<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, ar, &fi, s, 0,
-                               false)));
+int mst_outputs(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+                int32_t n, MstBuffers& B, int32_t* d_nf, int64_t* ids, W* wout,
+                int32_t* labels, Arena& ar, hipStream_t s) {
+  const int vb = vbits_for(n);
   // the forest count stays on the device: positions run over the bound n-1
   if (n > 1 && (ids || wout)) {
     k_forest_positions<W, HAS_OD><<<grid_for(8 * (int64_t)(n - 1), kGB,
                                              resident_grid(k_forest_positions<W, HAS_OD>, kGB)),
-                                    kGB, 0, s>>>(
-        B.fkeys, n - 1, fi.d_nf, dest, weight, fe, od, fi.vb, fi.wlo, ids, wout,
-        &B.st->missing);
+                                    kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od, vb,
+                                                 &B.st->plan, ids, wout, &B.st->missing);
     EAC_LAUNCH_CHECK();
   }
-  int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
   if (labels) {
     // flatten the parents to roots, then number components by their
     // smallest vertex (first occurrence)
@@ -1465,25 +1527,80 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     EAC_LAUNCH_CHECK();
     k_first_flag<<<g, 256, 0, s>>>(rootv, first, n, n, root, B.flag);
     EAC_LAUNCH_CHECK();
-    // component count in the slot after the forest count (which is still needed)
+    // component count in the slot after the forest count
     EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles + 1, ar, s)));
     k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
     EAC_LAUNCH_CHECK();
   }
-  prof_end(pr, s);
-  // one read at the end: the forest count and the lookup flag
-  k_pack_tail<<<1, 1, 0, s>>>(fi.d_nf, &B.st->missing, B.rank);
-  EAC_LAUNCH_CHECK();
-  int32_t tail[2] = {0, 0};
-  EAC_TRY(d2h_sync(tail, B.rank, sizeof(tail), s));
+  return EAC_OK;
+}
+
+// eac_boruvka_mst. Speculative: the first batch of rounds (usually all a
+// graph of this size needs), the forest keys, positions and labels are all
+// enqueued before the one read-back; if the rounds had not converged, the
+// remaining rounds run with a read per batch and the outputs are redone.
+template <typename W, bool HAS_OD>
+int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, int64_t m, int symmetric, int64_t* ids, W* wout, int32_t* labels,
+             int64_t* h_out, void* ws, size_t ws_bytes, hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  MstBuffers B = carve_mst(ar, n, m, symmetric);
+  MstTail* tail = ar.take<MstTail>(1);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "boruvka: workspace too small");
+  const size_t mark = ar.used;
+  const int vb = vbits_for(n);
+  const int64_t ftiles = (n + kGTile - 1) / kGTile;
+  int32_t* d_nf = B.tile_cnt + ftiles;
+  MstRecs rec;
+  EAC_TRY((mst_begin<W, HAS_OD>(dest, weight, fe, od, n, m, symmetric, B, rec, s)));
+  int enq = mst_rounds(B, n, 0, first_batch(n), rec, s);
+  MstTail ht;
+  for (int attempt = 0;; ++attempt) {
+    ar.used = mark;
+    EAC_TRY(mst_forest(B, n, d_nf, ar, s));
+    int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
+    EAC_TRY((mst_outputs<W, HAS_OD>(dest, weight, fe, od, n, B, d_nf, ids, wout, labels, ar,
+                                    s)));
+    prof_end(pr, s);
+    k_pack_tail<<<1, 1, 0, s>>>(B.st, enq, d_nf, tail);
+    EAC_LAUNCH_CHECK();
+    EAC_TRY(d2h_sync(&ht, tail, sizeof(ht), s));
+    if (ht.plan_err)
+      return fail(EAC_ERR_KEY_WIDTH,
+                  "boruvka: weight range needs %d bits and vertex ids 2x%d bits (> 64)",
+                  ht.wbits, vb);
+    if (ht.overflow & 2) return fail(EAC_ERR_ARG, "boruvka: dest holds ids outside [0, n)");
+    if (ht.overflow & 4)
+      return fail(EAC_ERR_ARG, "boruvka: graph passed as symmetric is not symmetric "
+                               "(pass symmetric=0)");
+    if (ht.overflow_rounds)
+      return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", kMaxRounds);
+    if (!ht.active) break;
+    if (attempt > 0 || enq >= kMaxRounds)
+      return fail(EAC_ERR_HIP, "boruvka: no convergence after %d rounds", enq);
+    // not converged yet: finish the rounds with a read per batch, then redo
+    MstState hs;
+    for (;;) {
+      enq = mst_rounds(B, n, enq, 4, rec, s);
+      EAC_TRY(d2h_sync(&hs, B.st, kCtlHead, s));
+      bool active = false;
+      EAC_TRY(mst_check(hs, enq, vb, &active));
+      if (!active) break;
+    }
+  }
+  if (prof_on()) {
+    MstState hs;
+    EAC_TRY(d2h_sync(&hs, B.st, kStateHead, s));
+    mst_prof_bytes(hs, enq, n, B.nchunks, rec);
+  }
   prof_flush();
-  if (tail[1]) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
-  const int64_t n_f = tail[0];
+  if (ht.missing) return fail(EAC_ERR_HIP, "boruvka: forest edge not found in its row");
+  const int64_t n_f = ht.n_f;
   if (h_out) {
     h_out[0] = n_f;
     h_out[1] = n - n_f;
-    h_out[2] = fi.rounds;
-    h_out[3] = fi.wbits;
+    h_out[2] = ht.rounds;
+    h_out[3] = ht.wbits;
   }
   return EAC_OK;
 }
