@@ -727,21 +727,16 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
           x[2 * q] = a;
           x[2 * q + 1] = b;
         }
-        if (use_gmap) {
-          // both ends in the largest component: internal, no second root
-          // lookup (the mark loads go out with the first lookups)
-          uint8_t gm[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            gm[q] = gmap[x[2 * q + 1]];
-            x[2 * q] = P[x[2 * q]];
-          }
+        for (int q = 0; q < 4; ++q) x[2 * q] = P[x[2 * q]];
+        if (use_gmap) {
+          // both ends in the largest component: internal, no second lookup
 #pragma unroll
           for (int q = 0; q < 4; ++q)
-            x[2 * q + 1] = (x[2 * q] == G && gm[q]) ? G : P[x[2 * q + 1]];
+            x[2 * q + 1] = (x[2 * q] == G && gmap[x[2 * q + 1]]) ? G : P[x[2 * q + 1]];
         } else {
 #pragma unroll
-          for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
+          for (int q = 0; q < 4; ++q) x[2 * q + 1] = P[x[2 * q + 1]];
         }
         bool keep[4];
 #pragma unroll
