@@ -621,23 +621,6 @@ __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* 
   atomicMin(&best[root], (unsigned long long)key);  // table crowded: direct
 }
 
-// Plurality root of 7 fixed sample vertices: the largest component's root
-// whenever that component holds most vertices (a miss only costs speed).
-__device__ __forceinline__ int32_t giant_root(const int32_t* P, int32_t n) {
-  int32_t c[7];
-#pragma unroll
-  for (int i = 0; i < 7; ++i) c[i] = P[(int32_t)(((int64_t)(2 * i + 1) * n) / 14)];
-  int best = 0, bc = -1;
-#pragma unroll
-  for (int i = 0; i < 7; ++i) {
-    int k = 0;
-#pragma unroll
-    for (int j = 0; j < 7; ++j) k += c[j] == c[i];
-    if (k > bc) { bc = k; best = i; }
-  }
-  return c[best];
-}
-
 // min_insert from a wavefront: lanes holding the same root (the giant
 // component's, typically) are first reduced to one lane, for up to two
 // distinct roots per call, so a hot root costs one LDS atomic per wave
@@ -672,18 +655,13 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                int32_t* P, unsigned long long* __restrict__ best,
                                                int32_t n, int vb, int max_rounds,
                                                unsigned long long* __restrict__ rlog,
-                                               const uint8_t* __restrict__ gmap,
                                                unsigned int* __restrict__ overflow) {
   __shared__ int32_t s_root[kMinSlots];
   __shared__ unsigned long long s_min[kMinSlots];
   __shared__ unsigned long long s_io[2 * (kGB / kWave)];
-  const RoundCtl c0 = ctl[r & 1];
-  const RoundCtl c = next_ctl(c0, r, had_edge[r], max_rounds);
+  const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
-  // first heavy pass: the giant-component marks of k_compress are current
-  const bool use_gmap = gmap && c0.phase == 0 && c.phase == 1;
-  const int32_t G = use_gmap ? giant_root(P, n) : -1;
   for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
     s_root[i] = -1;
     s_min[i] = kInf;
@@ -728,16 +706,7 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
           x[2 * q + 1] = b;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) x[2 * q] = P[x[2 * q]];
-        if (use_gmap) {
-          // both ends in the largest component: internal, no second lookup
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            x[2 * q + 1] = (x[2 * q] == G && gmap[x[2 * q + 1]]) ? G : P[x[2 * q + 1]];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 4; ++q) x[2 * q + 1] = P[x[2 * q + 1]];
-        }
+        for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
         bool keep[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
@@ -825,28 +794,11 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
 // here and parents only ever move up) and writes only that vertex, so when
 // the kernel ends the parents are exactly flat and the filter finds a root
 // with one load.
-//
-// In the round that ends the light phase (no link, parents flat) it instead
-// marks the vertices of the largest light component: gmap[v] = 1 iff v's
-// root is G, the plurality root of 7 fixed sample vertices (all threads
-// compute the same G). The heavy filter of that round then drops an edge
-// with both ends marked without looking its second root up; a vote that
-// misses the largest component only makes fewer edges take that path.
 __global__ __launch_bounds__(kGB) void k_compress(int r, int32_t n,
                                                   const RoundCtl* __restrict__ ctl,
                                                   const unsigned int* __restrict__ had_edge,
-                                                  int32_t* P, uint8_t* __restrict__ gmap,
-                                                  unsigned int* __restrict__ overflow) {
-  const RoundCtl c = ctl[r & 1];
-  if (!c.active) return;
-  if (!had_edge[r]) {  // nothing linked: still flat
-    if (c.phase != 0 || !gmap) return;
-    const int32_t G = giant_root(P, n);
-    for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
-         v += (int64_t)gridDim.x * kGB)
-      gmap[v] = P[v] == G;
-    return;
-  }
+                                                  int32_t* P, unsigned int* __restrict__ overflow) {
+  if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked: still flat
   for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
        v += (int64_t)gridDim.x * kGB) {
     const int32_t x0 = (int32_t)v;
@@ -1242,7 +1194,6 @@ struct MstBuffers {
   uint64_t* heavy;
   int32_t* cnt_light;    // keys held by each chunk
   int32_t* cnt_heavy;
-  uint8_t* gmap;         // largest light component marks (first heavy pass)
   uint64_t* fkeys;
   int32_t* tile_cnt;
   int32_t* flag;
@@ -1267,7 +1218,6 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   b.heavy = ar.take<uint64_t>(nchunks * kGTile);
   b.cnt_light = ar.take<int32_t>(nchunks);
   b.cnt_heavy = ar.take<int32_t>(nchunks);
-  b.gmap = ar.take<uint8_t>(n);
   b.fkeys = ar.take<uint64_t>(n);
   b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 2);  // + forest count, + labels count
   b.flag = ar.take<int32_t>(n);
@@ -1413,14 +1363,13 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
                               B.mst_key, B.st->had_edge, vb, &B.st->overflow);
     EAC_LAUNCH_CHECK();
     prof_end(rec.hook.back(), s);
-    k_compress<<<gc, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, B.gmap,
-                                  &B.st->overflow);
+    k_compress<<<gc, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
     EAC_LAUNCH_CHECK();
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
     k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
                                B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
                                kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
-                               B.gmap, &B.st->overflow);
+                               &B.st->overflow);
     EAC_LAUNCH_CHECK();
     prof_end(rec.round.back(), s);
   }
