@@ -1799,10 +1799,10 @@ __global__ void k_cut_decide(const uint64_t* __restrict__ sk, int64_t nf, int32_
 
 // ------------------------------------------------- batched single-link cut
 // Graph of a forest key (by its smaller endpoint) for disjoint-union batches;
-// absent keys (kInf) get the sentinel id G and are not counted.
+// absent keys (kInf) get the sentinel id G.
 __global__ void k_graph_of_key(const uint64_t* __restrict__ sk, int64_t nf, int vb,
                                const int64_t* __restrict__ goff, int G,
-                               uint32_t* __restrict__ gid, int32_t* __restrict__ gcount) {
+                               uint32_t* __restrict__ gid) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t key = sk[i];
@@ -1818,35 +1818,79 @@ __global__ void k_graph_of_key(const uint64_t* __restrict__ sk, int64_t nf, int 
       if (goff[mid] <= a) lo = mid; else hi = mid;
     }
     gid[i] = (uint32_t)lo;
-    atomicAdd(&gcount[lo], 1);
   }
 }
 
-// Per-graph largest weight word over the graph's CSR entries (rows of a
-// graph are contiguous): one block per graph.
+// Segment starts of the graph-sorted keys: seg[g] = first i with gid[i] >= g
+// (g = 0..G; seg[G] = keys of real graphs). No counters: binary searches.
+__global__ void k_seg_bounds(const uint32_t* __restrict__ gid_s, int64_t nf, int G,
+                             int64_t* __restrict__ seg) {
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g <= G; g += gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = nf;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (gid_s[mid] < (uint32_t)g) lo = mid + 1; else hi = mid;
+    }
+    seg[g] = lo;
+  }
+}
+
+// Per-graph largest weight word over the graph's CSR entries. Rows of a
+// graph are contiguous, so (offsets without gaps) graph g owns entries
+// [fe[goff[g]], fe[goff[g+1]]): every block takes an equal entry range and
+// folds the part of each graph it overlaps, one atomicMax per (block, graph).
+// With explicit out-degrees (rows may have gaps) one block walks each graph.
 template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_graph_wmax(const W* __restrict__ weight,
                                                     const int64_t* __restrict__ fe,
                                                     const int32_t* __restrict__ od,
                                                     const int64_t* __restrict__ goff, int G,
+                                                    int64_t m,
                                                     unsigned long long* __restrict__ gmax) {
   __shared__ unsigned long long s_m[kGB / kWave];
+  if (!HAS_OD) {
+    const int64_t per = (m + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = (int64_t)blockIdx.x * per;
+    const int64_t b1 = b0 + per < m ? b0 + per : m;
+    if (b0 >= b1) return;
+    // graph holding entry b0: last g with fe[goff[g]] <= b0
+    int lo = 0, hi = G;
+    while (lo + 1 < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (fe[goff[mid]] <= b0) lo = mid; else hi = mid;
+    }
+    for (int g = lo; g < G; ++g) {  // uniform over the block
+      const int64_t gs = fe[goff[g]], ge = fe[goff[g + 1]];
+      if (gs >= b1) break;
+      const int64_t e0 = gs > b0 ? gs : b0, e1 = ge < b1 ? ge : b1;
+      unsigned long long mx = 0;
+      for (int64_t q = e0 + threadIdx.x; q < e1; q += kGB) {
+        const unsigned long long x = word_of<W>(weight[q]);
+        mx = x > mx ? x : mx;
+      }
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, off, kWave);
+        mx = o > mx ? o : mx;
+      }
+      if (lane_id() == 0) s_m[threadIdx.x / kWave] = mx;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int q = 1; q < kGB / kWave; ++q) mx = s_m[q] > mx ? s_m[q] : mx;
+        if (e1 > e0) atomicMax(&gmax[g], mx);
+      }
+      __syncthreads();
+    }
+    return;
+  }
   for (int g = blockIdx.x; g < G; g += gridDim.x) {
     unsigned long long mx = 0;
     const int64_t v0 = goff[g], v1 = goff[g + 1];
-    if (!HAS_OD) {  // the graph's rows are one contiguous entry range
-      const int64_t e = v1 > v0 ? fe[v1] : 0;
-      for (int64_t q = (v1 > v0 ? fe[v0] : 0) + threadIdx.x; q < e; q += kGB) {
+    for (int64_t v = v0; v < v1; ++v) {
+      int64_t b = fe[v], e = b + od[v];
+      for (int64_t q = b + threadIdx.x; q < e; q += kGB) {
         unsigned long long x = word_of<W>(weight[q]);
         mx = x > mx ? x : mx;
-      }
-    } else {
-      for (int64_t v = v0; v < v1; ++v) {
-        int64_t b = fe[v], e = b + od[v];
-        for (int64_t q = b + threadIdx.x; q < e; q += kGB) {
-          unsigned long long x = word_of<W>(weight[q]);
-          mx = x > mx ? x : mx;
-        }
       }
     }
 #pragma unroll
@@ -1858,7 +1902,7 @@ __global__ __launch_bounds__(kGB) void k_graph_wmax(const W* __restrict__ weight
     __syncthreads();
     if (threadIdx.x == 0) {
       for (int q = 1; q < kGB / kWave; ++q) mx = s_m[q] > mx ? s_m[q] : mx;
-      gmax[g] = mx;
+      atomicMax(&gmax[g], mx);
     }
     __syncthreads();
   }
@@ -2144,7 +2188,6 @@ int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, con
   uint32_t* gid_s = ar.take<uint32_t>(N);
   int64_t* goff_ws = ar.take<int64_t>(G + 1);
   int64_t* seg = ar.take<int64_t>(G + 2);
-  int32_t* gcount = ar.take<int32_t>(G + 2);
   unsigned long long* gmax = ar.take<unsigned long long>(G);
   GraphCut* cut = ar.take<GraphCut>(G);
   int32_t* base = ar.take<int32_t>(G + 1);
@@ -2187,18 +2230,20 @@ int sl_batched_impl(const int32_t* dest, const W* weight, const int64_t* fe, con
   }
   EAC_TRY((radix_sort<uint64_t, NoValue>(B.fkeys, sk, nullptr, nullptr, nf, 0, key_bits, ar, s)));
   ar.used = mark0;
-  EAC_TRY(fill<int32_t>(gcount, G + 2, 0, s));
   if (nf > 0) {
-    k_graph_of_key<<<g1, kGB, 0, s>>>(sk, nf, vb, goff, G, gid, gcount);
+    k_graph_of_key<<<g1, kGB, 0, s>>>(sk, nf, vb, goff, G, gid);
     EAC_LAUNCH_CHECK();
     const int gb = bits_for((uint64_t)G);
     EAC_TRY((radix_sort<uint32_t, uint64_t>(gid, gid_s, sk, sk2, nf, 0, gb < 1 ? 1 : gb, ar, s)));
     ar.used = mark0;
   }
   // seg[G] = present forest keys; they lead sk2, grouped by graph
-  EAC_TRY((scan_excl<int32_t, int64_t>(gcount, seg, G + 1, 0, nullptr, ar, s)));
-  ar.used = mark0;
-  k_graph_wmax<W, HAS_OD><<<G < 4096 ? G : 4096, kGB, 0, s>>>(weight, fe, od, goff, G, gmax);
+  k_seg_bounds<<<grid_for(G + 1, 256), 256, 0, s>>>(gid_s, nf, G, seg);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY(fill<uint64_t>((uint64_t*)gmax, G, 0ull, s));
+  k_graph_wmax<W, HAS_OD><<<HAS_OD ? (G < 4096 ? G : 4096)
+                                    : grid_for(M, 1, resident_grid(k_graph_wmax<W, HAS_OD>, kGB)),
+                            kGB, 0, s>>>(weight, fe, od, goff, G, M, gmax);
   EAC_LAUNCH_CHECK();
   k_seg_cut<W><<<G < 4096 ? G : 4096, kGB, 0, s>>>(sk2, seg, goff, G, vb, fi.plan, gmax, rule,
                                                   max_dist, n_clusters, cut);
