@@ -161,6 +161,12 @@ def run_mst(args):
         return boruvka_mst(d_dest, d_w, d_ip, None, want_ids=True, want_weights=True,
                            want_labels=True)
 
+    # knobs (light/heavy split, first round batch, filter blocks per CU) from
+    # the autotuner: timed candidates on this graph, same forest for each
+    from masters_final_amd.tune import autotune_mst
+    tuning = autotune_mst(d_dest, d_w, d_ip, use_cache=not args.retune)
+    log(f"tuning: {tuning['knobs']} ({tuning['ms']:.3f} ms, cached={tuning['cached']})")
+
     for _ in range(args.warmup):
         r = step()
     torch.cuda.synchronize()
@@ -214,6 +220,7 @@ def run_mst(args):
                        "traffic_source": "profiles/boruvka_mst_knn1m_hbm_traffic.md",
                        "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl}
     out["kernels_ms_per_step"] = {MST_SLOTS[s]: prof[s][0] / args.steps for s in prof}
+    out["tuning"] = tuning
     out["cpu_baseline"] = cpu
     if not args.no_eac:
         log("end-to-end EAC: batched+BF16 vs plain per-call FP32 (n=10,000, P=100)")
@@ -461,6 +468,8 @@ def main():
     ap.add_argument("--vmax", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eac", action="store_true")
+    ap.add_argument("--retune", action="store_true",
+                    help="ignore the autotuner cache and time the candidates again")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a 256 MiB elementwise copy kernel 5 times (counter "
                          "calibration: 256 MiB read + 256 MiB written per launch)")

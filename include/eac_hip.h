@@ -72,6 +72,23 @@ int eac_abi_version(void);
 int eac_profile_enable(int32_t on);
 int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t* alg_bytes);
 
+/* ---- tuning knobs (no counterpart in the reference) ---------------------
+ * Launch and algorithm parameters that change speed, never results; set by
+ * the autotuner in masters_final_amd/tune.py (or by hand) for the calls
+ * that follow on this process. 0 selects the built-in choice.
+ *   EAC_TUNE_LIGHT_X10      light/heavy split of Borůvka: light undirected
+ *                           edges per vertex x 10 (default 15; -1 = no split)
+ *   EAC_TUNE_FIRST_BATCH    rounds enqueued before the first read-back
+ *                           (default 3/4 of bits(n-1), at least 6)
+ *   EAC_TUNE_FILTER_BPCU    blocks per compute unit of the edge-list filter
+ *                           (default: as many as are resident) */
+#define EAC_TUNE_LIGHT_X10 0
+#define EAC_TUNE_FIRST_BATCH 1
+#define EAC_TUNE_FILTER_BPCU 2
+#define EAC_TUNE_COUNT 8
+int eac_tune_set(int32_t knob, int32_t value);
+int eac_tune_get(int32_t knob, int32_t* value);
+
 /* ---- exclusive prefix sum ----------------------------------------------
  * out[i] = init + sum(in[0:i]); *d_total (device, may be NULL) = init + sum(in).
  * in == out is allowed (in place).

@@ -57,6 +57,8 @@ SIGNATURES = {
     "eac_sl_lifetime_batched_graph_launch": (C.c_int, [_p, _p]),
     "eac_sl_lifetime_batched_graph_rounds": (C.c_int, [_p]),
     "eac_sl_lifetime_batched_graph_destroy": (C.c_int, [_p]),
+    "eac_tune_set": (C.c_int, [_i32, _i32]),
+    "eac_tune_get": (C.c_int, [_i32, _p]),
     "eac_sort_workspace_bytes": (_sz, [_i64, _i32, _i32]),
     "eac_radix_sort_u64": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, _p, _sz, _p]),
     "eac_kmeans_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),

@@ -70,6 +70,7 @@ struct ProfRec {
   hipEvent_t a, b;
   int64_t bytes;
 };
+int tune_value(int knob);  // EAC_TUNE_* setting, 0 = built-in choice
 bool prof_on();
 void prof_suspend(bool on);  // no records while on (hipGraph capture)
 int prof_begin(int slot, int64_t bytes, hipStream_t s);  // returns record id or -1

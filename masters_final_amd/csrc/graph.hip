@@ -1327,10 +1327,11 @@ int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int
   // forest's light part; heavy edges then need one filtering pass against
   // the light components before the remaining rounds. Only worth it on
   // graphs with many edges per vertex. The threshold keeps about 1.5 light
-  // undirected edges per vertex: a fraction 3n/m of the entries of a
-  // symmetric CSR, 1.5n/m of a one-entry-per-edge CSR.
-  const bool try_split = m >= 6 * (int64_t)n && m >= (1 << 16);
-  const double frac = (symmetric ? 3.0 : 1.5) * n / (double)m;
+  // undirected edges per vertex (EAC_TUNE_LIGHT_X10): a fraction 3n/m of the
+  // entries of a symmetric CSR, 1.5n/m of a one-entry-per-edge CSR.
+  const int light_x10 = tune_value(EAC_TUNE_LIGHT_X10) ? tune_value(EAC_TUNE_LIGHT_X10) : 15;
+  const bool try_split = light_x10 > 0 && m >= 6 * (int64_t)n && m >= (1 << 16);
+  const double frac = (symmetric ? 2.0 : 1.0) * (light_x10 / 10.0) * n / (double)m;
   k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb,
                                        try_split ? 1 : 0, frac, &B.st->plan);
   EAC_LAUNCH_CHECK();
@@ -1355,7 +1356,15 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
   const int vb = vbits_for(n);
   const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
   const int gc = grid_for(n, kGB, resident_grid(k_compress, kGB));
-  const int gr = grid_for(B.nchunks, 1, resident_grid(k_round, kGB));
+  int gr_cap = resident_grid(k_round, kGB);
+  if (tune_value(EAC_TUNE_FILTER_BPCU) > 0) {
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int want = cus * tune_value(EAC_TUNE_FILTER_BPCU);
+    gr_cap = want < gr_cap ? want : gr_cap;
+  }
+  const int gr = grid_for(B.nchunks, 1, gr_cap);
   int r = r0;
   for (; r < r0 + count && r < kMaxRounds; ++r) {
     rec.hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
@@ -1424,6 +1433,7 @@ int mst_forest(MstBuffers& B, int32_t n, int32_t* d_nf, Arena& ar, hipStream_t s
 inline int first_batch(int32_t n) {
   // about the rounds a graph of this size needs (idle rounds cost a few
   // microseconds, a host read tens)
+  if (tune_value(EAC_TUNE_FIRST_BATCH) > 0) return tune_value(EAC_TUNE_FIRST_BATCH);
   const int vb = vbits_for(n);
   return vb * 3 / 4 > 6 ? vb * 3 / 4 : 6;
 }

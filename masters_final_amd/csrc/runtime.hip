@@ -126,3 +126,23 @@ extern "C" int eac_profile_read(int32_t slot, double* total_ms, int64_t* launche
 extern "C" const char* eac_last_error(void) { return eac::g_err; }
 
 extern "C" int eac_abi_version(void) { return 1; }
+
+// ------------------------------------------------------------ tuning knobs
+namespace eac {
+static int g_tune[EAC_TUNE_COUNT] = {0};
+int tune_value(int knob) { return (knob >= 0 && knob < EAC_TUNE_COUNT) ? g_tune[knob] : 0; }
+}  // namespace eac
+
+extern "C" int eac_tune_set(int32_t knob, int32_t value) {
+  if (knob < 0 || knob >= EAC_TUNE_COUNT) return eac::fail(EAC_ERR_ARG, "tune: bad knob %d", knob);
+  if (value < -1 || value > (1 << 20)) return eac::fail(EAC_ERR_ARG, "tune: bad value %d", value);
+  eac::g_tune[knob] = value;
+  return EAC_OK;
+}
+
+extern "C" int eac_tune_get(int32_t knob, int32_t* value) {
+  if (knob < 0 || knob >= EAC_TUNE_COUNT || !value)
+    return eac::fail(EAC_ERR_ARG, "tune: bad knob %d", knob);
+  *value = eac::g_tune[knob];
+  return EAC_OK;
+}

@@ -54,3 +54,18 @@ def test_no_cpu_fallback_without_gpu():
     with pytest.raises(_lib.EACUnavailable):
         boruvka_minho_seq(np.array([1, 0], np.int32), np.array([1, 1], np.int32),
                           np.array([0, 1], np.int32), np.array([1, 1], np.int32))
+
+
+def test_tuning_knobs_roundtrip():
+    from masters_final_amd import tune
+    from masters_final_amd._lib import EACError
+    before = tune.get_knobs()
+    try:
+        tune.set_knobs(light_x10=25, first_batch=9, filter_bpcu=3)
+        assert tune.get_knobs() == {"light_x10": 25, "first_batch": 9, "filter_bpcu": 3}
+        tune.set_knobs(light_x10=-1)
+        assert tune.get_knobs()["light_x10"] == -1
+        lib = __import__("masters_final_amd._lib", fromlist=["x"]).load_library()
+        assert lib.eac_tune_set(99, 1) != 0
+    finally:
+        tune.set_knobs(**before)

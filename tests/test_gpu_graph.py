@@ -247,3 +247,40 @@ def test_graph_from_edges_matches_oracle(torch, elt4):
     assert np.array_equal(nd.cpu().numpy(), want2[2]) and np.array_equal(od2.cpu().numpy(), want2[0])
     assert getGraphFromEdges_seq(g["dest"], g["weight"], g["fe"], g["od"],
                                  np.array([len(g["dest"]) + 3]), nod, nfe, ndest, nw) == -1
+
+
+@pytest.mark.parametrize("light_x10,first_batch,bpcu", [
+    (-1, 0, 0), (5, 2, 1), (40, 3, 2), (15, 30, 0)])
+def test_boruvka_tuning_knobs_keep_the_forest(torch, light_x10, first_batch, bpcu):
+    """Every tuning knob setting gives the canonical forest (knobs change
+    speed only): no split, tiny and large light fractions, a first round
+    batch far too short (rounds continue after the read-back) or long."""
+    from masters_final_amd import tune
+    from masters_final_amd.graph.mst import boruvka_mst
+    g = synth.random_graph(30000, 16, 21, wmax=60, n_parts=3)
+    before = tune.get_knobs()
+    try:
+        tune.set_knobs(light_x10=light_x10, first_batch=first_batch, filter_bpcu=bpcu)
+        r = boruvka_mst(g["dest"], g["weight"], g["indptr"], None, want_weights=True,
+                        want_labels=True)
+        _check_forest(g, r)
+    finally:
+        tune.set_knobs(**before)
+
+
+def test_autotune_mst_applies_a_candidate(torch, tmp_path, monkeypatch):
+    from masters_final_amd import tune
+    monkeypatch.setenv("EAC_TUNE_CACHE", str(tmp_path / "tune.json"))
+    g = synth.random_graph(20000, 16, 22, wmax=50)
+    d = {k: torch.from_numpy(g[k]).cuda() for k in ("dest", "weight", "indptr")}
+    before = tune.get_knobs()
+    try:
+        res = tune.autotune_mst(d["dest"], d["weight"], d["indptr"], reps=1,
+                                grid={"light_x10": (-1, 15), "first_batch": (0,),
+                                      "filter_bpcu": (0,)})
+        assert not res["cached"] and len(res["candidates"]) == 2
+        assert tune.get_knobs()["light_x10"] == res["knobs"]["light_x10"]
+        again = tune.autotune_mst(d["dest"], d["weight"], d["indptr"])
+        assert again["cached"] and again["knobs"] == res["knobs"]
+    finally:
+        tune.set_knobs(**before)
