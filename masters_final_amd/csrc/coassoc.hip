@@ -133,16 +133,22 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
     uint32_t nnz_local = 0;
     for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += kWin) {
       const int64_t c1 = c0 + kWin < n ? c0 + kWin : n;
-      for (int v = t; v < kWin / 2; v += kCB) s_cnt[v] = 0;
+      const int words_used = (int)((c1 - c0 + 1) >> 1);
+      for (int v = t; v < words_used; v += kCB) s_cnt[v] = 0;
       __syncthreads();
-      // members of this window: [cur, hi) with member < c1 (lists ascending)
-      for (int p = t; p < P; p += kCB) {
-        int32_t lo = s_cur[p], hi = s_end[p];
-        while (lo < hi) {
-          int32_t mid = (lo + hi) >> 1;
-          if (pl.members[mid] < c1) lo = mid + 1; else hi = mid;
+      // members of this window: [cur, hi) with member < c1 (lists ascending);
+      // the last window takes every remaining member, no search
+      if (c1 == n) {
+        for (int p = t; p < P; p += kCB) s_hi[p] = s_end[p];
+      } else {
+        for (int p = t; p < P; p += kCB) {
+          int32_t lo = s_cur[p], hi = s_end[p];
+          while (lo < hi) {
+            int32_t mid = (lo + hi) >> 1;
+            if (pl.members[mid] < c1) lo = mid + 1; else hi = mid;
+          }
+          s_hi[p] = lo;
         }
-        s_hi[p] = lo;
       }
       __syncthreads();
       // flattened ranges: prefix of (hi - cur) over partitions
@@ -159,12 +165,23 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
       }
       __syncthreads();
       const int32_t tot = s_pre[P];
+      // four member loads in flight per thread (f, f + kCB, ...)
       int p = 0;
-      for (int32_t f = t; f < tot; f += kCB) {
-        while (s_pre[p + 1] <= f) ++p;
-        int32_t j = pl.members[s_cur[p] + (f - s_pre[p])];
-        if (j != (int32_t)i) {
-          const int jj = (int)(j - c0);
+      for (int32_t f0 = t; f0 < tot; f0 += 4 * kCB) {
+        int32_t j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t f = f0 + u * kCB;
+          j[u] = -1;
+          if (f < tot) {
+            while (s_pre[p + 1] <= f) ++p;
+            j[u] = pl.members[s_cur[p] + (f - s_pre[p])];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (j[u] < 0 || j[u] == (int32_t)i) continue;
+          const int jj = (int)(j[u] - c0);
           atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
         }
       }
