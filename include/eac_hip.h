@@ -81,10 +81,14 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *   EAC_TUNE_FIRST_BATCH    rounds enqueued before the first read-back
  *                           (default 3/4 of bits(n-1), at least 6)
  *   EAC_TUNE_FILTER_BPCU    blocks per compute unit of the edge-list filter
- *                           (default: as many as are resident) */
+ *                           (default: as many as are resident)
+ *   EAC_TUNE_CO_WINDOW      co-association LDS window in columns (even,
+ *                           <= 32768; default: the row range when n is
+ *                           smaller, else 32768) */
 #define EAC_TUNE_LIGHT_X10 0
 #define EAC_TUNE_FIRST_BATCH 1
 #define EAC_TUNE_FILTER_BPCU 2
+#define EAC_TUNE_CO_WINDOW 3
 #define EAC_TUNE_COUNT 8
 int eac_tune_set(int32_t knob, int32_t value);
 int eac_tune_get(int32_t knob, int32_t* value);

@@ -92,7 +92,7 @@ struct CoPlan {
 // full rows) window by window. PASS 0: per-row nnz and max count. PASS 1:
 // write (col, value) at out_off.
 template <int PASS>
-__global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
+__global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, int win,
                                                       unsigned long long* __restrict__ row_ctr,
                                                       int32_t* __restrict__ row_nnz,
                                                       unsigned int* __restrict__ max_count,
@@ -100,11 +100,15 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
                                                       int32_t w_offset,
                                                       int32_t* __restrict__ out_col,
                                                       int32_t* __restrict__ out_val) {
-  __shared__ uint32_t s_cnt[kWin / 2];
-  __shared__ int32_t s_cur[kMaxP];   // next member position per partition
-  __shared__ int32_t s_end[kMaxP];   // end of the cluster per partition
-  __shared__ int32_t s_hi[kMaxP];    // end of this window's members
-  __shared__ int32_t s_pre[kMaxP + 1];
+  // launch-time LDS: window counters (win / 2 words), then per partition
+  // the next member position, the cluster end, this window's member end and
+  // the flattened-range prefix (P + 1)
+  extern __shared__ uint32_t s_dyn[];
+  uint32_t* s_cnt = s_dyn;
+  int32_t* s_cur = reinterpret_cast<int32_t*>(s_dyn + win / 2);
+  int32_t* s_end = s_cur + pl.P;
+  int32_t* s_hi = s_end + pl.P;
+  int32_t* s_pre = s_hi + pl.P;
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
   __shared__ unsigned int s_max;
@@ -131,8 +135,8 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows,
     int64_t row_base = PASS == 1 ? indptr[i] : 0;
     int64_t written = 0;
     uint32_t nnz_local = 0;
-    for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += kWin) {
-      const int64_t c1 = c0 + kWin < n ? c0 + kWin : n;
+    for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += win) {
+      const int64_t c1 = c0 + win < n ? c0 + win : n;
       const int words_used = (int)((c1 - c0 + 1) >> 1);
       for (int v = t; v < words_used; v += kCB) s_cnt[v] = 0;
       __syncthreads();
@@ -310,6 +314,34 @@ int co_check(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P)
 }
 
 }  // namespace
+// Columns per LDS window (even, <= kWin): the whole row range when n is
+// small (fewer LDS bytes, more resident workgroups), else kWin, or the
+// EAC_TUNE_CO_WINDOW setting (the autotuner trades windows per row against
+// workgroups per compute unit).
+int co_window(int64_t n) {
+  int w = tune_value(EAC_TUNE_CO_WINDOW);
+  if (w <= 0) w = kWin;
+  if (w > kWin) w = kWin;
+  const int64_t need = ((n + 63) / 64) * 64;
+  if (need < w) w = (int)need;
+  return w < 64 ? 64 : (w & ~1);
+}
+inline size_t co_lds(int win, int P) { return (size_t)(win / 2 + 4 * P + 1) * 4; }
+// Every workgroup resident at once (rows are handed out dynamically).
+template <int PASS>
+int co_grid(int win, int P) {
+  int dev = 0, cus = 256, per = 1;
+  if (hipGetDevice(&dev) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  // allow launch-time LDS beyond 64 KB (the window of large graphs)
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_coassoc_rows<PASS>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)co_lds(win, P));
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coassoc_rows<PASS>, kCB,
+                                                   co_lds(win, P)) != hipSuccess || per < 1)
+    per = 1;
+  return cus * per;
+}
+
 }  // namespace eac
 
 // ------------------------------------------------------------------ C ABI
@@ -366,7 +398,9 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
   prof_end(pr, s);
   CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart};
   int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
-  k_coassoc_rows<0><<<2048, kCB, 0, s>>>(pl, full_rows, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0,
+  const int win = co_window(n);
+  k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(pl, full_rows, win, B.ctr, B.row_nnz,
+                                                          B.misc + 1, nullptr, 0,
                                          nullptr, nullptr);
   EAC_LAUNCH_CHECK();
   prof_end(pr2, s);
@@ -404,7 +438,9 @@ extern "C" int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t*
   int64_t nnz = 0;
   EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
   int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
-  k_coassoc_rows<1><<<2048, kCB, 0, s>>>(pl, full_rows, B.ctr, nullptr, nullptr, indptr, w_offset,
+  const int win = co_window(n);
+  k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(pl, full_rows, win, B.ctr, nullptr,
+                                                          nullptr, indptr, w_offset,
                                          indices, data);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
