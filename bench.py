@@ -325,6 +325,9 @@ def run_coassoc(args):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     log(f"ensemble {t1 - t0:.2f}s, rescored {fit['n_rescored']}")
+    from masters_final_amd.tune import autotune_coassoc
+    tuning = autotune_coassoc(labels, ks, use_cache=not args.retune)
+    log(f"tuning: {tuning['knobs']} ({tuning['ms']:.1f} ms, cached={tuning['cached']})")
     L.check(lib.eac_profile_enable(1))
     (ip, ind, dat, mx), ms = timed(lambda: coassoc_device(labels, ks), args.steps, args.warmup,
                                    torch)
@@ -345,6 +348,7 @@ def run_coassoc(args):
                      "ensemble_seconds": t1 - t0, "rescored": fit["n_rescored"],
                      "count_ms": prof[9][0] / max(prof[9][1], 1), "fill_ms": fill_ms,
                      "plan_ms": prof[8][0] / max(prof[8][1], 1)}
+    out["tuning"] = tuning
     ach = fill_bytes / (fill_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "kernel": "k_coassoc_rows<fill>", "achieved": ach,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,

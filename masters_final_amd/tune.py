@@ -2,8 +2,8 @@
 
 The knobs change speed only, never results: the Borůvka light/heavy split
 (light undirected edges per vertex x 10, -1 = no split), the number of
-rounds enqueued before the first read-back, and the blocks per compute unit
-of the edge-list filter. `autotune_mst` times a grid of candidate settings
+rounds enqueued before the first read-back, the blocks per compute unit of
+the edge-list filter, and the co-association LDS window (columns). `autotune_mst` times a grid of candidate settings
 on the caller's own graph with HIP events, checks that every candidate gives
 the same forest, keeps the fastest, and caches the choice per device, weight
 type and graph-shape bucket (JSON under $EAC_TUNE_CACHE, default
@@ -19,7 +19,7 @@ from pathlib import Path
 
 from . import _lib as L
 
-KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2}
+KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3}
 
 DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),
@@ -108,6 +108,48 @@ def autotune_mst(dest, weight, indptr, grid=None, reps=3, use_cache=True):
         ev1.record()
         ev1.synchronize()
         results.append((ev0.elapsed_time(ev1) / reps, knobs))
+    results.sort(key=lambda x: x[0])
+    best_ms, best = results[0]
+    set_knobs(**best)
+    cache[key] = {"knobs": best, "ms": best_ms}
+    if use_cache:
+        _store_cache(cache)
+    return {"knobs": best, "ms": best_ms, "cached": False,
+            "candidates": [{"ms": ms, **k} for ms, k in results]}
+
+
+def autotune_coassoc(labels, ks, windows=(8192, 16384, 32768), use_cache=True):
+    """Picks the co-association LDS window for this ensemble (device labels
+    [P][n], cluster counts ks): timed candidates, identical output checked
+    (row offsets and the largest count). Returns dict(knobs, ms, ...)."""
+    from .EAC.sparse import coassoc_device
+    t = L.torch()
+    P, n = labels.shape
+    dev = t.cuda.get_device_name(t.cuda.current_device())
+    key = "coassoc|%s|n2^%d|P%d|k%d" % (dev, int(n).bit_length(), int(P),
+                                         int(sum(int(k) for k in ks)).bit_length())
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        set_knobs(**cache[key]["knobs"])
+        return {"knobs": cache[key]["knobs"], "ms": cache[key]["ms"], "candidates": None,
+                "cached": True}
+    results, ref = [], None
+    for w in windows:
+        if w > 32768:
+            continue
+        set_knobs(co_window=w)
+        ev0, ev1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        ev0.record()
+        ip, ind, dat, mx = coassoc_device(labels, ks)
+        ev1.record()
+        ev1.synchronize()
+        sig = (int(ip[-1].item()), int(mx), int(ip[n // 2].item()))
+        if ref is None:
+            ref = sig
+        elif sig != ref:
+            raise RuntimeError(f"co_window={w} changed the co-association: {sig} vs {ref}")
+        del ip, ind, dat
+        results.append((ev0.elapsed_time(ev1), {"co_window": w}))
     results.sort(key=lambda x: x[0])
     best_ms, best = results[0]
     set_knobs(**best)

@@ -133,3 +133,25 @@ def test_eac_class_end_to_end(torch):
     e.buildMatrix(ens)
     got = e.finalClustering()
     assert e.n_fclusts == nc and np.array_equal(got, lab)
+
+
+@pytest.mark.parametrize("window", [64, 1000, 4096])
+def test_coassoc_window_knob_keeps_the_matrix(torch, window):
+    """Any co-association window (several windows per row included) gives
+    the oracle's matrix."""
+    from masters_final_amd import tune
+    from masters_final_amd.EAC.sparse import coassoc_device
+    rng = np.random.default_rng(window)
+    n, P = 3000, 12
+    ks = rng.integers(3, 40, P)
+    lab = np.stack([rng.integers(0, k, n) for k in ks]).astype(np.int32)
+    before = tune.get_knobs()
+    try:
+        tune.set_knobs(co_window=window)
+        ip, ind, dat, mx = coassoc_device(torch.from_numpy(lab).cuda(), ks, w_offset=0)
+        wip, wind, wcnt = O.coassoc_condensed(lab, ks)
+        assert np.array_equal(ip.cpu().numpy(), wip)
+        assert np.array_equal(ind.cpu().numpy(), wind)
+        assert np.array_equal(dat.cpu().numpy(), wcnt)
+    finally:
+        tune.set_knobs(**before)
