@@ -84,11 +84,15 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *                           (default: as many as are resident)
  *   EAC_TUNE_CO_WINDOW      co-association LDS window in columns (even,
  *                           <= 32768; default: the row range when n is
- *                           smaller, else 32768) */
+ *                           smaller, else 32768)
+ *   EAC_TUNE_LEVEL_STEP     growth of the light edges per vertex from one
+ *                           weight level of the split to the next (default
+ *                           4, up to 4 levels; 1 or -1 = one split only) */
 #define EAC_TUNE_LIGHT_X10 0
 #define EAC_TUNE_FIRST_BATCH 1
 #define EAC_TUNE_FILTER_BPCU 2
 #define EAC_TUNE_CO_WINDOW 3
+#define EAC_TUNE_LEVEL_STEP 4
 #define EAC_TUNE_COUNT 8
 int eac_tune_set(int32_t knob, int32_t value);
 int eac_tune_get(int32_t knob, int32_t* value);
@@ -214,7 +218,8 @@ int eac_sl_lifetime_batched(const int32_t* dest, const void* weight, int32_t wei
  * 2 dest ids outside [0, N), 4 (unused), 8 key
  * wider than 64 bits, 16 rounds not converged within max_rounds, 32 bad
  * goff); results are valid only when the flags are 0. max_rounds <= 0
- * takes the bound 2 * (bits(N-1) + 2) that always suffices; a smaller value
+ * takes the bound L * (bits(N-1) + 3) that always suffices (L: weight
+ * levels of the Borůvka split, 1 to 4 by edges per vertex); a smaller value
  * shortens replays and sets flag 16 when a batch needs more. Launches on
  * one graph must not overlap (they share ws). */
 typedef struct eac_sl_graph eac_sl_graph;
@@ -227,6 +232,10 @@ int eac_sl_lifetime_batched_graph_create(const int32_t* dest, const void* weight
                                          int64_t* summary, void* ws, size_t ws_bytes,
                                          eac_sl_graph** out);
 int eac_sl_lifetime_batched_graph_launch(eac_sl_graph* graph, void* stream);
+/* Borůvka rounds that always suffice for a graph of n vertices and m CSR
+ * entries under the current tuning knobs (the max_rounds <= 0 choice of
+ * _graph_create); 0 for bad sizes. No counterpart in the reference. */
+int eac_mst_round_bound(int32_t n, int64_t m, int32_t symmetric);
 int eac_sl_lifetime_batched_graph_rounds(const eac_sl_graph* graph);
 int eac_sl_lifetime_batched_graph_destroy(eac_sl_graph* graph);
 

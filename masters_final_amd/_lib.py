@@ -56,6 +56,7 @@ SIGNATURES = {
                                                        _p, _p, _p, _p, _sz, _p]),
     "eac_sl_lifetime_batched_graph_launch": (C.c_int, [_p, _p]),
     "eac_sl_lifetime_batched_graph_rounds": (C.c_int, [_p]),
+    "eac_mst_round_bound": (C.c_int, [_i32, _i64, _i32]),
     "eac_sl_lifetime_batched_graph_destroy": (C.c_int, [_p]),
     "eac_tune_set": (C.c_int, [_i32, _i32]),
     "eac_tune_get": (C.c_int, [_i32, _p]),

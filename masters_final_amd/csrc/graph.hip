@@ -282,45 +282,52 @@ constexpr int kHistBins = 4096;
 
 // Weight offset, key width and light/heavy threshold of one call, computed
 // on the device so that no host round trip sits between the passes.
+constexpr int kLevels = 4;  // weight levels of the split (the last is unbounded)
 struct MstPlan {
   unsigned long long wlo;        // weight word of the lightest entry
   unsigned long long whi;        // weight word of the heaviest entry
   unsigned long long light_lim;  // keys with (word - wlo) >= light_lim are heavy
+  // level thresholds: level L takes the keys with lim[L-1] <= word - wlo <
+  // lim[L] (lim[0] = light_lim, lim[kLevels-1] = ~0; ~0 ends the levels)
+  unsigned long long lim[kLevels];
   int wbits;                     // bits of whi - wlo
   int err;                       // 1: wbits + 2 * vb > 64
 };
-// One block: folds the exact weight range, then (try_split) estimates the
-// weight quantile `frac` from kPlanSamples evenly spaced entries with a
-// 4096-bin histogram of the range. The threshold only steers the split;
-// any value gives the same forest.
+// One block: folds the exact weight range, then (levels > 1) estimates the
+// weight quantiles frac, frac * step, frac * step^2 ... (levels - 1 of them)
+// from kPlanSamples evenly spaced entries with a 4096-bin histogram of the
+// range: the level thresholds of the split. They only steer the split; any
+// values give the same forest.
 constexpr int kPlanThreads = 256;
 constexpr int kPlanSamples = 8192;
 template <typename W>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(
     const W* __restrict__ weight, int64_t m, const unsigned long long* __restrict__ range_lo,
-    const unsigned long long* __restrict__ range_hi, int vb, int try_split, double frac,
-    MstPlan* __restrict__ plan) {
+    const unsigned long long* __restrict__ range_hi, int vb, int levels, double frac,
+    double step, MstPlan* __restrict__ plan) {
   __shared__ unsigned int h[kHistBins];
   __shared__ unsigned int s_tmp[kPlanThreads / kWave + 1];
   __shared__ uint64_t s_lo, s_hi;
-  __shared__ int s_found;
+  __shared__ int s_found[kLevels];
   if (threadIdx.x < kWave) {
     uint64_t l, hh;
     fold_range(range_lo, range_hi, &l, &hh);
     if (threadIdx.x == 0) {
       s_lo = l;
       s_hi = hh;
-      s_found = kHistBins;
     }
   }
+  if (threadIdx.x < kLevels) s_found[threadIdx.x] = kHistBins;
   for (int i = threadIdx.x; i < kHistBins; i += kPlanThreads) h[i] = 0;
   __syncthreads();
   uint64_t wlo = s_lo, whi = s_hi;
   if (wlo == kInf) { wlo = 0; whi = 0; }
   int wbits = 0;
   while (wbits < 64 && ((whi - wlo) >> wbits) != 0) ++wbits;
-  uint64_t lim = kInf;
-  if (try_split && m > 0) {  // uniform over the block
+  uint64_t lim[kLevels];
+#pragma unroll
+  for (int L = 0; L < kLevels; ++L) lim[L] = kInf;
+  if (levels > 1 && m > 0) {  // uniform over the block
     const int shift = wbits > 12 ? wbits - 12 : 0;
     for (int i = threadIdx.x; i < kPlanSamples; i += kPlanThreads) {
       const int64_t e = (int64_t)((double)i * (double)m / kPlanSamples);
@@ -333,27 +340,39 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(
     for (int q = 0; q < per; ++q) part += h[threadIdx.x * per + q];
     unsigned int total;
     const unsigned int excl = block_excl_scan<unsigned int>(part, s_tmp, &total);
-    const double target = frac * kPlanSamples;
-    if ((double)excl < target && (double)(excl + part) >= target) {
-      unsigned int cum = excl;
-      for (int q = 0; q < per; ++q) {
-        cum += h[threadIdx.x * per + q];
-        if ((double)cum >= target) {
-          atomicMin(&s_found, threadIdx.x * per + q);
-          break;
+    double f = frac;
+    for (int L = 0; L + 1 < levels; ++L, f *= step) {
+      const double target = f * kPlanSamples;
+      if (f < 1.0 && (double)excl < target && (double)(excl + part) >= target) {
+        unsigned int cum = excl;
+        for (int q = 0; q < per; ++q) {
+          cum += h[threadIdx.x * per + q];
+          if ((double)cum >= target) {
+            atomicMin(&s_found[L], threadIdx.x * per + q);
+            break;
+          }
         }
       }
     }
     __syncthreads();
-    if (s_found < kHistBins) lim = (uint64_t)(s_found + 1) << shift;
-    if (lim != kInf && lim > whi - wlo) lim = kInf;
+    uint64_t prev = 0;
+    for (int L = 0; L + 1 < levels; ++L) {
+      uint64_t v = s_found[L] < kHistBins ? (uint64_t)(s_found[L] + 1) << shift : kInf;
+      if (v != kInf && v > whi - wlo) v = kInf;
+      if (v != kInf && v <= prev) v = prev + 1;  // strictly increasing levels
+      if (prev == kInf) v = kInf;
+      lim[L] = v;
+      prev = v;
+    }
   }
   if (threadIdx.x == 0) {
     plan->wlo = wlo;
     plan->whi = whi;
     plan->wbits = wbits;
     plan->err = wbits + 2 * vb > 64;
-    plan->light_lim = lim;
+    plan->light_lim = lim[0];
+#pragma unroll
+    for (int L = 0; L < kLevels; ++L) plan->lim[L] = lim[L];
   }
 }
 
@@ -568,25 +587,29 @@ __device__ __forceinline__ void roots_live2(int32_t* P, int32_t* a, int32_t* b, 
 }
 
 // ------------------------------------------------------ device round control
-// A round is hook(r) -> jump(r) -> filter(r). The control state of round r
-// lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
+// A round is hook(r) -> flatten(r) -> filter(r). The control state of round
+// r lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
 // from it and from had_edge[r] (some representative had a candidate edge in
-// hook(r)), and block 0 stores it in ctl[(r + 1) & 1]. Phase 0 filters the
-// light list; when a hook finds no candidate the heavy list is filtered
-// (phase 1); when that happens in phase 1 the forest is complete.
-constexpr int kMaxRounds = 64;
+// hook(r)), and block 0 stores it in ctl[(r + 1) & 1]. Rounds filter the
+// light list; when a hook finds no candidate, the next weight level starts:
+// one routing pass moves the heavy keys of that level that still cross
+// components into the light list (dropping internal ones); after the last
+// level the forest is complete.
+constexpr int kMaxRounds = kLevels * 34;  // kLevels * (31 + 2 + 1)
 struct RoundCtl {
-  int phase;            // 0: light list, 1: heavy list
+  int phase;            // weight level whose keys the light list holds
   int active;
   int rounds;           // rounds started
   int overflow_rounds;  // stopped at the round limit
 };
 
+// A level ends when a hook finds no candidate: the next level (if its lower
+// threshold is finite) starts with a routing pass over the heavy list.
 __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had,
-                                             int max_rounds) {
+                                             int max_rounds, const MstPlan* plan) {
   if (!c.active) return c;
   if (!had) {
-    if (c.phase == 0) c.phase = 1;
+    if (c.phase + 1 < kLevels && plan->lim[c.phase] != kInf) c.phase += 1;
     else c.active = 0;
   }
   c.rounds = r + 1;
@@ -646,39 +669,23 @@ __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long l
   if (active) min_insert(s_root, s_min, root, key, best);
 }
 
-__global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
-                                               uint64_t* __restrict__ heavy,
-                                               int32_t* __restrict__ cnt_light,
-                                               int32_t* __restrict__ cnt_heavy,
-                                               int64_t nchunks, RoundCtl* ctl,
-                                               const unsigned int* __restrict__ had_edge,
-                                               int32_t* P, unsigned long long* __restrict__ best,
-                                               int32_t n, int vb, int max_rounds,
-                                               unsigned long long* __restrict__ rlog,
-                                               unsigned int* __restrict__ overflow) {
-  __shared__ int32_t s_root[kMinSlots];
-  __shared__ unsigned long long s_min[kMinSlots];
-  __shared__ unsigned long long s_io[2 * (kGB / kWave)];
-  const RoundCtl c = next_ctl(ctl[r & 1], r, had_edge[r], max_rounds);
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
-  if (!c.active) return;
-  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
-    s_root[i] = -1;
-    s_min[i] = kInf;
-  }
-  __syncthreads();
-  uint64_t* list = c.phase ? heavy : light;
-  int32_t* cnt = c.phase ? cnt_heavy : cnt_light;
+// Filters one wavefront's run of chunks of `list`. ROUTE (the first round of
+// a weight level): `list` is the heavy list, whose cross keys below `lim`
+// (the level's upper threshold, on the key's weight bits) move to the same
+// chunk of the (empty) light list and become candidates, while heavier cross
+// keys stay in the heavy chunk; otherwise `list` is the light list, filtered
+// in place.
+template <bool ROUTE>
+__device__ __forceinline__ void filter_run(uint64_t* __restrict__ list, int32_t* __restrict__ cnt,
+                                           uint64_t* __restrict__ to, int32_t* __restrict__ cnt_to,
+                                           int64_t c_begin, int64_t c_end, uint64_t lim, int vb,
+                                           const int32_t* P, int32_t* s_root,
+                                           unsigned long long* s_min,
+                                           unsigned long long* __restrict__ best,
+                                           unsigned long long* sum_in,
+                                           unsigned long long* sum_out) {
   const int lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
-  // each wavefront owns a contiguous run of chunks and reads their counts
-  // 64 at a time, so empty chunks (most of them in late rounds) cost nothing
-  const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
-  const int64_t wid = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave;
-  const int64_t per = (nchunks + nwaves - 1) / nwaves;
-  const int64_t c_begin = wid * per;
-  const int64_t c_end = c_begin + per < nchunks ? c_begin + per : nchunks;
-  unsigned long long sum_in = 0, sum_out = 0;
   for (int64_t g0 = c_begin; g0 < c_end; g0 += kWave) {
     int32_t myn = g0 + lane < c_end ? cnt[g0 + lane] : 0;
     myn = myn < 0 ? 0 : (myn > kGTile ? kGTile : myn);
@@ -689,7 +696,8 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
       const int64_t cc = g0 + l;
       const int32_t n_in = __shfl(myn, l, kWave);
       uint64_t* lc = list + cc * kGTile;
-      int32_t out = 0;
+      uint64_t* tc = ROUTE ? to + cc * kGTile : nullptr;
+      int32_t out = 0, out_to = 0;
       for (int base = 0; base < n_in; base += 4 * kWave) {
         uint64_t k[4];
 #pragma unroll
@@ -707,25 +715,80 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
-        bool keep[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], keep[i / 2], best);
+        bool keep[4], cand[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const uint64_t m = __ballot(keep[q]);
-          if (keep[q]) lc[out + __popcll(m & below)] = k[q];
-          out += __popcll(m);
+          keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
+          cand[q] = keep[q] && (!ROUTE || (k[q] >> (2 * vb)) < lim);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          if (ROUTE) {
+            const uint64_t mt = __ballot(cand[q]);
+            if (cand[q]) tc[out_to + __popcll(mt & below)] = k[q];
+            out_to += __popcll(mt);
+            const bool stay = keep[q] && !cand[q];
+            const uint64_t ms = __ballot(stay);
+            if (stay) lc[out + __popcll(ms & below)] = k[q];
+            out += __popcll(ms);
+          } else {
+            const uint64_t m = __ballot(keep[q]);
+            if (keep[q]) lc[out + __popcll(m & below)] = k[q];
+            out += __popcll(m);
+          }
         }
       }
       if (lane == 0) {
         cnt[cc] = out;
-        sum_in += (unsigned long long)n_in;
-        sum_out += (unsigned long long)out;
+        if (ROUTE) cnt_to[cc] = out_to;
+        *sum_in += (unsigned long long)n_in;
+        *sum_out += (unsigned long long)(out + out_to);
       }
     }
   }
+}
+
+__global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
+                                               uint64_t* __restrict__ heavy,
+                                               int32_t* __restrict__ cnt_light,
+                                               int32_t* __restrict__ cnt_heavy,
+                                               int64_t nchunks, RoundCtl* ctl,
+                                               const unsigned int* __restrict__ had_edge,
+                                               const MstPlan* __restrict__ plan,
+                                               int32_t* P, unsigned long long* __restrict__ best,
+                                               int32_t n, int vb, int max_rounds,
+                                               unsigned long long* __restrict__ rlog,
+                                               unsigned int* __restrict__ overflow) {
+  __shared__ int32_t s_root[kMinSlots];
+  __shared__ unsigned long long s_min[kMinSlots];
+  __shared__ unsigned long long s_io[2 * (kGB / kWave)];
+  const RoundCtl prev = ctl[r & 1];
+  const RoundCtl c = next_ctl(prev, r, had_edge[r], max_rounds, plan);
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
+  if (!c.active) return;
+  const bool route = c.phase != prev.phase;
+  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
+    s_root[i] = -1;
+    s_min[i] = kInf;
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  // each wavefront owns a contiguous run of chunks and reads their counts
+  // 64 at a time, so empty chunks (most of them in late rounds) cost nothing
+  const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
+  const int64_t wid = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave;
+  const int64_t per = (nchunks + nwaves - 1) / nwaves;
+  const int64_t c_begin = wid * per;
+  const int64_t c_end = c_begin + per < nchunks ? c_begin + per : nchunks;
+  unsigned long long sum_in = 0, sum_out = 0;
+  if (route)
+    filter_run<true>(heavy, cnt_heavy, light, cnt_light, c_begin, c_end, plan->lim[c.phase], vb,
+                     P, s_root, s_min, best, &sum_in, &sum_out);
+  else
+    filter_run<false>(light, cnt_light, nullptr, nullptr, c_begin, c_end, kInf, vb, P, s_root,
+                      s_min, best, &sum_in, &sum_out);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < kMinSlots / kGB; ++j) {
@@ -1279,12 +1342,44 @@ inline int vbits_for(int32_t n) {
   return b < 1 ? 1 : b;
 }
 
+// Weight levels of the split (Filter-Kruskal applied to Borůvka): rounds run
+// first on the lightest keys, whose forest is exactly the forest's part below
+// the first threshold; each further level starts with one routing pass over
+// the remaining (heavier) keys against the components so far, which drops
+// the internal ones. Only worth it on graphs with many edges per vertex. The
+// first threshold keeps about 1.5 light undirected edges per vertex
+// (EAC_TUNE_LIGHT_X10): a fraction 3n/m of the entries of a symmetric CSR,
+// 1.5n/m of a one-entry-per-edge CSR; each further level takes step (4,
+// EAC_TUNE_LEVEL_STEP) times as many, up to kLevels levels.
+struct SplitPlan {
+  int levels;   // 1: no split
+  double frac;  // sampled fraction below the first threshold
+  double step;  // growth of that fraction per level
+};
+inline SplitPlan split_plan(int32_t n, int64_t m, int symmetric) {
+  SplitPlan sp{1, 0.0, 1.0};
+  const int light_x10 = tune_value(EAC_TUNE_LIGHT_X10) ? tune_value(EAC_TUNE_LIGHT_X10) : 15;
+  if (light_x10 <= 0 || m < 6 * (int64_t)n || m < (1 << 16)) return sp;
+  sp.frac = (symmetric ? 2.0 : 1.0) * (light_x10 / 10.0) * n / (double)m;
+  const int step_knob = tune_value(EAC_TUNE_LEVEL_STEP);
+  const int step = step_knob == 0 ? 4 : step_knob;
+  sp.step = step > 1 ? (double)step : 1.0;
+  const int max_levels = step > 1 ? kLevels : 2;
+  double f = sp.frac;
+  sp.levels = 1;
+  while (sp.levels < max_levels && f < 1.0) {
+    ++sp.levels;
+    f *= sp.step;
+  }
+  return sp;
+}
+
 // Rounds that always suffice: every round at least halves the components
-// still active in a phase (light edges, then heavy), and each phase ends
-// with one round whose hook finds no candidate, so each phase ends within
-// bits(n) + 2 rounds.
-inline int mst_round_bound(int32_t n) {
-  const int r = 2 * (vbits_for(n) + 2);
+// still active in a level, and each level ends with one round whose hook
+// finds no candidate, so each level (its routing round included) ends
+// within bits(n) + 3 rounds.
+inline int mst_round_bound(int32_t n, int levels) {
+  const int r = levels * (vbits_for(n) + 3);
   return r < kMaxRounds ? r : kMaxRounds;
 }
 
@@ -1322,18 +1417,9 @@ int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int
   k_weight_range<W, HAS_OD><<<grange, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
                                                     B.st->range_lo, B.st->range_hi);
   EAC_LAUNCH_CHECK();
-  // Light/heavy split (Filter-Kruskal applied to Boruvka): rounds run first
-  // on the light prefix of the key order, whose forest is exactly the
-  // forest's light part; heavy edges then need one filtering pass against
-  // the light components before the remaining rounds. Only worth it on
-  // graphs with many edges per vertex. The threshold keeps about 1.5 light
-  // undirected edges per vertex (EAC_TUNE_LIGHT_X10): a fraction 3n/m of the
-  // entries of a symmetric CSR, 1.5n/m of a one-entry-per-edge CSR.
-  const int light_x10 = tune_value(EAC_TUNE_LIGHT_X10) ? tune_value(EAC_TUNE_LIGHT_X10) : 15;
-  const bool try_split = light_x10 > 0 && m >= 6 * (int64_t)n && m >= (1 << 16);
-  const double frac = (symmetric ? 2.0 : 1.0) * (light_x10 / 10.0) * n / (double)m;
-  k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb,
-                                       try_split ? 1 : 0, frac, &B.st->plan);
+  const SplitPlan sp = split_plan(n, m, symmetric);
+  k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb, sp.levels,
+                                       sp.frac, sp.step, &B.st->plan);
   EAC_LAUNCH_CHECK();
   k_vertex_init<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, B.best[0], B.mst_key, B.comp);
   EAC_LAUNCH_CHECK();
@@ -1376,7 +1462,8 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
     EAC_LAUNCH_CHECK();
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
     k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
-                               B.st->ctl, B.st->had_edge, B.comp, B.best[(r + 1) & 1], n, vb,
+                               B.st->ctl, B.st->had_edge, &B.st->plan, B.comp,
+                               B.best[(r + 1) & 1], n, vb,
                                kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
                                &B.st->overflow);
     EAC_LAUNCH_CHECK();
@@ -2581,6 +2668,11 @@ extern "C" int eac_sl_lifetime_batched(const int32_t* dest, const void* weight,
                                      ws_bytes, s);
 }
 
+extern "C" int eac_mst_round_bound(int32_t n, int64_t m, int32_t symmetric) {
+  if (n < 1 || m < 0) return 0;
+  return eac::mst_round_bound(n, eac::split_plan(n, m, symmetric).levels);
+}
+
 // ------------------------------------------- batched cut as one hipGraph
 struct eac_sl_graph {
   hipGraph_t graph = nullptr;
@@ -2607,7 +2699,7 @@ extern "C" int eac_sl_lifetime_batched_graph_create(
   size_t need = eac_sl_lifetime_batched_workspace_bytes(N, M, G);
   if (!ws || ws_bytes < need)
     return fail(EAC_ERR_WORKSPACE, "sl_graph: workspace %zu < %zu bytes", ws_bytes, need);
-  const int bound = mst_round_bound(N);
+  const int bound = mst_round_bound(N, split_plan(N, M, symmetric).levels);
   const int rounds = max_rounds <= 0 || max_rounds > bound ? bound : max_rounds;
   eac_sl_graph* g = new eac_sl_graph();
   g->rounds = rounds;

@@ -103,7 +103,8 @@ class LifetimeCutGraph:
     M == m_cap; and G < g_cap whenever N < n_cap.
 
     Replays run a fixed number of Boruvka rounds. max_rounds=0 records the
-    bound that always suffices (2 * (bits(n_cap - 1) + 2)); with tune=True
+    bound that always suffices (L * (bits(n_cap - 1) + 3), L = 1 to 4 weight
+    levels of the Borůvka split, by edges per vertex); with tune=True
     the first replay's round count plus `margin` is recorded instead. A batch
     that needs more rounds than recorded is detected on the device (summary
     flag 16) and run again on a graph recorded at the bound, so results never
@@ -136,8 +137,8 @@ class LifetimeCutGraph:
         self.bound_rounds = self._bound()
 
     def _bound(self):
-        b = 2 * (max(int(self.n_cap - 1).bit_length(), 1) + 2)
-        return min(b, 64)
+        return int(self._lib.eac_mst_round_bound(self.n_cap, self.m_cap,
+                                                  1 if self.symmetric else 0))
 
     def _capture(self, max_rounds):
         self.close()

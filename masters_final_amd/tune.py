@@ -1,9 +1,11 @@
 """Autotuner for libeac_hip.so's launch and algorithm knobs (EAC_TUNE_*).
 
-The knobs change speed only, never results: the Borůvka light/heavy split
-(light undirected edges per vertex x 10, -1 = no split), the number of
-rounds enqueued before the first read-back, the blocks per compute unit of
-the edge-list filter, and the co-association LDS window (columns). `autotune_mst` times a grid of candidate settings
+The knobs change speed only, never results: the Borůvka weight-level split
+(light undirected edges per vertex x 10 below the first threshold, -1 = no
+split; growth of that count per further level, -1 = one split only), the
+number of rounds enqueued before the first read-back, the blocks per compute
+unit of the edge-list filter, and the co-association LDS window (columns).
+`autotune_mst` times a grid of candidate settings
 on the caller's own graph with HIP events, checks that every candidate gives
 the same forest, keeps the fastest, and caches the choice per device, weight
 type and graph-shape bucket (JSON under $EAC_TUNE_CACHE, default
@@ -19,10 +21,11 @@ from pathlib import Path
 
 from . import _lib as L
 
-KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3}
+KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3, "level_step": 4}
 
 DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),
+    "level_step": (-1, 0, 8),
     "first_batch": (0,),
     "filter_bpcu": (0, 4),
 }

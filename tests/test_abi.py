@@ -61,12 +61,35 @@ def test_tuning_knobs_roundtrip():
     from masters_final_amd._lib import EACError
     before = tune.get_knobs()
     try:
-        tune.set_knobs(light_x10=25, first_batch=9, filter_bpcu=3, co_window=4096)
+        tune.set_knobs(light_x10=25, first_batch=9, filter_bpcu=3, co_window=4096,
+                       level_step=8)
         assert tune.get_knobs() == {"light_x10": 25, "first_batch": 9, "filter_bpcu": 3,
-                                    "co_window": 4096}
+                                    "co_window": 4096, "level_step": 8}
         tune.set_knobs(light_x10=-1)
         assert tune.get_knobs()["light_x10"] == -1
         lib = __import__("masters_final_amd._lib", fromlist=["x"]).load_library()
         assert lib.eac_tune_set(99, 1) != 0
+    finally:
+        tune.set_knobs(**before)
+
+
+def test_round_bound_counts_weight_levels():
+    """bits(n-1) + 3 rounds per weight level of the Borůvka split: one level
+    for sparse graphs, up to four for dense ones, one split with the level
+    step knob at -1, none with the split off."""
+    from masters_final_amd import tune
+    lib = _lib.load_library()
+    before = tune.get_knobs()
+    try:
+        tune.set_knobs(light_x10=0, level_step=0)
+        assert lib.eac_mst_round_bound(1 << 20, 1 << 21, 1) == 23  # 2 entries/vertex
+        assert lib.eac_mst_round_bound(1 << 20, 64 << 20, 1) == 4 * 23
+        # 32 entries/vertex: fractions 3/32, 12/32, 48/32 -> three levels
+        assert lib.eac_mst_round_bound(1 << 20, 32 << 20, 1) == 3 * 23
+        tune.set_knobs(level_step=-1)
+        assert lib.eac_mst_round_bound(1 << 20, 40 << 20, 1) == 2 * 23
+        tune.set_knobs(light_x10=-1, level_step=0)
+        assert lib.eac_mst_round_bound(1 << 20, 40 << 20, 1) == 23
+        assert lib.eac_mst_round_bound(0, 10, 1) == 0
     finally:
         tune.set_knobs(**before)

@@ -249,18 +249,22 @@ def test_graph_from_edges_matches_oracle(torch, elt4):
                                  np.array([len(g["dest"]) + 3]), nod, nfe, ndest, nw) == -1
 
 
-@pytest.mark.parametrize("light_x10,first_batch,bpcu", [
-    (-1, 0, 0), (5, 2, 1), (40, 3, 2), (15, 30, 0)])
-def test_boruvka_tuning_knobs_keep_the_forest(torch, light_x10, first_batch, bpcu):
+@pytest.mark.parametrize("light_x10,first_batch,bpcu,step,wmax", [
+    (-1, 0, 0, 0, 60), (5, 2, 1, 2, 60), (40, 3, 2, 0, 60), (15, 30, 0, -1, 60),
+    (10, 0, 0, 2, 3), (5, 4, 0, 3, 100000)])
+def test_boruvka_tuning_knobs_keep_the_forest(torch, light_x10, first_batch, bpcu, step, wmax):
     """Every tuning knob setting gives the canonical forest (knobs change
-    speed only): no split, tiny and large light fractions, a first round
-    batch far too short (rounds continue after the read-back) or long."""
+    speed only): no split, one split, up to four weight levels (also with
+    fewer distinct weights than levels), tiny and large light fractions, a
+    first round batch far too short (rounds continue after the read-back)
+    or long."""
     from masters_final_amd import tune
     from masters_final_amd.graph.mst import boruvka_mst
-    g = synth.random_graph(30000, 16, 21, wmax=60, n_parts=3)
+    g = synth.random_graph(30000, 16, 21, wmax=wmax, n_parts=3)
     before = tune.get_knobs()
     try:
-        tune.set_knobs(light_x10=light_x10, first_batch=first_batch, filter_bpcu=bpcu)
+        tune.set_knobs(light_x10=light_x10, first_batch=first_batch, filter_bpcu=bpcu,
+                       level_step=step)
         r = boruvka_mst(g["dest"], g["weight"], g["indptr"], None, want_weights=True,
                         want_labels=True)
         _check_forest(g, r)
