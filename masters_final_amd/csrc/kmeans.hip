@@ -400,42 +400,44 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
   }
 }
 
-// Direct-difference labeling, one thread per point, the point in registers
-// (zero padded to DX dimensions) and centroid tiles staged in LDS (zero
-// padded rows, so the padding adds exact zeros). T = float: plain FP32;
-// T = double: the oracle's FP64 arithmetic (ascending dimensions, strict <).
+// Direct-difference labeling: one thread per (point, partition), blocks of
+// 256 points x one partition (grid.y over partitions, so a small data set
+// with many partitions still fills the chip). The point sits in registers
+// (DX = d exactly for d <= 4, else zero padded to DX: padding adds exact
+// zeros) and the partition's centroids are staged in LDS, already converted
+// to T, a tile of at most 16 KiB at a time. T = float: plain FP32; T =
+// double: the oracle's FP64 arithmetic (ascending dimensions, strict <).
 template <typename T, int DX>
 __global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ X, int64_t n,
                                                       int d, const float* __restrict__ C,
                                                       const int64_t* __restrict__ k_off, int P,
                                                       int32_t* __restrict__ labels,
                                                       float* __restrict__ dists) {
-  constexpr int kTile = 64;
-  __shared__ float s_c[kTile * DX];
+  constexpr int kTileRaw = 16384 / (DX * (int)sizeof(T));
+  constexpr int kTile = kTileRaw > 256 ? 256 : kTileRaw;
+  __shared__ T s_c[kTile * DX];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < n;
-  float xr[DX];
+  T xr[DX];
 #pragma unroll
-  for (int j = 0; j < DX; ++j) xr[j] = (valid && j < d) ? X[i * d + j] : 0.f;
-  for (int p = 0; p < P; ++p) {
+  for (int j = 0; j < DX; ++j) xr[j] = (valid && j < d) ? (T)X[i * d + j] : T(0);
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
     const int64_t kbeg = k_off[p], kend = k_off[p + 1];
     T best = 0;
     int32_t bl = -1;
     for (int64_t c0 = kbeg; c0 < kend; c0 += kTile) {
       const int rows = (int)((kend - c0) < kTile ? (kend - c0) : kTile);
       __syncthreads();
-      for (int v = threadIdx.x; v < kTile * DX; v += blockDim.x) {
-        int r = v / DX, j = v % DX;
-        s_c[v] = (r < rows && j < d) ? C[(c0 + r) * d + j] : 0.f;
+      for (int v = threadIdx.x; v < rows * DX; v += blockDim.x) {
+        const int r = v / DX, j = v - (v / DX) * DX;
+        s_c[v] = j < d ? (T)C[(c0 + r) * d + j] : T(0);
       }
       __syncthreads();
       if (valid) {
         for (int r = 0; r < rows; ++r) {
           T dist = 0;
 #pragma unroll
-          for (int j = 0; j < DX; ++j) {
-            dist = sq_acc(dist, (T)xr[j] - (T)s_c[r * DX + j]);
-          }
+          for (int j = 0; j < DX; ++j) dist = sq_acc(dist, xr[j] - s_c[r * DX + j]);
           if (bl < 0 || dist < best) { best = dist; bl = (int32_t)(c0 + r - kbeg); }
         }
       }
@@ -471,8 +473,12 @@ __global__ __launch_bounds__(256) void k_sum_dists(const float* __restrict__ dis
 template <typename T>
 int launch_direct(const float* X, int64_t n, int d, const float* C, const int64_t* k_off, int P,
                   int32_t* labels, float* dists, hipStream_t s) {
-  int g = grid_for(n, 256);
-  if (d <= 4) k_label_direct<T, 4><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  const dim3 g(grid_for(n, 256), P < 65535 ? P : 65535);
+  if (d == 1) k_label_direct<T, 1><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d == 2) k_label_direct<T, 2><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d == 3) k_label_direct<T, 3><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d == 4) k_label_direct<T, 4><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 8) k_label_direct<T, 8><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
   else if (d <= 16) k_label_direct<T, 16><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
   else if (d <= 32) k_label_direct<T, 32><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
   else if (d <= 64) k_label_direct<T, 64><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
@@ -483,37 +489,136 @@ int launch_direct(const float* X, int64_t n, int d, const float* C, const int64_
 }
 
 // ------------------------------------------------------- centroid update
-__global__ void k_label_keys(const int32_t* __restrict__ labels, int64_t n, int P,
-                             const int64_t* __restrict__ k_off, uint32_t* __restrict__ key,
-                             uint32_t* __restrict__ idx, int32_t* __restrict__ count) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < (int64_t)P * n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int p = (int)(t / n);
-    int64_t i = t - (int64_t)p * n;
-    uint32_t c = (uint32_t)(k_off[p] + labels[t]);
-    key[t] = c;
-    idx[t] = (uint32_t)i;
-    atomicAdd(&count[c], 1);
+// Sort keys (global centroid id) and point indices of every (partition,
+// point), and the member count of every centroid. Blocks cover a run of one
+// partition's points (grid.y over partitions); when that run is long next to
+// the partition's k, counts meet in LDS first (one global atomic per
+// centroid and block instead of one per point).
+constexpr int kKeyLds = 4096;
+__global__ __launch_bounds__(256) void k_label_keys(const int32_t* __restrict__ labels,
+                                                    int64_t n, int P,
+                                                    const int64_t* __restrict__ k_off,
+                                                    uint32_t* __restrict__ key,
+                                                    uint32_t* __restrict__ idx,
+                                                    int32_t* __restrict__ count) {
+  __shared__ int32_t s_cnt[kKeyLds];
+  const int64_t per_block = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per_block;
+  const int64_t i1 = i0 + per_block < n ? i0 + per_block : n;
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    const int64_t kb = k_off[p], kp = k_off[p + 1] - kb;
+    const bool lds = kp <= kKeyLds && 4 * kp <= i1 - i0;  // uniform over the block
+    if (lds) {
+      for (int c = threadIdx.x; c < kp; c += blockDim.x) s_cnt[c] = 0;
+      __syncthreads();
+    }
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+      const int64_t t = (int64_t)p * n + i;
+      const int32_t l = labels[t];
+      key[t] = (uint32_t)(kb + l);
+      idx[t] = (uint32_t)i;
+      if (lds) atomicAdd(&s_cnt[l], 1);
+      else atomicAdd(&count[kb + l], 1);
+    }
+    if (lds) {
+      __syncthreads();
+      for (int c = threadIdx.x; c < kp; c += blockDim.x)
+        if (s_cnt[c]) atomicAdd(&count[kb + c], s_cnt[c]);
+      __syncthreads();
+    }
   }
 }
 
 // new[c][j] = fp32 sequential sum over members in point order, / count in
-// double (KMeans.py:736-745). One thread per (centroid, dimension).
-__global__ void k_centroid_sums(const float* __restrict__ X, int d,
-                                const uint32_t* __restrict__ members,
-                                const int64_t* __restrict__ mstart,
-                                const int32_t* __restrict__ count, int64_t ktot,
-                                float* __restrict__ Cnew) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ktot * d;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t c = t / d;
-    int j = (int)(t - c * d);
-    int32_t cnt = count[c];
-    if (cnt == 0) continue;
+// double (KMeans.py:736-745). One single-wave block per centroid and group
+// of 128 dimensions: the lanes gather up to 2048 member coordinates at a
+// time into LDS (many loads in flight), then lane j adds dimension j's
+// values in member order (lane j + 64 too), so the serial chain runs on LDS
+// reads.
+constexpr int kSumStage = 2048;
+constexpr int kSumDims = 2 * kWave;
+__global__ __launch_bounds__(kWave) void k_centroid_sums(const float* __restrict__ X, int d,
+                                                         const uint32_t* __restrict__ members,
+                                                         const int64_t* __restrict__ mstart,
+                                                         const int32_t* __restrict__ count,
+                                                         int64_t ktot, float* __restrict__ Cnew) {
+  __shared__ float s_x[kSumStage];
+  const int lane = threadIdx.x;
+  for (int64_t c = blockIdx.x; c < ktot; c += gridDim.x) {
+    const int32_t cnt = count[c];
+    if (cnt == 0) continue;  // uniform over the block
     const uint32_t* m = members + mstart[c];
-    float s = 0.f;
-    for (int32_t q = 0; q < cnt; ++q) s = s + X[(int64_t)m[q] * d + j];
-    Cnew[t] = (float)((double)s / (double)cnt);
+    for (int j0 = 0; j0 < d; j0 += kSumDims) {
+      const int dd = d - j0 < kSumDims ? d - j0 : kSumDims;
+      const int q_max = kSumStage / dd;  // members per stage
+      float s0 = 0.f, s1 = 0.f;
+      for (int32_t b = 0; b < cnt; b += q_max) {
+        const int q = cnt - b < q_max ? cnt - b : q_max;
+        __syncthreads();
+        for (int v = lane; v < q * dd; v += kWave) {
+          const int r = v / dd;
+          s_x[v] = X[(int64_t)m[b + r] * d + j0 + (v - r * dd)];
+        }
+        __syncthreads();
+        if (lane < dd)
+          for (int r = 0; r < q; ++r) s0 = s0 + s_x[r * dd + lane];
+        if (lane + kWave < dd)
+          for (int r = 0; r < q; ++r) s1 = s1 + s_x[r * dd + lane + kWave];
+      }
+      if (lane < dd) Cnew[c * d + j0 + lane] = (float)((double)s0 / (double)cnt);
+      if (lane + kWave < dd)
+        Cnew[c * d + j0 + lane + kWave] = (float)((double)s1 / (double)cnt);
+    }
+  }
+}
+
+// Copies the new centroids of the partitions still running (upd[p] != 0).
+__global__ void k_copy_running(float* __restrict__ C, const float* __restrict__ Cnew, int d,
+                               const int64_t* __restrict__ k_off, int P,
+                               const int32_t* __restrict__ upd) {
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    if (!upd[p]) continue;
+    const int64_t a = k_off[p] * d, b = k_off[p + 1] * d;
+    for (int64_t t = a + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < b;
+         t += (int64_t)gridDim.x * blockDim.x)
+      C[t] = Cnew[t];
+  }
+}
+
+// KMeans.py:117-189 stopping rule of iteration `it` for every partition
+// still running: err = |inertia - previous|; stop when err <= tol or it >=
+// max_iter, else mark it for a centroid update. any[0] = some partition
+// continues.
+__global__ void k_converge(const double* __restrict__ inertia, double* __restrict__ prev,
+                           int32_t* __restrict__ active, int32_t* __restrict__ iters,
+                           int32_t* __restrict__ upd, int P, int it, int max_iter, double tol,
+                           int32_t* __restrict__ any) {
+  __shared__ int s_any;
+  if (threadIdx.x == 0) s_any = 0;
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    upd[p] = 0;
+    if (!active[p]) continue;
+    iters[p] = it;
+    const double err = fabs(inertia[p] - prev[p]);
+    prev[p] = inertia[p];
+    if (err <= tol || it >= max_iter) {
+      active[p] = 0;
+      continue;
+    }
+    upd[p] = 1;
+    s_any = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) any[0] = s_any;
+}
+
+__global__ void k_fit_init(double* __restrict__ prev, int32_t* __restrict__ active,
+                           int32_t* __restrict__ iters, int P) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+    prev[p] = INFINITY;
+    active[p] = 1;
+    iters[p] = 0;
   }
 }
 
@@ -614,13 +719,18 @@ size_t kmeans_label_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
 // nullable) their per-partition sums.
 int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t* h_koff,
                  int P, int mode, int32_t* labels, float* dists, double* inertia_out,
-                 int64_t* n_rescored, Arena& ar, hipStream_t s) {
+                 int64_t* n_rescored, Arena& ar, hipStream_t s,
+                 const int64_t* d_koff = nullptr) {
   const int64_t ktot = h_koff[P];
   LabelBufs B = carve_label(ar, n, d, ktot, P);
   if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans: workspace too small");
-  EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1),
-                               hipMemcpyHostToDevice, s));
-  EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_koff may be a pageable stack array
+  if (d_koff) {
+    B.k_off = const_cast<int64_t*>(d_koff);  // read only
+  } else {
+    EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1),
+                                 hipMemcpyHostToDevice, s));
+    EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_koff may be a pageable stack array
+  }
   if (n_rescored) *n_rescored = 0;
   if (mode == EAC_KM_F32 || mode == EAC_KM_F64 || (mode == EAC_KM_BF16X3 && d < 16)) {
     int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
@@ -711,7 +821,7 @@ size_t kmeans_update_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
 // the points of smallest (dist, index) in ascending cluster order.
 int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const float* dists,
                   const int64_t* h_koff, int P, float* Cnew, int64_t* n_empty, Arena& ar,
-                  hipStream_t s) {
+                  hipStream_t s, const int64_t* d_koff = nullptr) {
   const int64_t ktot = h_koff[P];
   const int64_t items = (int64_t)P * n;
   if (items > (int64_t)UINT32_MAX || ktot >= (int64_t)UINT32_MAX)
@@ -728,12 +838,21 @@ int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const
   uint64_t* dk = ar.take<uint64_t>(n);
   uint64_t* dks = ar.take<uint64_t>(n);
   if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans update: workspace too small");
-  EAC_HIP_CHECK(hipMemcpyAsync(k_off, h_koff, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice,
-                               s));
-  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  if (d_koff) {
+    k_off = const_cast<int64_t*>(d_koff);  // read only
+  } else {
+    EAC_HIP_CHECK(hipMemcpyAsync(k_off, h_koff, sizeof(int64_t) * (P + 1),
+                                 hipMemcpyHostToDevice, s));
+    EAC_HIP_CHECK(hipStreamSynchronize(s));
+  }
   EAC_TRY(fill<int32_t>(count, ktot + 1, 0, s));
-  k_label_keys<<<grid_for(items, 256, 8192), 256, 0, s>>>(labels, n, P, k_off, key, idx, count);
-  EAC_LAUNCH_CHECK();
+  {
+    // about 8192 blocks in all, each over a run of at least 1024 points
+    const int py = P < 65535 ? P : 65535;
+    int gx = grid_for(n, 1024, 8192 / py > 1 ? 8192 / py : 1);
+    k_label_keys<<<dim3(gx, py), 256, 0, s>>>(labels, n, P, k_off, key, idx, count);
+    EAC_LAUNCH_CHECK();
+  }
   // stable by centroid: members of each centroid in ascending point order
   const int bits = bits_for((uint64_t)ktot);
   size_t mark = ar.used;
@@ -742,8 +861,8 @@ int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const
   ar.used = mark;
   EAC_TRY((scan_excl<int32_t, int64_t>(count, mstart, ktot, 0, nullptr, ar, s)));
   ar.used = mark;
-  k_centroid_sums<<<grid_for(ktot * d, 256, 16384), 256, 0, s>>>(X, d, members, mstart, count,
-                                                                  ktot, Cnew);
+  k_centroid_sums<<<grid_for(ktot, 1, 65536), kWave, 0, s>>>(X, d, members, mstart, count, ktot,
+                                                              Cnew);
   EAC_LAUNCH_CHECK();
   // empty clusters
   k_empty_flags<<<grid_for(ktot, 256, 4096), 256, 0, s>>>(count, ktot, eflag);
@@ -785,8 +904,19 @@ extern "C" size_t eac_kmeans_workspace_bytes(int64_t n, int32_t d, int64_t k_tot
   if (P < 1) P = 1;
   size_t a = kmeans_label_ws_bytes(n, d, k_total, P);
   size_t b = kmeans_update_ws_bytes(n, d, k_total, P);
-  // fit: labels/dists are caller buffers; two centroid buffers + inertia
-  size_t c = 2 * (((size_t)k_total * d * 4 + 255) & ~size_t(255)) + (size_t)P * 16 + 4096;
+  // fit: labels/dists are caller buffers; new centroids and the per-partition
+  // state (inertia, previous inertia, offsets, active, iterations, update
+  // flags, one flag), each rounded up to 256 bytes
+  Arena fa(nullptr, 0, true);
+  fa.take<float>((size_t)k_total * d);
+  fa.take<double>(P);
+  fa.take<double>(P);
+  fa.take<int64_t>(P + 1);
+  fa.take<int32_t>(P);
+  fa.take<int32_t>(P);
+  fa.take<int32_t>(P);
+  fa.take<int32_t>(1);
+  size_t c = fa.used + 4096;
   return (a > b ? a : b) + c;
 }
 
@@ -860,10 +990,19 @@ extern "C" int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_ino
   Arena ar(ws, ws_bytes);
   float* Cnew = ar.take<float>((size_t)ktot * d);
   double* inertia = ar.take<double>(P);
+  double* prev = ar.take<double>(P);
+  int64_t* d_koff = ar.take<int64_t>(P + 1);
+  int32_t* active = ar.take<int32_t>(P);
+  int32_t* d_iters = ar.take<int32_t>(P);
+  int32_t* upd = ar.take<int32_t>(P);
+  int32_t* d_any = ar.take<int32_t>(1);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans fit: workspace too small");
   const size_t mark = ar.used;
-  std::vector<double> prev(P, INFINITY), cur(P);
-  std::vector<int32_t> iters(P, 0);
-  std::vector<char> active(P, 1);
+  // the partition offsets go up once; the stopping rule runs on the device,
+  // so an iteration costs one read-back (does any partition continue?)
+  EAC_TRY(h2d(d_koff, h_koff, sizeof(int64_t) * (P + 1), s));
+  k_fit_init<<<grid_for(P, 256, 1024), 256, 0, s>>>(prev, active, d_iters, P);
+  EAC_LAUNCH_CHECK();
   int64_t tot_res = 0, tot_empty = 0;
   for (int it = 1; it <= max_iter; ++it) {
     // label every partition (stopped ones relabel identically: unchanged
@@ -871,36 +1010,26 @@ extern "C" int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_ino
     int64_t nres = 0;
     ar.used = mark;
     EAC_TRY(kmeans_label(X, n, d, C_inout, h_koff, P, mode, labels, dists, inertia, &nres, ar,
-                         s));
+                         s, d_koff));
     tot_res += nres;
-    EAC_TRY(d2h(cur.data(), inertia, sizeof(double) * P, s));
-    bool any = false;
-    std::vector<char> upd(P, 0);
-    for (int p = 0; p < P; ++p) {
-      if (!active[p]) continue;
-      iters[p] = it;
-      double err = std::fabs(cur[p] - prev[p]);
-      prev[p] = cur[p];
-      if (err <= tol || it >= max_iter) { active[p] = 0; continue; }
-      upd[p] = 1;
-      any = true;
-    }
+    k_converge<<<1, 1024, 0, s>>>(inertia, prev, active, d_iters, upd, P, it, max_iter, tol,
+                                  d_any);
+    EAC_LAUNCH_CHECK();
+    if (it >= max_iter) break;  // every partition stops at max_iter
+    int32_t any = 0;
+    EAC_TRY(d2h(&any, d_any, sizeof(any), s));
     if (!any) break;
     ar.used = mark;
     int64_t ne = 0;
-    EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, Cnew, &ne, ar, s));
+    EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, Cnew, &ne, ar, s, d_koff));
     tot_empty += ne;
-    // copy updated centroids of partitions still running
-    for (int p = 0; p < P; ++p) {
-      if (!upd[p]) continue;
-      int64_t a = h_koff[p] * d, b = h_koff[p + 1] * d;
-      EAC_HIP_CHECK(hipMemcpyAsync(C_inout + a, Cnew + a, sizeof(float) * (b - a),
-                                   hipMemcpyDeviceToDevice, s));
-    }
+    const dim3 gc(grid_for(ktot * d / P + 1, 256, 64), P < 65535 ? P : 65535);
+    k_copy_running<<<gc, 256, 0, s>>>(C_inout, Cnew, d, d_koff, P, upd);
+    EAC_LAUNCH_CHECK();
   }
+  if (h_iters) EAC_TRY(d2h(h_iters, d_iters, sizeof(int32_t) * P, s));
   EAC_HIP_CHECK(hipStreamSynchronize(s));
   prof_flush();
-  if (h_iters) for (int p = 0; p < P; ++p) h_iters[p] = iters[p];
   if (h_out) { h_out[0] = tot_res; h_out[1] = tot_empty; }
   return EAC_OK;
 }

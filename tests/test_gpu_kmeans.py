@@ -24,7 +24,9 @@ def _data(n, d, k, seed, spread=2.0):
 
 
 @pytest.mark.parametrize("n,d,k", [(5000, 16, 37), (4000, 24, 100), (3000, 64, 256),
-                                   (2000, 100, 64), (3000, 2, 75), (777, 33, 5)])
+                                   (2000, 100, 64), (3000, 2, 75), (777, 33, 5),
+                                   (3000, 2, 300), (2500, 1, 20), (2500, 3, 40),
+                                   (2000, 5, 30)])
 def test_label_bf16x3_equals_fp64_oracle(torch, n, d, k):
     from masters_final_amd.cluster.KMeans import label_device
     X, Cc = _data(n, d, k, seed=d * 7 + k)
@@ -86,6 +88,40 @@ def test_centroid_update_matches_oracle(torch):
     got, ne = update_device(X, lab[None], dists[None], [k])
     assert ne == 2
     assert np.array_equal(got.cpu().numpy(), want)
+
+
+def test_centroid_update_wide_rows_matches_oracle(torch):
+    """More than 128 dimensions (two dimension groups) and clusters larger
+    than one LDS stage of members."""
+    from masters_final_amd.cluster.KMeans import update_device
+    rng = np.random.default_rng(14)
+    X = rng.standard_normal((3000, 130)).astype(np.float32)
+    k = 6
+    lab = rng.integers(0, k, 3000).astype(np.int32)
+    dists = rng.random(3000).astype(np.float32)
+    want = O.recompute_centroids(X, lab, dists, k, reseed_mode=1)
+    got, ne = update_device(X, lab[None], dists[None], [k])
+    assert ne == 0
+    assert np.array_equal(got.cpu().numpy(), want)
+
+
+def test_fit_batched_many_small_partitions_2d(torch):
+    """The ensemble shape: 2-D points, many partitions of small k, partitions
+    that stop at different iterations."""
+    from masters_final_amd.cluster.KMeans import fit_batched
+    rng = np.random.default_rng(15)
+    X, _ = synth.gmm_points(3000, 2, 6, 15, mean_scale=10.0, std=1.0)
+    ks = [int(k) for k in rng.integers(2, 60, 40)]
+    init = np.concatenate([X[rng.choice(3000, k, replace=False)] for k in ks])
+    r = fit_batched(X, init, ks, max_iter=6, tol=1e-2, precision="bf16x3")
+    off = np.concatenate([[0], np.cumsum(ks)])
+    for p, k in enumerate(ks):
+        lab, dist, cen, it = O.kmeans_fit(X, init[off[p]:off[p + 1]], max_iter=6, tol=1e-2,
+                                          reseed_mode=1)
+        assert int(r["iters"][p]) == it, p
+        assert np.array_equal(r["labels"][p].cpu().numpy(), lab), p
+        assert np.array_equal(r["dists"][p].cpu().numpy(), dist), p
+        assert np.array_equal(r["centroids"][off[p]:off[p + 1]].cpu().numpy(), cen), p
 
 
 @pytest.mark.parametrize("prec", ["bf16x3", "fp64"])
