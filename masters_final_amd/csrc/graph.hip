@@ -39,6 +39,8 @@
 //    labels follow from one scan of root flags.
 #include <climits>
 #include <cstddef>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include <type_traits>
@@ -1493,6 +1495,12 @@ int mst_check(const MstState& hs, int enq, int vb, bool* active) {
 void mst_prof_bytes(const MstState& hs, int enq, int32_t n, int64_t nchunks, MstRecs& rec) {
   if (!prof_on()) return;
   const unsigned long long l0 = fold_slots(hs.tot0[0]), h0 = fold_slots(hs.tot0[1]);
+  if (std::getenv("EAC_ROUND_LOG")) {  // diagnostics: keys per round of profiled calls
+    std::fprintf(stderr, "[boruvka] n=%d round0: light %llu heavy %llu\n", n, l0, h0);
+    for (int r = 0; r < hs.ctl[enq & 1].rounds && r < kMaxRounds; ++r)
+      std::fprintf(stderr, "[boruvka] round %d: candidates %u, keys in %llu, kept %llu\n", r,
+                   hs.had_edge[r], fold_slots(hs.rlog[r][0]), fold_slots(hs.rlog[r][1]));
+  }
   prof_set_bytes(rec.pr0, rec.round0_bytes + 8 * (int64_t)(l0 + h0));
   for (int r = 0; r < (int)rec.hook.size(); ++r) {
     const bool ran = r < hs.ctl[enq & 1].rounds;

@@ -25,6 +25,7 @@
 //    FP64 oracle's; distances are then recomputed in FP64 for every point
 //    (d flops) so they equal the oracle's too.
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -60,34 +61,61 @@ __device__ __forceinline__ int part_of(const int64_t* k_off, int P, int64_t c) {
 constexpr int kStat = 4;
 
 // Split every centroid into bf16 halves (row length KD, zero padded) and
-// record norms.
+// record norms. The per-partition maxima are reduced over the wavefront
+// first when its lanes share one partition (one atomic per statistic instead
+// of one per centroid: same-address atomics serialise).
 __global__ void k_cent_prep(const float* __restrict__ C, int64_t ktot, int d, int KD,
                             const int64_t* __restrict__ k_off, int P,
                             uint16_t* __restrict__ Ch, uint16_t* __restrict__ Cl,
                             float* __restrict__ ncf, unsigned int* __restrict__ pstat) {
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ktot;
-       c += (int64_t)gridDim.x * blockDim.x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ktot; base += stride) {
+    const int64_t c = base + threadIdx.x;
+    const bool valid = c < ktot;
     double n2 = 0, h2 = 0, l2 = 0, e2 = 0;
-    for (int j = 0; j < KD; ++j) {
-      float x = j < d ? C[c * d + j] : 0.f;
-      uint16_t h = bf16_rne(x);
-      float r = x - bf16_val(h);  // exact
-      uint16_t l = bf16_rne(r);
-      float e = r - bf16_val(l);  // exact
-      Ch[c * KD + j] = h;
-      Cl[c * KD + j] = l;
-      n2 += (double)x * x;
-      h2 += (double)bf16_val(h) * bf16_val(h);
-      l2 += (double)bf16_val(l) * bf16_val(l);
-      e2 += (double)e * e;
+    int p = -1;
+    if (valid) {
+      for (int j = 0; j < KD; ++j) {
+        float x = j < d ? C[c * d + j] : 0.f;
+        uint16_t h = bf16_rne(x);
+        float r = x - bf16_val(h);  // exact
+        uint16_t l = bf16_rne(r);
+        float e = r - bf16_val(l);  // exact
+        Ch[c * KD + j] = h;
+        Cl[c * KD + j] = l;
+        n2 += (double)x * x;
+        h2 += (double)bf16_val(h) * bf16_val(h);
+        l2 += (double)bf16_val(l) * bf16_val(l);
+        e2 += (double)e * e;
+      }
+      ncf[c] = (float)n2;
+      p = part_of(k_off, P, c);
     }
-    ncf[c] = (float)n2;
-    int p = part_of(k_off, P, c);
-    // round norms up so the bound stays rigorous
-    atomicMax(&pstat[p * kStat + 0], __float_as_uint((float)sqrt(n2) * (1.f + 1e-6f)));
-    atomicMax(&pstat[p * kStat + 1], __float_as_uint((float)sqrt(h2) * (1.f + 1e-6f)));
-    atomicMax(&pstat[p * kStat + 2], __float_as_uint((float)sqrt(l2) * (1.f + 1e-6f)));
-    atomicMax(&pstat[p * kStat + 3], __float_as_uint((float)sqrt(e2) * (1.f + 1e-6f)));
+    // norms rounded up so the bound stays rigorous; non-negative floats order
+    // like their bit patterns, and 0 is the identity of the maximum
+    unsigned int v[kStat] = {
+        valid ? __float_as_uint((float)sqrt(n2) * (1.f + 1e-6f)) : 0u,
+        valid ? __float_as_uint((float)sqrt(h2) * (1.f + 1e-6f)) : 0u,
+        valid ? __float_as_uint((float)sqrt(l2) * (1.f + 1e-6f)) : 0u,
+        valid ? __float_as_uint((float)sqrt(e2) * (1.f + 1e-6f)) : 0u};
+    int pmin = valid ? p : INT_MAX, pmax = p;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      pmin = min(pmin, __shfl_xor(pmin, off, kWave));
+      pmax = max(pmax, __shfl_xor(pmax, off, kWave));
+    }
+    if (pmin == pmax) {  // every valid lane in one partition (uniform)
+#pragma unroll
+      for (int q = 0; q < kStat; ++q)
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v[q] = max(v[q], __shfl_xor(v[q], off, kWave));
+      if (lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < kStat; ++q) atomicMax(&pstat[pmax * kStat + q], v[q]);
+    } else if (valid) {
+#pragma unroll
+      for (int q = 0; q < kStat; ++q) atomicMax(&pstat[p * kStat + q], v[q]);
+    }
   }
 }
 
@@ -119,9 +147,9 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
     unsigned long long* __restrict__ nflag, int64_t flag_cap) {
   constexpr int KB = KD / 32;
   constexpr int LROW = KD + 8;  // padded LDS row (bf16 elements)
-  __shared__ uint16_t s_h[kBN * LROW];
-  __shared__ uint16_t s_l[kBN * LROW];
-  __shared__ float s_nc[kBN];
+  __shared__ uint16_t s_h[2][kBN * LROW];
+  __shared__ uint16_t s_l[2][kBN * LROW];
+  __shared__ float s_nc[2][kBN];
   const int t = threadIdx.x;
   const int w = t / kWave;
   const int l = lane_id();
@@ -169,14 +197,50 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
     xln_a[rb] = sqrtf(sl) * (1.f + 1e-5f);
     exn_a[rb] = sqrtf(se) * (1.f + 1e-5f);
   }
-  // squared norms of the 8 output rows this lane covers: row 4*lq + i
-  float nx_o[2][4];
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) nx_o[rb][i] = __shfl(nx_a[rb], lq * 4 + i, kWave);
 
-  for (int p = 0; p < P; ++p) {
+  // Centroid tiles of every partition in sequence, double buffered in LDS:
+  // the next tile is fetched into registers before the current one is
+  // multiplied and stored to the other buffer after it, so one barrier per
+  // tile and the L2 latency overlaps the MFMAs.
+  constexpr int VPR = KD / 8;  // uint4 per row
+  constexpr int NV = (kBN * VPR + kLB - 1) / kLB;
+  uint4 ph[NV], pl[NV];
+  float pnc = 0.f;
+  // rows past the partition stay zero; their ||c||^2 of +inf keeps them out
+  // of the epilogue
+  auto fetch = [&](int64_t c0, int64_t kend) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int v = t + i * kLB;
+      ph[i] = make_uint4(0, 0, 0, 0);
+      pl[i] = make_uint4(0, 0, 0, 0);
+      if (v < kBN * VPR) {
+        const int64_t c = c0 + v / VPR;
+        if (c < kend) {
+          ph[i] = reinterpret_cast<const uint4*>(Ch + c * KD)[v % VPR];
+          pl[i] = reinterpret_cast<const uint4*>(Cl + c * KD)[v % VPR];
+        }
+      }
+    }
+    pnc = (t < kBN && c0 + t < kend) ? ncf[c0 + t] : INFINITY;
+  };
+  auto stash = [&](int b) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int v = t + i * kLB;
+      if (v < kBN * VPR) {
+        *reinterpret_cast<uint4*>(&s_h[b][(v / VPR) * LROW + (v % VPR) * 8]) = ph[i];
+        *reinterpret_cast<uint4*>(&s_l[b][(v / VPR) * LROW + (v % VPR) * 8]) = pl[i];
+      }
+    }
+    if (t < kBN) s_nc[b][t] = pnc;
+  };
+  int p = 0, buf = 0;
+  int64_t c0 = k_off[0];
+  fetch(c0, k_off[1]);
+  stash(0);
+  __syncthreads();
+  for (; p < P; ++p) {
     const int64_t kbeg = k_off[p], kend = k_off[p + 1];
     float d1[2][4], d2[2][4];
     int32_t k1[2][4];
@@ -184,68 +248,82 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) { d1[rb][i] = FLT_MAX; d2[rb][i] = FLT_MAX; k1[rb][i] = -1; }
-    for (int64_t c0 = kbeg; c0 < kend; c0 += kBN) {
-      __syncthreads();
-      // stage the tile: kBN rows of KD bf16 (hi and lo), 16-byte vectors
-      constexpr int VPR = KD / 8;  // uint4 per row
-      for (int v = t; v < kBN * VPR; v += kLB) {
-        int r = v / VPR, q = v % VPR;
-        int64_t c = c0 + r;
-        uint4 h = make_uint4(0, 0, 0, 0), lo = make_uint4(0, 0, 0, 0);
-        if (c < kend) {
-          h = reinterpret_cast<const uint4*>(Ch + c * KD)[q];
-          lo = reinterpret_cast<const uint4*>(Cl + c * KD)[q];
-        }
-        *reinterpret_cast<uint4*>(&s_h[r * LROW + q * 8]) = h;
-        *reinterpret_cast<uint4*>(&s_l[r * LROW + q * 8]) = lo;
+    for (c0 = kbeg; c0 < kend; c0 += kBN) {
+      // the next tile: this partition's, else the next partition's first
+      int64_t nc0 = c0 + kBN, nend = kend;
+      if (nc0 >= kend && p + 1 < P) {
+        nc0 = kend;
+        nend = k_off[p + 2];
       }
-      if (t < kBN) s_nc[t] = (c0 + t < kend) ? ncf[c0 + t] : 0.f;
-      __syncthreads();
+      const bool more = nc0 < nend;
+      if (more) fetch(nc0, nend);
       f32x4 acc[2][4];
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+      const uint16_t* sh = s_h[buf];
+      const uint16_t* sl = s_l[buf];
+      // all B fragments of the tile first (one wait), then the MFMAs in
+      // rounds of 8 independent accumulators, so no MFMA waits on the one
+      // before it
+      bf16x8 Bh[KB][4], Bl[KB][4];
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
-        bf16x8 Bh[4], Bl[4];
+      for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
-          int r = cb * 16 + lr;
-          Bh[cb] = *reinterpret_cast<const bf16x8*>(&s_h[r * LROW + kb * 32 + lq * 8]);
-          Bl[cb] = *reinterpret_cast<const bf16x8*>(&s_l[r * LROW + kb * 32 + lq * 8]);
+          const int r = cb * 16 + lr;
+          Bh[kb][cb] = *reinterpret_cast<const bf16x8*>(&sh[r * LROW + kb * 32 + lq * 8]);
+          Bl[kb][cb] = *reinterpret_cast<const bf16x8*>(&sl[r * LROW + kb * 32 + lq * 8]);
         }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
-          for (int rb = 0; rb < 2; ++rb) {
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bh[cb],
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bh[kb][cb],
                                                                   acc[rb][cb], 0, 0, 0);
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bl[cb],
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bl[kb][cb],
                                                                   acc[rb][cb], 0, 0, 0);
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[rb][kb], Bh[cb],
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[rb][kb], Bh[kb][cb],
                                                                   acc[rb][cb], 0, 0, 0);
-          }
       }
-      // epilogue: lane holds column cb*16 + lr, rows 4*lq + i
+      __builtin_amdgcn_sched_barrier(0);
+      // epilogue: lane holds column cb*16 + lr, rows 4*lq + i. The running
+      // top two per row compare D' = ||c||^2 - 2 x.c (the row's ||x||^2 is
+      // common to all columns, so it changes neither the argmin nor the
+      // margin); columns past the partition carry ||c||^2 = +inf. Four VALU
+      // operations per element: fma, med3 (new second), compare, select.
+      const int32_t colb = (int32_t)(c0 - kbeg) + lr;
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        int64_t c = c0 + cb * 16 + lr;
-        if (c >= kend) continue;
-        float nc = s_nc[cb * 16 + lr];
+        const float nc = s_nc[buf][cb * 16 + lr];
+        const int32_t col = colb + cb * 16;
 #pragma unroll
         for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
-            float D = fmaf(-2.f, acc[rb][cb][i], nx_o[rb][i] + nc);
-            if (D < d1[rb][i]) {
-              d2[rb][i] = d1[rb][i];
-              d1[rb][i] = D;
-              k1[rb][i] = (int32_t)(c - kbeg);
-            } else if (D < d2[rb][i]) {
-              d2[rb][i] = D;
-            }
+            const float D = fmaf(-2.f, acc[rb][cb][i], nc);
+            const bool take = D < d1[rb][i];
+            d2[rb][i] = __builtin_amdgcn_fmed3f(d1[rb][i], d2[rb][i], D);
+            d1[rb][i] = take ? D : d1[rb][i];
+            k1[rb][i] = take ? col : k1[rb][i];
           }
+      }
+      if (more) {
+        stash(buf ^ 1);  // last read in the previous tile, before its barrier
+        __syncthreads();
+        buf ^= 1;
       }
     }
     // merge the 16 lanes that share rows (same lq)
@@ -281,9 +359,10 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
         float xhn = __shfl(xhn_a[rb], src, kWave);
         float xln = __shfl(xln_a[rb], src, kWave);
         float exn = __shfl(exn_a[rb], src, kWave);
+        float nx = __shfl(nx_a[rb], src, kWave);
         int64_t row = p0 + rb * 16 + lq * 4 + i;
         if (lr == 0 && row < n && k1[rb][i] >= 0) {
-          float E = point_bound(nx_o[rb][i], xn, xhn, xln, exn, st, KD, d);
+          float E = point_bound(nx, xn, xhn, xln, exn, st, KD, d);
           labels[(int64_t)p * n + row] = k1[rb][i];
           if (!(d2[rb][i] - d1[rb][i] > 2.f * E)) mine[nm++] = ((uint64_t)p << 40) | (uint64_t)row;
         }
@@ -787,7 +866,7 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if (n_rescored) *n_rescored = (int64_t)nf;
   if (dists || inertia_out) {
     if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
-    dim3 grid(grid_for(n, 256, 1024), P < 65535 ? P : 65535);
+    dim3 grid(grid_for(n, 256, 16384), P < 65535 ? P : 65535);
     k_exact_dists<<<grid, 256, 0, s>>>(X, n, d, C, B.k_off, P, labels, dists, inertia_out);
     EAC_LAUNCH_CHECK();
   }
