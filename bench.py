@@ -163,9 +163,14 @@ def run_mst(args):
 
     # knobs (light/heavy split, first round batch, filter blocks per CU) from
     # the autotuner: timed candidates on this graph, same forest for each
-    from masters_final_amd.tune import autotune_mst
-    tuning = autotune_mst(d_dest, d_w, d_ip, use_cache=not args.retune)
-    log(f"tuning: {tuning['knobs']} ({tuning['ms']:.3f} ms, cached={tuning['cached']})")
+    from masters_final_amd.tune import autotune_mst, set_knobs
+    if args.knobs:  # fixed settings (profiling runs): no candidate timing
+        knobs = {k: int(v) for k, v in (kv.split("=") for kv in args.knobs.split(","))}
+        set_knobs(**knobs)
+        tuning = {"knobs": knobs, "ms": None, "cached": None, "source": "--knobs"}
+    else:
+        tuning = autotune_mst(d_dest, d_w, d_ip, use_cache=not args.retune)
+    log(f"tuning: {tuning['knobs']} (cached={tuning['cached']})")
 
     for _ in range(args.warmup):
         r = step()
@@ -472,6 +477,9 @@ def main():
     ap.add_argument("--vmax", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eac", action="store_true")
+    ap.add_argument("--knobs", default="",
+                    help="fixed tuning knobs for the MST workload, e.g. light_x10=20,level_step=-1 "
+                         "(skips the autotuner)")
     ap.add_argument("--retune", action="store_true",
                     help="ignore the autotuner cache and time the candidates again")
     ap.add_argument("--copy-calib", action="store_true",

@@ -67,7 +67,8 @@ constexpr int kStat = 4;
 __global__ void k_cent_prep(const float* __restrict__ C, int64_t ktot, int d, int KD,
                             const int64_t* __restrict__ k_off, int P,
                             uint16_t* __restrict__ Ch, uint16_t* __restrict__ Cl,
-                            float* __restrict__ ncf, unsigned int* __restrict__ pstat) {
+                            float* __restrict__ Ct, float* __restrict__ ncf,
+                            unsigned int* __restrict__ pstat) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ktot; base += stride) {
     const int64_t c = base + threadIdx.x;
@@ -83,6 +84,7 @@ __global__ void k_cent_prep(const float* __restrict__ C, int64_t ktot, int d, in
         float e = r - bf16_val(l);  // exact
         Ch[c * KD + j] = h;
         Cl[c * KD + j] = l;
+        if (j < d) Ct[(int64_t)j * ktot + c] = x;  // transposed copy for the re-scoring
         n2 += (double)x * x;
         h2 += (double)bf16_val(h) * bf16_val(h);
         l2 += (double)bf16_val(l) * bf16_val(l);
@@ -414,8 +416,12 @@ __device__ __forceinline__ double dist2_f64(const float* __restrict__ x,
   return acc;
 }
 
+// The lanes stride the partition's centroids and read them from the
+// transposed copy Ct [d][ktot] (64 consecutive floats per load, the point's
+// coordinates are wave-uniform).
 __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X, int64_t n,
-                                                     int d, const float* __restrict__ C,
+                                                     int d, const float* __restrict__ Ct,
+                                                     int64_t ktot,
                                                      const int64_t* __restrict__ k_off,
                                                      const uint64_t* __restrict__ flags,
                                                      int64_t nflag, int32_t* __restrict__ labels) {
@@ -429,11 +435,19 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
     const int64_t kbeg = k_off[p], kend = k_off[p + 1];
     double best = DBL_MAX;
     int64_t bk = -1;
-    const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
-                                      reinterpret_cast<uintptr_t>(C)) & 15) == 0;
     for (int64_t c = kbeg + l; c < kend; c += kWave) {
-      const double dist = dist2_f64(x, C + c * d, d, v4);
-      if (bk < 0 || dist < best) { best = dist; bk = c; }
+      const float* ct = Ct + c;
+      double acc = 0.0;
+      int j = 0;
+      for (; j + 8 <= d; j += 8) {  // eight loads in flight, then the chain
+        float cv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) cv[q] = ct[(int64_t)(j + q) * ktot];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc = sq_acc(acc, (double)x[j + q] - (double)cv[q]);
+      }
+      for (; j < d; ++j) acc = sq_acc(acc, (double)x[j] - (double)ct[(int64_t)j * ktot]);
+      if (bk < 0 || acc < best) { best = acc; bk = c; }
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -447,7 +461,12 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
 
 // FP64 distance of every point to its labelled centroid (oracle arithmetic),
 // stored as float32 like the reference's dists array, and per-partition
-// inertia sums in double.
+// inertia sums in double. Sixteen lanes share a point: lane s loads
+// dimensions 4s..4s+3 of each 64-dimension chunk of the point's row and its
+// centroid's row (coalesced) and squares the differences in parallel; the
+// running sum then visits the lanes in order (4 adds in lane s, then a
+// shuffle to lane s+1), so the additions happen in ascending dimension
+// order exactly as in the oracle. Padding dimensions add exact zeros.
 __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X, int64_t n,
                                                      int d, const float* __restrict__ C,
                                                      const int64_t* __restrict__ k_off, int P,
@@ -455,25 +474,65 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      float* __restrict__ dists,
                                                      double* __restrict__ inertia) {
   __shared__ double s_acc[256 / kWave];
+  const int lane = lane_id(), sub = lane & 15, base = lane & ~15;
+  const int64_t groups = (int64_t)gridDim.x * (256 / 16);
   const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
                                     reinterpret_cast<uintptr_t>(C)) & 15) == 0;
   for (int p = blockIdx.y; p < P; p += gridDim.y) {
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-      int64_t c = k_off[p] + labels[(int64_t)p * n + i];
-      const double dist = dist2_f64(X + i * d, C + c * d, d, v4);
-      float df = (float)dist;
-      if (dists) dists[(int64_t)p * n + i] = df;
-      acc += (double)df;
+    const int64_t kbeg = k_off[p];
+    double part = 0.0;
+    // wave-uniform trip count (the shuffles need every lane)
+    const int64_t g0 = (int64_t)blockIdx.x * (256 / 16) + threadIdx.x / 16;
+    const int64_t first = g0 - (threadIdx.x & (kWave - 1)) / 16;  // group of lane 0
+    for (int64_t gb = first; gb < n; gb += groups) {
+      const int64_t i = gb + base / 16;
+      const bool valid = i < n;
+      const int64_t c = valid ? kbeg + labels[(int64_t)p * n + i] : kbeg;
+      const float* xr = X + (valid ? i : 0) * d;
+      const float* cr = C + c * d;
+      double acc = 0.0;
+      for (int j0 = 0; j0 < d; j0 += 64) {
+        double sq[4];
+        const int jb = j0 + 4 * sub;
+        if (v4 && jb + 4 <= d) {  // one 16-byte load per row
+          const float4 xv = *reinterpret_cast<const float4*>(xr + jb);
+          const float4 cv = *reinterpret_cast<const float4*>(cr + jb);
+          const double e0 = (double)xv.x - (double)cv.x, e1 = (double)xv.y - (double)cv.y;
+          const double e2 = (double)xv.z - (double)cv.z, e3 = (double)xv.w - (double)cv.w;
+          sq[0] = __dmul_rn(e0, e0);
+          sq[1] = __dmul_rn(e1, e1);
+          sq[2] = __dmul_rn(e2, e2);
+          sq[3] = __dmul_rn(e3, e3);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int j = jb + q;
+            const double diff = j < d ? (double)xr[j] - (double)cr[j] : 0.0;
+            sq[q] = __dmul_rn(diff, diff);
+          }
+        }
+        const int steps = d - j0 >= 64 ? 16 : (d - j0 + 3) / 4;
+        for (int st = 0; st < steps; ++st) {
+          if (sub == st) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = __dadd_rn(acc, sq[q]);
+          }
+          acc = __shfl(acc, base + st, kWave);
+        }
+      }
+      if (valid && sub == 0) {
+        const float df = (float)acc;
+        if (dists) dists[(int64_t)p * n + i] = df;
+        part += (double)df;
+      }
     }
-    acc = wave_sum(acc);
-    if (lane_id() == 0) s_acc[threadIdx.x / kWave] = acc;
+    part = wave_sum(part);
+    if (lane == 0) s_acc[threadIdx.x / kWave] = part;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0;
-      for (int q = 0; q < 256 / kWave; ++q) s += s_acc[q];
-      if (inertia) atomicAdd(&inertia[p], s);
+    if (threadIdx.x == 0 && inertia) {
+      double t = 0;
+      for (int q = 0; q < 256 / kWave; ++q) t += s_acc[q];
+      atomicAdd(&inertia[p], t);
     }
     __syncthreads();
   }
@@ -762,6 +821,7 @@ struct LabelBufs {
   int64_t* k_off;
   uint16_t* Ch;
   uint16_t* Cl;
+  float* Ct;  // centroids transposed: [d][ktot]
   float* ncf;
   unsigned int* pstat;
   uint64_t* flags;
@@ -776,6 +836,7 @@ LabelBufs carve_label(Arena& ar, int64_t n, int d, int64_t ktot, int P) {
   b.k_off = ar.take<int64_t>(P + 1);
   b.Ch = ar.take<uint16_t>((size_t)ktot * KD + 64);
   b.Cl = ar.take<uint16_t>((size_t)ktot * KD + 64);
+  b.Ct = ar.take<float>((size_t)ktot * d);
   b.ncf = ar.take<float>(ktot);
   b.pstat = ar.take<unsigned int>((size_t)P * kStat);
   b.flag_cap = (int64_t)P * n;
@@ -833,7 +894,7 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   EAC_TRY(fill<int32_t>((int32_t*)B.pstat, (int64_t)P * kStat, 0, s));
   EAC_TRY(fill<uint64_t>((uint64_t*)B.nflag, 1, 0ull, s));
   k_cent_prep<<<grid_for(ktot, 256, 8192), 256, 0, s>>>(C, ktot, d, KD, B.k_off, P, B.Ch, B.Cl,
-                                                        B.ncf, B.pstat);
+                                                        B.Ct, B.ncf, B.pstat);
   EAC_LAUNCH_CHECK();
   int g = (int)((n + kBM - 1) / kBM);
   int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
@@ -859,14 +920,14 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if (nf > 0) {
     int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 3 * (int64_t)nf * d * (ktot / P), s);
     k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
-        X, n, d, C, B.k_off, B.flags, (int64_t)nf, labels);
+        X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels);
     EAC_LAUNCH_CHECK();
     prof_end(pr2, s);
   }
   if (n_rescored) *n_rescored = (int64_t)nf;
   if (dists || inertia_out) {
     if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
-    dim3 grid(grid_for(n, 256, 16384), P < 65535 ? P : 65535);
+    dim3 grid(grid_for(n, 16, 16384), P < 65535 ? P : 65535);  // 16 points per block
     k_exact_dists<<<grid, 256, 0, s>>>(X, n, d, C, B.k_off, P, labels, dists, inertia_out);
     EAC_LAUNCH_CHECK();
   }
