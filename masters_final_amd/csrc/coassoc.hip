@@ -115,6 +115,9 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   const int t = threadIdx.x;
   const int64_t n = pl.n;
   const int P = pl.P;
+  // the window is cleared once here; afterwards every window clears the
+  // words it used while reading them, so a clean window costs no pass
+  for (int v = t; v < win / 2; v += kCB) s_cnt[v] = 0;
   for (;;) {
     if (t == 0) s_row = (int64_t)atomicAdd(row_ctr, 1ull);
     __syncthreads();
@@ -137,16 +140,24 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
     uint32_t nnz_local = 0;
     for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += win) {
       const int64_t c1 = c0 + win < n ? c0 + win : n;
-      const int words_used = (int)((c1 - c0 + 1) >> 1);
-      for (int v = t; v < words_used; v += kCB) s_cnt[v] = 0;
-      __syncthreads();
       // members of this window: [cur, hi) with member < c1 (lists ascending);
-      // the last window takes every remaining member, no search
+      // the last window takes every remaining member, no search. Otherwise
+      // an exponential probe from cur (a window holds few members of each
+      // partition), then a binary search inside the bracket.
       if (c1 == n) {
         for (int p = t; p < P; p += kCB) s_hi[p] = s_end[p];
       } else {
         for (int p = t; p < P; p += kCB) {
           int32_t lo = s_cur[p], hi = s_end[p];
+          for (int32_t step = 16; lo < hi; step <<= 1) {
+            const int32_t b = hi - lo > step ? lo + step : hi;
+            if (pl.members[b - 1] < c1) {
+              lo = b;  // all of [lo, b) lie in the window
+            } else {
+              hi = b;
+              break;
+            }
+          }
           while (lo < hi) {
             int32_t mid = (lo + hi) >> 1;
             if (pl.members[mid] < c1) lo = mid + 1; else hi = mid;
@@ -198,6 +209,8 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         uint32_t c = 0, mx = 0;
         for (int v = t; v < words; v += kCB) {
           uint32_t x = s_cnt[v];
+          if (x == 0u) continue;
+          s_cnt[v] = 0u;
           uint32_t lo = x & 0xffffu, hi = x >> 16;
           c += (lo != 0) + (hi != 0);
           mx = lo > mx ? lo : mx;
@@ -227,6 +240,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         for (int v0 = ra; v0 < rb; v0 += kWave) {
           int v = v0 + l;
           uint32_t x = v < rb ? s_cnt[v] : 0u;
+          if (x) s_cnt[v] = 0u;
           uint32_t lo = x & 0xffffu, hi = x >> 16;
           uint32_t cnt = (lo != 0) + (hi != 0);
           uint32_t incl = wave_incl_scan(cnt);
