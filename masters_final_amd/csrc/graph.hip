@@ -129,9 +129,8 @@ struct ChunkCtx {
 // Stage row bounds of one chunk in LDS (offsets relative to e0).
 template <bool HAS_OD>
 __device__ __forceinline__ ChunkCtx stage_chunk(int64_t c, int64_t m, const int64_t* fe,
-                                                const int32_t* od, int32_t n,
-                                                const int32_t* chunk_row, int32_t* s_beg,
-                                                int32_t* s_end) {
+                                                const int32_t* od, const int32_t* chunk_row,
+                                                int32_t* s_beg, int32_t* s_end) {
   ChunkCtx cx;
   cx.e0 = c * kGTile;
   cx.e1 = cx.e0 + kGTile < m ? cx.e0 + kGTile : m;
@@ -229,7 +228,7 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
     }
   } else {
     for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-      ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+      ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, chunk_row, s_beg, s_end);
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < kGI; ++j) {
@@ -427,7 +426,7 @@ __global__ __launch_bounds__(kGB) void k_round0(
     }
     const int64_t e0 = c * kGTile;
     const int64_t e1 = e0 + kGTile < m ? e0 + kGTile : m;
-    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, chunk_row, s_beg, s_end);
     if (cx.staged) {
       for (int i = threadIdx.x; i < kGTile; i += kGB) s_row[i] = -1;
       for (int i = threadIdx.x; i < cx.r_cnt; i += kGB) s_best[i] = kInf;
@@ -760,9 +759,8 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                const unsigned int* __restrict__ had_edge,
                                                const MstPlan* __restrict__ plan,
                                                int32_t* P, unsigned long long* __restrict__ best,
-                                               int32_t n, int vb, int max_rounds,
-                                               unsigned long long* __restrict__ rlog,
-                                               unsigned int* __restrict__ overflow) {
+                                               int vb, int max_rounds,
+                                               unsigned long long* __restrict__ rlog) {
   __shared__ int32_t s_root[kMinSlots];
   __shared__ unsigned long long s_min[kMinSlots];
   __shared__ unsigned long long s_io[2 * (kGB / kWave)];
@@ -1064,7 +1062,7 @@ __global__ __launch_bounds__(kGB) void k_unite_csr(const int32_t* __restrict__ d
                                                    unsigned int* bad) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, n, chunk_row, s_beg, s_end);
+    ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, chunk_row, s_beg, s_end);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kGI; ++j) {
@@ -1465,9 +1463,8 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
     k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
                                B.st->ctl, B.st->had_edge, &B.st->plan, B.comp,
-                               B.best[(r + 1) & 1], n, vb,
-                               kMaxRounds, prof_on() ? &B.st->rlog[0][0][0] : nullptr,
-                               &B.st->overflow);
+                               B.best[(r + 1) & 1], vb, kMaxRounds,
+                               prof_on() ? &B.st->rlog[0][0][0] : nullptr);
     EAC_LAUNCH_CHECK();
     prof_end(rec.round.back(), s);
   }

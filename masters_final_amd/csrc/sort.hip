@@ -58,7 +58,6 @@ __global__ __launch_bounds__(kRB) void k_scatter(const K* __restrict__ kin,
   __shared__ int32_t s_wcnt[kRB / kWave][kRadix];
   const int t = threadIdx.x;
   const int w = t / kWave;
-  const int l = lane_id();
   s_run[t] = 0;
   s_off[t] = offs[(int64_t)t * ntiles + blockIdx.x];
 #pragma unroll
