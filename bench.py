@@ -496,7 +496,7 @@ def main():
         print(json.dumps({"copy_calib_bytes": int(x.numel()) * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the library has no "
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
     fn = {"mst": run_mst, "eac": run_eac, "kmeans": run_kmeans, "coassoc": run_coassoc,

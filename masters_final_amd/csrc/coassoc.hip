@@ -8,8 +8,8 @@
 // _condense (:212-225, condense_eac_csr :336-349) packs the rows with a
 // sequential offset sum; make_diassoc (:295-311) turns counts into
 // (max + 1) - count. The reference caps each row at max_assocs and silently
-// drops associations beyond it (sparse.py:501-503, :995-997); this build is
-// exact (no cap) and its row offsets are int64.
+// drops associations beyond it (sparse.py:501-503, :995-997); this library
+// is exact (no cap) and its row offsets are int64.
 //
 // MI355X design:
 //  * Plan: a stable radix sort of (global cluster id, point) turns labels

@@ -214,13 +214,14 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int v = t + i * kLB;
-      ph[i] = make_uint4(0, 0, 0, 0);
-      pl[i] = make_uint4(0, 0, 0, 0);
       if (v < kBN * VPR) {
         const int64_t c = c0 + v / VPR;
         if (c < kend) {
           ph[i] = reinterpret_cast<const uint4*>(Ch + c * KD)[v % VPR];
           pl[i] = reinterpret_cast<const uint4*>(Cl + c * KD)[v % VPR];
+        } else {  // only a partition's last tile takes this branch
+          ph[i] = make_uint4(0, 0, 0, 0);
+          pl[i] = make_uint4(0, 0, 0, 0);
         }
       }
     }
