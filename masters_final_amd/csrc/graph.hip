@@ -310,6 +310,15 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(
   __shared__ unsigned int s_tmp[kPlanThreads / kWave + 1];
   __shared__ uint64_t s_lo, s_hi;
   __shared__ int s_found[kLevels];
+  // the sample loads go out first (they do not need the range), all at once
+  constexpr int kPer = kPlanSamples / kPlanThreads;
+  const bool sample = levels > 1 && m > 0;  // uniform over the block
+  uint64_t sw[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int64_t e = (int64_t)((double)(threadIdx.x + q * kPlanThreads) * (double)m / kPlanSamples);
+    sw[q] = sample ? word_of<W>(weight[e < m ? e : m - 1]) : 0ull;
+  }
   if (threadIdx.x < kWave) {
     uint64_t l, hh;
     fold_range(range_lo, range_hi, &l, &hh);
@@ -328,11 +337,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_plan(
   uint64_t lim[kLevels];
 #pragma unroll
   for (int L = 0; L < kLevels; ++L) lim[L] = kInf;
-  if (levels > 1 && m > 0) {  // uniform over the block
+  if (sample) {
     const int shift = wbits > 12 ? wbits - 12 : 0;
-    for (int i = threadIdx.x; i < kPlanSamples; i += kPlanThreads) {
-      const int64_t e = (int64_t)((double)i * (double)m / kPlanSamples);
-      const uint64_t w = word_of<W>(weight[e < m ? e : m - 1]);
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const uint64_t w = sw[q];
       if (w >= wlo && w <= whi) atomicAdd(&h[(w - wlo) >> shift], 1u);
     }
     __syncthreads();
