@@ -760,13 +760,7 @@ __device__ __forceinline__ void filter_run(uint64_t* __restrict__ list, int32_t*
   }
 }
 
-// Block sizes of the per-round kernels. The filter's first rounds (most keys,
-// roots still small) run faster with more, smaller blocks; later rounds
-// (fewer keys, larger components) with 512-thread blocks, which merge more
-// keys per root in one LDS table and flush it with two slots per thread.
-constexpr int kVertB = 1024;  // per-vertex passes (hook, flatten)
-template <int kRoundB>
-__global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__ light,
+__global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
                                                int32_t* __restrict__ cnt_heavy,
@@ -778,13 +772,13 @@ __global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__
                                                unsigned long long* __restrict__ rlog) {
   __shared__ int32_t s_root[kMinSlots];
   __shared__ unsigned long long s_min[kMinSlots];
-  __shared__ unsigned long long s_io[2 * (kRoundB / kWave)];
+  __shared__ unsigned long long s_io[2 * (kGB / kWave)];
   const RoundCtl prev = ctl[r & 1];
   const RoundCtl c = next_ctl(prev, r, had_edge[r], max_rounds, plan);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
   const bool route = c.phase != prev.phase;
-  for (int i = threadIdx.x; i < kMinSlots; i += kRoundB) {
+  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
     s_root[i] = -1;
     s_min[i] = kInf;
   }
@@ -792,8 +786,8 @@ __global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__
   const int lane = lane_id();
   // each wavefront owns a contiguous run of chunks and reads their counts
   // 64 at a time, so empty chunks (most of them in late rounds) cost nothing
-  const int64_t nwaves = (int64_t)gridDim.x * (kRoundB / kWave);
-  const int64_t wid = (int64_t)blockIdx.x * (kRoundB / kWave) + threadIdx.x / kWave;
+  const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
+  const int64_t wid = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave;
   const int64_t per = (nchunks + nwaves - 1) / nwaves;
   const int64_t c_begin = wid * per;
   const int64_t c_end = c_begin + per < nchunks ? c_begin + per : nchunks;
@@ -806,8 +800,8 @@ __global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__
                       s_min, best, &sum_in, &sum_out);
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kMinSlots / kRoundB; ++j) {
-    const int i = j * kRoundB + threadIdx.x;
+  for (int j = 0; j < kMinSlots / kGB; ++j) {
+    const int i = j * kGB + threadIdx.x;
     const int32_t root = s_root[i];
     if (root < 0) continue;
     const unsigned long long key = s_min[i];
@@ -822,7 +816,7 @@ __global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__
     if (lane == 0) { s_io[2 * w] = sum_in; s_io[2 * w + 1] = sum_out; }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int i = 1; i < kRoundB / kWave; ++i) { sum_in += s_io[2 * i]; sum_out += s_io[2 * i + 1]; }
+      for (int i = 1; i < kGB / kWave; ++i) { sum_in += s_io[2 * i]; sum_out += s_io[2 * i + 1]; }
       const int slot = blockIdx.x & (kLogSlots - 1);
       if (sum_in) {
         atomicAdd(&rlog[(2 * r) * kLogSlots + slot], sum_in);
@@ -841,7 +835,7 @@ __global__ __launch_bounds__(kRoundB) void k_round(int r, uint64_t* __restrict__
 // timing). Slots of round r are read-only here (the next round's slots are
 // cleared instead), so the mirror test sees every choice of the round.
 // If s's side already hooked under v (the mirror case with v < s), v stays.
-__global__ __launch_bounds__(kVertB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
+__global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
                                               const unsigned long long* __restrict__ best,
                                               unsigned long long* __restrict__ best_next,
                                               int32_t* P, uint64_t* __restrict__ mst_key,
@@ -849,8 +843,8 @@ __global__ __launch_bounds__(kVertB) void k_hook(int r, int32_t n, const RoundCt
                                               unsigned int* __restrict__ overflow) {
   if (!ctl[r & 1].active) return;
   bool any = false;
-  for (int64_t v = (int64_t)blockIdx.x * kVertB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kVertB) {
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
     best_next[v] = kInf;
     const uint64_t k = best[v];
     if (k == kInf) continue;
@@ -872,13 +866,13 @@ __global__ __launch_bounds__(kVertB) void k_hook(int r, int32_t n, const RoundCt
 // here and parents only ever move up) and writes only that vertex, so when
 // the kernel ends the parents are exactly flat and the filter finds a root
 // with one load.
-__global__ __launch_bounds__(kVertB) void k_compress(int r, int32_t n,
+__global__ __launch_bounds__(kGB) void k_compress(int r, int32_t n,
                                                   const RoundCtl* __restrict__ ctl,
                                                   const unsigned int* __restrict__ had_edge,
                                                   int32_t* P, unsigned int* __restrict__ overflow) {
   if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked: still flat
-  for (int64_t v = (int64_t)blockIdx.x * kVertB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kVertB) {
+  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * kGB) {
     const int32_t x0 = (int32_t)v;
     int32_t x = P[x0];
     if (x == x0) continue;
@@ -1455,38 +1449,31 @@ int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int
 // number of enqueued rounds. Rounds after convergence exit at once.
 int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStream_t s) {
   const int vb = vbits_for(n);
-  const int gh = grid_for(n, kVertB, resident_grid(k_hook, kVertB));
-  const int gc = grid_for(n, kVertB, resident_grid(k_compress, kVertB));
-  int gr_cap[2] = {resident_grid(k_round<256>, 256), resident_grid(k_round<512>, 512)};
+  const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
+  const int gc = grid_for(n, kGB, resident_grid(k_compress, kGB));
+  int gr_cap = resident_grid(k_round, kGB);
   if (tune_value(EAC_TUNE_FILTER_BPCU) > 0) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const int want = cus * tune_value(EAC_TUNE_FILTER_BPCU);
-    for (int q = 0; q < 2; ++q) gr_cap[q] = want < gr_cap[q] ? want : gr_cap[q];
+    gr_cap = want < gr_cap ? want : gr_cap;
   }
-  const int gr[2] = {grid_for(B.nchunks, 1, gr_cap[0]), grid_for(B.nchunks, 1, gr_cap[1])};
-  constexpr int kBigFrom = 3;  // first round run with 512-thread blocks
+  const int gr = grid_for(B.nchunks, 1, gr_cap);
   int r = r0;
   for (; r < r0 + count && r < kMaxRounds; ++r) {
     rec.hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
-    k_hook<<<gh, kVertB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
+    k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
                               B.mst_key, B.st->had_edge, vb, &B.st->overflow);
     EAC_LAUNCH_CHECK();
     prof_end(rec.hook.back(), s);
-    k_compress<<<gc, kVertB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
+    k_compress<<<gc, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
     EAC_LAUNCH_CHECK();
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
-    if (r < kBigFrom)
-      k_round<256><<<gr[0], 256, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy,
-                                         B.nchunks, B.st->ctl, B.st->had_edge, &B.st->plan,
-                                         B.comp, B.best[(r + 1) & 1], vb, kMaxRounds,
-                                         prof_on() ? &B.st->rlog[0][0][0] : nullptr);
-    else
-      k_round<512><<<gr[1], 512, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy,
-                                         B.nchunks, B.st->ctl, B.st->had_edge, &B.st->plan,
-                                         B.comp, B.best[(r + 1) & 1], vb, kMaxRounds,
-                                         prof_on() ? &B.st->rlog[0][0][0] : nullptr);
+    k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
+                               B.st->ctl, B.st->had_edge, &B.st->plan, B.comp,
+                               B.best[(r + 1) & 1], vb, kMaxRounds,
+                               prof_on() ? &B.st->rlog[0][0][0] : nullptr);
     EAC_LAUNCH_CHECK();
     prof_end(rec.round.back(), s);
   }
