@@ -181,51 +181,76 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       __syncthreads();
       const int32_t tot = s_pre[P];
       // Flattened member index f -> partition p with s_pre[p] <= f <
-      // s_pre[p+1]. Each wavefront takes 256 consecutive f per step (four
-      // member loads in flight per lane, lanes consecutive: coalesced). A
-      // wave-uniform cursor finds the partition of the step's first f by a
-      // ballot over 64 boundaries at a time; each lane then locates its f
-      // among the next 64 boundaries (held one per lane) with a 6-step
-      // shuffle search, so no lane walks the boundaries one LDS read at a
-      // time. Steps spanning more than 64 partitions walk instead.
-      const int wv = t / kWave, ln = lane_id();
-      int pw = 0;
-      for (int32_t base = wv * 4 * kWave; base < tot; base += 4 * kCB) {
-        for (;;) {  // uniform: advance the cursor to the partition of `base`
-          const int q = pw + 1 + ln;
-          const int32_t b = q <= P ? s_pre[q] : INT32_MAX;
-          const uint64_t m = __ballot(b <= base);
-          if (m == ~0ull) {
-            pw += kWave;
-            continue;
+      // s_pre[p+1]; four member loads in flight per lane, consecutive lanes
+      // on consecutive f (coalesced). Long runs (32+ members per partition
+      // on average): lanes stride the block (f = t, t + kCB, ...) and walk
+      // the few boundaries each step crosses. Short runs: each wavefront
+      // takes 256 consecutive f per step; a wave-uniform cursor finds the
+      // partition of the step's first f by a ballot over 64 boundaries at a
+      // time, and each lane locates its f among the next 64 boundaries (held
+      // one per lane) with a 6-step shuffle search, so no lane walks the
+      // boundaries one LDS read at a time (steps spanning more than 64
+      // partitions walk instead).
+      if (tot >= 32 * P) {
+        int p = 0;
+        for (int32_t f0 = t; f0 < tot; f0 += 4 * kCB) {
+          int32_t j[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int32_t f = f0 + u * kCB;
+            j[u] = -1;
+            if (f < tot) {
+              while (s_pre[p + 1] <= f) ++p;
+              j[u] = pl.members[s_cur[p] + (f - s_pre[p])];
+            }
           }
-          pw += __popcll(m);
-          break;
-        }
-        const int qb = pw + 1 + ln;
-        const int32_t bnd = qb <= P ? s_pre[qb] : INT32_MAX;  // ascending by lane
-        const bool covered = __shfl(bnd, kWave - 1, kWave) > base + 4 * kWave - 1;
-        int32_t j[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int32_t f = base + u * kWave + ln;
-          int p = pw;
-          if (covered) {
-            int c = 0;  // boundaries <= f among the 64 held (at most 63 here)
-#pragma unroll
-            for (int step = kWave / 2; step > 0; step >>= 1)
-              if (__shfl(bnd, c + step - 1, kWave) <= f) c += step;
-            p += c;
-          } else if (f < tot) {
-            while (s_pre[p + 1] <= f) ++p;
+          for (int u = 0; u < 4; ++u) {
+            if (j[u] < 0 || j[u] == (int32_t)i) continue;
+            const int jj = (int)(j[u] - c0);
+            atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
           }
-          j[u] = f < tot ? pl.members[s_cur[p] + (f - s_pre[p])] : -1;
         }
+      } else {
+        const int wv = t / kWave, ln = lane_id();
+        int pw = 0;
+        for (int32_t base = wv * 4 * kWave; base < tot; base += 4 * kCB) {
+          for (;;) {  // uniform: advance the cursor to the partition of `base`
+            const int q = pw + 1 + ln;
+            const int32_t b = q <= P ? s_pre[q] : INT32_MAX;
+            const uint64_t m = __ballot(b <= base);
+            if (m == ~0ull) {
+              pw += kWave;
+              continue;
+            }
+            pw += __popcll(m);
+            break;
+          }
+          const int qb = pw + 1 + ln;
+          const int32_t bnd = qb <= P ? s_pre[qb] : INT32_MAX;  // ascending by lane
+          const bool covered = __shfl(bnd, kWave - 1, kWave) > base + 4 * kWave - 1;
+          int32_t j[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (j[u] < 0 || j[u] == (int32_t)i) continue;
-          const int jj = (int)(j[u] - c0);
-          atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
+          for (int u = 0; u < 4; ++u) {
+            const int32_t f = base + u * kWave + ln;
+            int p = pw;
+            if (covered) {
+              int c = 0;  // boundaries <= f among the 64 held (at most 63 here)
+#pragma unroll
+              for (int step = kWave / 2; step > 0; step >>= 1)
+                if (__shfl(bnd, c + step - 1, kWave) <= f) c += step;
+              p += c;
+            } else if (f < tot) {
+              while (s_pre[p + 1] <= f) ++p;
+            }
+            j[u] = f < tot ? pl.members[s_cur[p] + (f - s_pre[p])] : -1;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (j[u] < 0 || j[u] == (int32_t)i) continue;
+            const int jj = (int)(j[u] - c0);
+            atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
+          }
         }
       }
       __syncthreads();
