@@ -460,6 +460,91 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
   }
 }
 
+// FP64 re-scoring with the partition's centroids staged in LDS, for flags
+// sorted by partition: a block takes 64 flags; four threads per flag split
+// the centroids (c = s, s+4, ... ascending, two FP64 chains at a time) and
+// read them from LDS tiles shared by all 64 flags, so each centroid is read
+// from memory once per 64 flagged points instead of once per point. Strict <
+// in ascending order inside a subset, then the lowest index among equal
+// minima across the four subsets: the oracle's first minimum.
+constexpr int kRtF = 64;  // flags per block
+constexpr int kRtS = 4;   // threads per flag
+constexpr int kRtT = 64;  // centroids per LDS tile
+inline size_t rescore_tiled_lds(int d) { return (size_t)(kRtF + kRtT) * (d + 1) * sizeof(float); }
+__global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
+    const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C,
+    const int64_t* __restrict__ k_off, const uint64_t* __restrict__ flags, int64_t nflag,
+    int32_t* __restrict__ labels) {
+  extern __shared__ float s_dyn[];
+  __shared__ double s_best[kRtS][kRtF];
+  __shared__ int32_t s_bk[kRtS][kRtF];
+  constexpr uint64_t kRowMask = (1ull << 40) - 1;
+  const int RS = d + 1;  // padded rows: conflict-free column reads
+  float* s_x = s_dyn;
+  float* s_c = s_dyn + kRtF * RS;
+  const int t = threadIdx.x, g = t & (kRtF - 1), sub = t / kRtF;
+  const int64_t f0 = (int64_t)blockIdx.x * kRtF;
+  const int cnt = nflag - f0 < kRtF ? (int)(nflag - f0) : kRtF;
+  for (int done = 0; done < cnt;) {  // runs of one partition (block-uniform)
+    const int p = (int)(flags[f0 + done] >> 40);
+    int end = done + 1;
+    while (end < cnt && (int)(flags[f0 + end] >> 40) == p) ++end;
+    for (int v = t; v < (end - done) * d; v += kRtF * kRtS) {
+      const int r = v / d, j = v - r * d;
+      s_x[r * RS + j] = X[(int64_t)(flags[f0 + done + r] & kRowMask) * d + j];
+    }
+    const bool active = g >= done && g < end;
+    const float* xa = s_x + (active ? g - done : 0) * RS;
+    const int64_t kbeg = k_off[p], kend = k_off[p + 1];
+    double best = DBL_MAX;
+    int64_t bk = -1;
+    for (int64_t c0 = kbeg; c0 < kend; c0 += kRtT) {
+      const int rows = kend - c0 < kRtT ? (int)(kend - c0) : kRtT;
+      __syncthreads();  // the rows are staged, the previous tile is consumed
+      for (int v = t; v < rows * d; v += kRtF * kRtS) {
+        const int r = v / d, j = v - r * d;
+        s_c[r * RS + j] = C[(c0 + r) * d + j];
+      }
+      __syncthreads();
+      if (!active) continue;
+      int c = sub;
+      for (; c + kRtS < rows; c += 2 * kRtS) {
+        const float* ca = s_c + c * RS;
+        const float* cb = ca + kRtS * RS;
+        double a0 = 0.0, a1 = 0.0;
+        for (int j = 0; j < d; ++j) {
+          const double xv = (double)xa[j];
+          a0 = sq_acc(a0, xv - (double)ca[j]);
+          a1 = sq_acc(a1, xv - (double)cb[j]);
+        }
+        if (bk < 0 || a0 < best) { best = a0; bk = c0 + c; }
+        if (a1 < best) { best = a1; bk = c0 + c + kRtS; }
+      }
+      for (; c < rows; c += kRtS) {
+        const float* ca = s_c + c * RS;
+        double a0 = 0.0;
+        for (int j = 0; j < d; ++j) a0 = sq_acc(a0, (double)xa[j] - (double)ca[j]);
+        if (bk < 0 || a0 < best) { best = a0; bk = c0 + c; }
+      }
+    }
+    s_best[sub][g] = best;
+    s_bk[sub][g] = bk < 0 ? -1 : (int32_t)(bk - kbeg);
+    __syncthreads();
+    if (active && sub == 0) {
+      double b = s_best[0][g];
+      int32_t k = s_bk[0][g];
+      for (int q = 1; q < kRtS; ++q) {
+        const double ob = s_best[q][g];
+        const int32_t ok = s_bk[q][g];
+        if (ok >= 0 && (k < 0 || ob < b || (ob == b && ok < k))) { b = ob; k = ok; }
+      }
+      labels[(int64_t)p * n + (int64_t)(flags[f0 + g] & kRowMask)] = k;
+    }
+    __syncthreads();
+    done = end;
+  }
+}
+
 // FP64 distance of every point to its labelled centroid (oracle arithmetic),
 // stored as float32 like the reference's dists array, and per-partition
 // inertia sums in double. Sixteen lanes share a point: lane s loads
@@ -920,8 +1005,35 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if ((int64_t)nf > B.flag_cap) return fail(EAC_ERR_HIP, "kmeans: flag list overflow");
   if (nf > 0) {
     int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 3 * (int64_t)nf * d * (ktot / P), s);
-    k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
-        X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels);
+    // Many flags (at least 1024 blocks of 64): flags sorted by partition
+    // (one radix pass for up to 256 partitions) feed the LDS-tiled
+    // re-scoring. Few flags, or no room for the sort: one wavefront per flag
+    // over the transposed centroids (more parallelism per flag).
+    const uint64_t* fl = B.flags;
+    bool tiled = nf >= (unsigned long long)kRtF * 1024;
+    if (tiled && P > 1) {
+      const size_t mark2 = ar.used;
+      uint64_t* sorted = ar.take<uint64_t>(nf);
+      if (ar.overflow() || ar.cap - ar.used < radix_sort_ws_bytes((int64_t)nf, 8, 0)) {
+        tiled = false;
+        ar.used = mark2;
+      } else {
+        const int pb = bits_for((uint64_t)(P - 1));
+        EAC_TRY((radix_sort<uint64_t, NoValue>(B.flags, sorted, nullptr, nullptr, (int64_t)nf, 40,
+                                               40 + (pb < 1 ? 1 : pb), ar, s)));
+        fl = sorted;
+      }
+    }
+    if (tiled) {
+      const size_t lds = rescore_tiled_lds(d);
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_rescore_tiled),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      k_rescore_tiled<<<(unsigned)(((int64_t)nf + kRtF - 1) / kRtF), kRtF * kRtS, lds, s>>>(
+          X, n, d, C, B.k_off, fl, (int64_t)nf, labels);
+    } else {
+      k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
+          X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels);
+    }
     EAC_LAUNCH_CHECK();
     prof_end(pr2, s);
   }

@@ -52,6 +52,25 @@ def test_label_ties_lowest_index(torch):
     assert nres > 0  # tied points cannot be certified: they are re-scored
 
 
+def test_rescore_many_flags_tiled(torch):
+    """More than 65,536 flagged points (ties between exact duplicate
+    centroids) take the partition-sorted, LDS-tiled FP64 re-scoring; labels
+    must still be the oracle's (lowest index among equal distances)."""
+    from masters_final_amd.cluster.KMeans import label_device
+    rng = np.random.default_rng(31)
+    X, _ = synth.gmm_points(70000, 16, 20, 31, mean_std=2.0)
+    base0 = X[rng.choice(70000, 20, replace=False)]
+    base1 = X[rng.choice(70000, 25, replace=False)]
+    C0 = np.concatenate([base0, base0])                  # every point ties
+    C1 = np.concatenate([base1[:5], base1, base1[5:9]])  # some points tie
+    ks = [40, 34]
+    lab, _, nres = label_device(X, np.concatenate([C0, C1]), k_list=ks, precision="bf16x3")
+    assert nres >= 65536
+    for p, Cp in enumerate((C0, C1)):
+        want, _ = O.kmeans_label_f64(X, Cp)
+        assert np.array_equal(lab[p].cpu().numpy(), want), p
+
+
 def test_label_fp32_plain(torch):
     from masters_final_amd.cluster.KMeans import label_device
     X, Cc = _data(4000, 16, 50, seed=9)
