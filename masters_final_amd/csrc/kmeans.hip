@@ -125,6 +125,7 @@ constexpr int kLB = 256;       // threads per labeling block (4 waves)
 constexpr int kRowsW = 32;     // points per wave
 constexpr int kBM = 128;       // points per block
 constexpr int kBN = 64;        // centroids per LDS tile
+constexpr int kFlagSlots = 64; // flag list regions (one counter each)
 
 // Rigorous bound on |approx - oracle| for one point and one partition.
 __device__ __forceinline__ float point_bound(float nx, float xn, float xhn, float xln, float exn,
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
     const uint16_t* __restrict__ Cl, const float* __restrict__ ncf,
     const int64_t* __restrict__ k_off, int P, const unsigned int* __restrict__ pstat,
     int32_t* __restrict__ labels, uint64_t* __restrict__ flags,
-    unsigned long long* __restrict__ nflag, int64_t flag_cap) {
+    unsigned long long* __restrict__ nflag, int64_t region) {
   constexpr int KB = KD / 32;
   constexpr int LROW = KD + 8;  // padded LDS row (bf16 elements)
   __shared__ uint16_t s_h[2][kBN * LROW];
@@ -375,11 +376,14 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
     uint32_t incl = wave_incl_scan(cnt);
     uint32_t tot = __shfl(incl, kWave - 1, kWave);
     unsigned long long base = 0;
-    if (l == kWave - 1 && tot) base = atomicAdd(nflag, (unsigned long long)tot);
+    // flags go to one of kFlagSlots regions (by block): counters on one
+    // address would serialise one atomic per wave and partition
+    const int slot = blockIdx.x % kFlagSlots;
+    if (l == kWave - 1 && tot) base = atomicAdd(&nflag[slot], (unsigned long long)tot);
     base = __shfl(base, kWave - 1, kWave);
     for (int q = 0; q < nm; ++q) {
       unsigned long long o = base + incl - cnt + q;
-      if (o < (unsigned long long)flag_cap) flags[o] = mine[q];
+      if (o < (unsigned long long)region) flags[(int64_t)slot * region + (int64_t)o] = mine[q];
     }
   }
 }
@@ -542,6 +546,31 @@ __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
     }
     __syncthreads();
     done = end;
+  }
+}
+
+// Concatenates the kFlagSlots flag regions (counts cnt[slot]) into one list.
+__global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t region,
+                               const unsigned long long* __restrict__ cnt,
+                               uint64_t* __restrict__ out, int64_t total) {
+  __shared__ unsigned long long s_pre[kFlagSlots + 1];
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0;
+    for (int q = 0; q < kFlagSlots; ++q) {
+      s_pre[q] = a;
+      a += cnt[q];
+    }
+    s_pre[kFlagSlots] = a;
+  }
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = kFlagSlots;  // slot q with s_pre[q] <= i < s_pre[q+1]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (s_pre[mid] <= (unsigned long long)i) lo = mid; else hi = mid;
+    }
+    out[i] = regions[(int64_t)lo * region + (i - (int64_t)s_pre[lo])];
   }
 }
 
@@ -910,10 +939,12 @@ struct LabelBufs {
   float* Ct;  // centroids transposed: [d][ktot]
   float* ncf;
   unsigned int* pstat;
-  uint64_t* flags;
-  unsigned long long* nflag;
+  uint64_t* flags_r;  // kFlagSlots regions of `region` flags
+  uint64_t* flags;    // the flags of all regions, contiguous
+  unsigned long long* nflag;  // per-region counts
   double* inertia;
   int64_t flag_cap;
+  int64_t region;
 };
 
 LabelBufs carve_label(Arena& ar, int64_t n, int d, int64_t ktot, int P) {
@@ -926,8 +957,12 @@ LabelBufs carve_label(Arena& ar, int64_t n, int d, int64_t ktot, int P) {
   b.ncf = ar.take<float>(ktot);
   b.pstat = ar.take<unsigned int>((size_t)P * kStat);
   b.flag_cap = (int64_t)P * n;
+  // a region holds every (partition, row) its blocks can flag
+  const int64_t nblocks = (n + kBM - 1) / kBM;
+  b.region = ((nblocks + kFlagSlots - 1) / kFlagSlots) * kBM * (int64_t)P;
+  b.flags_r = ar.take<uint64_t>((size_t)kFlagSlots * b.region);
   b.flags = ar.take<uint64_t>(b.flag_cap);
-  b.nflag = ar.take<unsigned long long>(1);
+  b.nflag = ar.take<unsigned long long>(kFlagSlots);
   b.inertia = ar.take<double>(P);
   return b;
 }
@@ -978,7 +1013,7 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   const int KD = kd_for(d);
   if (!KD) return fail(EAC_ERR_UNSUPPORTED, "kmeans bf16x3: d=%d > 128", d);
   EAC_TRY(fill<int32_t>((int32_t*)B.pstat, (int64_t)P * kStat, 0, s));
-  EAC_TRY(fill<uint64_t>((uint64_t*)B.nflag, 1, 0ull, s));
+  EAC_TRY(fill<uint64_t>((uint64_t*)B.nflag, kFlagSlots, 0ull, s));
   k_cent_prep<<<grid_for(ktot, 256, 8192), 256, 0, s>>>(C, ktot, d, KD, B.k_off, P, B.Ch, B.Cl,
                                                         B.Ct, B.ncf, B.pstat);
   EAC_LAUNCH_CHECK();
@@ -987,23 +1022,31 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   switch (KD) {
     case 32:
       k_label_bf16x3<32><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                           labels, B.flags, B.nflag, B.flag_cap);
+                                           labels, B.flags_r, B.nflag, B.region);
       break;
     case 64:
       k_label_bf16x3<64><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                           labels, B.flags, B.nflag, B.flag_cap);
+                                           labels, B.flags_r, B.nflag, B.region);
       break;
     default:
       k_label_bf16x3<128><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                            labels, B.flags, B.nflag, B.flag_cap);
+                                            labels, B.flags_r, B.nflag, B.region);
       break;
   }
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
+  unsigned long long hcnt[kFlagSlots];
+  EAC_TRY(d2h(hcnt, B.nflag, sizeof(hcnt), s));
   unsigned long long nf = 0;
-  EAC_TRY(d2h(&nf, B.nflag, sizeof(nf), s));
+  for (int q = 0; q < kFlagSlots; ++q) {
+    if ((int64_t)hcnt[q] > B.region) return fail(EAC_ERR_HIP, "kmeans: flag list overflow");
+    nf += hcnt[q];
+  }
   if ((int64_t)nf > B.flag_cap) return fail(EAC_ERR_HIP, "kmeans: flag list overflow");
   if (nf > 0) {
+    k_gather_flags<<<grid_for((int64_t)nf, 256, 65536), 256, 0, s>>>(B.flags_r, B.region, B.nflag,
+                                                                     B.flags, (int64_t)nf);
+    EAC_LAUNCH_CHECK();
     int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 3 * (int64_t)nf * d * (ktot / P), s);
     // Many flags (at least 1024 blocks of 64): flags sorted by partition
     // (one radix pass for up to 256 partitions) feed the LDS-tiled
