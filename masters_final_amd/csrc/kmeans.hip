@@ -576,12 +576,14 @@ __global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t reg
 
 // FP64 distance of every point to its labelled centroid (oracle arithmetic),
 // stored as float32 like the reference's dists array, and per-partition
-// inertia sums in double. Sixteen lanes share a point: lane s loads
-// dimensions 4s..4s+3 of each 64-dimension chunk of the point's row and its
-// centroid's row (coalesced) and squares the differences in parallel; the
-// running sum then visits the lanes in order (4 adds in lane s, then a
-// shuffle to lane s+1), so the additions happen in ascending dimension
-// order exactly as in the oracle. Padding dimensions add exact zeros.
+// inertia sums in double. LPP lanes share a point (4, 8 or 16: about d/4):
+// lane s loads dimensions 4s..4s+3 of each (4 LPP)-dimension chunk of the
+// point's row and its centroid's row (coalesced) and squares the
+// differences in parallel; the running sum then visits the lanes in order
+// (4 adds in lane s, then a shuffle to lane s+1), so the additions happen in
+// ascending dimension order exactly as in the oracle. Padding dimensions add
+// exact zeros.
+template <int LPP>
 __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X, int64_t n,
                                                      int d, const float* __restrict__ C,
                                                      const int64_t* __restrict__ k_off, int P,
@@ -589,24 +591,25 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      float* __restrict__ dists,
                                                      double* __restrict__ inertia) {
   __shared__ double s_acc[256 / kWave];
-  const int lane = lane_id(), sub = lane & 15, base = lane & ~15;
-  const int64_t groups = (int64_t)gridDim.x * (256 / 16);
+  constexpr int PPB = 256 / LPP;  // points per block and step
+  const int lane = lane_id(), sub = lane & (LPP - 1), base = lane & ~(LPP - 1);
+  const int64_t groups = (int64_t)gridDim.x * PPB;
   const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
                                     reinterpret_cast<uintptr_t>(C)) & 15) == 0;
   for (int p = blockIdx.y; p < P; p += gridDim.y) {
     const int64_t kbeg = k_off[p];
     double part = 0.0;
     // wave-uniform trip count (the shuffles need every lane)
-    const int64_t g0 = (int64_t)blockIdx.x * (256 / 16) + threadIdx.x / 16;
-    const int64_t first = g0 - (threadIdx.x & (kWave - 1)) / 16;  // group of lane 0
+    const int64_t g0 = (int64_t)blockIdx.x * PPB + threadIdx.x / LPP;
+    const int64_t first = g0 - (threadIdx.x & (kWave - 1)) / LPP;  // group of lane 0
     for (int64_t gb = first; gb < n; gb += groups) {
-      const int64_t i = gb + base / 16;
+      const int64_t i = gb + base / LPP;
       const bool valid = i < n;
       const int64_t c = valid ? kbeg + labels[(int64_t)p * n + i] : kbeg;
       const float* xr = X + (valid ? i : 0) * d;
       const float* cr = C + c * d;
       double acc = 0.0;
-      for (int j0 = 0; j0 < d; j0 += 64) {
+      for (int j0 = 0; j0 < d; j0 += 4 * LPP) {
         double sq[4];
         const int jb = j0 + 4 * sub;
         if (v4 && jb + 4 <= d) {  // one 16-byte load per row
@@ -626,7 +629,7 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
             sq[q] = __dmul_rn(diff, diff);
           }
         }
-        const int steps = d - j0 >= 64 ? 16 : (d - j0 + 3) / 4;
+        const int steps = d - j0 >= 4 * LPP ? LPP : (d - j0 + 3) / 4;
         for (int st = 0; st < steps; ++st) {
           if (sub == st) {
 #pragma unroll
@@ -1083,8 +1086,19 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if (n_rescored) *n_rescored = (int64_t)nf;
   if (dists || inertia_out) {
     if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
-    dim3 grid(grid_for(n, 16, 16384), P < 65535 ? P : 65535);  // 16 points per block
-    k_exact_dists<<<grid, 256, 0, s>>>(X, n, d, C, B.k_off, P, labels, dists, inertia_out);
+    const int py = P < 65535 ? P : 65535;
+    if (d <= 16)
+      k_exact_dists<4><<<dim3(grid_for(n, 64, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
+                                                                          labels, dists,
+                                                                          inertia_out);
+    else if (d <= 32)
+      k_exact_dists<8><<<dim3(grid_for(n, 32, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
+                                                                          labels, dists,
+                                                                          inertia_out);
+    else
+      k_exact_dists<16><<<dim3(grid_for(n, 16, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
+                                                                           labels, dists,
+                                                                           inertia_out);
     EAC_LAUNCH_CHECK();
   }
   return EAC_OK;
