@@ -78,6 +78,29 @@ __global__ void k_member_pos(const uint32_t* __restrict__ skey, const uint32_t* 
   }
 }
 
+// Window table: wtab[g * (nw + 1) + w] = first position in cluster g's
+// member list whose member is >= w * win (windows aligned to multiples of
+// win), so a window's end in every partition is one load instead of a
+// search. Built from the sorted (cluster, point) keys: each position writes
+// the windows that start after its predecessor's member and up to its own,
+// the last member of a cluster also the windows after it.
+__global__ void k_win_table(const uint32_t* __restrict__ skey, const int32_t* __restrict__ members,
+                            const int32_t* __restrict__ cstart, int64_t total, int64_t ktot,
+                            int win, int nw, int32_t* __restrict__ wtab) {
+  const int64_t ts = nw + 1;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t g = skey[q];
+    if (g >= (uint32_t)ktot) continue;  // unassigned points sort last
+    const int wq = members[q] / win;
+    const int wp = q == cstart[g] ? -1 : members[q - 1] / win;
+    int32_t* t = wtab + (int64_t)g * ts;
+    for (int w = wp + 1; w <= wq; ++w) t[w] = (int32_t)q;
+    if (q + 1 == cstart[g + 1])
+      for (int w = wq + 1; w <= nw; ++w) t[w] = (int32_t)(q + 1);
+  }
+}
+
 struct CoPlan {
   const int32_t* labels;
   int64_t n;
@@ -86,6 +109,8 @@ struct CoPlan {
   const int32_t* members;
   const int32_t* pos;     // [P*n]
   const int32_t* cstart;  // [ktot+1] start of each global cluster in members
+  const int32_t* wtab;    // window table (nullptr: search the window ends)
+  int nw;                 // windows of the table: ceil(n / win)
 };
 
 // One row: walk every partition's co-members > i (or all co-members != i for
@@ -109,6 +134,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   int32_t* s_end = s_cur + pl.P;
   int32_t* s_hi = s_end + pl.P;
   int32_t* s_pre = s_hi + pl.P;
+  int32_t* s_g = s_pre + pl.P + 1;  // global cluster of the row in each partition
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
   __shared__ unsigned int s_max;
@@ -133,12 +159,14 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         int64_t g = pl.k_off[p] + pl.labels[(int64_t)p * n + i];
         s_cur[p] = full_rows ? pl.cstart[g] : q + 1;
         s_end[p] = pl.cstart[g + 1];
+        s_g[p] = (int32_t)g;
       }
     }
     int64_t row_base = PASS == 1 ? indptr[i] : 0;
     int64_t written = 0;
     uint32_t nnz_local = 0;
-    for (int64_t c0 = full_rows ? 0 : i + 1; c0 < n; c0 += win) {
+    // windows aligned to multiples of win (columns <= i get no counts)
+    for (int64_t c0 = full_rows ? 0 : ((i + 1) / win) * win; c0 < n; c0 += win) {
       const int64_t c1 = c0 + win < n ? c0 + win : n;
       // members of this window: [cur, hi) with member < c1 (lists ascending);
       // the last window takes every remaining member, no search. Otherwise
@@ -146,6 +174,11 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       // partition), then a binary search inside the bracket.
       if (c1 == n) {
         for (int p = t; p < P; p += kCB) s_hi[p] = s_end[p];
+      } else if (pl.wtab) {
+        const int64_t w1 = c1 / win;
+        for (int p = t; p < P; p += kCB)
+          s_hi[p] = s_cur[p] >= s_end[p] ? s_end[p]
+                                         : pl.wtab[(int64_t)s_g[p] * (pl.nw + 1) + w1];
       } else {
         for (int p = t; p < P; p += kCB) {
           int32_t lo = s_cur[p], hi = s_end[p];
@@ -393,7 +426,21 @@ int co_window(int64_t n) {
   if (need < w) w = (int)need;
   return w < 64 ? 64 : (w & ~1);
 }
-inline size_t co_lds(int win, int P) { return (size_t)(win / 2 + 4 * P + 1) * 4; }
+inline size_t co_lds(int win, int P) { return (size_t)(win / 2 + 5 * P + 1) * 4; }
+
+// Builds the window table in the sort staging (free after the plan) when it
+// fits; returns nullptr otherwise (the rows then search their window ends).
+int32_t* build_win_table(const CoBufs& B, int64_t n, int P, int64_t ktot, int win, int* nw,
+                         hipStream_t s) {
+  *nw = (int)((n + win - 1) / win);
+  const int64_t items = (int64_t)P * n;
+  if (ktot * (int64_t)(*nw + 1) > items) return nullptr;
+  int32_t* wtab = reinterpret_cast<int32_t*>(B.key);
+  k_win_table<<<grid_for(items, 256, 16384), 256, 0, s>>>(B.skey, B.members, B.cstart, items,
+                                                          ktot, win, *nw, wtab);
+  if (hipGetLastError() != hipSuccess) return nullptr;
+  return wtab;
+}
 // Every workgroup resident at once (rows are handed out dynamically).
 template <int PASS>
 int co_grid(int win, int P) {
@@ -463,12 +510,13 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
                                                           B.members, B.pos);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart};
-  int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
   const int win = co_window(n);
-  k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(pl, full_rows, win, B.ctr, B.row_nnz,
-                                                          B.misc + 1, nullptr, 0,
-                                         nullptr, nullptr);
+  int nw = 0;
+  const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw};
+  int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
+  k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(
+      pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr);
   EAC_LAUNCH_CHECK();
   prof_end(pr2, s);
   EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
@@ -501,14 +549,15 @@ extern "C" int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t*
   Arena ar(ws, ws_bytes);
   CoBufs B = carve_co(ar, n, P, ktot);
   EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart};
   int64_t nnz = 0;
   EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
-  int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
   const int win = co_window(n);
-  k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(pl, full_rows, win, B.ctr, nullptr,
-                                                          nullptr, indptr, w_offset,
-                                         indices, data);
+  int nw = 0;
+  const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw};
+  int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
+  k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(
+      pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
   EAC_HIP_CHECK(hipStreamSynchronize(s));

@@ -121,7 +121,7 @@ def autotune_mst(dest, weight, indptr, grid=None, reps=3, use_cache=True):
             "candidates": [{"ms": ms, **k} for ms, k in results]}
 
 
-def autotune_coassoc(labels, ks, windows=(8192, 16384, 32768), use_cache=True):
+def autotune_coassoc(labels, ks, windows=(4096, 8192, 16384, 32768), use_cache=True):
     """Picks the co-association LDS window for this ensemble (device labels
     [P][n], cluster counts ks): timed candidates, identical output checked
     (row offsets and the largest count). Returns dict(knobs, ms, ...)."""
