@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   // launch-time LDS: window counters (win / 2 words), then per partition
   // the next member position, the cluster end, this window's member end and
   // the flattened-range prefix (P + 1)
-  extern __shared__ uint32_t s_dyn[];
+  extern __shared__ __align__(16) uint32_t s_dyn[];
   uint32_t* s_cnt = s_dyn;
   int32_t* s_cur = reinterpret_cast<int32_t*>(s_dyn + win / 2);
   int32_t* s_end = s_cur + pl.P;
@@ -288,31 +288,55 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       }
       __syncthreads();
       for (int p2 = t; p2 < P; p2 += kCB) s_cur[p2] = s_hi[p2];
-      // read the window: lane-strided words (two columns per word)
+      // read the window four words (eight columns) per lane and load, clearing
+      // the words it used; PASS 1 emits them in column order
       const int width = (int)(c1 - c0);
       const int words = (width + 1) >> 1;
+      const int groups = (words + 3) >> 2;  // 4-word groups, the last may be partial
+      uint4* s4 = reinterpret_cast<uint4*>(s_cnt);
+      // words 4g..4g+3 of group g (zeros past the window), cleared when used
+      auto read_group = [&](int g, uint32_t* xw) {
+        if (4 * g + 4 <= words) {
+          const uint4 x = s4[g];
+          xw[0] = x.x; xw[1] = x.y; xw[2] = x.z; xw[3] = x.w;
+          if (x.x | x.y | x.z | x.w) s4[g] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int v = 4 * g + q;
+            xw[q] = v < words ? s_cnt[v] : 0u;
+            if (xw[q]) s_cnt[v] = 0u;
+          }
+        }
+      };
       if (PASS == 0) {
         uint32_t c = 0, mx = 0;
-        for (int v = t; v < words; v += kCB) {
-          uint32_t x = s_cnt[v];
-          if (x == 0u) continue;
-          s_cnt[v] = 0u;
-          uint32_t lo = x & 0xffffu, hi = x >> 16;
-          c += (lo != 0) + (hi != 0);
-          mx = lo > mx ? lo : mx;
-          mx = hi > mx ? hi : mx;
+        for (int g = t; g < groups; g += kCB) {
+          uint32_t xw[4];
+          read_group(g, xw);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t lo = xw[q] & 0xffffu, hi = xw[q] >> 16;
+            c += (lo != 0) + (hi != 0);
+            mx = lo > mx ? lo : mx;
+            mx = hi > mx ? hi : mx;
+          }
         }
         nnz_local += c;
         if (mx) atomicMax(&s_max, mx);
       } else {
-        // wave w emits words [w*R, (w+1)*R) in column order
+        // wave w emits groups [w*RG, (w+1)*RG) in column order
         const int w = t / kWave, l = lane_id();
-        const int R = ((words + 4 * kWave - 1) / (4 * kWave)) * kWave;
-        const int ra = w * R, rb = (w + 1) * R < words ? (w + 1) * R : words;
+        const int RG = ((groups + 4 * kWave - 1) / (4 * kWave)) * kWave;
+        const int ga = w * RG, gb = (w + 1) * RG < groups ? (w + 1) * RG : groups;
         uint32_t c = 0;
-        for (int v = ra + l; v < rb; v += kWave) {
-          uint32_t x = s_cnt[v];
-          c += ((x & 0xffffu) != 0) + ((x >> 16) != 0);
+        for (int g = ga + l; g < gb; g += kWave) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int v = 4 * g + q;
+            const uint32_t x = v < words ? s_cnt[v] : 0u;
+            c += ((x & 0xffffu) != 0) + ((x >> 16) != 0);
+          }
         }
         c = wave_sum(c);
         if (l == 0) s_tmp[w] = c;
@@ -323,22 +347,29 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           if (q < w) o += s_tmp[q];
           wtot += s_tmp[q];
         }
-        for (int v0 = ra; v0 < rb; v0 += kWave) {
-          int v = v0 + l;
-          uint32_t x = v < rb ? s_cnt[v] : 0u;
-          if (x) s_cnt[v] = 0u;
-          uint32_t lo = x & 0xffffu, hi = x >> 16;
-          uint32_t cnt = (lo != 0) + (hi != 0);
-          uint32_t incl = wave_incl_scan(cnt);
+        for (int g0 = ga; g0 < gb; g0 += kWave) {
+          const int g = g0 + l;
+          uint32_t xw[4] = {0u, 0u, 0u, 0u};
+          if (g < gb) read_group(g, xw);
+          uint32_t cnt = 0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cnt += ((xw[q] & 0xffffu) != 0) + ((xw[q] >> 16) != 0);
+          const uint32_t incl = wave_incl_scan(cnt);
           int64_t at = o + (incl - cnt);
-          if (lo) {
-            out_col[at] = (int32_t)(c0 + 2 * v);
-            out_val[at] = w_offset ? w_offset - (int32_t)lo : (int32_t)lo;
-            ++at;
-          }
-          if (hi) {
-            out_col[at] = (int32_t)(c0 + 2 * v + 1);
-            out_val[at] = w_offset ? w_offset - (int32_t)hi : (int32_t)hi;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const uint32_t lo = xw[q] & 0xffffu, hi = xw[q] >> 16;
+            const int32_t col = (int32_t)(c0 + 2 * (4 * g + q));
+            if (lo) {
+              out_col[at] = col;
+              out_val[at] = w_offset ? w_offset - (int32_t)lo : (int32_t)lo;
+              ++at;
+            }
+            if (hi) {
+              out_col[at] = col + 1;
+              out_val[at] = w_offset ? w_offset - (int32_t)hi : (int32_t)hi;
+              ++at;
+            }
           }
           o += __shfl(incl, kWave - 1, kWave);
         }
