@@ -144,11 +144,22 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   // the window is cleared once here; afterwards every window clears the
   // words it used while reading them, so a clean window costs no pass
   for (int v = t; v < win / 2; v += kCB) s_cnt[v] = 0;
+  // Rows are taken in the order of partition 0's member list (points
+  // grouped by their partition-0 cluster) when every point is assigned
+  // there: rows in flight at the same time then share most of their
+  // clusters, so their member runs are read from cache. The output does not
+  // depend on the order (each row writes its own segment).
+  const int32_t* order = nullptr;
+  {
+    const int32_t a = pl.cstart[pl.k_off[0]], b = pl.cstart[pl.k_off[1]];
+    if ((int64_t)(b - a) == n) order = pl.members + a;
+  }
   for (;;) {
     if (t == 0) s_row = (int64_t)atomicAdd(row_ctr, 1ull);
     __syncthreads();
-    const int64_t i = s_row;
-    if (i >= n) break;
+    const int64_t r = s_row;
+    if (r >= n) break;
+    const int64_t i = order ? (int64_t)order[r] : r;
     if (t == 0) s_max = 0;
     for (int p = t; p < P; p += kCB) {
       int32_t q = pl.pos[(int64_t)p * n + i];
