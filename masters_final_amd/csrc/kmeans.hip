@@ -206,38 +206,44 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
   // multiplied and stored to the other buffer after it, so one barrier per
   // tile and the L2 latency overlaps the MFMAs.
   constexpr int VPR = KD / 8;  // uint4 per row
-  constexpr int NV = (kBN * VPR + kLB - 1) / kLB;
+  static_assert((kBN * VPR) % kLB == 0 && kBN <= kLB, "a tile splits evenly over the block");
+  constexpr int NV = kBN * VPR / kLB;
   uint4 ph[NV], pl[NV];
-  float pnc = 0.f;
-  // rows past the partition stay zero; their ||c||^2 of +inf keeps them out
-  // of the epilogue
+  float pnc;
+  // Rows past the partition are stored as zeros with ||c||^2 = +inf, which
+  // keeps them out of the epilogue. Fetch and store are branch-free: the
+  // loads are unconditional (row clamped to the partition's last centroid;
+  // every partition has one, the host checks k >= 1), every thread loads a
+  // norm (kLB / kBN threads store the same value to each entry) and the
+  // defaults are chosen when the tile is stored. A register write to a load's
+  // destination next to the load, or a load skipped on some lanes, would make
+  // the wave wait for the new tile there, before the MFMAs, and undo the
+  // double buffering.
+  int64_t f_c0 = 0, f_end = 0;
   auto fetch = [&](int64_t c0, int64_t kend) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int v = t + i * kLB;
-      if (v < kBN * VPR) {
-        const int64_t c = c0 + v / VPR;
-        if (c < kend) {
-          ph[i] = reinterpret_cast<const uint4*>(Ch + c * KD)[v % VPR];
-          pl[i] = reinterpret_cast<const uint4*>(Cl + c * KD)[v % VPR];
-        } else {  // only a partition's last tile takes this branch
-          ph[i] = make_uint4(0, 0, 0, 0);
-          pl[i] = make_uint4(0, 0, 0, 0);
-        }
-      }
+      const int64_t c = min(c0 + v / VPR, kend - 1);
+      ph[i] = reinterpret_cast<const uint4*>(Ch + c * KD)[v % VPR];
+      pl[i] = reinterpret_cast<const uint4*>(Cl + c * KD)[v % VPR];
     }
-    pnc = (t < kBN && c0 + t < kend) ? ncf[c0 + t] : INFINITY;
+    pnc = ncf[min(c0 + (t % kBN), kend - 1)];
+    f_c0 = c0;
+    f_end = kend;
   };
+  // (a mask, not a select of the two values: a select of array elements can
+  // be turned into a load through a selected pointer, i.e. a stack array)
+  auto masked = [](uint4 a, uint32_t m) { return make_uint4(a.x & m, a.y & m, a.z & m, a.w & m); };
   auto stash = [&](int b) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int v = t + i * kLB;
-      if (v < kBN * VPR) {
-        *reinterpret_cast<uint4*>(&s_h[b][(v / VPR) * LROW + (v % VPR) * 8]) = ph[i];
-        *reinterpret_cast<uint4*>(&s_l[b][(v / VPR) * LROW + (v % VPR) * 8]) = pl[i];
-      }
+      const uint32_t m = f_c0 + v / VPR < f_end ? ~0u : 0u;
+      *reinterpret_cast<uint4*>(&s_h[b][(v / VPR) * LROW + (v % VPR) * 8]) = masked(ph[i], m);
+      *reinterpret_cast<uint4*>(&s_l[b][(v / VPR) * LROW + (v % VPR) * 8]) = masked(pl[i], m);
     }
-    if (t < kBN) s_nc[b][t] = pnc;
+    s_nc[b][t % kBN] = f_c0 + (t % kBN) < f_end ? pnc : INFINITY;
   };
   int p = 0, buf = 0;
   int64_t c0 = k_off[0];
@@ -324,6 +330,14 @@ __global__ __launch_bounds__(kLB) void k_label_bf16x3(
             k1[rb][i] = take ? col : k1[rb][i];
           }
       }
+      // the epilogue stays ahead of the tile store: its VALU work covers the
+      // latency of the next tile's loads (left alone, the compiler sinks it
+      // below the store and the barrier, which then wait right after the MFMAs)
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          asm volatile("" : "+v"(d1[rb][i]), "+v"(d2[rb][i]), "+v"(k1[rb][i])::"memory");
       if (more) {
         stash(buf ^ 1);  // last read in the previous tile, before its barrier
         __syncthreads();

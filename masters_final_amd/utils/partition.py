@@ -78,9 +78,15 @@ def generate_ensemble_batched(data, k_list, iters=3, seed=0, precision="bf16x3",
     X = data
     n = X.shape[0]
     if init is None:
+        # the sample indices come from the host generator; the rows are
+        # gathered where X lives (a device gather avoids copying X back)
         rng = np.random.default_rng(seed)
-        Xh = X if not L.is_torch(X) else X.cpu().numpy()
-        init = np.concatenate([Xh[rng.choice(n, int(k), replace=False)] for k in k_list])
+        idx = np.concatenate([rng.choice(n, int(k), replace=False) for k in k_list])
+        if L.is_torch(X):
+            t = L.torch()
+            init = X.index_select(0, t.from_numpy(idx).to(X.device))
+        else:
+            init = np.asarray(X)[idx]
     r = fit_batched(X, init, list(k_list), max_iter=iters, tol=1e-4, precision=precision)
     return r["labels"], r
 

@@ -120,6 +120,23 @@ def test_eac_pipeline_matches_oracle(torch):
             assert np.array_equal(got, lab)
 
 
+def test_seeded_ensemble_init_on_device(torch):
+    """generate_ensemble_batched without init draws k distinct rows per
+    partition from PCG64(seed) and gathers them where X lives: device and
+    host inputs give the same init, and the labels equal the oracle's fits."""
+    from masters_final_amd.utils.partition import generate_ensemble_batched
+    X, _ = synth.gmm_points(3000, 3, 5, 4, mean_scale=10.0)
+    ks = np.array([7, 12, 3, 20])
+    rng = np.random.default_rng(11)
+    init = np.concatenate([X[rng.choice(3000, int(k), replace=False)] for k in ks])
+    off = np.concatenate([[0], np.cumsum(ks)])
+    want = np.stack([O.kmeans_fit(X, init[off[p]:off[p + 1]], max_iter=3, reseed_mode=1)[0]
+                     for p in range(len(ks))])
+    for data in (torch.from_numpy(X).cuda(), X):
+        labels, _ = generate_ensemble_batched(data, ks, iters=3, seed=11)
+        assert np.array_equal(labels.cpu().numpy(), want)
+
+
 def test_eac_class_end_to_end(torch):
     from masters_final_amd.EAC.eac import EAC
     from masters_final_amd.utils.partition import convertClusterStringToIndex
