@@ -480,29 +480,59 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
 
 // FP64 re-scoring with the partition's centroids staged in LDS, for flags
 // sorted by partition: a block takes 64 flags; four threads per flag split
-// the centroids (c = s, s+4, ... ascending, two FP64 chains at a time) and
-// read them from LDS tiles shared by all 64 flags, so each centroid is read
-// from memory once per 64 flagged points instead of once per point. Strict <
-// in ascending order inside a subset, then the lowest index among equal
-// minima across the four subsets: the oracle's first minimum.
+// the centroids (c = s, s+4, ... ascending, two chains at a time) and read
+// them from LDS tiles shared by all 64 flags, so each centroid is read from
+// memory once per 64 flagged points instead of once per point.
+//
+// Screen first: the FP32 direct difference S = sum fl(x_j - c_j)^2 (fused
+// multiply-add) is within a factor (1 +- g) of the exact distance D, g =
+// gamma_{d+2} (d + 2 roundings, unit 2^-23 for safety), plus an absolute
+// underflow slack A; the oracle's FP64 sum is within (1 +- g64) of D. With m
+// the smallest screened value, a centroid whose screened value exceeds
+// T = (m + A) R + A, R = (1+g)(1+g64) / ((1-g)(1-g64)), has an oracle distance
+// strictly above the oracle distance of the true nearest centroid, so it can
+// be neither the oracle's minimum nor tied with it. Each thread keeps its two
+// smallest screened values; the candidates (screened value <= T, usually one
+// per flag) are re-scored with the oracle's FP64 arithmetic, lowest index on
+// equal values (the oracle's first minimum). A flag where some thread's
+// second-smallest value is also <= T may have more candidates than were kept:
+// the block then re-scores its run in FP64 against every centroid, strict < in
+// ascending order inside a subset, lowest index among equal minima across the
+// four subsets.
 constexpr int kRtF = 64;  // flags per block
 constexpr int kRtS = 4;   // threads per flag
 constexpr int kRtT = 64;  // centroids per LDS tile
-inline size_t rescore_tiled_lds(int d) { return (size_t)(kRtF + kRtT) * (d + 1) * sizeof(float); }
+// point rows: stride d + 1 (lanes read different rows: conflict-free);
+// centroid rows: stride rounded up to 4 floats, zero padded (every lane of a
+// wave reads the same row: broadcast 16-byte reads)
+inline int rescore_cstride(int d) { return (d + 3) & ~3; }
+inline size_t rescore_tiled_lds(int d) {
+  return ((size_t)kRtF * (d + 1) + (size_t)kRtT * rescore_cstride(d)) * sizeof(float);
+}
+// DX > 0: the screen keeps the point's d <= DX coordinates in registers.
+template <int DX>
 __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
     const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C,
     const int64_t* __restrict__ k_off, const uint64_t* __restrict__ flags, int64_t nflag,
     int32_t* __restrict__ labels) {
-  extern __shared__ float s_dyn[];
+  extern __shared__ __align__(16) float s_dyn[];
   __shared__ double s_best[kRtS][kRtF];
   __shared__ int32_t s_bk[kRtS][kRtF];
+  __shared__ float s_v1[kRtS][kRtF];
   constexpr uint64_t kRowMask = (1ull << 40) - 1;
-  const int RS = d + 1;  // padded rows: conflict-free column reads
+  const int RS = d + 1;
+  const int RC = (d + 3) & ~3;
   float* s_x = s_dyn;
-  float* s_c = s_dyn + kRtF * RS;
+  float* s_c = s_dyn + kRtF * RS;  // 16-byte aligned: kRtF * RS * 4 = 256 (d + 1) bytes
   const int t = threadIdx.x, g = t & (kRtF - 1), sub = t / kRtF;
   const int64_t f0 = (int64_t)blockIdx.x * kRtF;
   const int cnt = nflag - f0 < kRtF ? (int)(nflag - f0) : kRtF;
+  // screen bounds (double, then rounded up into float)
+  const double ug = 1.1920928955078125e-07, u64 = 1.1102230246251565e-16;
+  const double gs = (d + 2) * ug / (1.0 - (d + 2) * ug);
+  const double g64 = (d + 2) * 2 * u64 / (1.0 - (d + 2) * 2 * u64);
+  const float R = (float)((1.0 + gs) * (1.0 + g64) / ((1.0 - gs) * (1.0 - g64)) * (1.0 + 1e-6));
+  const float A = (float)d * 4.7e-38f;  // >= d * 2^-124
   for (int done = 0; done < cnt;) {  // runs of one partition (block-uniform)
     const int p = (int)(flags[f0 + done] >> 40);
     int end = done + 1;
@@ -514,40 +544,110 @@ __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
     const bool active = g >= done && g < end;
     const float* xa = s_x + (active ? g - done : 0) * RS;
     const int64_t kbeg = k_off[p], kend = k_off[p + 1];
-    double best = DBL_MAX;
-    int64_t bk = -1;
+    // pass 1: FP32 screen, two smallest values per thread
+    float v1 = INFINITY, v2 = INFINITY;
+    int64_t k1 = -1;
+    auto keep = [&](float v, int64_t k) {
+      const bool lt1 = v < v1;
+      v2 = __builtin_amdgcn_fmed3f(v1, v2, v);
+      k1 = lt1 ? k : k1;
+      v1 = lt1 ? v : v1;
+    };
+    // staged centroid rows, zero padded to RC columns
+    auto stage = [&](int64_t c0, int rows) {
+      for (int v = t; v < rows * RC; v += kRtF * kRtS) {
+        const int r = v / RC, j = v - r * RC;
+        s_c[v] = j < d ? C[(c0 + r) * d + j] : 0.f;
+      }
+    };
+    constexpr int NX = DX > 0 ? DX : 1;
+    float xr[NX];
+    __syncthreads();  // the point rows are staged
+    if constexpr (DX > 0) {
+#pragma unroll
+      for (int j = 0; j < NX; ++j) xr[j] = j < d ? xa[j] : 0.f;
+    }
+    const int nq = RC / 4;
+    // screened value of one staged row (padding adds exact zeros)
+    auto screen = [&](const float* cr) {
+      float a = 0.f;
+      if constexpr (DX > 0) {
+        const float4* c4 = reinterpret_cast<const float4*>(cr);
+#pragma unroll
+        for (int q = 0; q < NX / 4; ++q) {
+          if (q < nq) {
+            const float4 u = c4[q];
+            float e = xr[4 * q] - u.x;
+            a = fmaf(e, e, a);
+            e = xr[4 * q + 1] - u.y;
+            a = fmaf(e, e, a);
+            e = xr[4 * q + 2] - u.z;
+            a = fmaf(e, e, a);
+            e = xr[4 * q + 3] - u.w;
+            a = fmaf(e, e, a);
+          }
+        }
+      } else {
+        for (int j = 0; j < d; ++j) {
+          const float e = xa[j] - cr[j];
+          a = fmaf(e, e, a);
+        }
+      }
+      return a;
+    };
     for (int64_t c0 = kbeg; c0 < kend; c0 += kRtT) {
       const int rows = kend - c0 < kRtT ? (int)(kend - c0) : kRtT;
       __syncthreads();  // the rows are staged, the previous tile is consumed
-      for (int v = t; v < rows * d; v += kRtF * kRtS) {
-        const int r = v / d, j = v - r * d;
-        s_c[r * RS + j] = C[(c0 + r) * d + j];
-      }
+      stage(c0, rows);
       __syncthreads();
       if (!active) continue;
       int c = sub;
       for (; c + kRtS < rows; c += 2 * kRtS) {
-        const float* ca = s_c + c * RS;
-        const float* cb = ca + kRtS * RS;
-        double a0 = 0.0, a1 = 0.0;
-        for (int j = 0; j < d; ++j) {
-          const double xv = (double)xa[j];
-          a0 = sq_acc(a0, xv - (double)ca[j]);
-          a1 = sq_acc(a1, xv - (double)cb[j]);
-        }
-        if (bk < 0 || a0 < best) { best = a0; bk = c0 + c; }
-        if (a1 < best) { best = a1; bk = c0 + c + kRtS; }
+        const float a0 = screen(s_c + c * RC);
+        const float a1 = screen(s_c + (c + kRtS) * RC);
+        keep(a0, c0 + c);
+        keep(a1, c0 + c + kRtS);
       }
-      for (; c < rows; c += kRtS) {
-        const float* ca = s_c + c * RS;
-        double a0 = 0.0;
-        for (int j = 0; j < d; ++j) a0 = sq_acc(a0, (double)xa[j] - (double)ca[j]);
-        if (bk < 0 || a0 < best) { best = a0; bk = c0 + c; }
-      }
+      for (; c < rows; c += kRtS) keep(screen(s_c + c * RC), c0 + c);
     }
-    s_best[sub][g] = best;
-    s_bk[sub][g] = bk < 0 ? -1 : (int32_t)(bk - kbeg);
+    s_v1[sub][g] = v1;
     __syncthreads();
+    bool full = false;
+    {
+      float m = s_v1[0][g];
+      for (int q = 1; q < kRtS; ++q) m = fminf(m, s_v1[q][g]);
+      const float T = (m + A) * R + A;
+      double best = DBL_MAX;
+      int32_t bk = -1;
+      if (active && k1 >= 0 && v1 <= T) {
+        best = dist2_f64(xa, C + k1 * d, d, false);
+        bk = (int32_t)(k1 - kbeg);
+      }
+      full = active && (k1 < 0 || !(v2 > T));  // (NaN screens take the full pass)
+      s_best[sub][g] = best;
+      s_bk[sub][g] = bk;
+    }
+    // pass 2 (rare): every centroid in FP64 for the whole run
+    if (__syncthreads_or(full)) {
+      double best = DBL_MAX;
+      int64_t bk = -1;
+      for (int64_t c0 = kbeg; c0 < kend; c0 += kRtT) {
+        const int rows = kend - c0 < kRtT ? (int)(kend - c0) : kRtT;
+        __syncthreads();
+        stage(c0, rows);
+        __syncthreads();
+        if (!active) continue;
+        for (int c = sub; c < rows; c += kRtS) {
+          const float* ca = s_c + c * RC;
+          double a0 = 0.0;
+          for (int j = 0; j < d; ++j) a0 = sq_acc(a0, (double)xa[j] - (double)ca[j]);
+          if (bk < 0 || a0 < best) { best = a0; bk = c0 + c; }
+        }
+      }
+      s_best[sub][g] = best;
+      s_bk[sub][g] = bk < 0 ? -1 : (int32_t)(bk - kbeg);
+      __syncthreads();
+    }
     if (active && sub == 0) {
       double b = s_best[0][g];
       int32_t k = s_bk[0][g];
@@ -1065,12 +1165,12 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
                                                                      B.flags, (int64_t)nf);
     EAC_LAUNCH_CHECK();
     int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 3 * (int64_t)nf * d * (ktot / P), s);
-    // Many flags (at least 1024 blocks of 64): flags sorted by partition
-    // (one radix pass for up to 256 partitions) feed the LDS-tiled
-    // re-scoring. Few flags, or no room for the sort: one wavefront per flag
+    // Many flags (at least 1024 blocks of 64) and d <= 127 (LDS tiles up to
+    // 64 KiB): flags sorted by partition (one radix pass for up to 256
+    // partitions) feed the LDS-tiled re-scoring. Few flags, or no room for the sort: one wavefront per flag
     // over the transposed centroids (more parallelism per flag).
     const uint64_t* fl = B.flags;
-    bool tiled = nf >= (unsigned long long)kRtF * 1024;
+    bool tiled = nf >= (unsigned long long)kRtF * 1024 && rescore_tiled_lds(d) <= 64 * 1024;
     if (tiled && P > 1) {
       const size_t mark2 = ar.used;
       uint64_t* sorted = ar.take<uint64_t>(nf);
@@ -1086,10 +1186,16 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
     }
     if (tiled) {
       const size_t lds = rescore_tiled_lds(d);
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_rescore_tiled),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      k_rescore_tiled<<<(unsigned)(((int64_t)nf + kRtF - 1) / kRtF), kRtF * kRtS, lds, s>>>(
-          X, n, d, C, B.k_off, fl, (int64_t)nf, labels);
+      const unsigned nb = (unsigned)(((int64_t)nf + kRtF - 1) / kRtF);
+      auto go = [&](auto kern) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        kern<<<nb, kRtF * kRtS, lds, s>>>(X, n, d, C, B.k_off, fl, (int64_t)nf, labels);
+      };
+      if (d <= 16) go(k_rescore_tiled<16>);
+      else if (d <= 32) go(k_rescore_tiled<32>);
+      else if (d <= 64) go(k_rescore_tiled<64>);
+      else go(k_rescore_tiled<0>);
     } else {
       k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
           X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels);

@@ -71,6 +71,32 @@ def test_rescore_many_flags_tiled(torch):
         assert np.array_equal(lab[p].cpu().numpy(), want), p
 
 
+@pytest.mark.parametrize("d", [16, 19, 40, 100])
+def test_rescore_tiled_screen_near_ties(torch, d):
+    """Far from the origin the split-BF16 bound flags nearly every point, so
+    the tiled re-scoring decides (point coordinates in registers for d <= 64,
+    centroid rows zero padded to 4 floats); centroids one ulp apart (and
+    points on them) put FP32 screened values within the screen margin of
+    each other. Labels must be the oracle's."""
+    from masters_final_amd.cluster.KMeans import label_device
+    rng = np.random.default_rng(37 + d)
+    X, _ = synth.gmm_points(70000, d, 12, 37, mean_std=2.0)
+    X = (X + 300.0).astype(np.float32)
+    base = X[rng.choice(70000, 24, replace=False)]
+    near = base.copy()
+    j = rng.integers(0, d, 24)
+    near[np.arange(24), j] = np.nextafter(near[np.arange(24), j], np.float32(np.inf))
+    C0 = np.concatenate([base, near])                   # pairs one ulp apart
+    C1 = np.concatenate([near[:9], base[3:20], near[9:]])
+    X[:48] = np.concatenate([base, near])              # points exactly on centroids
+    ks = [48, 41]
+    lab, _, nres = label_device(X, np.concatenate([C0, C1]), k_list=ks, precision="bf16x3")
+    assert nres >= 65536
+    for p, Cp in enumerate((C0, C1)):
+        want, _ = O.kmeans_label_f64(X, Cp)
+        assert np.array_equal(lab[p].cpu().numpy(), want), p
+
+
 def test_label_fp32_plain(torch):
     from masters_final_amd.cluster.KMeans import label_device
     X, Cc = _data(4000, 16, 50, seed=9)
