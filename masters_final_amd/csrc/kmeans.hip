@@ -942,6 +942,43 @@ __global__ __launch_bounds__(kWave) void k_centroid_sums(const float* __restrict
   }
 }
 
+// The same sums for d <= 64 with L lanes per centroid (L = the power of two
+// >= d, at least 4; 64 / L centroids per wavefront): lane j of a group adds
+// dimension j of its centroid's members in point order straight from X, 16
+// member rows loaded ahead of the (in-order) additions. The one-wave-per-
+// centroid form above leaves 64 - d lanes idle and waits on LDS for every
+// member.
+template <int L>
+__global__ __launch_bounds__(256) void k_centroid_sums_narrow(
+    const float* __restrict__ X, int d, const uint32_t* __restrict__ members,
+    const int64_t* __restrict__ mstart, const int32_t* __restrict__ count, int64_t ktot,
+    float* __restrict__ Cnew) {
+  constexpr int U = 16;
+  const int j = threadIdx.x % L;
+  const int64_t groups = (int64_t)gridDim.x * (blockDim.x / L);
+  for (int64_t c = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; c < ktot;
+       c += groups) {
+    const int32_t cnt = count[c];
+    if (cnt == 0 || j >= d) continue;
+    const uint32_t* m = members + mstart[c];
+    const float* xj = X + j;
+    float s = 0.f;
+    int32_t r = 0;
+    for (; r + U <= cnt; r += U) {
+      uint32_t mi[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) mi[u] = m[r + u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = xj[(int64_t)mi[u] * d];
+#pragma unroll
+      for (int u = 0; u < U; ++u) s = s + v[u];
+    }
+    for (; r < cnt; ++r) s = s + xj[(int64_t)m[r] * d];
+    Cnew[c * d + j] = (float)((double)s / (double)cnt);
+  }
+}
+
 // Copies the new centroids of the partitions still running (upd[p] != 0).
 __global__ void k_copy_running(float* __restrict__ C, const float* __restrict__ Cnew, int d,
                                const int64_t* __restrict__ k_off, int P,
@@ -1291,8 +1328,18 @@ int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const
   ar.used = mark;
   EAC_TRY((scan_excl<int32_t, int64_t>(count, mstart, ktot, 0, nullptr, ar, s)));
   ar.used = mark;
-  k_centroid_sums<<<grid_for(ktot, 1, 65536), kWave, 0, s>>>(X, d, members, mstart, count, ktot,
-                                                              Cnew);
+  auto narrow = [&](auto kern, int lanes) {
+    kern<<<grid_for(ktot, 256 / lanes, 65536), 256, 0, s>>>(X, d, members, mstart, count, ktot,
+                                                           Cnew);
+  };
+  if (d <= 4) narrow(k_centroid_sums_narrow<4>, 4);
+  else if (d <= 8) narrow(k_centroid_sums_narrow<8>, 8);
+  else if (d <= 16) narrow(k_centroid_sums_narrow<16>, 16);
+  else if (d <= 32) narrow(k_centroid_sums_narrow<32>, 32);
+  else if (d <= 64) narrow(k_centroid_sums_narrow<64>, 64);
+  else
+    k_centroid_sums<<<grid_for(ktot, 1, 65536), kWave, 0, s>>>(X, d, members, mstart, count, ktot,
+                                                                Cnew);
   EAC_LAUNCH_CHECK();
   // empty clusters
   k_empty_flags<<<grid_for(ktot, 256, 4096), 256, 0, s>>>(count, ktot, eflag);

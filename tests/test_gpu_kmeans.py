@@ -135,6 +135,25 @@ def test_centroid_update_matches_oracle(torch):
     assert np.array_equal(got.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("d", [1, 3, 8, 16, 33, 64])
+def test_centroid_update_narrow_rows_matches_oracle(torch, d):
+    """d <= 64: several centroids per wavefront (4 to 64 lanes each); sizes
+    from empty and single-member clusters to thousands of members (full
+    batches of 16 rows plus a remainder), sums in point order."""
+    from masters_final_amd.cluster.KMeans import update_device
+    rng = np.random.default_rng(40 + d)
+    n, k = 30000, 70
+    X = (rng.standard_normal((n, d)) * 3 + 5).astype(np.float32)
+    lab = np.minimum(rng.geometric(0.08, n) - 1, k - 1).astype(np.int32)  # skewed sizes
+    lab[lab == 5] = 4                                                     # one empty cluster
+    lab[:1] = 60                                                          # small clusters
+    dists = rng.random(n).astype(np.float32)
+    want = O.recompute_centroids(X, lab, dists, k, reseed_mode=1)
+    got, ne = update_device(X, lab[None], dists[None], [k])
+    assert ne == int(np.sum(np.bincount(lab, minlength=k) == 0))
+    assert np.array_equal(got.cpu().numpy(), want)
+
+
 def test_centroid_update_wide_rows_matches_oracle(torch):
     """More than 128 dimensions (two dimension groups) and clusters larger
     than one LDS stage of members."""
