@@ -141,8 +141,13 @@ __device__ __forceinline__ float point_bound(float nx, float xn, float xhn, floa
   return (2.f * cross + epi + orc) * 1.0001f + 1e-30f;
 }
 
+// Waves per SIMD the compiler must fit: 3 for KD = 32 (a few cold values
+// spill; measured faster than 2 waves without spills), otherwise no
+// constraint (at KD = 64 the same hint spills inside the tile loop).
+constexpr int label_waves(int KD) { return KD == 32 ? 3 : 1; }
 template <int KD>
-__global__ __launch_bounds__(kLB) void k_label_bf16x3(
+__global__ __launch_bounds__(kLB) __attribute__((amdgpu_waves_per_eu(label_waves(KD))))
+void k_label_bf16x3(
     const float* __restrict__ X, int64_t n, int d, const uint16_t* __restrict__ Ch,
     const uint16_t* __restrict__ Cl, const float* __restrict__ ncf,
     const int64_t* __restrict__ k_off, int P, const unsigned int* __restrict__ pstat,
