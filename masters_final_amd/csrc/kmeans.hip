@@ -133,7 +133,9 @@ __device__ __forceinline__ float point_bound(float nx, float xn, float xhn, floa
   const float u = 5.9604645e-08f;            // 2^-24
   const float ut = 1.1920929e-07f;           // 2^-23: fp32 accumulation, any rounding
   const float Cn = st[0], Chn = st[1], Cln = st[2], Cen = st[3];
-  const float nterms = 3.f * KD;
+  // nonzero products per dot: 3 d (columns d..KD-1 are zero in both
+  // operands; adding an exact zero neither rounds nor truncates)
+  const float nterms = 3.f * (d < KD ? d : KD);
   const float gam = nterms * ut / (1.f - nterms * ut);
   float cross = gam * (xhn * (Chn + Cln) + xln * Chn) + xhn * Cen + xln * (Cln + Cen) + exn * Cn;
   float epi = 6.f * u * (nx + Cn * Cn);
