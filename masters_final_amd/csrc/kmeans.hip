@@ -703,7 +703,10 @@ __global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t reg
 // differences in parallel; the running sum then visits the lanes in order
 // (4 adds in lane s, then a shuffle to lane s+1), so the additions happen in
 // ascending dimension order exactly as in the oracle. Padding dimensions add
-// exact zeros.
+// exact zeros. A block takes kPG partitions (grid.y over partition groups)
+// with the partitions innermost, so a point's row is read from memory once
+// per group and from cache for the other partitions.
+constexpr int kPG = 8;
 template <int LPP>
 __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X, int64_t n,
                                                      int d, const float* __restrict__ C,
@@ -711,67 +714,79 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      const int32_t* __restrict__ labels,
                                                      float* __restrict__ dists,
                                                      double* __restrict__ inertia) {
-  __shared__ double s_acc[256 / kWave];
+  __shared__ double s_acc[kPG][256 / kWave];
   constexpr int PPB = 256 / LPP;  // points per block and step
   const int lane = lane_id(), sub = lane & (LPP - 1), base = lane & ~(LPP - 1);
   const int64_t groups = (int64_t)gridDim.x * PPB;
   const bool v4 = (d & 3) == 0 && ((reinterpret_cast<uintptr_t>(X) |
                                     reinterpret_cast<uintptr_t>(C)) & 15) == 0;
-  for (int p = blockIdx.y; p < P; p += gridDim.y) {
-    const int64_t kbeg = k_off[p];
-    double part = 0.0;
+  for (int pg = blockIdx.y * kPG; pg < P; pg += gridDim.y * kPG) {
+    const int np = P - pg < kPG ? P - pg : kPG;
+    double part[kPG];
+#pragma unroll
+    for (int q = 0; q < kPG; ++q) part[q] = 0.0;
     // wave-uniform trip count (the shuffles need every lane)
     const int64_t g0 = (int64_t)blockIdx.x * PPB + threadIdx.x / LPP;
     const int64_t first = g0 - (threadIdx.x & (kWave - 1)) / LPP;  // group of lane 0
     for (int64_t gb = first; gb < n; gb += groups) {
       const int64_t i = gb + base / LPP;
       const bool valid = i < n;
-      const int64_t c = valid ? kbeg + labels[(int64_t)p * n + i] : kbeg;
       const float* xr = X + (valid ? i : 0) * d;
-      const float* cr = C + c * d;
-      double acc = 0.0;
-      for (int j0 = 0; j0 < d; j0 += 4 * LPP) {
-        double sq[4];
-        const int jb = j0 + 4 * sub;
-        if (v4 && jb + 4 <= d) {  // one 16-byte load per row
-          const float4 xv = *reinterpret_cast<const float4*>(xr + jb);
-          const float4 cv = *reinterpret_cast<const float4*>(cr + jb);
-          const double e0 = (double)xv.x - (double)cv.x, e1 = (double)xv.y - (double)cv.y;
-          const double e2 = (double)xv.z - (double)cv.z, e3 = (double)xv.w - (double)cv.w;
-          sq[0] = __dmul_rn(e0, e0);
-          sq[1] = __dmul_rn(e1, e1);
-          sq[2] = __dmul_rn(e2, e2);
-          sq[3] = __dmul_rn(e3, e3);
-        } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int j = jb + q;
-            const double diff = j < d ? (double)xr[j] - (double)cr[j] : 0.0;
-            sq[q] = __dmul_rn(diff, diff);
+      for (int q = 0; q < kPG; ++q) {
+        if (q >= np) break;  // uniform
+        const int p = pg + q;
+        const int64_t kbeg = k_off[p];
+        const int64_t c = valid ? kbeg + labels[(int64_t)p * n + i] : kbeg;
+        const float* cr = C + c * d;
+        double acc = 0.0;
+        for (int j0 = 0; j0 < d; j0 += 4 * LPP) {
+          double sq[4];
+          const int jb = j0 + 4 * sub;
+          if (v4 && jb + 4 <= d) {  // one 16-byte load per row
+            const float4 xv = *reinterpret_cast<const float4*>(xr + jb);
+            const float4 cv = *reinterpret_cast<const float4*>(cr + jb);
+            const double e0 = (double)xv.x - (double)cv.x, e1 = (double)xv.y - (double)cv.y;
+            const double e2 = (double)xv.z - (double)cv.z, e3 = (double)xv.w - (double)cv.w;
+            sq[0] = __dmul_rn(e0, e0);
+            sq[1] = __dmul_rn(e1, e1);
+            sq[2] = __dmul_rn(e2, e2);
+            sq[3] = __dmul_rn(e3, e3);
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int j = jb + u;
+              const double diff = j < d ? (double)xr[j] - (double)cr[j] : 0.0;
+              sq[u] = __dmul_rn(diff, diff);
+            }
+          }
+          const int steps = d - j0 >= 4 * LPP ? LPP : (d - j0 + 3) / 4;
+          for (int st = 0; st < steps; ++st) {
+            if (sub == st) {
+#pragma unroll
+              for (int u = 0; u < 4; ++u) acc = __dadd_rn(acc, sq[u]);
+            }
+            acc = __shfl(acc, base + st, kWave);
           }
         }
-        const int steps = d - j0 >= 4 * LPP ? LPP : (d - j0 + 3) / 4;
-        for (int st = 0; st < steps; ++st) {
-          if (sub == st) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc = __dadd_rn(acc, sq[q]);
-          }
-          acc = __shfl(acc, base + st, kWave);
+        if (valid && sub == 0) {
+          const float df = (float)acc;
+          if (dists) dists[(int64_t)p * n + i] = df;
+          part[q] += (double)df;
         }
-      }
-      if (valid && sub == 0) {
-        const float df = (float)acc;
-        if (dists) dists[(int64_t)p * n + i] = df;
-        part += (double)df;
       }
     }
-    part = wave_sum(part);
-    if (lane == 0) s_acc[threadIdx.x / kWave] = part;
+#pragma unroll
+    for (int q = 0; q < kPG; ++q) {
+      if (q >= np) break;
+      const double w = wave_sum(part[q]);
+      if (lane == 0) s_acc[q][threadIdx.x / kWave] = w;
+    }
     __syncthreads();
-    if (threadIdx.x == 0 && inertia) {
+    if (threadIdx.x < np && inertia) {
       double t = 0;
-      for (int q = 0; q < 256 / kWave; ++q) t += s_acc[q];
-      atomicAdd(&inertia[p], t);
+      for (int w = 0; w < 256 / kWave; ++w) t += s_acc[threadIdx.x][w];
+      atomicAdd(&inertia[pg + threadIdx.x], t);
     }
     __syncthreads();
   }
@@ -1250,7 +1265,8 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if (n_rescored) *n_rescored = (int64_t)nf;
   if (dists || inertia_out) {
     if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
-    const int py = P < 65535 ? P : 65535;
+    const int pgs = (P + kPG - 1) / kPG;
+    const int py = pgs < 65535 ? pgs : 65535;
     if (d <= 16)
       k_exact_dists<4><<<dim3(grid_for(n, 64, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
                                                                           labels, dists,
