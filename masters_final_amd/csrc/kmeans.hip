@@ -143,10 +143,10 @@ __device__ __forceinline__ float point_bound(float nx, float xn, float xhn, floa
   return (2.f * cross + epi + orc) * 1.0001f + 1e-30f;
 }
 
-// Waves per SIMD the compiler must fit: 3 for KD = 32 (a few cold values
+// Waves per SIMD the compiler must fit: 3 for KD <= 64 (a few values
 // spill; measured faster than 2 waves without spills), otherwise no
-// constraint (at KD = 64 the same hint spills inside the tile loop).
-constexpr int label_waves(int KD) { return KD == 32 ? 3 : 1; }
+// constraint.
+constexpr int label_waves(int KD) { return KD <= 64 ? 3 : 1; }
 template <int KD>
 __global__ __launch_bounds__(kLB) __attribute__((amdgpu_waves_per_eu(label_waves(KD))))
 void k_label_bf16x3(
@@ -281,38 +281,36 @@ void k_label_bf16x3(
         for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
       const uint16_t* sh = s_h[buf];
       const uint16_t* sl = s_l[buf];
-      // all B fragments of the tile first (one wait), then the MFMAs in
-      // rounds of 8 independent accumulators, so no MFMA waits on the one
-      // before it
-      bf16x8 Bh[KB][4], Bl[KB][4];
+      // per 32-wide K block: its B fragments (8 LDS reads), then the MFMAs
+      // in rounds of 8 independent accumulators, so no MFMA waits on the one
+      // before it (fragments of one K block at a time keep the register count
+      // within 3 waves per SIMD)
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb)
+      for (int kb = 0; kb < KB; ++kb) {
+        bf16x8 Bh[4], Bl[4];
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb) {
           const int r = cb * 16 + lr;
-          Bh[kb][cb] = *reinterpret_cast<const bf16x8*>(&sh[r * LROW + kb * 32 + lq * 8]);
-          Bl[kb][cb] = *reinterpret_cast<const bf16x8*>(&sl[r * LROW + kb * 32 + lq * 8]);
+          Bh[cb] = *reinterpret_cast<const bf16x8*>(&sh[r * LROW + kb * 32 + lq * 8]);
+          Bl[cb] = *reinterpret_cast<const bf16x8*>(&sl[r * LROW + kb * 32 + lq * 8]);
         }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
           for (int rb = 0; rb < 2; ++rb)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bh[kb][cb],
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bh[cb],
                                                                   acc[rb][cb], 0, 0, 0);
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
           for (int rb = 0; rb < 2; ++rb)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bl[kb][cb],
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah[rb][kb], Bl[cb],
                                                                   acc[rb][cb], 0, 0, 0);
 #pragma unroll
         for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
           for (int rb = 0; rb < 2; ++rb)
-            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[rb][kb], Bh[kb][cb],
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al[rb][kb], Bh[cb],
                                                                   acc[rb][cb], 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
