@@ -317,8 +317,9 @@ void k_label_bf16x3(
       // epilogue: lane holds column cb*16 + lr, rows 4*lq + i. The running
       // top two per row compare D' = ||c||^2 - 2 x.c (the row's ||x||^2 is
       // common to all columns, so it changes neither the argmin nor the
-      // margin); columns past the partition carry ||c||^2 = +inf. Four VALU
-      // operations per element: fma, med3 (new second), compare, select.
+      // margin); columns past the partition carry ||c||^2 = +inf. Five VALU
+      // operations per element: fma, compare, med3 (new second) and two
+      // selects (best value, best column).
       const int32_t colb = (int32_t)(c0 - kbeg) + lr;
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
