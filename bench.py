@@ -463,13 +463,80 @@ def run_sl_batched(args):
     return out
 
 
+def run_accuracy(args):
+    """ConsistencyIndex + HungarianIndex of one ensemble-shaped partition
+    (k=2048) against a 32-cluster ground truth over 500,000 samples: the
+    contingency table on the GPU, the matchings on the host."""
+    import numpy as np
+    import torch
+
+    from masters_final_amd import _lib as L
+    from masters_final_amd.validation.consistency import ConsistencyIndex, greedy_match_count
+    from masters_final_amd.validation.hindex import HungarianIndex, hungarian_match_count
+
+    lib = L.load_library()
+    n = args.n if args.n != 1_000_000 else 500_000
+    rng = np.random.default_rng(11)
+    truth = rng.integers(0, 32, n).astype(np.int32)
+    part = ((truth.astype(np.int64) * 64 + rng.integers(0, 64, n)) % 2048).astype(np.int32)
+    da, db = torch.from_numpy(part).cuda(), torch.from_numpy(truth).cuda()
+    out = torch.empty(2048 * 32 + 1, dtype=torch.int64, device="cuda")
+    stream = torch.cuda.current_stream()
+    sp = L.stream_ptr()
+
+    def kernel():
+        L.check(lib.eac_contingency(L.ptr(da), L.ptr(db), n, 0, 2048, 0, 32, L.ptr(out), sp))
+
+    for _ in range(args.warmup):
+        kernel()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        kernel()
+    e1.record(stream)
+    e1.synchronize()
+    kms = e0.elapsed_time(e1) / args.steps
+    T = out.cpu().numpy()[:-1].reshape(2048, 32)
+    want = np.bincount(part.astype(np.int64) * 32 + truth, minlength=2048 * 32).reshape(2048, 32)
+
+    def score():
+        return (ConsistencyIndex(N=n).score(da, db), HungarianIndex(n).score(da, db))
+
+    (ci, hi), ms = timed(score, args.steps, args.warmup, torch)
+
+    def cpu_score():  # the same indices with the table counted by numpy
+        t = np.bincount(part.astype(np.int64) * 32 + truth, minlength=2048 * 32).reshape(2048, 32)
+        return (float(greedy_match_count(t, t.sum(1), t.sum(0))) / n,
+                hungarian_match_count(t) / n)
+
+    c0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - c0 < 2.0:
+        cpu_res = cpu_score()
+        reps += 1
+    cpu_s = (time.perf_counter() - c0) / reps
+    out_line = base_line(args, n / (ms * 1e-3) / 1e6, "Msamples/s", ms, "int32",
+                         {"workload": "accuracy_indices", "n": n, "k1": 2048, "k2": 32,
+                          "seed": 11})
+    ach = n * 8 / (kms * 1e-3) / 1e9
+    out_line["result"] = {"consistency": ci, "hungarian": hi, "table_equal_numpy": bool(
+        np.array_equal(T, want)), "equal_to_cpu": cpu_res == (ci, hi), "contingency_ms": kms}
+    out_line["roofline"] = {"bound": "hbm", "kernel": "k_contingency", "achieved": ach,
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                            "traffic": None, "avg_launch_ms": kms}
+    out_line["cpu_baseline"] = {"value": n / cpu_s / 1e6, "unit": "Msamples/s", "cores": 1,
+                                "kind": "port", "sample": "whole workload: numpy bincount "
+                                "table + the same host matchings"}
+    return out_line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mst",
-                    choices=["mst", "eac", "kmeans", "coassoc", "sl_batched"])
+                    choices=["mst", "eac", "kmeans", "coassoc", "sl_batched", "accuracy"])
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--knn", type=int, default=20)
     ap.add_argument("--partitions", type=int, default=256)
@@ -500,7 +567,7 @@ def main():
               "multi-GPU path", file=sys.stderr)
         sys.exit(2)
     fn = {"mst": run_mst, "eac": run_eac, "kmeans": run_kmeans, "coassoc": run_coassoc,
-          "sl_batched": run_sl_batched}[args.workload]
+          "sl_batched": run_sl_batched, "accuracy": run_accuracy}[args.workload]
     print(json.dumps(fn(args)), flush=True)
 
 

@@ -337,6 +337,19 @@ int eac_canonical_relabel(const int32_t* labels_in, int32_t n,
                           int32_t n_values, int32_t* out, int64_t* h_count,
                           void* ws, size_t ws_bytes, void* stream);
 
+/* ---- contingency table of two partitions (accuracy indices) -------------
+ * out[(a[s] - a0) * kb + (b[s] - b0)] = number of samples s with those two
+ * labels; out[ka * kb] = samples whose labels fall outside
+ * [a0, a0 + ka) x [b0, b0 + kb) (the caller rejects those). out has
+ * ka * kb + 1 entries and is cleared by the call. For partitions this is the
+ * product of the binary cluster-by-sample matrices that
+ * MyML/MyML/validation/consistency.py:70-125 (ConsistencyIndex._match_: A,
+ * and B, C as its row and column sums) and
+ * MyML/MyML/validation/hindex.py:44 (HungarianIndex: match_matrix) compute
+ * with dense dot products. */
+int eac_contingency(const int32_t* a, const int32_t* b, int64_t n, int32_t a0, int32_t ka,
+                    int32_t b0, int32_t kb, int64_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,7 @@ SIGNATURES = {
     "eac_coassoc_fill": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _i32, _p, _p, _p, _sz, _p]),
     "eac_relabel_workspace_bytes": (_sz, [_i32, _i32]),
     "eac_canonical_relabel": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _sz, _p]),
+    "eac_contingency": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _i32, _p, _p]),
 }
 
 
