@@ -64,16 +64,19 @@ extern "C" int eac_contingency(const int32_t* a, const int32_t* b, int64_t n, in
                                int32_t ka, int32_t b0, int32_t kb, int64_t* out,
                                void* stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (n < 0 || ka < 1 || kb < 1) return fail(EAC_ERR_ARG, "contingency: need n >= 0, ka, kb >= 1");
-  if ((int64_t)ka * kb > ((int64_t)1 << 31)) return fail(EAC_ERR_UNSUPPORTED, "contingency: table too large");
+  if (n < 0 || ka < 1 || kb < 1)
+    return fail(EAC_ERR_ARG, "contingency: need n >= 0, ka, kb >= 1");
+  if ((int64_t)ka * kb > ((int64_t)1 << 31))
+    return fail(EAC_ERR_UNSUPPORTED, "contingency: table too large");
   if (!out || (n > 0 && (!a || !b))) return fail(EAC_ERR_ARG, "contingency: null array");
   const int64_t cells = (int64_t)ka * kb;
   EAC_TRY(eac::fill<int64_t>(out, cells + 1, 0, s));
   if (n == 0) return EAC_OK;
   unsigned long long* o = reinterpret_cast<unsigned long long*>(out);
   if (cells <= eac::kCtLds) {
-    // a few samples per thread, so the per-block table flush stays small
-    const int g = eac::grid_for(n, 256 * 64, 1024);
+    // 16 samples per thread, at most 512 blocks: enough blocks to fill the
+    // chip, few enough table flushes (one atomic per nonzero cell per block)
+    const int g = eac::grid_for(n, 256 * 16, 512);
     eac::k_contingency<true><<<g, 256, 0, s>>>(a, b, n, a0, ka, b0, kb, o);
   } else {
     eac::k_contingency<false><<<eac::grid_for(n, 256, 4096), 256, 0, s>>>(a, b, n, a0, ka, b0,
