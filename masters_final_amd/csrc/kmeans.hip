@@ -368,42 +368,56 @@ void k_label_bf16x3(
           d2[rb][i] = nd2;
         }
       }
-    // bounds in output layout, then write
+    // Bounds in output layout, then write. After the merge the 16 lanes of a
+    // row group hold the same top two for each of the group's 8 rows; lane
+    // lr < 8 finishes row slot (rb, i) = (lr / 4, lr % 4), so each lane
+    // computes one bound instead of the group's first lane computing eight.
     float st[kStat];
 #pragma unroll
     for (int q = 0; q < kStat; ++q) st[q] = __uint_as_float(pstat[p * kStat + q]);
-    uint64_t mine[8];
-    int nm = 0;
+    const int rs = lr & 7, rbs = rs >> 2, is = rs & 3;
+    float D1 = d1[0][0], D2 = d2[0][0];
+    int32_t K1 = k1[0][0];
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        int src = lq * 4 + i;
-        float xn = __shfl(xn_a[rb], src, kWave);
-        float xhn = __shfl(xhn_a[rb], src, kWave);
-        float xln = __shfl(xln_a[rb], src, kWave);
-        float exn = __shfl(exn_a[rb], src, kWave);
-        float nx = __shfl(nx_a[rb], src, kWave);
-        int64_t row = p0 + rb * 16 + lq * 4 + i;
-        if (lr == 0 && row < n && k1[rb][i] >= 0) {
-          float E = point_bound(nx, xn, xhn, xln, exn, st, KD, d);
-          labels[(int64_t)p * n + row] = k1[rb][i];
-          if (!(d2[rb][i] - d1[rb][i] > 2.f * E)) mine[nm++] = ((uint64_t)p << 40) | (uint64_t)row;
-        }
+    for (int q = 1; q < 8; ++q)
+      if (rs == q) {
+        D1 = d1[q >> 2][q & 3];
+        D2 = d2[q >> 2][q & 3];
+        K1 = k1[q >> 2][q & 3];
       }
+    // row statistics: lanes lr' = lq * 4 + i hold row lq * 4 + i of each block
+    const int src = lq * 4 + is;
+    float xs[2][5];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      xs[rb][0] = __shfl(nx_a[rb], src, kWave);
+      xs[rb][1] = __shfl(xn_a[rb], src, kWave);
+      xs[rb][2] = __shfl(xhn_a[rb], src, kWave);
+      xs[rb][3] = __shfl(xln_a[rb], src, kWave);
+      xs[rb][4] = __shfl(exn_a[rb], src, kWave);
+    }
+    const int64_t row = p0 + rbs * 16 + lq * 4 + is;
+    bool flagged = false;
+    if (lr < 8 && row < n && K1 >= 0) {
+      const float E = rbs ? point_bound(xs[1][0], xs[1][1], xs[1][2], xs[1][3], xs[1][4], st, KD, d)
+                          : point_bound(xs[0][0], xs[0][1], xs[0][2], xs[0][3], xs[0][4], st, KD, d);
+      labels[(int64_t)p * n + row] = K1;
+      flagged = !(D2 - D1 > 2.f * E);
+    }
     // append flagged (p, row) pairs: one atomic per wave
-    uint32_t cnt = nm;
-    uint32_t incl = wave_incl_scan(cnt);
-    uint32_t tot = __shfl(incl, kWave - 1, kWave);
+    const uint32_t cnt = flagged ? 1u : 0u;
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t tot = __shfl(incl, kWave - 1, kWave);
     unsigned long long base = 0;
     // flags go to one of kFlagSlots regions (by block): counters on one
     // address would serialise one atomic per wave and partition
     const int slot = blockIdx.x % kFlagSlots;
     if (l == kWave - 1 && tot) base = atomicAdd(&nflag[slot], (unsigned long long)tot);
     base = __shfl(base, kWave - 1, kWave);
-    for (int q = 0; q < nm; ++q) {
-      unsigned long long o = base + incl - cnt + q;
-      if (o < (unsigned long long)region) flags[(int64_t)slot * region + (int64_t)o] = mine[q];
+    if (flagged) {
+      const unsigned long long o = base + incl - 1;
+      if (o < (unsigned long long)region)
+        flags[(int64_t)slot * region + (int64_t)o] = ((uint64_t)p << 40) | (uint64_t)row;
     }
   }
 }
