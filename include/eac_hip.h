@@ -273,7 +273,8 @@ int eac_radix_sort_u64(const uint64_t* keys_in, uint64_t* keys_out,
  * order); h_out[0] = clusters re-seeded.
  * eac_kmeans_fit replaces KMeans.py:117-189 K_Means.fit with given initial
  * centroids (C_inout), batched: each partition stops on its own when
- * |sum(dists) - previous| <= tol or after max_iter labelings. */
+ * |sum(dists) - previous| <= tol (numpy's float32 sum and float32
+ * arithmetic, as eac_sum_f32_rows) or after max_iter labelings. */
 #define EAC_KM_F32 0
 #define EAC_KM_F64 1
 #define EAC_KM_BF16X3 2
@@ -288,6 +289,17 @@ int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_inout,
                    const int64_t* h_koff, int32_t P, int32_t max_iter, double tol,
                    int32_t mode, int32_t* labels, float* dists, int32_t* h_iters,
                    int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
+
+/* numpy's float32 sum of each row of x (rows x n, row-major): out[r] equals
+ * x[r].sum() of a contiguous float32 numpy array to the bit (numpy's add
+ * reduction: chunks of its 8192-element buffer, each summed pairwise with
+ * PW_BLOCKSIZE 128, chunk sums added in order from 0). It is the inertia of
+ * MyML/MyML/cluster/KMeans.py:164 (new_inertia = self._dists.sum()), which
+ * eac_kmeans_fit's stopping rule computes on the device: err =
+ * |new_inertia - previous| in float32, stop when err <= (float)tol. */
+size_t eac_sum_f32_workspace_bytes(int64_t n, int64_t rows);
+int eac_sum_f32_rows(const float* x, int64_t n, int64_t rows, float* out, void* ws,
+                     size_t ws_bytes, void* stream);
 
 /* ---- graph from a list of edge ids -------------------------------------
  * Replaces MyML/MyML/graph/build.py:49-93 getGraphFromEdges_seq and

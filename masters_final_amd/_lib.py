@@ -67,6 +67,8 @@ SIGNATURES = {
     "eac_kmeans_update": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _i32, _p, _p, _p, _sz, _p]),
     "eac_kmeans_fit": (C.c_int, [_p, _i64, _i32, _p, _p, _i32, _i32, C.c_double, _i32, _p, _p,
                                  _p, _p, _p, _sz, _p]),
+    "eac_sum_f32_workspace_bytes": (_sz, [_i64, _i64]),
+    "eac_sum_f32_rows": (C.c_int, [_p, _i64, _i64, _p, _p, _sz, _p]),
     "eac_graph_from_edges_workspace_bytes": (_sz, [_i32, _i64]),
     "eac_graph_from_edges": (C.c_int, [_p, _p, _i32, _p, _p, _i32, _i64, _p, _i64, _p, _p, _p,
                                        _p, _p, _p, _sz, _p]),

@@ -105,6 +105,21 @@ def fit_batched(X, init_centroids, k_list, max_iter=3, tol=1e-4, precision="bf16
             "n_rescored": int(h[0]), "n_reseeded": int(h[1])}
 
 
+def sum_f32_device(x):
+    """numpy's float32 sum of each row of a 2-D device tensor (1-D: one row),
+    bit-equal to ndarray.sum(): the inertia of KMeans.py:164."""
+    t = L.torch()
+    lib = L.load_library()
+    dx = L.as_device(x, t.float32)
+    x2 = dx.reshape(1, -1) if dx.dim() == 1 else dx
+    rows, n = x2.shape
+    out = t.empty(rows, dtype=t.float32, device=dx.device)
+    ws = L.workspace(lib.eac_sum_f32_workspace_bytes(n, rows))
+    L.check(lib.eac_sum_f32_rows(L.ptr(x2), n, rows, L.ptr(out), L.ptr(ws), ws.numel(),
+                                 L.stream_ptr()))
+    return out
+
+
 def numba_label(data, centroids, dists, converge):
     """KMeans.py:453-490 (numba_label = njit(py_label)): FP64 direct difference,
     lowest k on ties; fills dists (float32) when converge. Runs on the GPU."""
@@ -191,6 +206,8 @@ class K_Means:
         self.cluster_centers_ = r["centroids"].cpu().numpy()
         self.centroids = self.cluster_centers_
         self.iters_ = int(r["iters"][0])
-        self.inertia_ = float(self._dists.astype(np.float64).sum())
+        # KMeans.py:164: the inertia is numpy's float32 sum of the last dists
+        # (the device stopping rule used the same float32 sum, eac_sum_f32_rows)
+        self.inertia_ = self._dists.sum() if self._converge else np.inf
         self.n_rescored_ = r["n_rescored"]
         return self

@@ -709,8 +709,8 @@ __global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t reg
 }
 
 // FP64 distance of every point to its labelled centroid (oracle arithmetic),
-// stored as float32 like the reference's dists array, and per-partition
-// inertia sums in double. LPP lanes share a point (4, 8 or 16: about d/4):
+// stored as float32 like the reference's dists array. LPP lanes share a
+// point (4, 8 or 16: about d/4):
 // lane s loads dimensions 4s..4s+3 of each (4 LPP)-dimension chunk of the
 // point's row and its centroid's row (coalesced) and squares the
 // differences in parallel; the running sum then visits the lanes in order
@@ -725,9 +725,7 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      int d, const float* __restrict__ C,
                                                      const int64_t* __restrict__ k_off, int P,
                                                      const int32_t* __restrict__ labels,
-                                                     float* __restrict__ dists,
-                                                     double* __restrict__ inertia) {
-  __shared__ double s_acc[kPG][256 / kWave];
+                                                     float* __restrict__ dists) {
   constexpr int PPB = 256 / LPP;  // points per block and step
   const int lane = lane_id(), sub = lane & (LPP - 1), base = lane & ~(LPP - 1);
   const int64_t groups = (int64_t)gridDim.x * PPB;
@@ -735,9 +733,6 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                     reinterpret_cast<uintptr_t>(C)) & 15) == 0;
   for (int pg = blockIdx.y * kPG; pg < P; pg += gridDim.y * kPG) {
     const int np = P - pg < kPG ? P - pg : kPG;
-    double part[kPG];
-#pragma unroll
-    for (int q = 0; q < kPG; ++q) part[q] = 0.0;
     // wave-uniform trip count (the shuffles need every lane)
     const int64_t g0 = (int64_t)blockIdx.x * PPB + threadIdx.x / LPP;
     const int64_t first = g0 - (threadIdx.x & (kWave - 1)) / LPP;  // group of lane 0
@@ -782,26 +777,9 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
             acc = __shfl(acc, base + st, kWave);
           }
         }
-        if (valid && sub == 0) {
-          const float df = (float)acc;
-          if (dists) dists[(int64_t)p * n + i] = df;
-          part[q] += (double)df;
-        }
+        if (valid && sub == 0) dists[(int64_t)p * n + i] = (float)acc;
       }
     }
-#pragma unroll
-    for (int q = 0; q < kPG; ++q) {
-      if (q >= np) break;
-      const double w = wave_sum(part[q]);
-      if (lane == 0) s_acc[q][threadIdx.x / kWave] = w;
-    }
-    __syncthreads();
-    if (threadIdx.x < np && inertia) {
-      double t = 0;
-      for (int w = 0; w < 256 / kWave; ++w) t += s_acc[threadIdx.x][w];
-      atomicAdd(&inertia[pg + threadIdx.x], t);
-    }
-    __syncthreads();
   }
 }
 
@@ -854,25 +832,104 @@ __global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ 
   }
 }
 
-// Per-partition sums of float distances in double.
-__global__ __launch_bounds__(256) void k_sum_dists(const float* __restrict__ dists, int64_t n,
-                                                   int P, double* __restrict__ inertia) {
-  __shared__ double s_acc[256 / kWave];
-  for (int p = blockIdx.y; p < P; p += gridDim.y) {
-    double acc = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-      acc += (double)dists[(int64_t)p * n + i];
-    acc = wave_sum(acc);
-    if (lane_id() == 0) s_acc[threadIdx.x / kWave] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = 0;
-      for (int q = 0; q < 256 / kWave; ++q) t += s_acc[q];
-      atomicAdd(&inertia[p], t);
+// numpy's float32 sum of each partition's dists: the inertia of
+// KMeans.py:164 (new_inertia = self._dists.sum()). numpy's add reduction
+// starts from 0 and visits the array in chunks of its buffer size (8192
+// elements), adding each chunk's pairwise sum in order
+// (numpy/_core/src/umath/loops_utils.h.src @TYPE@_pairwise_sum,
+// PW_BLOCKSIZE 128): a range of len < 8 is summed sequentially from 0; len
+// <= 128 keeps eight partial sums over a[j], a[j+8], ... combined as
+// ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then adds the tail in order; longer
+// ranges split at n2 = len/2 rounded down to a multiple of 8. The tree of a
+// chunk is a pure function of its length, so one wavefront evaluates it
+// bottom-up: heap node h at depth t has its range found by descending from
+// the root along the bits of h, each level's nodes are computed in parallel
+// from the level below (held in LDS), and leaves (len <= 128) sum their
+// range directly. Every addition is a float32 addition, so the result is
+// numpy's to the bit.
+constexpr int kNpBuf = 8192;
+constexpr int kNpDepth = 8;  // chunk trees of len <= 8192 are at most 7 deep
+
+__device__ __forceinline__ bool np_node(int len0, int h, int t, int& off, int& len) {
+  off = 0;
+  len = len0;
+  for (int q = t - 1; q >= 0; --q) {
+    if (len <= 128) return false;  // an ancestor is a leaf
+    int n2 = len >> 1;
+    n2 -= n2 & 7;
+    if ((h >> q) & 1) { off += n2; len -= n2; } else { len = n2; }
+  }
+  return true;
+}
+
+__device__ __forceinline__ float np_leaf(const float* __restrict__ a, int len) {
+  if (len < 8) {
+    float res = 0.0f;
+    for (int i = 0; i < len; ++i) res = __fadd_rn(res, a[i]);
+    return res;
+  }
+  float r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i = 8;
+  for (; i < len - (len & 7); i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] = __fadd_rn(r[j], a[i + j]);
+  }
+  float res = __fadd_rn(__fadd_rn(__fadd_rn(r[0], r[1]), __fadd_rn(r[2], r[3])),
+                        __fadd_rn(__fadd_rn(r[4], r[5]), __fadd_rn(r[6], r[7])));
+  for (; i < len; ++i) res = __fadd_rn(res, a[i]);
+  return res;
+}
+
+// One wavefront per (chunk, row): csum[row * nchunks + chunk].
+__global__ __launch_bounds__(kWave) void k_np_chunk_sums(const float* __restrict__ x, int64_t n,
+                                                         int64_t nchunks,
+                                                         float* __restrict__ csum) {
+  __shared__ float v[2 << kNpDepth];
+  const int64_t c = blockIdx.x;
+  const int64_t row = blockIdx.y;
+  const int64_t base = c * kNpBuf;
+  const int len0 = (int)(n - base < kNpBuf ? n - base : kNpBuf);
+  const float* a = x + row * n + base;
+  for (int t = kNpDepth; t >= 0; --t) {
+    for (int h = (1 << t) + (int)threadIdx.x; h < (2 << t); h += kWave) {
+      int off, len;
+      if (!np_node(len0, h, t, off, len)) continue;
+      v[h] = len <= 128 ? np_leaf(a + off, len) : __fadd_rn(v[2 * h], v[2 * h + 1]);
     }
     __syncthreads();
   }
+  if (threadIdx.x == 0) csum[row * nchunks + c] = v[1];
+}
+
+// out[row] = ((0 + csum[row][0]) + csum[row][1]) + ...
+__global__ void k_np_finish(const float* __restrict__ csum, int64_t nchunks, int64_t rows,
+                            float* __restrict__ out) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  float r = 0.0f;
+  for (int64_t c = 0; c < nchunks; ++c) r = __fadd_rn(r, csum[row * nchunks + c]);
+  out[row] = r;
+}
+
+int64_t np_chunks(int64_t n) { return n > 0 ? (n + kNpBuf - 1) / kNpBuf : 0; }
+
+// numpy float32 sums of `rows` rows of length n; csum holds rows * np_chunks(n)
+int np_sum_rows(const float* x, int64_t n, int64_t rows, float* csum, float* out,
+                hipStream_t s) {
+  const int64_t nc = np_chunks(n);
+  if (nc > 0) {
+    for (int64_t r0 = 0; r0 < rows; r0 += 65535) {
+      const int64_t rr = rows - r0 < 65535 ? rows - r0 : 65535;
+      k_np_chunk_sums<<<dim3((unsigned)nc, (unsigned)rr), kWave, 0, s>>>(x + r0 * n, n, nc,
+                                                                          csum + r0 * nc);
+      EAC_LAUNCH_CHECK();
+    }
+  }
+  k_np_finish<<<grid_for(rows, 256), 256, 0, s>>>(csum, nc, rows, out);
+  EAC_LAUNCH_CHECK();
+  return EAC_OK;
 }
 
 template <typename T>
@@ -1028,12 +1085,14 @@ __global__ void k_copy_running(float* __restrict__ C, const float* __restrict__ 
 }
 
 // KMeans.py:117-189 stopping rule of iteration `it` for every partition
-// still running: err = |inertia - previous|; stop when err <= tol or it >=
-// max_iter, else mark it for a centroid update. any[0] = some partition
-// continues.
-__global__ void k_converge(const double* __restrict__ inertia, double* __restrict__ prev,
+// still running: new_inertia = numpy float32 sum of the dists (inertia[p]),
+// err = |new_inertia - previous| in float32 (previous starts at +inf); stop
+// when err <= tol (compared in float32, as numpy compares a float32 scalar
+// with a Python float) or it >= max_iter, else mark the partition for a
+// centroid update. any[0] = some partition continues.
+__global__ void k_converge(const float* __restrict__ inertia, float* __restrict__ prev,
                            int32_t* __restrict__ active, int32_t* __restrict__ iters,
-                           int32_t* __restrict__ upd, int P, int it, int max_iter, double tol,
+                           int32_t* __restrict__ upd, int P, int it, int max_iter, float tol,
                            int32_t* __restrict__ any) {
   __shared__ int s_any;
   if (threadIdx.x == 0) s_any = 0;
@@ -1042,7 +1101,7 @@ __global__ void k_converge(const double* __restrict__ inertia, double* __restric
     upd[p] = 0;
     if (!active[p]) continue;
     iters[p] = it;
-    const double err = fabs(inertia[p] - prev[p]);
+    const float err = fabsf(__fsub_rn(inertia[p], prev[p]));
     prev[p] = inertia[p];
     if (err <= tol || it >= max_iter) {
       active[p] = 0;
@@ -1055,7 +1114,7 @@ __global__ void k_converge(const double* __restrict__ inertia, double* __restric
   if (threadIdx.x == 0) any[0] = s_any;
 }
 
-__global__ void k_fit_init(double* __restrict__ prev, int32_t* __restrict__ active,
+__global__ void k_fit_init(float* __restrict__ prev, int32_t* __restrict__ active,
                            int32_t* __restrict__ iters, int P) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
     prev[p] = INFINITY;
@@ -1131,7 +1190,7 @@ struct LabelBufs {
   uint64_t* flags_r;  // kFlagSlots regions of `region` flags
   uint64_t* flags;    // the flags of all regions, contiguous
   unsigned long long* nflag;  // per-region counts
-  double* inertia;
+  float* csum;                // numpy-order chunk sums of the dists
   int64_t flag_cap;
   int64_t region;
 };
@@ -1152,7 +1211,7 @@ LabelBufs carve_label(Arena& ar, int64_t n, int d, int64_t ktot, int P) {
   b.flags_r = ar.take<uint64_t>((size_t)kFlagSlots * b.region);
   b.flags = ar.take<uint64_t>(b.flag_cap);
   b.nflag = ar.take<unsigned long long>(kFlagSlots);
-  b.inertia = ar.take<double>(P);
+  b.csum = ar.take<float>((size_t)P * np_chunks(n) + 1);
   return b;
 }
 
@@ -1166,9 +1225,9 @@ size_t kmeans_label_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
 
 // Label all partitions; dists (nullable) receive FP64-exact distances for
 // the F64 / BF16X3 modes and fp32 distances for F32; inertia[p] (device,
-// nullable) their per-partition sums.
+// nullable) their per-partition numpy float32 sums.
 int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t* h_koff,
-                 int P, int mode, int32_t* labels, float* dists, double* inertia_out,
+                 int P, int mode, int32_t* labels, float* dists, float* inertia_out,
                  int64_t* n_rescored, Arena& ar, hipStream_t s,
                  const int64_t* d_koff = nullptr) {
   const int64_t ktot = h_koff[P];
@@ -1191,10 +1250,7 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
     prof_end(pr, s);
     if (inertia_out) {
       if (!dists) return fail(EAC_ERR_ARG, "kmeans: inertia needs dists");
-      EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
-      dim3 grid(grid_for(n, 256, 1024), P < 65535 ? P : 65535);
-      k_sum_dists<<<grid, 256, 0, s>>>(dists, n, P, inertia_out);
-      EAC_LAUNCH_CHECK();
+      EAC_TRY(np_sum_rows(dists, n, P, B.csum, inertia_out, s));
     }
     return EAC_OK;
   }
@@ -1276,24 +1332,22 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
     prof_end(pr2, s);
   }
   if (n_rescored) *n_rescored = (int64_t)nf;
-  if (dists || inertia_out) {
-    if (inertia_out) EAC_TRY(fill<int64_t>((int64_t*)inertia_out, P, 0, s));
+  if (inertia_out && !dists) return fail(EAC_ERR_ARG, "kmeans: inertia needs dists");
+  if (dists) {
     const int pgs = (P + kPG - 1) / kPG;
     const int py = pgs < 65535 ? pgs : 65535;
     if (d <= 16)
       k_exact_dists<4><<<dim3(grid_for(n, 64, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                          labels, dists,
-                                                                          inertia_out);
+                                                                          labels, dists);
     else if (d <= 32)
       k_exact_dists<8><<<dim3(grid_for(n, 32, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                          labels, dists,
-                                                                          inertia_out);
+                                                                          labels, dists);
     else
       k_exact_dists<16><<<dim3(grid_for(n, 16, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                           labels, dists,
-                                                                           inertia_out);
+                                                                           labels, dists);
     EAC_LAUNCH_CHECK();
   }
+  if (inertia_out) EAC_TRY(np_sum_rows(dists, n, P, B.csum, inertia_out, s));
   return EAC_OK;
 }
 
@@ -1422,8 +1476,8 @@ extern "C" size_t eac_kmeans_workspace_bytes(int64_t n, int32_t d, int64_t k_tot
   // flags, one flag), each rounded up to 256 bytes
   Arena fa(nullptr, 0, true);
   fa.take<float>((size_t)k_total * d);
-  fa.take<double>(P);
-  fa.take<double>(P);
+  fa.take<float>(P);
+  fa.take<float>(P);
   fa.take<int64_t>(P + 1);
   fa.take<int32_t>(P);
   fa.take<int32_t>(P);
@@ -1481,6 +1535,25 @@ extern "C" int eac_kmeans_update(const float* X, int64_t n, int32_t d, const int
   return EAC_OK;
 }
 
+extern "C" size_t eac_sum_f32_workspace_bytes(int64_t n, int64_t rows) {
+  if (n < 0 || rows < 0) return 0;
+  return ((size_t)rows * (size_t)np_chunks(n) + 1) * sizeof(float) + 256;
+}
+
+// numpy's float32 x[r].sum() of every row (KMeans.py:164 inertia).
+extern "C" int eac_sum_f32_rows(const float* x, int64_t n, int64_t rows, float* out, void* ws,
+                                size_t ws_bytes, void* stream) {
+  if (n < 0 || rows < 0) return fail(EAC_ERR_ARG, "sum_f32: negative size");
+  if (rows == 0) return EAC_OK;
+  if (!out || (n > 0 && !x)) return fail(EAC_ERR_ARG, "sum_f32: null array");
+  if (n >= (1ll << 40)) return fail(EAC_ERR_UNSUPPORTED, "sum_f32: n too large");
+  const size_t need = eac_sum_f32_workspace_bytes(n, rows);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "sum_f32: workspace %zu < %zu bytes", ws_bytes, need);
+  EAC_TRY(np_sum_rows(x, n, rows, static_cast<float*>(ws), out, (hipStream_t)stream));
+  return EAC_OK;
+}
+
 // KMeans.py:117-189 per partition: label, count the iteration, stop when
 // |sum(dists) - previous| <= tol or iters >= max_iter, otherwise recompute the
 // centroids. Partitions stop independently; a stopped partition keeps its
@@ -1502,8 +1575,8 @@ extern "C" int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_ino
   hipStream_t s = (hipStream_t)stream;
   Arena ar(ws, ws_bytes);
   float* Cnew = ar.take<float>((size_t)ktot * d);
-  double* inertia = ar.take<double>(P);
-  double* prev = ar.take<double>(P);
+  float* inertia = ar.take<float>(P);
+  float* prev = ar.take<float>(P);
   int64_t* d_koff = ar.take<int64_t>(P + 1);
   int32_t* active = ar.take<int32_t>(P);
   int32_t* d_iters = ar.take<int32_t>(P);
@@ -1525,8 +1598,8 @@ extern "C" int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_ino
     EAC_TRY(kmeans_label(X, n, d, C_inout, h_koff, P, mode, labels, dists, inertia, &nres, ar,
                          s, d_koff));
     tot_res += nres;
-    k_converge<<<1, 1024, 0, s>>>(inertia, prev, active, d_iters, upd, P, it, max_iter, tol,
-                                  d_any);
+    k_converge<<<1, 1024, 0, s>>>(inertia, prev, active, d_iters, upd, P, it, max_iter,
+                                  (float)tol, d_any);
     EAC_LAUNCH_CHECK();
     if (it >= max_iter) break;  // every partition stops at max_iter
     int32_t any = 0;

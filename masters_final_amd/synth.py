@@ -5,6 +5,9 @@
                      1,000,000-vertex MST/CC workload and its smaller cases)
   gmm_points()       Gaussian-mixture points for the K-Means workloads
   random_graph()     small random symmetric CSR graphs for parity tests
+  graph_batch()      many independent graphs as one disjoint-union CSR (host)
+  knn_batch_device() the 512-graph batched single-link workload: k-NN graphs
+                     found on the host, symmetrized and laid out on the GPU
 """
 from __future__ import annotations
 
@@ -196,3 +199,71 @@ def graph_batch_device(n_graphs, v_range, deg_range, seed, wmax=256, device="cud
     del rows
     return {"indptr": indptr, "dest": cols.to(torch.int32), "weight": ww, "goff": goff,
             "n": N, "targets": deg}
+
+
+def _batch_sizes(n_graphs, v_range, deg_range, rng):
+    lo, hi = np.log(v_range[0]), np.log(v_range[1])
+    nv = np.exp(rng.uniform(lo, hi, n_graphs)).astype(np.int64)
+    deg = rng.uniform(deg_range[0], deg_range[1], n_graphs)
+    return nv, deg
+
+
+def _csr_on_device(a, b, w, N, torch):
+    """Symmetric CSR sorted by (row, column) from unique undirected device
+    edges a < b with weights w (both directions stored)."""
+    rows = torch.cat([a, b])
+    cols = torch.cat([b, a])
+    ww = torch.cat([w, w])
+    del a, b, w
+    order = torch.sort(rows * N + cols).indices
+    rows, cols, ww = rows[order], cols[order], ww[order]
+    del order
+    indptr = torch.zeros(N + 1, dtype=torch.int64, device=rows.device)
+    indptr[1:] = torch.cumsum(torch.bincount(rows, minlength=N), 0)
+    del rows
+    return indptr, cols.to(torch.int32), ww
+
+
+def knn_batch_device(n_graphs, v_range, deg_range, seed, wmax=256, device="cuda"):
+    """Independent 3-D k-NN graphs as one disjoint-union CSR on the GPU (the
+    batched single-link workload): graph g has V_g points uniform in [0,1)^3
+    (V_g log-uniform in v_range), k_g = round(0.75 * deg_g) exact nearest
+    neighbours (deg_g uniform in deg_range; cKDTree on the host, as
+    knn_graph), symmetrized as a union, int32 weights U[1, wmax] per
+    undirected edge from a torch CUDA generator seeded with `seed`. Sizes and
+    points come from numpy PCG64(seed). Returns dict(indptr int64, dest
+    int32, weight int32 (device tensors), goff (host int64), n, targets,
+    knn_k)."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(seed)
+    nv, deg = _batch_sizes(n_graphs, v_range, deg_range, rng)
+    goff = np.zeros(n_graphs + 1, np.int64)
+    np.cumsum(nv, out=goff[1:])
+    N = int(goff[-1])
+    dev = torch.device(device)
+    ks = np.zeros(n_graphs, np.int64)
+    keys = []
+    for g in range(n_graphs):
+        V = int(nv[g])
+        pts = np.random.default_rng(int(rng.integers(0, 2**31 - 1))).random((V, 3))
+        k = min(max(1, int(round(0.75 * deg[g]))), V - 1)
+        ks[g] = k
+        _, nb = cKDTree(pts).query(pts, k=k + 1, workers=-1)
+        dst = torch.from_numpy(nb[:, 1:].astype(np.int32)).to(dev).reshape(-1).to(torch.int64)
+        src = torch.arange(V, device=dev, dtype=torch.int64).repeat_interleave(k)
+        lo = torch.minimum(src, dst) + int(goff[g])
+        hi = torch.maximum(src, dst) + int(goff[g])
+        keys.append(lo * N + hi)
+        del src, dst, lo, hi
+    key = torch.unique(torch.cat(keys))  # sorted
+    del keys
+    a, b = key // N, key % N
+    del key
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    w = torch.randint(1, wmax + 1, (a.numel(),), generator=gen, device=dev, dtype=torch.int32)
+    indptr, dest, weight = _csr_on_device(a, b, w, N, torch)
+    return {"indptr": indptr, "dest": dest, "weight": weight, "goff": goff, "n": N,
+            "targets": deg, "knn_k": ks}

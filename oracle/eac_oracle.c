@@ -689,37 +689,85 @@ void orc_recompute_centroids(const float* X, const int32_t* labels, const float*
   free(acc); free(cnt); free(far);
 }
 
+/* numpy's float32 summation, as `dists.sum()` computes the inertia at
+ * KMeans.py:164 (numpy, the reference's array library, version unpinned).
+ * The add reduction starts from the identity 0 and visits the contiguous
+ * array in buffer-sized chunks (numpy's default buffer, 8192 elements),
+ * adding each chunk's pairwise sum to the running result in order; the
+ * pairwise sum is numpy/_core/src/umath/loops_utils.h.src
+ * @TYPE@_pairwise_sum with PW_BLOCKSIZE 128 (numpy >= 1.9). Checked
+ * bit-for-bit against numpy 2.2's float32 sum. Every addition is a float32
+ * addition:
+ *   n < 8:    sequential sum from 0;
+ *   n <= 128: eight partial sums r[j] over a[j], a[j+8], ... up to the last
+ *             multiple of 8, combined as ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)),
+ *             then the remaining elements added in order;
+ *   else:     split at n2 = n/2 rounded down to a multiple of 8, sum of the
+ *             two halves. */
+static float pairwise_f32(const float* a, int64_t n) {
+  if (n < 8) {
+    float res = 0.0f;
+    for (int64_t i = 0; i < n; ++i) res = res + a[i];
+    return res;
+  }
+  if (n <= 128) {
+    float r[8];
+    for (int j = 0; j < 8; ++j) r[j] = a[j];
+    int64_t i;
+    for (i = 8; i < n - (n % 8); i += 8)
+      for (int j = 0; j < 8; ++j) r[j] = r[j] + a[i + j];
+    float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+    for (; i < n; ++i) res = res + a[i];
+    return res;
+  }
+  int64_t n2 = n / 2;
+  n2 -= n2 % 8;
+  return pairwise_f32(a, n2) + pairwise_f32(a + n2, n - n2);
+}
+
+#define ORC_NUMPY_BUFSIZE 8192
+float orc_sum_f32(const float* a, int64_t n) {
+  float res = 0.0f;
+  for (int64_t i = 0; i < n; i += ORC_NUMPY_BUFSIZE)
+    res = res + pairwise_f32(a + i, n - i < ORC_NUMPY_BUFSIZE ? n - i : ORC_NUMPY_BUFSIZE);
+  return res;
+}
+
 /* KMeans.py:117-189 K_Means.fit with given initial centroids: label, count
- * the iteration, stop when |sum(dists) - previous| <= tol or
- * iters >= max_iter, else recompute centroids. The convergence sum is taken
- * over the float32 dists in double (numpy's float32 pairwise sum is not
- * restated; with tol=1e-4 both stop exactly when the distances repeat).
- * labels/dists of the last labeling and the final centroids are returned;
- * the iteration count is the return value. */
+ * the iteration, then (converge mode, KMeans.py:160-175) new_inertia =
+ * dists.sum() in float32 (orc_sum_f32), error = |new_inertia - inertia| in
+ * float32 (inertia starts at +inf, so the first error is +inf), stop when
+ * error <= tol (a float32 comparison: numpy compares a float32 scalar with a
+ * Python float in float32) or iters >= max_iter, else recompute centroids.
+ * A negative tol never stops early (converge=False). The labels/dists
+ * of the last labeling, the final centroids and the last inertia are
+ * returned; the iteration count is the return value. */
 int64_t orc_kmeans_fit(const float* X, const float* init, int64_t n, int64_t d, int64_t k,
                        int64_t max_iter, double tol, int reseed_mode, int32_t* labels,
-                       float* dists, float* centroids) {
+                       float* dists, float* centroids, float* inertia_out) {
   float* C = malloc(sizeof(float) * k * d);
   float* Cn = malloc(sizeof(float) * k * d);
   double* dd = malloc(sizeof(double) * (n > 0 ? n : 1));
   memcpy(C, init, sizeof(float) * k * d);
-  double inertia = INFINITY;
+  const float tol32 = (float)tol;
+  float inertia = INFINITY;
   int64_t iters = 0;
   for (;;) {
     orc_kmeans_label_f64(X, C, n, d, k, labels, dd);
-    double s = 0.0;
-    for (int64_t i = 0; i < n; ++i) { dists[i] = (float)dd[i]; s += dists[i]; }
+    for (int64_t i = 0; i < n; ++i) dists[i] = (float)dd[i];
     iters += 1;
     int stop = 0;
-    double err = fabs(s - inertia);
+    const float s = orc_sum_f32(dists, n);
+    const float err = fabsf(s - inertia);
     inertia = s;
-    if (err <= tol) stop = 1;
+    if (err <= tol32) stop = 1;
     if (iters >= max_iter) stop = 1;
     if (stop) break;
     orc_recompute_centroids(X, labels, dists, n, d, k, reseed_mode, Cn);
     float* t = C; C = Cn; Cn = t;
   }
   memcpy(centroids, C, sizeof(float) * k * d);
+  if (inertia_out) *inertia_out = inertia;
   free(C); free(Cn); free(dd);
   return iters;
 }

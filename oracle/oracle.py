@@ -40,7 +40,8 @@ def lib() -> C.CDLL:
             "orc_kmeans_label_f64": (None, [_P, _P, _I, _I, _I, _P, _P]),
             "orc_arg_k_select": (None, [_P, _I, _I, _P]),
             "orc_recompute_centroids": (None, [_P, _P, _P, _I, _I, _I, C.c_int, _P]),
-            "orc_kmeans_fit": (_I, [_P, _P, _I, _I, _I, _I, _D, C.c_int, _P, _P, _P]),
+            "orc_kmeans_fit": (_I, [_P, _P, _I, _I, _I, _I, _D, C.c_int, _P, _P, _P, _P]),
+            "orc_sum_f32": (C.c_float, [_P, _I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -227,13 +228,22 @@ def recompute_centroids(X, labels, dists, k, reseed_mode=0):
     return out
 
 
-def kmeans_fit(X, init, max_iter=3, tol=1e-4, reseed_mode=0):
+def sum_f32(a):
+    """numpy's float32 `a.sum()` (pairwise summation), restated in C."""
+    a = _a(a, np.float32)
+    return np.float32(lib().orc_sum_f32(_p(a), a.size))
+
+
+def kmeans_fit(X, init, max_iter=3, tol=1e-4, reseed_mode=0, return_inertia=False):
     X, init = _a(X, np.float32), _a(init, np.float32)
     n, d = X.shape
     k = init.shape[0]
     lab = np.zeros(n, np.int32)
     dist = np.zeros(n, np.float32)
     cen = np.zeros((k, d), np.float32)
+    inertia = C.c_float()
     it = lib().orc_kmeans_fit(_p(X), _p(init), n, d, k, max_iter, tol, reseed_mode, _p(lab),
-                              _p(dist), _p(cen))
+                              _p(dist), _p(cen), C.byref(inertia))
+    if return_inertia:
+        return lab, dist, cen, int(it), np.float32(inertia.value)
     return lab, dist, cen, int(it)

@@ -224,3 +224,38 @@ def test_k_means_class(torch):
     assert km.iters_ == it and np.array_equal(km.labels_, lab)
     with pytest.raises(Warning):
         K_Means(n_clusters=3).fit(X.astype(np.float64))
+
+
+def test_sum_f32_rows_equal_numpy(torch):
+    """eac_sum_f32_rows gives numpy's float32 sum to the bit (the inertia of
+    KMeans.py:164): row lengths around the pairwise block and the 8192-element
+    reduction buffer, magnitudes over 40 binades, many rows at once."""
+    from masters_final_amd.cluster.KMeans import sum_f32_device
+    rng = np.random.default_rng(21)
+    for n in [1, 7, 8, 9, 127, 128, 129, 1000, 8191, 8192, 8193, 10000, 65537, 500_000]:
+        rows = 3 if n > 100_000 else 17
+        x = (rng.standard_normal((rows, n)) * np.exp(rng.uniform(-20, 20, (rows, n)))).astype(
+            np.float32)
+        got = sum_f32_device(x).cpu().numpy()
+        want = np.array([x[r].sum() for r in range(rows)], np.float32)
+        assert got.tobytes() == want.tobytes(), n
+    big = rng.random(4_194_304).astype(np.float32)
+    assert sum_f32_device(big).cpu().numpy()[0] == big.sum()
+
+
+def test_fit_float32_stop_rule(torch):
+    """Two labelings: the second changes the dists by 0.013 but not their
+    float32 sum (~2e8), so the reference stops where float64 sums would not."""
+    from masters_final_amd.cluster.KMeans import K_Means, fit_batched
+    X = np.array([[-1.0], [0.0], [1.2], [20000.0], [40000.0]], np.float32)
+    init = np.array([[0.0], [30000.0]], np.float32)
+    lab, dist, cen, it, inert = O.kmeans_fit(X, init, max_iter=3, tol=1e-4, reseed_mode=1,
+                                             return_inertia=True)
+    assert it == 2
+    r = fit_batched(X, init, [2], max_iter=3, precision="fp64")
+    assert int(r["iters"][0]) == 2
+    assert np.array_equal(r["labels"][0].cpu().numpy(), lab)
+    assert np.array_equal(r["centroids"].cpu().numpy(), cen)
+    km = K_Means(n_clusters=2, max_iter=3, init=init)
+    km.fit(X)
+    assert km.iters_ == 2 and km.inertia_.tobytes() == inert.tobytes()

@@ -275,3 +275,159 @@ def test_recompute_centroids_closed_form():
                 s = (s + X[i]).astype(np.float32)
             assert np.array_equal(out[c], (s.astype(np.float64) / idx.size).astype(np.float32))
     assert np.array_equal(out[4], X[np.argmin(dists)])
+
+
+# ------------------------------------------------ K-Means inertia and fit
+@pytest.mark.parametrize("kind", ["unit", "wide", "signed"])
+def test_sum_f32_equals_numpy(kind):
+    """The oracle's float32 sum is numpy's `ndarray.sum()` to the bit (the
+    inertia of KMeans.py:164): sizes around the pairwise block (8, 128) and
+    numpy's 8192-element reduction buffer, magnitudes over 40 binades."""
+    rng = np.random.default_rng({"unit": 1, "wide": 2, "signed": 3}[kind])
+    sizes = list(range(0, 140)) + [255, 256, 257, 1000, 8191, 8192, 8193, 10000, 16383,
+                                   16384, 16385, 24577, 99999, 500_000, 1_000_003]
+    for n in sizes:
+        if kind == "unit":
+            a = rng.random(n).astype(np.float32)
+        elif kind == "wide":
+            a = np.exp(rng.uniform(-30, 30, n)).astype(np.float32)
+        else:
+            a = (rng.standard_normal(n) * np.exp(rng.uniform(-20, 20, n))).astype(np.float32)
+        assert O.sum_f32(a).tobytes() == a.sum().tobytes(), n
+
+
+def _py_pairwise(a):
+    """numpy's pairwise block sum over one range, in float32 (no chunking)."""
+    n = a.size
+    if n < 8:
+        r = np.float32(0)
+        for x in a:
+            r = np.float32(r + x)
+        return r
+    if n <= 128:
+        r = a[:8].copy()
+        i = 8
+        while i < n - n % 8:
+            r = (r + a[i:i + 8]).astype(np.float32)
+            i += 8
+        res = np.float32(np.float32(np.float32(r[0] + r[1]) + np.float32(r[2] + r[3])) +
+                         np.float32(np.float32(r[4] + r[5]) + np.float32(r[6] + r[7])))
+        for x in a[i:]:
+            res = np.float32(res + x)
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return np.float32(_py_pairwise(a[:n2]) + _py_pairwise(a[n2:]))
+
+
+def test_sum_f32_discriminates_orders():
+    """Plausible wrong orders give other bits on this input: one pairwise
+    tree over the whole array (no buffer chunks), a sequential float32 sum,
+    and a float64 sum rounded to float32."""
+    rng = np.random.default_rng(4)
+    a = (rng.standard_normal(40_000) * np.exp(rng.uniform(-3, 3, 40_000))).astype(np.float32)
+    want = a.sum()
+    assert O.sum_f32(a) == want
+    assert _py_pairwise(a[:8192]) == a[:8192].sum()  # one chunk is one tree
+    assert _py_pairwise(a) != want
+    seq = np.float32(0)
+    for x in a:
+        seq = np.float32(seq + x)
+    assert seq != want
+    assert np.float32(a.astype(np.float64).sum()) != want
+
+
+def _np_label(X, C):
+    """KMeans.py:453-490 in FP64: dimensions ascending, first minimum wins."""
+    X64, C64 = X.astype(np.float64), C.astype(np.float64)
+    dist = np.zeros((X.shape[0], C.shape[0]))
+    for j in range(X.shape[1]):
+        diff = X64[:, j:j + 1] - C64[None, :, j]
+        dist += diff * diff
+    lab = np.argmin(dist, axis=1).astype(np.int32)
+    return lab, dist[np.arange(X.shape[0]), lab].astype(np.float32)
+
+
+def _np_recompute(X, lab, dists, k):
+    """KMeans.py:709-754: float32 sums in point order (np.add.at is an
+    unbuffered in-order update), divided by the count in float64, stored as
+    float32; empty clusters take the points of smallest (dist, index)."""
+    acc = np.zeros((k, X.shape[1]), np.float32)
+    np.add.at(acc, lab, X)
+    cnt = np.bincount(lab, minlength=k)
+    out = np.empty_like(acc)
+    nz = cnt > 0
+    out[nz] = (acc[nz].astype(np.float64) / cnt[nz, None]).astype(np.float32)
+    empty = np.nonzero(~nz)[0]
+    order = np.lexsort((np.arange(dists.size), dists))[:empty.size]
+    out[empty] = X[order]
+    return out
+
+
+def _np_fit(X, init, max_iter, tol, f32_inertia=True):
+    """KMeans.py:117-189 written out with numpy: label, count, inertia =
+    dists.sum() (numpy float32), error = |new - old| (old starts at inf),
+    stop on error <= tol or iters >= max_iter, else recompute."""
+    C = init.copy()
+    inertia = np.inf
+    iters = 0
+    while True:
+        lab, dists = _np_label(X, C)
+        iters += 1
+        new = dists.sum() if f32_inertia else dists.astype(np.float64).sum()
+        error = np.abs(new - inertia)
+        inertia = new
+        stop = bool(error <= tol)
+        if iters >= max_iter:
+            stop = True
+        if stop:
+            break
+        C = _np_recompute(X, lab, dists, C.shape[0])
+    return lab, dists, C, iters, inertia
+
+
+@pytest.mark.parametrize("n,d,k,max_iter,tol,seed", [
+    (2000, 2, 40, 3, 1e-4, 1), (3000, 5, 25, 10, 1e-4, 2), (1500, 16, 60, 3, 1e-4, 3),
+    (9000, 3, 12, 30, 1e-4, 4), (20000, 2, 8, 30, 1e-2, 5), (600, 7, 90, 4, 1e-4, 6)])
+def test_kmeans_fit_vs_numpy_restatement(n, d, k, max_iter, tol, seed):
+    X, _ = synth.gmm_points(n, d, max(k // 3, 2), seed, mean_scale=10.0)
+    rng = np.random.default_rng(seed)
+    init = X[rng.choice(n, k, replace=False)].copy()
+    lab, dist, cen, it, inert = O.kmeans_fit(X, init, max_iter=max_iter, tol=tol, reseed_mode=1,
+                                             return_inertia=True)
+    wl, wd, wc, wit, winert = _np_fit(X, init, max_iter, tol)
+    assert it == wit
+    assert np.array_equal(lab, wl) and np.array_equal(dist, wd) and np.array_equal(cen, wc)
+    assert inert.tobytes() == np.float32(winert).tobytes()
+
+
+def _f32_stop_case():
+    """1-D data whose second labeling lowers the dists by 0.013 while their
+    float32 sum (about 2e8, float32 spacing 16) does not change."""
+    X = np.array([[-1.0], [0.0], [1.2], [20000.0], [40000.0]], np.float32)
+    init = np.array([[0.0], [30000.0]], np.float32)
+    return X, init
+
+
+def test_kmeans_fit_float32_stop_rule():
+    """The stop rule compares numpy float32 sums: on this input it stops after
+    two labelings, where sums in float64 would run a third."""
+    X, init = _f32_stop_case()
+    lab, dist, cen, it = O.kmeans_fit(X, init, max_iter=3, tol=1e-4, reseed_mode=1)
+    wl, wd, wc, wit, _ = _np_fit(X, init, 3, 1e-4)
+    *_, it64, _ = _np_fit(X, init, 3, 1e-4, f32_inertia=False)
+    assert it == wit == 2 and it64 == 3
+    assert np.array_equal(lab, wl) and np.array_equal(cen, wc)
+
+
+def test_kmeans_fit_reseeds_empty_clusters():
+    """Duplicate initial centroids leave clusters empty (the first equal
+    centroid takes every point); the update re-seeds them."""
+    X, _ = synth.gmm_points(3000, 3, 5, 9, mean_scale=10.0)
+    rng = np.random.default_rng(9)
+    base = X[rng.choice(3000, 10, replace=False)]
+    init = np.concatenate([base, base[:4]])
+    lab, dist, cen, it = O.kmeans_fit(X, init, max_iter=4, tol=1e-4, reseed_mode=1)
+    wl, wd, wc, wit, _ = _np_fit(X, init, 4, 1e-4)
+    assert it == wit and np.array_equal(lab, wl) and np.array_equal(cen, wc)
+    assert len(np.unique(lab)) > 10
