@@ -56,7 +56,8 @@ int eac_abi_version(void);
  * stream they run on; eac_profile_read returns their summed duration, the
  * number of launches and the algorithmic bytes those launches had to move
  * (the per-unit byte figures are documented with each slot). Enabling
- * resets the counters. Host-thread-local use only. */
+ * resets the counters. The switch, the records and the totals belong to the
+ * calling host thread: other threads' calls are neither timed nor counted. */
 #define EAC_PROF_SLOTS 16
 #define EAC_PROF_MST_ROUND0 0 /* CSR stream: 8 B/entry + 16 B/vertex + 8 B/edge out */
 #define EAC_PROF_MST_ROUND 1  /* list pass: 16 B/key in (key + 2 root lookups) + 8 B/kept key */
@@ -75,7 +76,8 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
 /* ---- tuning knobs (no counterpart in the reference) ---------------------
  * Launch and algorithm parameters that change speed, never results; set by
  * the autotuner in masters_final_amd/tune.py (or by hand) for the calls
- * that follow on this process. 0 selects the built-in choice.
+ * that follow on the calling host thread (each thread has its own
+ * settings, so threads never race on them). 0 selects the built-in choice.
  *   EAC_TUNE_LIGHT_X10      light/heavy split of Borůvka: light undirected
  *                           edges per vertex x 10 (default 15; -1 = no split)
  *   EAC_TUNE_FIRST_BATCH    rounds enqueued before the first read-back
@@ -263,7 +265,11 @@ int eac_radix_sort_u64(const uint64_t* keys_in, uint64_t* keys_out,
  *   EAC_KM_BF16X3  split-BF16 MFMA contraction with fp32 accumulation, a
  *                  rigorous per-point error bound, FP64 re-scoring of every
  *                  point whose top-2 margin is within it -> labels and
- *                  distances equal EAC_KM_F64's (d < 16 runs EAC_KM_F64).
+ *                  distances equal EAC_KM_F64's (d < 16 runs EAC_KM_F64);
+ *   EAC_KM_F32_NUMPY numpy's float32 arithmetic of KMeans.py:493-522
+ *                  np_label: ((x - c) ** 2).sum(axis=1) with a float32
+ *                  difference and square and numpy's pairwise row sum,
+ *                  strict < (lowest k on ties); d <= 128.
  * eac_kmeans_label replaces KMeans.py:296-412 _cu_label (kernel
  * KMeans.py:586-629); h_out[0] = points re-scored in FP64.
  * eac_kmeans_update replaces KMeans.py:709-754 numba_recompute_centroids
@@ -278,6 +284,7 @@ int eac_radix_sort_u64(const uint64_t* keys_in, uint64_t* keys_out,
 #define EAC_KM_F32 0
 #define EAC_KM_F64 1
 #define EAC_KM_BF16X3 2
+#define EAC_KM_F32_NUMPY 3
 size_t eac_kmeans_workspace_bytes(int64_t n, int32_t d, int64_t k_total, int32_t P);
 int eac_kmeans_label(const float* X, int64_t n, int32_t d, const float* C,
                      const int64_t* h_koff, int32_t P, int32_t mode, int32_t* labels,
@@ -339,6 +346,87 @@ int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff,
                      int32_t P, int32_t full_rows, const int64_t* indptr, int32_t w_offset,
                      int32_t* indices, int32_t* data, void* ws, size_t ws_bytes,
                      void* stream);
+/* eac_coassoc_fill (w_offset 0) that also writes rank[nnz]: for each entry,
+ * the position, in the reference's processing order, of the first partition
+ * in which the pair co-occurs. The order is the one of
+ * MyML/MyML/EAC/sparse.py:154-166 update_ensemble: first the partition with
+ * the fewest non-empty clusters (the first such), then the others by index
+ * (rank 0 = that first partition). Same workspace as the count pass. */
+int eac_coassoc_fill_ranked(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                            int32_t P, int32_t full_rows, const int64_t* indptr,
+                            int32_t* indices, int32_t* data, int32_t* rank, void* ws,
+                            size_t ws_bytes, void* stream);
+
+/* ---- capacity-capped co-association (the reference's lossy matrix) ------
+ * From the exact matrix (indptr, indices, count) and its ranks (above),
+ * the matrix the reference's EAC_CSR builds with per-row capacities:
+ * cap_off[n+1] are its capacity offsets (indptr_constant sparse.py:368-369
+ * or indptr_linear :371-413; condensed rows, full_rows = 0) or, with
+ * full_rows = 1, every row holds max_assocs (cap_off unused). A row keeps
+ * the first `capacity` columns in the order (rank, column) at their full
+ * counts, as the surgical updates (sparse.py:956-1040 condensed, :866-950
+ * full) admit new columns only while the row has room and discard every
+ * later co-occurrence of the others (counted, nnz_discarded); with full rows
+ * a row above id 405109 keeps only the first partition's columns at count 1
+ * (sparse.py:877-878). Replaces the capped update_ensemble + _condense.
+ * Output: out_indptr[n+1] (int64), out_indices/out_data (room for
+ * out_capacity entries; the reference's own allocation, cap_off[n], always
+ * suffices), h_out[4] (host): kept entries, discarded co-occurrences,
+ * largest kept count, -1. EAC_ERR_VALUE when the first partition alone
+ * gives a row more entries than its capacity (the reference then writes
+ * past the row into the next one). */
+size_t eac_coassoc_cap_workspace_bytes(int64_t n);
+int eac_coassoc_cap(const int64_t* indptr, const int32_t* indices, const int32_t* count,
+                    const int32_t* rank, int64_t n, int32_t P, int32_t full_rows,
+                    const int64_t* cap_off, int64_t max_assocs, int64_t out_capacity,
+                    int64_t* out_indptr, int32_t* out_indices, int32_t* out_data,
+                    int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
+
+/* ---- dense co-association and single linkage (small n) ------------------
+ * Matrix dtypes: EAC_W_U8, EAC_W_U16, EAC_W_I32, EAC_W_F32. condensed = 1:
+ * the n(n-1)/2 upper-triangle vector in SciPy's order (pdist/squareform);
+ * 0: the n x n matrix (diagonal 0).
+ * eac_coassoc_dense replaces MyML/MyML/EAC/full.py:8-166 EAC_FULL
+ * update_ensemble (numba_update_coassoc_with_cluster :141-153,
+ * numba_update_condensed_coassoc_with_cluster :155-166): out[i, j] = number
+ * of partitions with labels[p][i] == labels[p][j] >= 0, in the matrix dtype
+ * (uint8 wraps as the reference's += 1 does); *h_max (host, nullable)
+ * receives the largest stored value. ws: 256 bytes.
+ * eac_dense_diassoc replaces full.py:98-112 make_diassoc (_1d / _2d):
+ * mat[i] = val - mat[i] over len elements (integer types wrap).
+ * eac_dense_degree replaces full.py:83-95 get_degree: degree[i] = non-zero
+ * entries of row i (condensed: its columns j > i only), *h_nnz the total.
+ * eac_full_sl_lifetime replaces MyML/MyML/EAC/eac.py:337-364
+ * full_sl_lifetime on the dissimilarities: single linkage (the minimum
+ * spanning tree of the complete graph), the lifetime rule of eac.py:366-391
+ * on the float32 heights unless n_clusters > 0, labels of the clusters
+ * numbered by first occurrence (labels_from_Z_numba,
+ * MyML/MyML/cluster/linkage.py:303-357); h_out[2]: clusters, labels_empty
+ * (one cluster: labels not written). Z ((n-1) x 4 float32, nullable): the
+ * linkage matrix. EAC_ERR_VALUE where numpy raises (no lifetime within
+ * 1e-8 of zero, or fewer than two heights).
+ * eac_slink replaces MyML/MyML/cluster/linkage.py:144-149 (SL: SLINK,
+ * scipy_numba/.../_hierarchy_eac.py:884-927, and from_pointer_representation
+ * :842-881): Z rows (left, right, height, size), heights ascending, equal
+ * heights in (height, min id, max id) order of their tree edges.
+ * eac_labels_from_Z replaces linkage.py:303-357 labels_from_Z_numba for any
+ * valid Z: the first n - n_clusters merges, clusters numbered by first
+ * occurrence. n <= 65536 for the single-linkage calls. */
+int eac_coassoc_dense(const int32_t* labels, int64_t n, int32_t P, int32_t condensed,
+                      int32_t dtype, void* out, int64_t* h_max, void* ws, size_t ws_bytes,
+                      void* stream);
+int eac_dense_diassoc(void* mat, int64_t len, int32_t dtype, double val, void* stream);
+int eac_dense_degree(const void* mat, int32_t dtype, int64_t n, int32_t condensed,
+                     int32_t* degree, int64_t* h_nnz, void* ws, size_t ws_bytes, void* stream);
+size_t eac_dense_sl_workspace_bytes(int64_t n, int32_t dtype, int32_t condensed);
+int eac_full_sl_lifetime(const void* dists, int32_t dtype, int64_t n, int32_t condensed,
+                         int32_t n_clusters, float* Z, int32_t* labels, int64_t* h_out,
+                         void* ws, size_t ws_bytes, void* stream);
+int eac_slink(const void* dists, int32_t dtype, int64_t n, int32_t condensed, float* Z,
+              void* ws, size_t ws_bytes, void* stream);
+size_t eac_labels_from_Z_workspace_bytes(int64_t n);
+int eac_labels_from_Z(const float* Z, int64_t n, int32_t n_clusters, int32_t* labels, void* ws,
+                      size_t ws_bytes, void* stream);
 
 /* ---- first-occurrence relabelling --------------------------------------
  * out[v] = rank of labels_in[v] among distinct values ordered by first
@@ -348,6 +436,18 @@ size_t eac_relabel_workspace_bytes(int32_t n, int32_t n_values);
 int eac_canonical_relabel(const int32_t* labels_in, int32_t n,
                           int32_t n_values, int32_t* out, int64_t* h_count,
                           void* ws, size_t ws_bytes, void* stream);
+
+/* ---- binary cluster-by-sample matrix of a partition ---------------------
+ * out[k][N] (uint8, 16-byte aligned) is cleared, then out[rows[t]][cols[t]]
+ * = 1 for t < m (cols NULL: cols[t] = t, m <= N). Replaces
+ * MyML/MyML/utils/partition.py:17-35 convertIndexToBin (rows = the cluster
+ * of each listed index, cols = the indices, N = the number listed) and
+ * partition.py:37-70 convertClusterStringToBin (rows = label + offset, one
+ * column per sample). Entries outside [0, k) x [0, N) are skipped and
+ * counted into *d_bad (device int64, nullable), where numpy raises
+ * IndexError. */
+int eac_partition_to_bin(const int32_t* rows, const int64_t* cols, int64_t m, int32_t k,
+                         int64_t N, uint8_t* out, int64_t* d_bad, void* stream);
 
 /* ---- contingency table of two partitions (accuracy indices) -------------
  * out[(a[s] - a0) * kb + (b[s] - b0)] = number of samples s with those two

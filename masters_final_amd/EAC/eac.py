@@ -44,13 +44,15 @@ def sp_sl_lifetime(mat, max_dist=None, n_clusters=0):
 
 
 class EAC:
-    """eac.py:40-179 interface. buildMatrix counts the ensemble's
-    co-associations exactly on the GPU (EAC_CSR) and turns them into
-    dissimilarities (max + 1) - count; finalClustering runs the Boruvka forest,
-    the lifetime rule and the cut on the GPU (sp_sl_lifetime).
-    The dense EAC_FULL storage (eac.py:139-144, O(n^2)) and the disk-backed
-    Kruskal (eac.py:270-335) are not built: `sparse=False` uses the same exact
-    sparse matrix."""
+    """eac.py:40-179 interface. With sparse=True, buildMatrix counts the
+    ensemble's co-associations on the GPU (EAC_CSR: exact, or the
+    reference's capacity-capped matrix with sparse_exact=False) and turns
+    them into dissimilarities (max + 1) - count; finalClustering runs the
+    Boruvka forest, the lifetime rule and the cut on the GPU
+    (sp_sl_lifetime). With sparse=False (the reference's default) the dense
+    EAC_FULL matrix and full_sl_lifetime run instead (eac.py:139-144,
+    :166-169), on the GPU as well. The disk-backed Kruskal (eac.py:270-335)
+    is not built."""
 
     def __init__(self, n_samples, **kwargs):
         self.n_samples = n_samples
@@ -68,6 +70,9 @@ class EAC:
         self.sp_max_assocs_factor = kwargs.get("sparse_max_assocs_factor", 3)
         self.sp_max_assocs_mode = kwargs.get("sparse_max_assocs_mode", "linear")
         self.sp_keep_degree = kwargs.get("sparse_keep_degree", False)
+        # False: the reference's capacity-capped (lossy) matrix, for parity
+        # studies; True (default): exact counts
+        self.sp_exact = kwargs.get("sparse_exact", True)
         self.full = not self.sparse and not self.kNN
         self.coassoc_store = kwargs.get("coassoc_store", False)
         self.linkage = kwargs.get("linkage", "SL")
@@ -84,13 +89,24 @@ class EAC:
         if self.kNN:
             raise NotImplementedError("kNN matrix building has not been included in this "
                                       "version.")
+        if self.full:
+            # eac.py:139-144: dense counts, degrees, dissimilarities
+            from .full import EAC_FULL
+            coassoc = EAC_FULL(self.n_samples, condensed=self.condensed,
+                               dtype=self.assoc_dtype)
+            coassoc.update_ensemble(ensemble)
+            coassoc.get_degree()
+            coassoc.make_diassoc()
+            self.coassoc = coassoc
+            return
         if self.sp_max_assocs is None:
             self.sp_max_assocs = max(max(map(np.size, p)) for p in ensemble)
             self.sp_max_assocs *= self.sp_max_assocs_factor
         mode = self.sp_max_assocs_mode if self.condensed else "constant"
         coassoc = EAC_CSR(self.n_samples, max_assocs=int(self.sp_max_assocs),
                           condensed=self.condensed, max_assocs_type=mode,
-                          sort_mode=self.sp_sort, dtype=self.assoc_dtype)
+                          sort_mode=self.sp_sort, dtype=self.assoc_dtype,
+                          exact=self.sp_exact)
         coassoc.update_ensemble(ensemble)
         coassoc._condense(keep_degree=self.sp_keep_degree)
         coassoc.make_diassoc()
@@ -108,8 +124,15 @@ class EAC:
 
     def finalClustering(self, n_clusters=0):
         c = self.coassoc
-        n_fclusts, labels, _ = sp_sl_lifetime_device((c.indptr, c.indices, c.data),
-                                                     self.n_partitions, n_clusters=n_clusters)
+        if self.full:
+            # eac.py:166-169: full_sl_lifetime on the dense dissimilarities
+            from .full import full_sl_lifetime_device
+            n_fclusts, labels, _ = full_sl_lifetime_device(c.coassoc, self.n_samples,
+                                                           n_clusters=n_clusters)
+        else:
+            n_fclusts, labels, _ = sp_sl_lifetime_device((c.indptr, c.indices, c.data),
+                                                         self.n_partitions,
+                                                         n_clusters=n_clusters)
         self.n_fclusts = n_fclusts
         self.labels = (np.empty(0, dtype=np.int32) if labels is None
                        else labels.cpu().numpy())

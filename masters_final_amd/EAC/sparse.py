@@ -3,13 +3,16 @@
 
 EAC_CSR keeps the reference's constructor, update_ensemble / _condense /
 make_diassoc / tocsr / todense sequence (sparse.py:12-304). The counting runs
-in libeac_hip.so (eac_coassoc_count + eac_coassoc_fill) and is exact: the
-reference's per-row max_assocs capacity (indptr_constant / indptr_linear) and
-its silent discards (nnz_discarded, sparse.py:501-503) do not apply, so
-nnz_discarded is always 0 and the counts equal the uncapped reference. Counts
-are int32 (the reference's uint8 default wraps at 255, sparse.py:300).
-Arrays stay on the device (torch tensors); .csr builds a scipy matrix on
-request.
+in libeac_hip.so (eac_coassoc_count + eac_coassoc_fill). By default it is
+exact: the reference's per-row max_assocs capacity (indptr_constant /
+indptr_linear) and its silent discards (nnz_discarded, sparse.py:501-503) do
+not apply, so nnz_discarded is 0 and the counts equal the uncapped
+reference. With exact=False the capacities apply as in the reference
+(capped-compat mode: the same kept columns, counts and nnz_discarded as the
+reference's surgical build, for parity studies against its lossy matrix).
+Counts are int32 (the reference's uint8 default wraps at 255,
+sparse.py:300). Arrays stay on the device (torch tensors); .csr builds a
+scipy matrix on request.
 """
 from __future__ import annotations
 
@@ -62,14 +65,61 @@ def coassoc_device(labels, k_list, full_rows=False, w_offset=0):
     return indptr, indices[: nnz.value], data[: nnz.value], mx.value
 
 
+def coassoc_capped_device(labels, k_list, cap_off, full_rows=False, max_assocs=0):
+    """The reference's capacity-capped co-association (EAC_CSR with
+    max_assocs, sort_mode "surgical") on the GPU: the exact build with
+    per-entry partition ranks (eac_coassoc_fill_ranked), then the per-row
+    selection of the columns the reference keeps (eac_coassoc_cap).
+    cap_off: int64 [n+1] capacity offsets (indptr_constant / indptr_linear).
+    Returns (indptr, indices, data, max_count, nnz_discarded) with device
+    tensors; raises ValueError where the reference's first partition writes
+    past a row's capacity."""
+    t = L.torch()
+    lib = L.load_library()
+    dl = L.as_device(labels, t.int32)
+    if dl.dim() == 1:
+        dl = dl[None]
+    P, n = dl.shape
+    koff = np.zeros(P + 1, dtype=np.int64)
+    np.cumsum(np.asarray(k_list, dtype=np.int64), out=koff[1:])
+    indptr = t.empty(n + 1, dtype=t.int64, device=dl.device)
+    nnz, mx = C.c_int64(), C.c_int32()
+    ws = L.workspace(lib.eac_coassoc_workspace_bytes(n, P, int(koff[-1])))
+    fr = 1 if full_rows else 0
+    L.check(lib.eac_coassoc_count(L.ptr(dl), n, koff.ctypes.data_as(C.c_void_p), P, fr,
+                                  L.ptr(indptr), C.byref(nnz), C.byref(mx), L.ptr(ws),
+                                  ws.numel(), L.stream_ptr()))
+    m = max(nnz.value, 1)
+    ind = t.empty(m, dtype=t.int32, device=dl.device)
+    cnt = t.empty(m, dtype=t.int32, device=dl.device)
+    rank = t.empty(m, dtype=t.int32, device=dl.device)
+    L.check(lib.eac_coassoc_fill_ranked(L.ptr(dl), n, koff.ctypes.data_as(C.c_void_p), P, fr,
+                                        L.ptr(indptr), L.ptr(ind), L.ptr(cnt), L.ptr(rank),
+                                        L.ptr(ws), ws.numel(), L.stream_ptr()))
+    d_off = None if full_rows else L.as_device(np.asarray(cap_off, dtype=np.int64), t.int64)
+    cap_total = nnz.value if full_rows else min(nnz.value, int(np.asarray(cap_off)[-1]))
+    out_ip = t.empty(n + 1, dtype=t.int64, device=dl.device)
+    out_ind = t.empty(max(cap_total, 1), dtype=t.int32, device=dl.device)
+    out_dat = t.empty(max(cap_total, 1), dtype=t.int32, device=dl.device)
+    h = (C.c_int64 * 4)()
+    ws2 = t.empty(lib.eac_coassoc_cap_workspace_bytes(n), dtype=t.uint8, device=dl.device)
+    L.check(lib.eac_coassoc_cap(L.ptr(indptr), L.ptr(ind), L.ptr(cnt), L.ptr(rank), n, P, fr,
+                                L.ptr(d_off), int(max_assocs), max(cap_total, 1),
+                                L.ptr(out_ip), L.ptr(out_ind), L.ptr(out_dat), h, L.ptr(ws2),
+                                ws2.numel(), L.stream_ptr()))
+    kept = int(h[0])
+    return out_ip, out_ind[:kept], out_dat[:kept], int(h[2]), int(h[1])
+
+
 def indptr_constant(n_samples, max_assocs):
-    """sparse.py:368-369 capacity profile (kept for API compatibility; the
-    exact builder does not need capacities)."""
+    """sparse.py:368-369 capacity profile: every row holds max_assocs."""
     return np.arange(n_samples + 1, dtype=np.int64) * max_assocs
 
 
 def indptr_linear(n_samples, max_assocs, n_s, n_e, val_s, val_e):
-    """sparse.py:371-413 capacity profile (API compatibility only)."""
+    """sparse.py:371-413 capacity profile: val_s * max_assocs up to row
+    n_s * n, then a line down to val_e * max_assocs at row n_e * n (float
+    line values truncated to integers), returned as exclusive offsets."""
     x = np.arange(n_samples, dtype=np.int32)
     y = np.empty(n_samples + 1, dtype=np.int64)
     n_s, n_e = n_s * n_samples, n_e * n_samples
@@ -87,7 +137,7 @@ def indptr_linear(n_samples, max_assocs, n_s, n_e, val_s, val_e):
 
 
 class EAC_CSR:
-    """sparse.py:12-304 interface; exact GPU build."""
+    """sparse.py:12-304 interface; GPU build, exact unless exact=False."""
 
     def __init__(self, n_samples, max_assocs, max_assocs_type="constant", dtype=np.uint8,
                  condensed=False, sort_mode="numpy", **kwargs):
@@ -98,6 +148,12 @@ class EAC_CSR:
         self.condensed = condensed
         self.sort_mode = sort_mode
         self._validate_func_options()
+        # linear capacity profile arguments (sparse.py:57-61)
+        self.n_s = kwargs.get("n_s", 0.05)
+        self.n_e = kwargs.get("n_e", 1.0)
+        self.val_s = kwargs.get("val_s", 1.0)
+        self.val_e = kwargs.get("val_e", 0.05)
+        self.exact = kwargs.get("exact", True)
         self.nnz = 0
         self.nnz_discarded = 0
         self.csr_form = False
@@ -135,12 +191,27 @@ class EAC_CSR:
         labels, ks = labels_from_ensemble(ensemble, self.n_samples)
         self.update_labels(labels, ks)
 
+    def capacity_offsets(self):
+        """The reference's row capacities as offsets (_allocate_arrays,
+        sparse.py:67-80)."""
+        if self.max_assocs_type == "constant":
+            return indptr_constant(self.n_samples, self.max_assocs)
+        return indptr_linear(self.n_samples, self.max_assocs, self.n_s, self.n_e, self.val_s,
+                             self.val_e)
+
     def update_labels(self, labels, k_list):
         """Device entry: labels [P, n] (tensor or array), k per partition."""
         self._labels = labels
         self._ks = list(k_list)
-        self.indptr, self.indices, self.data, self.max_count = coassoc_device(
-            labels, self._ks, full_rows=not self.condensed)
+        if self.exact:
+            self.indptr, self.indices, self.data, self.max_count = coassoc_device(
+                labels, self._ks, full_rows=not self.condensed)
+            self.nnz_discarded = 0
+        else:
+            (self.indptr, self.indices, self.data, self.max_count,
+             self.nnz_discarded) = coassoc_capped_device(
+                labels, self._ks, self.capacity_offsets(), full_rows=not self.condensed,
+                max_assocs=self.max_assocs)
         self.nnz = int(self.indices.numel())
         self.csr_form = False
 

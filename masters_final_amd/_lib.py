@@ -75,9 +75,23 @@ SIGNATURES = {
     "eac_coassoc_workspace_bytes": (_sz, [_i64, _i32, _i64]),
     "eac_coassoc_count": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _p, _p, _p, _sz, _p]),
     "eac_coassoc_fill": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _i32, _p, _p, _p, _sz, _p]),
+    "eac_coassoc_fill_ranked": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _p, _p, _p, _p, _sz,
+                                          _p]),
+    "eac_coassoc_cap_workspace_bytes": (_sz, [_i64]),
+    "eac_coassoc_cap": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, _p, _i64, _i64, _p, _p, _p,
+                                  _p, _p, _sz, _p]),
     "eac_relabel_workspace_bytes": (_sz, [_i32, _i32]),
     "eac_canonical_relabel": (C.c_int, [_p, _i32, _i32, _p, _p, _p, _sz, _p]),
     "eac_contingency": (C.c_int, [_p, _p, _i64, _i32, _i32, _i32, _i32, _p, _p]),
+    "eac_partition_to_bin": (C.c_int, [_p, _p, _i64, _i32, _i64, _p, _p, _p]),
+    "eac_coassoc_dense": (C.c_int, [_p, _i64, _i32, _i32, _i32, _p, _p, _p, _sz, _p]),
+    "eac_dense_diassoc": (C.c_int, [_p, _i64, _i32, C.c_double, _p]),
+    "eac_dense_degree": (C.c_int, [_p, _i32, _i64, _i32, _p, _p, _p, _sz, _p]),
+    "eac_dense_sl_workspace_bytes": (_sz, [_i64, _i32, _i32]),
+    "eac_full_sl_lifetime": (C.c_int, [_p, _i32, _i64, _i32, _i32, _p, _p, _p, _p, _sz, _p]),
+    "eac_slink": (C.c_int, [_p, _i32, _i64, _i32, _p, _p, _sz, _p]),
+    "eac_labels_from_Z_workspace_bytes": (_sz, [_i64]),
+    "eac_labels_from_Z": (C.c_int, [_p, _i64, _i32, _p, _p, _sz, _p]),
 }
 
 

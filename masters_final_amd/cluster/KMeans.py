@@ -24,8 +24,9 @@ import numpy as np
 
 from .. import _lib as L
 
-EAC_KM_F32, EAC_KM_F64, EAC_KM_BF16X3 = 0, 1, 2
-_PREC = {"fp32": EAC_KM_F32, "fp64": EAC_KM_F64, "bf16x3": EAC_KM_BF16X3}
+EAC_KM_F32, EAC_KM_F64, EAC_KM_BF16X3, EAC_KM_F32_NUMPY = 0, 1, 2, 3
+_PREC = {"fp32": EAC_KM_F32, "fp64": EAC_KM_F64, "bf16x3": EAC_KM_BF16X3,
+         "numpy32": EAC_KM_F32_NUMPY}
 
 
 def _koff(ks):
@@ -130,6 +131,16 @@ def numba_label(data, centroids, dists, converge):
 
 
 py_label = numba_label
+
+
+def np_label(data, centroids, dists, converge):
+    """KMeans.py:493-522: labels in numpy's float32 arithmetic
+    (((data - c) ** 2).sum(axis=1) per centroid, strict <, lowest k on ties),
+    computed on the GPU (EAC_KM_F32_NUMPY). As in the reference, `dists` is
+    left untouched: its `dists = best_dist` (KMeans.py:519-520) rebinds a
+    local name, so the caller never sees the distances."""
+    labels, _, _ = label_device(data, centroids, precision="numpy32")
+    return labels[0].cpu().numpy()
 
 
 def numba_recompute_centroids(data, centroids, labels, dists):

@@ -8,8 +8,10 @@
 // _condense (:212-225, condense_eac_csr :336-349) packs the rows with a
 // sequential offset sum; make_diassoc (:295-311) turns counts into
 // (max + 1) - count. The reference caps each row at max_assocs and silently
-// drops associations beyond it (sparse.py:501-503, :995-997); this library
-// is exact (no cap) and its row offsets are int64.
+// drops associations beyond it (sparse.py:501-503, :995-997); the default
+// build here is exact (no cap, int64 row offsets), and a capacity-capped
+// selection (eac_coassoc_cap, at the end of this file) reproduces the
+// reference's lossy matrix from the exact one plus per-entry partition ranks.
 //
 // MI355X design:
 //  * Plan: a stable radix sort of (global cluster id, point) turns labels
@@ -111,12 +113,35 @@ struct CoPlan {
   const int32_t* cstart;  // [ktot+1] start of each global cluster in members
   const int32_t* wtab;    // window table (nullptr: search the window ends)
   int nw;                 // windows of the table: ceil(n / win)
+  const int32_t* first;   // ranked fill: the partition processed first (device)
 };
+
+// The partition the reference processes first: the one with the fewest
+// non-empty clusters, the first such (sparse.py:158: np.argmin over the
+// cluster-list lengths; clusters are the labels present, partition.py:73-86).
+// One block; cstart gives every global cluster's member count.
+__global__ __launch_bounds__(256) void k_first_partition(const int32_t* __restrict__ cstart,
+                                                         const int64_t* __restrict__ k_off,
+                                                         int P, int32_t* __restrict__ first) {
+  __shared__ unsigned long long s_best;
+  if (threadIdx.x == 0) s_best = ~0ull;
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) {
+    uint32_t nonempty = 0;
+    for (int64_t g = k_off[p]; g < k_off[p + 1]; ++g) nonempty += cstart[g + 1] > cstart[g];
+    atomicMin(&s_best, ((unsigned long long)nonempty << 32) | (unsigned)p);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) first[0] = (int32_t)(s_best & 0xffffffffu);
+}
 
 // One row: walk every partition's co-members > i (or all co-members != i for
 // full rows) window by window. PASS 0: per-row nnz and max count. PASS 1:
-// write (col, value) at out_off.
-template <int PASS>
+// write (col, value) at out_off. RANK (PASS 1 only): also write, per entry,
+// the processing rank of the first partition in which the pair co-occurs
+// (0 = the partition pl.first[0], then the others in index order), kept in a
+// second LDS window with one atomicMin per pair visit.
+template <int PASS, bool RANK = false>
 __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, int win,
                                                       unsigned long long* __restrict__ row_ctr,
                                                       int32_t* __restrict__ row_nnz,
@@ -124,17 +149,21 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
                                                       const int64_t* __restrict__ indptr,
                                                       int32_t w_offset,
                                                       int32_t* __restrict__ out_col,
-                                                      int32_t* __restrict__ out_val) {
-  // launch-time LDS: window counters (win / 2 words), then per partition
-  // the next member position, the cluster end, this window's member end and
-  // the flattened-range prefix (P + 1)
+                                                      int32_t* __restrict__ out_val,
+                                                      int32_t* __restrict__ out_rank) {
+  // launch-time LDS: window counters (win / 2 words), with RANK the rank
+  // window (win words), then per partition the next member position, the
+  // cluster end, this window's member end and the flattened-range prefix
+  // (P + 1), the row's global cluster, with RANK each partition's rank
   extern __shared__ __align__(16) uint32_t s_dyn[];
   uint32_t* s_cnt = s_dyn;
-  int32_t* s_cur = reinterpret_cast<int32_t*>(s_dyn + win / 2);
+  uint32_t* s_rank = s_dyn + win / 2;
+  int32_t* s_cur = reinterpret_cast<int32_t*>(s_dyn + win / 2 + (RANK ? win : 0));
   int32_t* s_end = s_cur + pl.P;
   int32_t* s_hi = s_end + pl.P;
   int32_t* s_pre = s_hi + pl.P;
   int32_t* s_g = s_pre + pl.P + 1;  // global cluster of the row in each partition
+  int32_t* s_prank = s_g + pl.P;
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
   __shared__ unsigned int s_max;
@@ -144,6 +173,11 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   // the window is cleared once here; afterwards every window clears the
   // words it used while reading them, so a clean window costs no pass
   for (int v = t; v < win / 2; v += kCB) s_cnt[v] = 0;
+  if (RANK) {
+    for (int v = t; v < win; v += kCB) s_rank[v] = ~0u;
+    const int f = pl.first[0];
+    for (int p = t; p < P; p += kCB) s_prank[p] = p == f ? 0 : (p < f ? p + 1 : p);
+  }
   // Rows are taken in the order of partition 0's member list (points
   // grouped by their partition-0 cluster) when every point is assigned
   // there: rows in flight at the same time then share most of their
@@ -239,6 +273,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         int p = 0;
         for (int32_t f0 = t; f0 < tot; f0 += 4 * kCB) {
           int32_t j[4];
+          int pu[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int32_t f = f0 + u * kCB;
@@ -247,12 +282,14 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
               while (s_pre[p + 1] <= f) ++p;
               j[u] = pl.members[s_cur[p] + (f - s_pre[p])];
             }
+            pu[u] = p;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (j[u] < 0 || j[u] == (int32_t)i) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
+            if (RANK) atomicMin(&s_rank[jj], (uint32_t)s_prank[pu[u]]);
           }
         }
       } else {
@@ -274,6 +311,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           const int32_t bnd = qb <= P ? s_pre[qb] : INT32_MAX;  // ascending by lane
           const bool covered = __shfl(bnd, kWave - 1, kWave) > base + 4 * kWave - 1;
           int32_t j[4];
+          int pu[4];
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             const int32_t f = base + u * kWave + ln;
@@ -288,12 +326,14 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
               while (s_pre[p + 1] <= f) ++p;
             }
             j[u] = f < tot ? pl.members[s_cur[p] + (f - s_pre[p])] : -1;
+            pu[u] = p;
           }
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
             if (j[u] < 0 || j[u] == (int32_t)i) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
+            if (RANK) atomicMin(&s_rank[jj], (uint32_t)s_prank[pu[u]]);
           }
         }
       }
@@ -374,11 +414,19 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
             if (lo) {
               out_col[at] = col;
               out_val[at] = w_offset ? w_offset - (int32_t)lo : (int32_t)lo;
+              if (RANK) {
+                out_rank[at] = (int32_t)s_rank[2 * (4 * g + q)];
+                s_rank[2 * (4 * g + q)] = ~0u;
+              }
               ++at;
             }
             if (hi) {
               out_col[at] = col + 1;
               out_val[at] = w_offset ? w_offset - (int32_t)hi : (int32_t)hi;
+              if (RANK) {
+                out_rank[at] = (int32_t)s_rank[2 * (4 * g + q) + 1];
+                s_rank[2 * (4 * g + q) + 1] = ~0u;
+              }
               ++at;
             }
           }
@@ -468,7 +516,12 @@ int co_window(int64_t n) {
   if (need < w) w = (int)need;
   return w < 64 ? 64 : (w & ~1);
 }
-inline size_t co_lds(int win, int P) { return (size_t)(win / 2 + 5 * P + 1) * 4; }
+inline size_t co_lds(int win, int P, bool rank = false) {
+  return (size_t)(win / 2 + (rank ? win + P : 0) + 5 * P + 1) * 4;
+}
+// The ranked fill keeps a 4-byte rank per column next to the counters: its
+// window is at most half as wide.
+constexpr int kWinRank = kWin / 2;
 
 // Builds the window table in the sort staging (free after the plan) when it
 // fits; returns nullptr otherwise (the rows then search their window ends).
@@ -484,16 +537,17 @@ int32_t* build_win_table(const CoBufs& B, int64_t n, int P, int64_t ktot, int wi
   return wtab;
 }
 // Every workgroup resident at once (rows are handed out dynamically).
-template <int PASS>
+template <int PASS, bool RANK = false>
 int co_grid(int win, int P) {
   int dev = 0, cus = 256, per = 1;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // allow launch-time LDS beyond 64 KB (the window of large graphs)
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_coassoc_rows<PASS>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)co_lds(win, P));
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coassoc_rows<PASS>, kCB,
-                                                   co_lds(win, P)) != hipSuccess || per < 1)
+  const size_t lds = co_lds(win, P, RANK);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_coassoc_rows<PASS, RANK>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coassoc_rows<PASS, RANK>, kCB,
+                                                   lds) != hipSuccess || per < 1)
     per = 1;
   return cus * per;
 }
@@ -555,10 +609,10 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
   const int win = co_window(n);
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr};
   int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
   k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(
-      pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr);
+      pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
   EAC_LAUNCH_CHECK();
   prof_end(pr2, s);
   EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
@@ -577,32 +631,289 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
 // Pass 2 (same ws as pass 1): indices[nnz] (int32 columns, ascending per
 // row) and data[nnz] = count, or w_offset - count when w_offset != 0
 // (make_diassoc, sparse.py:295-311, with w_offset = max + 1).
-extern "C" int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff,
-                                int32_t P, int32_t full_rows, const int64_t* indptr,
-                                int32_t w_offset, int32_t* indices, int32_t* data, void* ws,
-                                size_t ws_bytes, void* stream) {
+namespace eac {
+namespace {
+int coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P,
+                 int32_t full_rows, const int64_t* indptr, int32_t w_offset, int32_t* indices,
+                 int32_t* data, int32_t* rank, void* ws, size_t ws_bytes, hipStream_t s) {
   EAC_TRY(co_check(labels, n, h_koff, P));
   if (!indptr || !indices || !data) return fail(EAC_ERR_ARG, "coassoc: null output");
   const int64_t ktot = h_koff[P];
   size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
   if (!ws || ws_bytes < need)
     return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
-  hipStream_t s = (hipStream_t)stream;
   Arena ar(ws, ws_bytes);
   CoBufs B = carve_co(ar, n, P, ktot);
   EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
   int64_t nnz = 0;
   EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
-  const int win = co_window(n);
+  int win = co_window(n);
+  if (rank && win > kWinRank) win = kWinRank;
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw};
+  int32_t* d_first = reinterpret_cast<int32_t*>(B.misc + 2);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, d_first};
   int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
-  k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(
-      pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data);
+  if (rank) {
+    k_first_partition<<<1, 256, 0, s>>>(B.cstart, B.k_off, P, d_first);
+    EAC_LAUNCH_CHECK();
+    k_coassoc_rows<1, true><<<co_grid<1, true>(win, P), kCB, co_lds(win, P, true), s>>>(
+        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, rank);
+  } else {
+    k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(
+        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, nullptr);
+  }
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
   EAC_HIP_CHECK(hipStreamSynchronize(s));
   prof_flush();
+  return EAC_OK;
+}
+}  // namespace
+}  // namespace eac
+
+extern "C" int eac_coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                int32_t P, int32_t full_rows, const int64_t* indptr,
+                                int32_t w_offset, int32_t* indices, int32_t* data, void* ws,
+                                size_t ws_bytes, void* stream) {
+  return coassoc_fill(labels, n, h_koff, P, full_rows, indptr, w_offset, indices, data, nullptr,
+                      ws, ws_bytes, (hipStream_t)stream);
+}
+
+// Pass 2 with, per entry, the processing rank of the first partition in
+// which the pair co-occurs (the input of the capacity-capped selection).
+extern "C" int eac_coassoc_fill_ranked(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                       int32_t P, int32_t full_rows, const int64_t* indptr,
+                                       int32_t* indices, int32_t* data, int32_t* rank, void* ws,
+                                       size_t ws_bytes, void* stream) {
+  if (!rank) return fail(EAC_ERR_ARG, "coassoc: null rank output");
+  return coassoc_fill(labels, n, h_koff, P, full_rows, indptr, 0, indices, data, rank, ws,
+                      ws_bytes, (hipStream_t)stream);
+}
+
+// ------------------------------------------------ capacity-capped selection
+// The reference's EAC_CSR gives row r a fixed capacity (cap_off[r+1] -
+// cap_off[r] condensed, max_assocs with full rows) and, processing the
+// partitions in their order (the one with the fewest clusters first), adds
+// a new column to a row only while the row is below capacity; a column met
+// when the row is full is discarded, and discarded again at each later
+// co-occurrence (sparse.py:956-1040 / :866-950). Rows never shrink, so a row
+// keeps exactly the first `cap` of its columns in the order (rank of the
+// first partition with the pair, column), and each kept column ends with
+// its full count. From the exact matrix with per-entry ranks, the kept set
+// of a row is {rank < R*} plus the first `need` columns of rank R*, where
+// R* is the smallest rank whose cumulative entry count reaches cap. Full
+// rows above id 405109 keep only the first partition's entries at count 1
+// (the update returns before them, sparse.py:877-878). A first partition
+// that gives a row more entries than its capacity makes the reference write
+// past the row: reported as an error.
+namespace eac {
+namespace {
+constexpr int kCapB = 256;
+constexpr int64_t kDebugRow = 405109;
+
+__global__ __launch_bounds__(kCapB) void k_cap_select(const int64_t* __restrict__ indptr,
+                                                      const int32_t* __restrict__ rank,
+                                                      int64_t n, int P, int full_rows,
+                                                      const int64_t* __restrict__ cap_off,
+                                                      int64_t max_assocs,
+                                                      int32_t* __restrict__ kept,
+                                                      int32_t* __restrict__ rstar,
+                                                      int32_t* __restrict__ need,
+                                                      unsigned long long* __restrict__ bad_row) {
+  extern __shared__ int32_t s_hist[];  // P bins
+  __shared__ int32_t s_tmp[kCapB / kWave + 1];
+  __shared__ int32_t s_sel[2];
+  const int t = threadIdx.x;
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int64_t a = indptr[i], b = indptr[i + 1], total = b - a;
+    const int64_t cap = full_rows ? max_assocs : cap_off[i + 1] - cap_off[i];
+    const bool debug_row = full_rows && i > kDebugRow;
+    if (total <= cap && !debug_row) {  // block-uniform: the row fits
+      if (t == 0) {
+        kept[i] = (int32_t)total;
+        rstar[i] = INT32_MAX;
+        need[i] = 0;
+      }
+      continue;
+    }
+    for (int v = t; v < P; v += kCapB) s_hist[v] = 0;
+    if (t == 0) { s_sel[0] = 0; s_sel[1] = 0; }
+    __syncthreads();
+    for (int64_t e = a + t; e < b; e += kCapB) {
+      const int32_t R = rank[e];
+      if ((uint32_t)R < (uint32_t)P) atomicAdd(&s_hist[R], 1);
+      else atomicAdd(&bad_row[3], 1ull);  // a rank outside [0, P): rejected by the caller
+    }
+    __syncthreads();
+    const int32_t n0 = s_hist[0];
+    if (t == 0 && n0 > cap) atomicMin(bad_row, (unsigned long long)i);
+    if (debug_row) {
+      if (t == 0) {
+        kept[i] = n0;
+        rstar[i] = 1;  // keep rank 0 only
+        need[i] = 0;
+      }
+    } else {
+      // contiguous bins per thread, block scan of their sums
+      const int per = (P + kCapB - 1) / kCapB;
+      const int b0 = t * per, b1 = b0 + per < P ? b0 + per : P;
+      int32_t mine = 0;
+      for (int r = b0; r < b1; ++r) mine += s_hist[r];
+      int32_t tot;
+      int32_t cum = block_excl_scan<int32_t>(mine, s_tmp, &tot);
+      for (int r = b0; r < b1; ++r) {
+        if (cum < cap && cum + s_hist[r] >= cap) {
+          s_sel[0] = r;
+          s_sel[1] = (int32_t)(cap - cum);
+        }
+        cum += s_hist[r];
+      }
+      __syncthreads();
+      if (t == 0) {
+        kept[i] = (int32_t)cap;
+        rstar[i] = cap > 0 ? s_sel[0] : 0;
+        need[i] = cap > 0 ? s_sel[1] : 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kCapB) void k_cap_compact(
+    const int64_t* __restrict__ indptr, const int32_t* __restrict__ indices,
+    const int32_t* __restrict__ count, const int32_t* __restrict__ rank, int64_t n,
+    int full_rows, const int32_t* __restrict__ rstar, const int32_t* __restrict__ need,
+    const int64_t* __restrict__ out_off, int32_t* __restrict__ out_col,
+    int32_t* __restrict__ out_val, unsigned long long* __restrict__ discarded,
+    unsigned int* __restrict__ max_kept) {
+  __shared__ int32_t s_tmp[kCapB / kWave + 1];
+  const int t = threadIdx.x;
+  unsigned long long disc = 0;
+  unsigned int mx = 0;
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int64_t a = indptr[i], b = indptr[i + 1], o = out_off[i];
+    const int32_t rs = rstar[i], nd = need[i];
+    const bool debug_row = full_rows && i > kDebugRow;
+    if (rs == INT32_MAX) {  // the whole row is kept
+      for (int64_t e = a + t; e < b; e += kCapB) {
+        const int32_t v = count[e];
+        out_col[o + (e - a)] = indices[e];
+        out_val[o + (e - a)] = v;
+        mx = (unsigned)v > mx ? (unsigned)v : mx;
+      }
+      continue;
+    }
+    int32_t carry_star = 0, carry_keep = 0;
+    for (int64_t base = a; base < b; base += kCapB) {  // block-uniform trip count
+      const int64_t e = base + t;
+      const bool valid = e < b;
+      const int32_t R = valid ? rank[e] : INT32_MAX;
+      const int32_t is_star = valid && R == rs;
+      int32_t star_tot, keep_tot;
+      const int32_t idx = carry_star + block_excl_scan<int32_t>(is_star, s_tmp, &star_tot);
+      const int32_t keep = valid && (R < rs || (is_star && idx < nd));
+      const int32_t pos = carry_keep + block_excl_scan<int32_t>(keep, s_tmp, &keep_tot);
+      if (keep) {
+        const int32_t v = debug_row ? 1 : count[e];
+        out_col[o + pos] = indices[e];
+        out_val[o + pos] = v;
+        mx = (unsigned)v > mx ? (unsigned)v : mx;
+      } else if (valid && !debug_row) {
+        disc += (unsigned long long)count[e];
+      }
+      carry_star += star_tot;
+      carry_keep += keep_tot;
+    }
+  }
+  disc = wave_sum(disc);
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const unsigned int o2 = __shfl_xor(mx, off, kWave);
+    mx = o2 > mx ? o2 : mx;
+  }
+  if (lane_id() == 0) {
+    if (disc) atomicAdd(discarded, disc);
+    if (mx) atomicMax(max_kept, mx);
+  }
+}
+
+struct CapBufs {
+  int32_t* kept;
+  int32_t* rstar;
+  int32_t* need;
+  unsigned long long* misc;  // [0] bad row (min), [1] discarded, [2] max kept, [3] bad ranks
+};
+
+CapBufs carve_cap(Arena& ar, int64_t n) {
+  CapBufs c;
+  c.kept = ar.take<int32_t>(n + 1);
+  c.rstar = ar.take<int32_t>(n);
+  c.need = ar.take<int32_t>(n);
+  c.misc = ar.take<unsigned long long>(4);
+  return c;
+}
+}  // namespace
+}  // namespace eac
+
+extern "C" size_t eac_coassoc_cap_workspace_bytes(int64_t n) {
+  if (n < 1) n = 1;
+  Arena ar(nullptr, 0, true);
+  (void)carve_cap(ar, n);
+  return ar.used + scan_ws_bytes(n + 1, 8) + 4096;
+}
+
+extern "C" int eac_coassoc_cap(const int64_t* indptr, const int32_t* indices,
+                               const int32_t* count, const int32_t* rank, int64_t n, int32_t P,
+                               int32_t full_rows, const int64_t* cap_off, int64_t max_assocs,
+                               int64_t out_capacity, int64_t* out_indptr, int32_t* out_indices,
+                               int32_t* out_data, int64_t* h_out, void* ws, size_t ws_bytes,
+                               void* stream) {
+  if (n < 1 || P < 1 || P > kMaxP) return fail(EAC_ERR_ARG, "coassoc cap: need n >= 1, 1 <= P <= %d", kMaxP);
+  if (!indptr || !count || !rank || !out_indptr || (!full_rows && !cap_off))
+    return fail(EAC_ERR_ARG, "coassoc cap: null array");
+  if (full_rows && max_assocs < 0) return fail(EAC_ERR_ARG, "coassoc cap: max_assocs < 0");
+  const size_t need_ws = eac_coassoc_cap_workspace_bytes(n);
+  if (!ws || ws_bytes < need_ws)
+    return fail(EAC_ERR_WORKSPACE, "coassoc cap: workspace %zu < %zu bytes", ws_bytes, need_ws);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  CapBufs B = carve_cap(ar, n);
+  const size_t mark = ar.used;
+  uint64_t* misc64 = reinterpret_cast<uint64_t*>(B.misc);
+  EAC_TRY(fill<uint64_t>(misc64, 4, 0ull, s));
+  EAC_TRY(fill<uint64_t>(misc64, 1, ~0ull, s));
+  EAC_TRY(fill<int32_t>(B.kept + n, 1, 0, s));
+  const int g = grid_for(n, 1, 4096);
+  k_cap_select<<<g, kCapB, sizeof(int32_t) * P, s>>>(indptr, rank, n, P, full_rows, cap_off,
+                                                     max_assocs, B.kept, B.rstar, B.need, B.misc);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int64_t>(B.kept, out_indptr, n + 1, 0, nullptr, ar, s)));
+  ar.used = mark;
+  unsigned long long bad[4] = {0, 0, 0, 0};
+  int64_t total = 0;
+  EAC_TRY(d2h(bad, B.misc, sizeof(bad), s));
+  EAC_TRY(d2h(&total, out_indptr + n, sizeof(total), s));
+  if (bad[3]) return fail(EAC_ERR_ARG, "coassoc cap: %llu ranks outside [0, P)", bad[3]);
+  if (bad[0] != ~0ull)
+    return fail(EAC_ERR_VALUE,
+                "coassoc cap: the first partition gives row %llu more entries than its "
+                "capacity (the reference writes past the row)", bad[0]);
+  if (total > out_capacity)
+    return fail(EAC_ERR_ARG, "coassoc cap: %lld kept entries exceed the output capacity %lld",
+                (long long)total, (long long)out_capacity);
+  if (total > 0 && (!out_indices || !out_data))
+    return fail(EAC_ERR_ARG, "coassoc cap: null output");
+  k_cap_compact<<<g, kCapB, 0, s>>>(indptr, indices, count, rank, n, full_rows, B.rstar, B.need,
+                                    out_indptr, out_indices, out_data, B.misc + 1,
+                                    reinterpret_cast<unsigned int*>(B.misc + 2));
+  EAC_LAUNCH_CHECK();
+  unsigned long long misc[3];
+  EAC_TRY(d2h(misc, B.misc, sizeof(misc), s));
+  if (h_out) {
+    h_out[0] = total;
+    h_out[1] = (int64_t)misc[1];
+    h_out[2] = (int64_t)(unsigned int)misc[2];
+    h_out[3] = -1;
+  }
   return EAC_OK;
 }

@@ -832,6 +832,106 @@ __global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ 
   }
 }
 
+// numpy's float32 sum of one short row v[0..d), d <= 128 (numpy's add
+// reduction along the last axis: 0 + the pairwise sum of the row; a row of
+// at most PW_BLOCKSIZE = 128 elements is one leaf: fewer than 8 elements
+// summed in order from 0, otherwise eight partial sums over v[j], v[j+8],
+// ... up to the last multiple of 8, combined as ((r0+r1)+(r2+r3)) +
+// ((r4+r5)+(r6+r7)), then the remaining elements added in order). Every
+// operation is a separately rounded float32 addition.
+template <int DX>
+__device__ __forceinline__ float np_row_sum(const float* v, int d) {
+  if (d < 8) {
+    float r = 0.0f;
+#pragma unroll
+    for (int j = 0; j < (DX < 8 ? DX : 8); ++j)
+      if (j < d) r = __fadd_rn(r, v[j]);
+    return r;
+  }
+  float r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = j < DX ? v[j < DX ? j : 0] : 0.0f;
+  const int full = d - (d & 7);
+#pragma unroll
+  for (int i = 8; i + 8 <= DX; i += 8)
+    if (i < full) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = __fadd_rn(r[j], v[i + j]);
+    }
+  float res = __fadd_rn(__fadd_rn(__fadd_rn(r[0], r[1]), __fadd_rn(r[2], r[3])),
+                        __fadd_rn(__fadd_rn(r[4], r[5]), __fadd_rn(r[6], r[7])));
+#pragma unroll
+  for (int i = 0; i < DX; ++i)
+    if (i >= full && i < d) res = __fadd_rn(res, v[i]);
+  return __fadd_rn(0.0f, res);
+}
+
+// Labeling in numpy's float32 arithmetic (KMeans.py:493-522 np_label):
+// dist_c = ((x - c) ** 2).sum(axis=1) with a float32 difference, a float32
+// square and numpy's row sum (np_row_sum); a centroid replaces the best one
+// only when strictly closer (labels[dist < best_dist] = c), so the lowest k
+// wins ties. Same grid and LDS staging as k_label_direct.
+template <int DX>
+__global__ __launch_bounds__(256) void k_label_np(const float* __restrict__ X, int64_t n, int d,
+                                                  const float* __restrict__ C,
+                                                  const int64_t* __restrict__ k_off, int P,
+                                                  int32_t* __restrict__ labels,
+                                                  float* __restrict__ dists) {
+  constexpr int kTile = 16384 / (DX * 4) > 256 ? 256 : 16384 / (DX * 4);
+  __shared__ float s_c[kTile * DX];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  float xr[DX];
+#pragma unroll
+  for (int j = 0; j < DX; ++j) xr[j] = (valid && j < d) ? X[i * d + j] : 0.0f;
+  for (int p = blockIdx.y; p < P; p += gridDim.y) {
+    const int64_t kbeg = k_off[p], kend = k_off[p + 1];
+    float best = 0.0f;
+    int32_t bl = -1;
+    for (int64_t c0 = kbeg; c0 < kend; c0 += kTile) {
+      const int rows = (int)((kend - c0) < kTile ? (kend - c0) : kTile);
+      __syncthreads();
+      for (int v = threadIdx.x; v < rows * DX; v += blockDim.x) {
+        const int r = v / DX, j = v - (v / DX) * DX;
+        s_c[v] = j < d ? C[(c0 + r) * d + j] : 0.0f;
+      }
+      __syncthreads();
+      if (valid) {
+        for (int r = 0; r < rows; ++r) {
+          float sq[DX];
+#pragma unroll
+          for (int j = 0; j < DX; ++j) {
+            const float df = __fsub_rn(xr[j], s_c[r * DX + j]);
+            sq[j] = __fmul_rn(df, df);
+          }
+          const float dist = np_row_sum<DX>(sq, d);
+          if (bl < 0 || dist < best) { best = dist; bl = (int32_t)(c0 + r - kbeg); }
+        }
+      }
+    }
+    if (valid) {
+      labels[(int64_t)p * n + i] = bl;
+      if (dists) dists[(int64_t)p * n + i] = best;
+    }
+  }
+}
+
+int launch_np(const float* X, int64_t n, int d, const float* C, const int64_t* k_off, int P,
+              int32_t* labels, float* dists, hipStream_t s) {
+  const dim3 g(grid_for(n, 256), P < 65535 ? P : 65535);
+  if (d == 1) k_label_np<1><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d == 2) k_label_np<2><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 4) k_label_np<4><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 8) k_label_np<8><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 16) k_label_np<16><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 32) k_label_np<32><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 64) k_label_np<64><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else if (d <= 128) k_label_np<128><<<g, 256, 0, s>>>(X, n, d, C, k_off, P, labels, dists);
+  else return fail(EAC_ERR_UNSUPPORTED, "kmeans numpy arithmetic: d=%d > 128", d);
+  EAC_LAUNCH_CHECK();
+  return EAC_OK;
+}
+
 // numpy's float32 sum of each partition's dists: the inertia of
 // KMeans.py:164 (new_inertia = self._dists.sum()). numpy's add reduction
 // starts from 0 and visits the array in chunks of its buffer size (8192
@@ -1241,9 +1341,12 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
     EAC_HIP_CHECK(hipStreamSynchronize(s));  // h_koff may be a pageable stack array
   }
   if (n_rescored) *n_rescored = 0;
-  if (mode == EAC_KM_F32 || mode == EAC_KM_F64 || (mode == EAC_KM_BF16X3 && d < 16)) {
+  if (mode == EAC_KM_F32 || mode == EAC_KM_F64 || mode == EAC_KM_F32_NUMPY ||
+      (mode == EAC_KM_BF16X3 && d < 16)) {
     int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
-    if (mode == EAC_KM_F32)
+    if (mode == EAC_KM_F32_NUMPY)
+      EAC_TRY(launch_np(X, n, d, C, B.k_off, P, labels, dists, s));
+    else if (mode == EAC_KM_F32)
       EAC_TRY(launch_direct<float>(X, n, d, C, B.k_off, P, labels, dists, s));
     else
       EAC_TRY(launch_direct<double>(X, n, d, C, B.k_off, P, labels, dists, s));

@@ -51,11 +51,13 @@ struct ProfSlot {
   int64_t launches = 0;
   int64_t bytes = 0;
 };
-static bool g_prof = false;
+// Profiler state is per host thread: a thread's records, totals and event
+// pool are its own, so calls from several host threads never share them.
+static thread_local bool g_prof = false;
 static thread_local bool g_prof_suspended = false;  // set while a hipGraph is captured
-static ProfSlot g_slots[EAC_PROF_SLOTS];
-static std::vector<ProfRec> g_recs;
-static std::vector<hipEvent_t> g_pool;
+static thread_local ProfSlot g_slots[EAC_PROF_SLOTS];
+static thread_local std::vector<ProfRec> g_recs;
+static thread_local std::vector<hipEvent_t> g_pool;
 
 bool prof_on() { return g_prof && !g_prof_suspended; }
 
@@ -129,7 +131,8 @@ extern "C" int eac_abi_version(void) { return 1; }
 
 // ------------------------------------------------------------ tuning knobs
 namespace eac {
-static int g_tune[EAC_TUNE_COUNT] = {0};
+// Per host thread: a setting applies to the calls its own thread makes.
+static thread_local int g_tune[EAC_TUNE_COUNT] = {0};
 int tune_value(int knob) { return (knob >= 0 && knob < EAC_TUNE_COUNT) ? g_tune[knob] : 0; }
 }  // namespace eac
 

@@ -36,6 +36,68 @@ def _group_by_label(labels):
     return order, vals
 
 
+def _to_bin_device(rows, cols, k, N):
+    """k x N uint8 device matrix with ones at (rows[t], cols[t]) (cols None:
+    column t). Raises IndexError, as numpy's fancy assignment does, when an
+    entry falls outside the matrix."""
+    t = L.torch()
+    lib = L.load_library()
+    d_rows = L.as_device(rows, t.int32)
+    d_cols = None if cols is None else L.as_device(cols, t.int64)
+    m = d_rows.numel()
+    out = t.empty((k, N), dtype=t.uint8, device=d_rows.device if m else L.device())
+    bad = t.zeros(1, dtype=t.int64, device=out.device)
+    L.check(lib.eac_partition_to_bin(L.ptr(d_rows) if m else None, L.ptr(d_cols), m, k, N,
+                                     L.ptr(out), L.ptr(bad), L.stream_ptr()))
+    nbad = int(bad.item())
+    if nbad:
+        raise IndexError(f"{nbad} cluster/sample indices outside the {k} x {N} matrix")
+    return out
+
+
+def index_to_bin_device(clusts):
+    """convertIndexToBin on the GPU, result left on the device."""
+    if clusts is None:
+        raise Exception("A clustering partition must be provided.")
+    sizes = [int(np.asarray(c).size) for c in clusts]
+    N, k = sum(sizes), len(clusts)
+    if N == 0:
+        return _to_bin_device(np.zeros(0, np.int32), np.zeros(0, np.int64), k, 0)
+    rows = np.repeat(np.arange(k, dtype=np.int32), sizes)
+    cols = np.concatenate([np.asarray(c).astype(np.int64).reshape(-1) for c in clusts])
+    return _to_bin_device(rows, cols, k, N)
+
+
+def convertIndexToBin(clusts):
+    """partition.py:17-35: list of index arrays -> k x N uint8 binary matrix
+    (N = number of listed indices), built on the GPU."""
+    return index_to_bin_device(clusts).cpu().numpy()
+
+
+def string_to_bin_device(clusts):
+    """convertClusterStringToBin on the GPU, result left on the device."""
+    if clusts is None:
+        raise Exception("A clustering partition must be provided.")
+    t = L.torch()
+    d = L.as_device(np.asarray(clusts).astype(np.int64) if not L.is_torch(clusts) else clusts,
+                    t.int64)
+    u = t.unique(d)
+    mx, mn = int(d.max().item()), int(d.min().item())
+    if u.numel() not in (mx, mx + 1):
+        raise ValueError('cluster numbers go from [0, biggest_clust[ or'
+                         '[1, biggest_clust[ uninterrupted')
+    offset = -1 if mn == 1 else 0
+    return _to_bin_device((d + offset).to(t.int32), None, u.numel(), d.numel())
+
+
+def convertClusterStringToBin(clusts):
+    """partition.py:37-70: label array -> k x N uint8 binary matrix (rows are
+    labels, shifted down by one when the smallest label is 1), built on the
+    GPU. A gapped numbering the reference's check lets through (e.g. {0, 2})
+    raises IndexError, as the reference's assignment does."""
+    return string_to_bin_device(clusts).cpu().numpy()
+
+
 def convertClusterStringToIndex(partition):
     """partition.py:73-86: label array -> list of uint64 index arrays, one per
     distinct label in ascending label order, members ascending."""

@@ -584,6 +584,337 @@ int64_t orc_coassoc_condensed(const int64_t* labels, const int64_t* kp, int64_t 
   return nnz;
 }
 
+/* binary_search_interval, MyML/MyML/EAC/sparse.py:1069-1099: index of key in
+ * the sorted ary[start:end), else -(insertion point) - 1. */
+static int64_t orc_bsearch_interval(int64_t key, const int64_t* ary, int64_t start, int64_t end) {
+  int64_t imin = start, imax = end;
+  while (imin < imax) {
+    int64_t imid = (imax + imin) / 2;
+    if (key < ary[imid]) imax = imid;
+    else if (key > ary[imid]) imin = imid + 1;
+    else return imid;
+  }
+  return -imin - 1;
+}
+
+/* The reference's capacity-capped co-association, EAC_CSR with its default
+ * sort_mode "surgical" (MyML/MyML/EAC/sparse.py):
+ *  - storage: one flat indices/data pair with row r owning
+ *    [cap_off[r], cap_off[r+1]) (indptr_constant :368-369 / indptr_linear
+ *    :371-413, computed by the caller) and degree[r] entries in use;
+ *  - update_ensemble (:154-166): the partition with the fewest clusters
+ *    (np.argmin of the cluster-list lengths, first minimum; clusters are the
+ *    distinct labels present, partition.py:73-86) goes first, then every
+ *    other partition in order;
+ *  - first partition: update_cluster_fp_condensed (:445-465) writes count 1
+ *    for every later member of the cluster (update_cluster_fp :425-443 in
+ *    full mode: every other member), with no capacity check;
+ *  - other partitions: update_cluster_sorted_surgical_condensed (:956-1040)
+ *    / update_cluster_sorted_surgical (:866-950, full mode): for each row, a
+ *    co-member found in the row (binary search over the entries the row had
+ *    before this cluster) gets +1; a new one is queued while the row's
+ *    degree is below its capacity (cap_off[r+1] - cap_off[r] condensed,
+ *    max_assocs in full mode) and otherwise discarded and counted; the
+ *    queued ones are merged into the row keeping it sorted. In full mode a
+ *    row id above 405109 returns from the cluster (:877-878);
+ *  - _condense (:212-225, condense_eac_csr :336-349) packs the rows.
+ * labels: P x n (-1 = not listed in that partition). Outputs: indptr[n+1],
+ * indices/counts (room for cap_off[n] entries), *nnz, *discarded (the sum of
+ * the per-cluster discard counts the surgical update returns, i.e. every
+ * discarded co-occurrence). Returns ORC_ERR_VALUE, with *bad_row set, when
+ * the first partition gives a row more entries than its capacity: the
+ * reference then writes past the row into the next row's storage, which is
+ * not restated. */
+int orc_coassoc_capped(const int64_t* labels, int64_t P, int64_t n, int full_rows,
+                       const int64_t* cap_off, int64_t max_assocs, int64_t* indptr,
+                       int64_t* indices, int64_t* counts, int64_t* nnz_out,
+                       int64_t* discarded_out, int64_t* bad_row) {
+  const int64_t total = cap_off[n];
+  int64_t* ind = malloc(sizeof(int64_t) * (total > 0 ? total : 1));
+  int64_t* dat = malloc(sizeof(int64_t) * (total > 0 ? total : 1));
+  int64_t* deg = calloc(n > 0 ? n : 1, sizeof(int64_t));
+  int64_t kmax = 0;
+  for (int64_t t = 0; t < P * n; ++t) kmax = labels[t] + 1 > kmax ? labels[t] + 1 : kmax;
+  int64_t* cnt = malloc(sizeof(int64_t) * (kmax + 1));
+  int64_t* off = malloc(sizeof(int64_t) * (kmax + 1));
+  int64_t* mem = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* q_col = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* q_at = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* tmp_i = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* tmp_d = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  /* first partition: fewest distinct labels present, first minimum */
+  int64_t first = 0, best = -1;
+  for (int64_t p = 0; p < P; ++p) {
+    memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
+    int64_t distinct = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      int64_t l = labels[p * n + i];
+      if (l >= 0 && cnt[l]++ == 0) ++distinct;
+    }
+    if (best < 0 || distinct < best) { best = distinct; first = p; }
+  }
+  int rc = ORC_OK;
+  int64_t discarded = 0;
+  *bad_row = -1;
+  for (int64_t step = 0; step < P && rc == ORC_OK; ++step) {
+    int64_t p = step == 0 ? first : (step <= first ? step - 1 : step);
+    const int64_t* lab = labels + p * n;
+    memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
+    for (int64_t i = 0; i < n; ++i) if (lab[i] >= 0) cnt[lab[i]]++;
+    off[0] = 0;
+    for (int64_t c = 0; c < kmax; ++c) off[c + 1] = off[c] + cnt[c];
+    memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
+    for (int64_t i = 0; i < n; ++i) if (lab[i] >= 0) mem[off[lab[i]] + cnt[lab[i]]++] = i;
+    for (int64_t c = 0; c < kmax && rc == ORC_OK; ++c) {
+      const int64_t* S = mem + off[c];
+      const int64_t s = off[c + 1] - off[c];
+      if (s == 0) continue;  /* absent label: no cluster in the list format */
+      if (step == 0) {
+        for (int64_t a = 0; a < s; ++a) {
+          const int64_t r = S[a], fa = cap_off[r];
+          int64_t add = 0;
+          for (int64_t b = full_rows ? 0 : a + 1; b < s; ++b) {
+            if (b == a) continue;
+            if (fa + add >= cap_off[r + 1]) { rc = ORC_ERR_VALUE; *bad_row = r; break; }
+            ind[fa + add] = S[b];
+            dat[fa + add] = 1;
+            ++add;
+          }
+          if (rc != ORC_OK) break;
+          deg[r] = add;
+        }
+        continue;
+      }
+      for (int64_t a = 0; a < (full_rows ? s : s - 1); ++a) {
+        const int64_t r = S[a];
+        if (full_rows && r > 405109) break;
+        const int64_t fa = cap_off[r], n_degree = deg[r], la = fa + n_degree;
+        const int64_t cap = full_rows ? max_assocs : cap_off[r + 1] - fa;
+        int64_t new_deg = n_degree, nq = 0;
+        for (int64_t b = full_rows ? 0 : a + 1; b < s; ++b) {
+          if (b == a) continue;
+          const int64_t col = S[b];
+          const int64_t at = orc_bsearch_interval(col, ind, fa, la);
+          if (at >= 0) { dat[at] += 1; continue; }
+          if (new_deg >= cap) { ++discarded; continue; }
+          q_col[nq] = col;
+          q_at[nq] = -(at + 1);
+          ++nq;
+          ++new_deg;
+        }
+        if (nq == 0) continue;
+        /* merge the queued columns into the row (sorted), as the shifting
+         * loop of :1015-1038 does */
+        int64_t o = 0, e = fa;
+        for (int64_t t = 0; t < nq; ++t) {
+          while (e < q_at[t]) { tmp_i[o] = ind[e]; tmp_d[o] = dat[e]; ++o; ++e; }
+          tmp_i[o] = q_col[t]; tmp_d[o] = 1; ++o;
+        }
+        while (e < la) { tmp_i[o] = ind[e]; tmp_d[o] = dat[e]; ++o; ++e; }
+        memcpy(ind + fa, tmp_i, sizeof(int64_t) * o);
+        memcpy(dat + fa, tmp_d, sizeof(int64_t) * o);
+        deg[r] = new_deg;
+      }
+    }
+  }
+  if (rc == ORC_OK) {
+    int64_t nnz = 0;
+    indptr[0] = 0;
+    for (int64_t r = 0; r < n; ++r) {
+      memcpy(indices + nnz, ind + cap_off[r], sizeof(int64_t) * deg[r]);
+      memcpy(counts + nnz, dat + cap_off[r], sizeof(int64_t) * deg[r]);
+      nnz += deg[r];
+      indptr[r + 1] = nnz;
+    }
+    *nnz_out = nnz;
+    *discarded_out = discarded;
+  }
+  free(ind); free(dat); free(deg); free(cnt); free(off); free(mem);
+  free(q_col); free(q_at); free(tmp_i); free(tmp_d);
+  return rc;
+}
+
+/* ------------------------------------------------ dense EAC + single link */
+
+/* condensed_index, MyML/MyML/EAC/full.py:168-181 (SciPy's): (i, j), i != j. */
+static int64_t orc_cidx(int64_t n, int64_t i, int64_t j) {
+  if (i > j) { int64_t t = i; i = j; j = t; }
+  return n * i - (i * (i + 1)) / 2 + (j - i - 1);
+}
+
+/* EAC_FULL (MyML/MyML/EAC/full.py:8-131) update_ensemble: every partition,
+ * every cluster (labels present, ascending), every pair of members i < j
+ * (positions) gets +1: condensed (numba_update_condensed_coassoc_with_cluster
+ * :155-166) or both (i, j) and (j, i) of the n x n matrix
+ * (numba_update_coassoc_with_cluster :141-153). labels: P x n (-1: not
+ * listed). out: n(n-1)/2 or n*n counts, zeroed by the caller. */
+void orc_coassoc_full(const int64_t* labels, int64_t P, int64_t n, int condensed,
+                      int64_t* out) {
+  for (int64_t p = 0; p < P; ++p) {
+    const int64_t* lab = labels + p * n;
+    for (int64_t i = 0; i < n; ++i) {
+      if (lab[i] < 0) continue;
+      for (int64_t j = i + 1; j < n; ++j) {
+        if (lab[j] != lab[i]) continue;
+        if (condensed) out[orc_cidx(n, i, j)] += 1;
+        else { out[i * n + j] += 1; out[j * n + i] += 1; }
+      }
+    }
+  }
+}
+
+/* argsort1D, scipy_numba/scipy_numba/cluster/_argsort.py:6-64: quicksort
+ * with the first element as pivot (partition :6-29) and an explicit stack of
+ * (left, right) ranges, pushing the left part, then the right part, popping
+ * the last pushed. Its order under ties is a property of this procedure, so
+ * it is restated step by step. */
+static int64_t orc_qs_partition(const double* values, int64_t* idxs, int64_t left,
+                                int64_t right) {
+  double piv = values[idxs[left]];
+  int64_t i = left + 1, j = right;
+  for (;;) {
+    while (i <= j && values[idxs[i]] <= piv) ++i;
+    while (j >= i && values[idxs[j]] >= piv) --j;
+    if (j <= i) break;
+    int64_t t = idxs[i]; idxs[i] = idxs[j]; idxs[j] = t;
+  }
+  int64_t t = idxs[left]; idxs[left] = idxs[j]; idxs[j] = t;
+  return j;
+}
+
+void orc_argsort1d(const double* values, int64_t n, int64_t* idxs) {
+  for (int64_t i = 0; i < n; ++i) idxs[i] = i;
+  if (n < 2) return;
+  int64_t* stk = malloc(sizeof(int64_t) * 2 * (n + 1));
+  int64_t nd = 0;
+  stk[0] = 0; stk[1] = n - 1; nd = 1;
+  while (nd > 0) {
+    --nd;
+    int64_t left = stk[2 * nd], right = stk[2 * nd + 1];
+    int64_t piv = orc_qs_partition(values, idxs, left, right);
+    if (piv - 1 > left) { stk[2 * nd] = left; stk[2 * nd + 1] = piv - 1; ++nd; }
+    if (piv + 1 < right) { stk[2 * nd] = piv + 1; stk[2 * nd + 1] = right; ++nd; }
+  }
+  free(stk);
+}
+
+/* The SLINK algorithm (R. Sibson, 1973) as
+ * scipy_numba/scipy_numba/cluster/_hierarchy_eac.py:884-927 writes it, then
+ * from_pointer_representation (:842-881, argsort1D order) and
+ * calculate_cluster_sizes (:329-356). dists: condensed, carried in double
+ * (the reference's arrays take the weights' dtype, where np.inf of
+ * Lambda[0] does not fit an integer type; every integer and float32 weight
+ * is exact in double). Z: (n-1) x 4 (left, right, height, size). */
+void orc_slink(const double* dists, int64_t n, double* Z) {
+  double* M = malloc(sizeof(double) * (n > 0 ? n : 1));
+  double* Lambda = malloc(sizeof(double) * (n > 0 ? n : 1));
+  int64_t* Pi = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  Pi[0] = 0;
+  Lambda[0] = INFINITY;
+  for (int64_t i = 1; i < n; ++i) {
+    Pi[i] = i;
+    Lambda[i] = INFINITY;
+    for (int64_t j = 0; j < i; ++j) M[j] = dists[orc_cidx(n, i, j)];
+    for (int64_t j = 0; j < i; ++j) {
+      if (Lambda[j] >= M[j]) {
+        M[Pi[j]] = M[Pi[j]] < Lambda[j] ? M[Pi[j]] : Lambda[j];
+        Lambda[j] = M[j];
+        Pi[j] = i;
+      } else {
+        M[Pi[j]] = M[Pi[j]] < M[j] ? M[Pi[j]] : M[j];
+      }
+    }
+    for (int64_t j = 0; j < i; ++j)
+      if (Lambda[j] >= Lambda[Pi[j]]) Pi[j] = i;
+  }
+  int64_t* sorted_idx = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* node_ids = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  orc_argsort1d(Lambda, n, sorted_idx);
+  for (int64_t i = 0; i < n; ++i) node_ids[i] = i;
+  for (int64_t i = 0; i + 1 < n; ++i) {
+    int64_t leaf = sorted_idx[i], pi = Pi[leaf];
+    if (node_ids[leaf] < node_ids[pi]) {
+      Z[4 * i] = (double)node_ids[leaf];
+      Z[4 * i + 1] = (double)node_ids[pi];
+    } else {
+      Z[4 * i] = (double)node_ids[pi];
+      Z[4 * i + 1] = (double)node_ids[leaf];
+    }
+    Z[4 * i + 2] = Lambda[leaf];
+    node_ids[pi] = n + i;
+  }
+  for (int64_t i = 0; i + 1 < n; ++i) {
+    int64_t l = (int64_t)Z[4 * i], r = (int64_t)Z[4 * i + 1];
+    double cs = 0.0;
+    cs += l >= n ? Z[4 * (l - n) + 3] : 1.0;
+    cs += r >= n ? Z[4 * (r - n) + 3] : 1.0;
+    Z[4 * i + 3] = cs;
+  }
+  free(M); free(Lambda); free(Pi); free(sorted_idx); free(node_ids);
+}
+
+/* labels_from_Z_numba, MyML/MyML/cluster/linkage.py:303-357 (the function
+ * eac.py imports as labels_from_Z): every sample starts as its own cluster,
+ * the first n - n_clusters rows of Z merge their two clusters into cluster
+ * n + row, and the final clusters are numbered in order of first
+ * occurrence over the samples. */
+void orc_labels_from_Z(const double* Z, int64_t n, int64_t n_clusters, int64_t* labels) {
+  int64_t* track = labels;
+  for (int64_t i = 0; i < n; ++i) track[i] = i;
+  for (int64_t zp = 0; zp != n - n_clusters && zp + 1 < n; ++zp) {
+    int64_t c0 = (int64_t)Z[4 * zp], c1 = (int64_t)Z[4 * zp + 1];
+    for (int64_t i = 0; i < n; ++i)
+      if (track[i] == c0 || track[i] == c1) track[i] = n + zp;
+  }
+  int64_t* key = malloc(sizeof(int64_t) * (n_clusters > 0 ? n_clusters : 1));
+  for (int64_t k = 0; k < n_clusters; ++k) key[k] = -1;
+  for (int64_t l = 0; l < n; ++l) {
+    int64_t c = track[l], k = 0;
+    for (; k < n_clusters; ++k)
+      if (key[k] == c || key[k] == -1) break;
+    if (k < n_clusters && key[k] == -1) key[k] = c;
+    track[l] = k;
+  }
+  free(key);
+}
+
+/* full_sl_lifetime, MyML/MyML/EAC/eac.py:337-364: SLINK on the condensed
+ * dissimilarities, the lifetime rule (lifetime_n_clusters :366-391) on the
+ * float32 heights Z[:, 2] unless n_clusters is given, labels from Z when
+ * more than one cluster remains (empty labels otherwise). Returns
+ * ORC_ERR_VALUE where numpy raises (fewer than two heights; no lifetime
+ * close to zero). */
+int orc_full_sl_lifetime(const double* dists, int64_t n, int64_t n_clusters, double* Z,
+                         int64_t* labels, int64_t* nc_out, int64_t* labels_empty) {
+  orc_slink(dists, n, Z);
+  int64_t nc = n_clusters;
+  if (n_clusters == 0) {
+    if (n < 3) return ORC_ERR_VALUE;  /* np.argmax of an empty array */
+    float best = 0.0f;
+    int64_t m = 0;
+    int have_zero = 0;
+    double minimum = 0.0;
+    for (int64_t i = 0; i + 2 < n; ++i) {
+      float lt = (float)Z[4 * (i + 1) + 2] - (float)Z[4 * i + 2];
+      if (i == 0 || lt > best) { best = lt; m = i; }
+      if (fabs((double)lt) <= 1e-8) {
+        if (!have_zero || (double)lt < minimum) minimum = (double)lt;
+        have_zero = 1;
+      }
+    }
+    if (!have_zero) return ORC_ERR_VALUE;  /* np.min of an empty selection */
+    float wm = (float)Z[4 * m + 2];
+    int64_t above = 0;
+    for (int64_t i = 0; i + 1 < n; ++i) above += (float)Z[4 * i + 2] > wm;
+    nc = above == 0 ? 1 : above + 1;
+    if ((double)best < 2.0 * minimum) nc = 1;
+  }
+  *nc_out = nc;
+  *labels_empty = nc > 1 ? 0 : 1;
+  if (nc > 1) orc_labels_from_Z(Z, n, nc, labels);
+  return ORC_OK;
+}
+
 /* ------------------------------------------------------------- K-Means */
 
 /* MyML/MyML/cluster/KMeans.py:453-490 py_label / numba_label as the FP64

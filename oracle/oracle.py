@@ -37,6 +37,13 @@ def lib() -> C.CDLL:
             "orc_sp_sl_lifetime": (C.c_int, [_P, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _P]),
             "orc_sl_mst_lifetime": (C.c_int, [_P, _P, _P, _I, _I, _D, _P, _P, _P]),
             "orc_coassoc_condensed": (_I, [_P, _P, _I, _I, _P, _P, _P, _I]),
+            "orc_coassoc_capped": (C.c_int, [_P, _I, _I, C.c_int, _P, _I, _P, _P, _P, _P, _P,
+                                             _P]),
+            "orc_coassoc_full": (None, [_P, _I, _I, C.c_int, _P]),
+            "orc_argsort1d": (None, [_P, _I, _P]),
+            "orc_slink": (None, [_P, _I, _P]),
+            "orc_labels_from_Z": (None, [_P, _I, _I, _P]),
+            "orc_full_sl_lifetime": (C.c_int, [_P, _I, _I, _P, _P, _P, _P]),
             "orc_kmeans_label_f64": (None, [_P, _P, _I, _I, _I, _P, _P]),
             "orc_arg_k_select": (None, [_P, _I, _I, _P]),
             "orc_recompute_centroids": (None, [_P, _P, _P, _I, _I, _I, C.c_int, _P]),
@@ -198,6 +205,75 @@ def coassoc_condensed(labels, kp):
     nnz = lib().orc_coassoc_condensed(_p(L), _p(kp), P, n, _p(indptr), _p(ind), _p(cnt), cap)
     assert nnz >= 0
     return indptr, ind[:nnz], cnt[:nnz]
+
+
+def coassoc_capped(labels, cap_off, full_rows=False, max_assocs=0):
+    """The reference's capacity-capped surgical build (orc_coassoc_capped).
+    labels (P, n), -1 = not listed; cap_off int64 [n+1] capacity offsets.
+    Returns (indptr, indices, counts, discarded); raises ValueError (with the
+    row) when the first partition overflows a row."""
+    L = _a(labels, np.int64)
+    off = _a(cap_off, np.int64)
+    P, n = L.shape
+    tot = max(int(off[-1]), 1)
+    indptr = np.zeros(n + 1, np.int64)
+    ind = np.zeros(tot, np.int64)
+    cnt = np.zeros(tot, np.int64)
+    nnz, disc, bad = C.c_int64(), C.c_int64(), C.c_int64()
+    rc = lib().orc_coassoc_capped(_p(L), P, n, 1 if full_rows else 0, _p(off), int(max_assocs),
+                                  _p(indptr), _p(ind), _p(cnt), C.byref(nnz), C.byref(disc),
+                                  C.byref(bad))
+    if rc != 0:
+        raise ValueError(f"first partition overflows row {bad.value}")
+    return indptr, ind[:nnz.value], cnt[:nnz.value], int(disc.value)
+
+
+# ------------------------------------------------- dense EAC + single link
+def coassoc_full(labels, condensed=True):
+    """EAC_FULL counts (full.py:141-166): condensed n(n-1)/2 or n x n int64."""
+    L = _a(labels, np.int64)
+    P, n = L.shape
+    out = np.zeros(n * (n - 1) // 2 if condensed else n * n, np.int64)
+    lib().orc_coassoc_full(_p(L), P, n, 1 if condensed else 0, _p(out))
+    return out if condensed else out.reshape(n, n)
+
+
+def argsort1d(values):
+    """scipy_numba's argsort1D (quicksort, first-element pivot)."""
+    v = _a(values, np.float64)
+    out = np.zeros(v.size, np.int64)
+    lib().orc_argsort1d(_p(v), v.size, _p(out))
+    return out
+
+
+def slink(dists, n):
+    """SLINK + from_pointer_representation: Z (n-1, 4) float64."""
+    d = _a(dists, np.float64)
+    Z = np.zeros((max(n - 1, 1), 4), np.float64)
+    lib().orc_slink(_p(d), n, _p(Z))
+    return Z[: n - 1]
+
+
+def labels_from_Z(Z, n_clusters):
+    Z = _a(Z, np.float64)
+    n = Z.shape[0] + 1
+    lab = np.zeros(n, np.int64)
+    lib().orc_labels_from_Z(_p(Z), n, int(n_clusters), _p(lab))
+    return lab
+
+
+def full_sl_lifetime(dists, n, n_clusters=0):
+    """eac.py:337-364 on condensed dissimilarities: (nc, labels, Z); labels
+    empty for one cluster; ValueError where numpy raises."""
+    d = _a(dists, np.float64)
+    Z = np.zeros((max(n - 1, 1), 4), np.float64)
+    lab = np.zeros(n, np.int64)
+    nc, empty = C.c_int64(), C.c_int64()
+    rc = lib().orc_full_sl_lifetime(_p(d), n, int(n_clusters), _p(Z), _p(lab), C.byref(nc),
+                                    C.byref(empty))
+    if rc != 0:
+        raise ValueError("lifetime rule: no lifetime close to zero (numpy raises)")
+    return int(nc.value), (np.zeros(0, np.int64) if empty.value else lab), Z[: n - 1]
 
 
 # ---------------------------------------------------------------- K-Means

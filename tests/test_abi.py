@@ -12,7 +12,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def _declared():
     text = (ROOT / "include" / "eac_hip.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(eac_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(eac_[A-Za-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_entry_points():
