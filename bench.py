@@ -6,12 +6,16 @@ k-NN graph (k=20, int32 weights uniform in [1,100], 10% of the vertices in 4
 separate islands, seed 1). One step = one call of the forest + component-label
 path, inputs resident in HBM; value = directed CSR entries / step time
 (Medges/s). The CPU baseline is the single-threaded C restatement of the
-reference's boruvka_minho_seq on the same graph. The same JSON line carries
-the end-to-end EAC comparison on the 10,000-point workload (batched ensemble +
-BF16 path vs plain per-call FP32 path, exact labels).
+reference's boruvka_minho_seq on the same graph, with an all-core OpenMP
+Borůvka line beside it. The same JSON line carries the end-to-end EAC
+comparison on the 10,000-point workload (batched ensemble vs plain per-call
+FP32 path, exact labels) and, unless --no-extras, every other workload as an
+extra key with its own roofline and CPU baseline: the same 1M-vertex graph at
+mean degree ~40 (k=37), the 4,194,304 x 64 x 2048 labeling, the 500,000-point
+256-partition ensemble + co-association, and the 512-graph batched cut.
 
-Other workloads (--workload): eac, kmeans, coassoc, sl_batched. Each prints
-one JSON line with the same top-level keys.
+Other workloads (--workload): eac, kmeans, coassoc, sl_batched, accuracy.
+Each prints one JSON line with the same top-level keys.
 """
 from __future__ import annotations
 
@@ -47,6 +51,23 @@ def profile_traffic(kernel):
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs, the CPUs this process
+    may use, OMP_NUM_THREADS and the CPU model (/proc/cpuinfo)."""
+    import os
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
 
 def read_prof(lib, L, slots):
@@ -113,6 +134,10 @@ def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu
     rb, tb = timed(batched, reps, 1, torch)
     rp, tp = timed(plain, reps, 1, torch)
     out = {"n": n, "partitions": P, "k_range": [kmin, kmax], "iters": iters,
+           "batched_labeling": "FP64 direct difference: d = 2 is below the d >= 16 where the "
+                               "split-BF16 MFMA path applies, so precision 'bf16x3' runs the "
+                               "FP64 labeler here; the gain is from batching the partitions",
+           "plain_labeling": "FP32 direct difference, one K-Means call per partition",
            "batched_bf16_ms": tb, "plain_fp32_ms": tp, "speedup": tp / tb,
            "batched_pts_per_s": n / (tb * 1e-3), "plain_pts_per_s": n / (tp * 1e-3),
            "n_clusters": rb["n_clusters"], "nnz": rb["nnz"], "rescored": rb["n_rescored"]}
@@ -138,6 +163,50 @@ def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu
 
 # --------------------------------------------------------------- workloads
 def run_mst(args):
+    out = mst_case(args, args.knn)
+    if not args.no_eac:
+        log("end-to-end EAC: batched+BF16 vs plain per-call FP32 (n=10,000, P=100)")
+        out["eac_small"] = eac_compare(cpu=not args.no_cpu)
+        log(f"EAC speedup {out['eac_small']['speedup']:.2f}x")
+    if not args.no_extras:
+        out.update(extras(args))
+    out["host"] = host_info()
+    return out
+
+
+_LINE_KEYS = ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling",
+              "vs_baseline", "data")
+
+
+def _compact(line, steps, warmup):
+    """An extra workload's line without the keys the headline line holds."""
+    d = {k: v for k, v in line.items() if k not in _LINE_KEYS}
+    d["steps"], d["warmup"] = steps, warmup
+    return d
+
+
+def extras(args):
+    """The other benchmark workloads, each with its own roofline and CPU
+    baseline, as extra keys of the default line: the 1M-vertex graph at mean
+    degree ~40 (k=37), the 4,194,304 x 64 x 2048 labeling, the 500,000-point
+    256-partition ensemble + co-association, the 512-graph batched cut."""
+    import argparse as _ap
+    ns = _ap.Namespace(**vars(args))
+    ns.n = 1_000_000  # each workload's own default size
+    ns.steps, ns.warmup = min(args.steps, 5), 1
+    out = {}
+    for key, fn in (("mst_knn_deg40", lambda a: mst_case(a, 37)),
+                    ("kmeans_4m_d64_k2048", run_kmeans),
+                    ("coassoc_500k_p256", run_coassoc),
+                    ("sl_batched_512_knn", run_sl_batched)):
+        log(f"extra workload {key}")
+        t0 = time.perf_counter()
+        out[key] = _compact(fn(ns), ns.steps, ns.warmup)
+        log(f"extra workload {key} done in {time.perf_counter() - t0:.1f}s")
+    return out
+
+
+def mst_case(args, knn):
     import numpy as np
     import torch
 
@@ -146,9 +215,9 @@ def run_mst(args):
     from masters_final_amd.graph.mst import boruvka_mst
 
     lib = L.load_library()
-    log(f"generating k-NN graph n={args.n} k={args.knn}")
+    log(f"generating k-NN graph n={args.n} k={knn}")
     t0 = time.perf_counter()
-    g = synth.knn_graph(n=args.n, k=args.knn, seed=1)
+    g = synth.knn_graph(n=args.n, k=knn, seed=1)
     n, m = g["n"], int(g["dest"].size)
     log(f"graph: {n} vertices, {m} directed entries ({m / n:.2f}/vertex), "
         f"{time.perf_counter() - t0:.1f}s")
@@ -194,7 +263,7 @@ def run_mst(args):
     total_w = int(r["weights"][: r["n_mst"]].to(torch.int64).sum().item())
     labels_crc = int(torch.sum(r["labels"].to(torch.int64) *
                                torch.arange(n, device=dev, dtype=torch.int64) % 1000003).item())
-    cpu = None
+    cpu = cpu_all = None
     if not args.no_cpu:
         from oracle import oracle as O
         log("timing the CPU restatement of boruvka_minho_seq (1 thread)")
@@ -211,9 +280,22 @@ def run_mst(args):
                          "restatement of mst.py:143-407 boruvka_minho_seq",
                "seconds": c1 - c0}
         log(f"CPU baseline {c1 - c0:.2f}s")
+        # all cores: OpenMP Borůvka over the same CSR (the canonical forest)
+        O.boruvka_omp(g["indptr"], g["dest"], g["weight"])  # warm-up (threads, pages)
+        c0 = time.perf_counter()
+        lo, hi, omp_w, threads = O.boruvka_omp(g["indptr"], g["dest"], g["weight"])
+        c1 = time.perf_counter()
+        if omp_w != total_w or lo.size != r["n_mst"]:
+            raise SystemExit(f"all-core CPU/GPU forest mismatch: weight {omp_w} vs {total_w}")
+        cpu_all = {"value": m / (c1 - c0) / 1e6, "unit": "Medges/s", "cores": threads,
+                   "kind": "port", "sample": "full graph, OpenMP Borůvka with the same "
+                   "(weight, min id, max id) order (oracle/eac_oracle.c orc_boruvka_omp)",
+                   "seconds": c1 - c0}
+        log(f"all-core CPU baseline {c1 - c0:.2f}s on {threads} threads")
     out = base_line(args, m / (ms * 1e-3) / 1e6, "Medges/s", ms, "int32",
                     {"workload": "boruvka_mst_cc_knn", "n_vertices": n, "m_entries": m,
-                     "knn_k": args.knn, "weights": "int32 U[1,100]", "islands": 4, "seed": 1,
+                     "mean_degree": m / n, "knn_k": knn, "weights": "int32 U[1,100]",
+                     "islands": 4, "seed": 1,
                      "step": "forest ids+weights+first-occurrence component labels"})
     out["result"] = {"forest_edges": r["n_mst"], "forest_weight": total_w,
                      "components": r["n_components"], "rounds": r["rounds"],
@@ -221,16 +303,14 @@ def run_mst(args):
     kname = {0: "k_round0", 1: "k_round"}.get(dom)
     out["roofline"] = {"bound": "hbm", "kernel": MST_SLOTS[dom], "achieved": achieved,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                       "traffic": profile_traffic(kname) if kname else None,
+                       # the committed counter summary covers the k=20 graph only
+                       "traffic": profile_traffic(kname) if kname and knn == 20 else None,
                        "traffic_source": "profiles/boruvka_mst_knn1m_hbm_traffic.md",
                        "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl}
     out["kernels_ms_per_step"] = {MST_SLOTS[s]: prof[s][0] / args.steps for s in prof}
     out["tuning"] = tuning
     out["cpu_baseline"] = cpu
-    if not args.no_eac:
-        log("end-to-end EAC: batched+BF16 vs plain per-call FP32 (n=10,000, P=100)")
-        out["eac_small"] = eac_compare(cpu=not args.no_cpu)
-        log(f"EAC speedup {out['eac_small']['speedup']:.2f}x")
+    out["cpu_baseline_all_cores"] = cpu_all
     return out
 
 
@@ -238,7 +318,8 @@ def run_eac(args):
     import torch
     r = eac_compare(n=args.n if args.n != 1_000_000 else 10_000, cpu=not args.no_cpu,
                     reps=max(args.steps, 1))
-    out = base_line(args, r["batched_pts_per_s"], "pts/s", r["batched_bf16_ms"], "f32/f64",
+    out = base_line(args, r["batched_pts_per_s"], "pts/s", r["batched_bf16_ms"],
+                    "f64 labeling (d=2), int32 counts",
                     {"workload": "eac_e2e_gmm2d", "n": r["n"], "partitions": r["partitions"],
                      "k_range": r["k_range"], "iters": r["iters"]})
     out["result"] = r
@@ -355,19 +436,51 @@ def run_coassoc(args):
                      "plan_ms": prof[8][0] / max(prof[8][1], 1)}
     out["tuning"] = tuning
     ach = fill_bytes / (fill_ms * 1e-3) / 1e9
+    whole = 4.0 * P * n + 8.0 * nnz + 8.0 * (n + 1)  # labels read, (col, count) + offsets
     out["roofline"] = {"bound": "hbm", "kernel": "k_coassoc_rows<fill>", "achieved": ach,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                       "traffic": None, "avg_launch_ms": fill_ms}
+                       "traffic": None, "avg_launch_ms": fill_ms,
+                       "alg_bytes_per_launch": fill_bytes,
+                       "whole_build_alg_bytes": whole,
+                       "whole_build_frac": whole / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out["cpu_baseline"] = None
+    if not args.no_cpu:
+        # the C restatement of the reference's surgical build on a sample of
+        # the points (all 256 partitions, labels kept), checked against the
+        # GPU build of the same sample
+        from oracle import oracle as O
+        ns = 20_000
+        samp = np.sort(np.random.default_rng(33).choice(n, ns, replace=False))
+        Ls = np.ascontiguousarray(lab_h[:, samp])
+        sinc = 0
+        for p in range(P):
+            s = np.bincount(Ls[p], minlength=int(ks[p])).astype(np.int64)
+            sinc += int((s * (s - 1) // 2).sum())
+        c0 = time.perf_counter()
+        sip, sind, scnt = O.coassoc_condensed(Ls, ks)
+        c1 = time.perf_counter()
+        gip, gind, gdat, _ = coassoc_device(torch.from_numpy(Ls).cuda(), ks)
+        eq = bool(np.array_equal(gip.cpu().numpy(), sip) and
+                  np.array_equal(gind.cpu().numpy(), sind) and
+                  np.array_equal(gdat.cpu().numpy(), scnt))
+        out["cpu_baseline"] = {"value": sinc / (c1 - c0) / 1e9, "unit": "Gpair-increments/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{ns} of the {n} points (every partition), C "
+                                         "restatement of EAC_CSR's surgical build "
+                                         "(sparse.py:154-225, :445-465, :956-1099)",
+                               "seconds": c1 - c0, "pair_increments": sinc,
+                               "equal_to_gpu": eq}
+        log(f"CPU sample {c1 - c0:.2f}s, equal {eq}")
     return out
 
 
 def run_sl_batched(args):
     """512 independent symmetric CSR graphs (vertex counts log-uniform in
-    [1,000, vmax], mean degree U[8, 64], int32 weights U[1, 256]) as one
-    disjoint union, generated on the GPU; one batched lifetime cut (eac.py
-    rule) per step, as an eager launch sequence and as one hipGraph replay,
-    against one call per graph."""
+    [1,000, vmax], 3-D k-NN graphs with k = 0.75 x a target mean degree
+    U[8, 64], symmetrized, int32 weights U[1, 256]) as one disjoint union on
+    the GPU; one batched lifetime cut (eac.py rule) per step, as an eager
+    launch sequence and as one hipGraph replay, against one call per
+    graph."""
     import numpy as np
     import torch
 
@@ -376,9 +489,9 @@ def run_sl_batched(args):
                                                     lifetime_cut_batched)
 
     G = args.graphs
-    log(f"generating {G} random graphs on the GPU")
+    log(f"generating {G} k-NN graphs")
     t0 = time.perf_counter()
-    b = synth.graph_batch_device(G, (1000, args.vmax), (8, 64), seed=4, wmax=256)
+    b = synth.knn_batch_device(G, (1000, args.vmax), (8, 64), seed=4, wmax=256)
     torch.cuda.synchronize()
     d_dest, d_w, d_ip, goff = b["dest"], b["weight"], b["indptr"], b["goff"]
     N, M = b["n"], int(d_dest.numel())
@@ -386,6 +499,23 @@ def run_sl_batched(args):
     (lab, gout, info), ms = timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, goff, rule=0),
                                   args.steps, args.warmup, torch)
     log(f"batched eager {ms:.2f} ms")
+    # dominant kernel of the batched cut from a separate profiled pass (the
+    # library's HIP events bracket the Borůvka kernels on their stream)
+    from masters_final_amd import _lib as L
+    lib = L.load_library()
+    L.check(lib.eac_profile_enable(1))
+    timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, goff, rule=0), args.steps, 0, torch)
+    prof = read_prof(lib, L, MST_SLOTS)
+    L.check(lib.eac_profile_enable(0))
+    dom = max(prof, key=lambda s: prof[s][0])
+    tm, nl, nb = prof[dom]
+    avg_ms = tm / max(nl, 1)
+    bpl = nb / max(nl, 1)
+    ach = bpl / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roof = {"bound": "hbm", "kernel": MST_SLOTS[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "avg_launch_ms": avg_ms, "alg_bytes_per_launch": bpl,
+            "kernels_ms_per_step": {MST_SLOTS[s]: prof[s][0] / args.steps for s in prof}}
     # the same cut recorded as one hipGraph (rounds tuned on the first
     # replay); the batch is loaded into the recorded buffers before timing
     cg = LifetimeCutGraph(N + 1, M + 1, G + 1, torch.int32, rule=0, tune=True)
@@ -450,16 +580,47 @@ def run_sl_batched(args):
                          "per-graph eac.py:199-267 sp_sl_lifetime (Kruskal order)",
                "seconds": secs, "equal_to_gpu": ok}
         log(f"CPU sample {ng} graphs {secs:.2f}s, equal {ok}")
+        # all cores: the same per-graph restatement on every graph of the
+        # batch at once, one host thread per core (the C calls release the
+        # GIL), graphs handed out largest first
+        import concurrent.futures as cf
+        threads = len(__import__("os").sched_getaffinity(0))
+        order = np.argsort(-np.diff(goff), kind="stable")
+
+        def one(g):
+            a, z = int(goff[g]), int(goff[g + 1])
+            e0, e1 = int(ip_h[a]), int(ip_h[z])
+            u, v, w, _ = O.csr_edges(dest_h[e0:e1] - a, w_h[e0:e1], ip_h[a:z] - e0,
+                                     np.diff(ip_h[a:z + 1]))
+            try:
+                O.sp_sl_lifetime(u, v, w, z - a)
+            except ValueError:
+                pass
+            return e1 - e0
+
+        c0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            all_e = sum(ex.map(one, order.tolist()))
+        c1 = time.perf_counter()
+        cpu_all = {"value": all_e / (c1 - c0) / 1e6, "unit": "Medges/s", "cores": threads,
+                   "kind": "port", "sample": f"all {G} graphs ({all_e} entries), the per-graph "
+                   "C restatement on one host thread per core", "seconds": c1 - c0}
+        log(f"all-core CPU {c1 - c0:.2f}s on {threads} threads")
+    else:
+        cpu_all = None
     out = base_line(args, M / (mg * 1e-3) / 1e6, "Medges/s", mg, "int32",
-                    {"workload": "single_link_batched_random_graphs", "graphs": G,
+                    {"workload": "single_link_batched_knn_graphs", "graphs": G,
                      "v_range": [1000, args.vmax], "deg_range": [8, 64], "w_range": [1, 256],
                      "seed": 4, "n_vertices": N, "m_entries": M,
+                     "mean_degree": M / N,
                      "step": "hipGraph replay of the batched lifetime cut (eac.py rule)"})
     out["result"] = {"graph_replay_ms": mg, "batched_eager_ms": ms, "per_graph_calls_ms": mp,
                      "speedup_vs_per_graph": mp / mg, "graph_recorded_rounds": cg.rounds,
                      "graph_equal_to_batched": same_graph, "batched_equal_to_per_graph": same,
                      **info}
+    out["roofline"] = roof
     out["cpu_baseline"] = cpu
+    out["cpu_baseline_all_cores"] = cpu_all
     return out
 
 
@@ -544,6 +705,8 @@ def main():
     ap.add_argument("--vmax", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eac", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the other workloads' extra keys")
     ap.add_argument("--knobs", default="",
                     help="fixed tuning knobs for the MST workload, e.g. light_x10=20,level_step=-1 "
                          "(skips the autotuner)")

@@ -29,11 +29,16 @@ def _labels_from_list(clusts, N=None):
     n = int(N) if N is not None else int(sum(sizes))
     k = len(clusts)
     lab = np.full(n, k, dtype=np.int32)  # k = "in no cluster"
+    seen = np.zeros(n, dtype=np.int64)
     for i, c in enumerate(clusts):
         idx = np.asarray(c, dtype=np.int64)
         if idx.size and (idx.min() < 0 or idx.max() >= n):
             raise IndexError("cluster index out of range")
+        np.add.at(seen, idx, 1)
         lab[idx] = i
+    if np.any(seen > 1):
+        # a partition: the 'bin' format rejects the same overlap
+        raise ValueError("list format: a sample is listed in more than one cluster")
     return lab, k, np.asarray(sizes, dtype=np.int64)
 
 
@@ -53,7 +58,11 @@ def _as_rows(clusts, fmt, N=None):
     cluster sizes or None)."""
     t = L.torch()
     if fmt == "bin":
+        width = int(np.asarray(clusts).shape[1]) if np.ndim(clusts) == 2 else None
+        if N is not None and width is not None and int(N) != width:
+            raise ValueError(f"bin format: N = {N} but the matrix has {width} columns")
         clusts, fmt = _labels_from_bin(clusts), "list"
+        N = width  # every column is a sample, with or without a cluster
     if fmt == "list":
         lab, k, sizes = _labels_from_list(clusts, N)
         return L.as_device(lab, t.int32), 0, k + 1, True, sizes
@@ -82,11 +91,16 @@ def contingency(clusts1, clusts2, format1="array", format2="array", N=None):
         mm = mm.cpu().tolist()
         for e, q in enumerate(ext):
             lo, hi = int(mm[2 * e]), int(mm[2 * e + 1])
-            if lo < 0 or hi >= 2 ** 31 - 1:
+            # partition.py:56-59: labels [0, K) or [1, K] without gaps; the
+            # distinct count is checked before any table is allocated
+            if lo not in (0, 1) or hi >= 2 ** 31 - 1 or \
+                    int(t.unique(rows[q][0]).numel()) != hi - lo + 1:
                 raise ValueError('cluster numbers go from [0, biggest_clust[ or'
                                  '[1, biggest_clust[ uninterrupted')
             rows[q] = [rows[q][0].to(t.int32).contiguous(), lo, hi - lo + 1]
     (a, a0, ka), (b, b0, kb) = rows
+    if ka * kb > 2 ** 31:
+        raise ValueError(f"contingency table of {ka} x {kb} cells is over the 2^31 limit")
     out = t.empty(ka * kb + 1, dtype=t.int64, device=a.device)
     L.check(lib.eac_contingency(L.ptr(a), L.ptr(b), a.numel(), a0, ka, b0, kb, L.ptr(out),
                                 L.stream_ptr()))
@@ -94,14 +108,6 @@ def contingency(clusts1, clusts2, format1="array", format2="array", N=None):
     if h[-1]:
         raise ValueError("labels outside the partition's cluster range")
     T = h[:-1].reshape(ka, kb)
-    for fmt, lo, k, sz, axis in ((format1, a0, ka, size1, 1), (format2, b0, kb, size2, 0)):
-        if fmt == "array":
-            # partition.py:56-59: labels must be [0, K) or [1, K] without gaps
-            present = int(np.count_nonzero(T.sum(axis=axis)))
-            top = lo + k - 1
-            if present not in (top, top + 1):
-                raise ValueError('cluster numbers go from [0, biggest_clust[ or'
-                                 '[1, biggest_clust[ uninterrupted')
     s1 = T.sum(axis=1) if size1 is None else size1
     s2 = T.sum(axis=0) if size2 is None else size2
     if none1:

@@ -20,6 +20,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define ORC_OK 0
 #define ORC_ERR_VALUE (-7)
@@ -250,6 +253,100 @@ int64_t orc_kruskal_canonical(const int64_t* u, const int64_t* v, const double* 
   }
   free(E); free(p);
   return nf;
+}
+
+/* All-core CPU baseline of the same forest: Borůvka on OpenMP threads over
+ * the symmetric CSR (indptr[n+1], dest, int32 weights), keys (w - wmin,
+ * min id, max id) so the forest is the canonical one. Each round: every
+ * vertex's lightest key to another component, a compare-and-swap minimum
+ * per component, hooks (a mutual choice keeps the smaller root), then every
+ * vertex walks to its root. Writes the forest's (lo, hi) pairs and returns
+ * their count (-1 when the key does not fit 64 bits). Not the reference's
+ * algorithm (mst.py is single-threaded): it is the parallel CPU line the
+ * measurements report beside it. */
+int64_t orc_boruvka_omp(const int64_t* indptr, const int32_t* dest, const int32_t* w,
+                        int64_t n, int64_t* lo_out, int64_t* hi_out, double* weight_out) {
+  int b = 0;
+  while (b < 63 && (((uint64_t)(n > 1 ? n - 1 : 1)) >> b) != 0) ++b;
+  int64_t wmin = INT64_MAX, wmax = INT64_MIN;
+  const int64_t m = indptr[n];
+#pragma omp parallel for reduction(min : wmin) reduction(max : wmax)
+  for (int64_t e = 0; e < m; ++e) {
+    wmin = w[e] < wmin ? w[e] : wmin;
+    wmax = w[e] > wmax ? w[e] : wmax;
+  }
+  if (m == 0) { wmin = 0; wmax = 0; }
+  int wb = 0;
+  while (wb < 63 && (((uint64_t)(wmax - wmin)) >> wb) != 0) ++wb;
+  if (wb + 2 * b > 64) return -1;
+  const uint64_t idmask = (b >= 64) ? ~0ull : ((1ull << b) - 1);
+  int64_t* comp = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  int64_t* parent = malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  uint64_t* best = malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+  uint64_t* tree = malloc(sizeof(uint64_t) * (n > 0 ? n : 1));
+#pragma omp parallel for
+  for (int64_t v = 0; v < n; ++v) { comp[v] = v; parent[v] = v; tree[v] = ~0ull; }
+  for (;;) {
+    int any = 0;
+#pragma omp parallel for
+    for (int64_t v = 0; v < n; ++v) best[v] = ~0ull;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : any)
+    for (int64_t u = 0; u < n; ++u) {
+      const int64_t cu = comp[u];
+      uint64_t k = ~0ull;
+      for (int64_t e = indptr[u]; e < indptr[u + 1]; ++e) {
+        const int64_t v = dest[e];
+        if (comp[v] == cu) continue;
+        const uint64_t a = (uint64_t)(u < v ? u : v), c = (uint64_t)(u < v ? v : u);
+        const uint64_t key = ((uint64_t)(w[e] - wmin) << (2 * b)) | (a << b) | c;
+        k = key < k ? key : k;
+      }
+      if (k == ~0ull) continue;
+      any = 1;
+      uint64_t cur = __atomic_load_n(&best[cu], __ATOMIC_RELAXED);
+      while (k < cur && !__atomic_compare_exchange_n(&best[cu], &cur, k, 1, __ATOMIC_RELAXED,
+                                                     __ATOMIC_RELAXED)) {
+      }
+    }
+    if (!any) break;
+#pragma omp parallel for
+    for (int64_t r = 0; r < n; ++r) {
+      if (comp[r] != r || best[r] == ~0ull) continue;
+      const uint64_t k = best[r];
+      const int64_t a = (int64_t)((k >> b) & idmask), c = (int64_t)(k & idmask);
+      const int64_t other = comp[a] == r ? comp[c] : comp[a];
+      if (best[other] == k && r < other) continue;
+      parent[r] = other;
+      tree[r] = k;
+    }
+#pragma omp parallel for
+    for (int64_t v = 0; v < n; ++v) {
+      int64_t x = v;
+      while (parent[x] != x) x = parent[x];
+      comp[v] = x;
+    }
+  }
+  int64_t nf = 0;
+  double tw = 0.0;
+  for (int64_t r = 0; r < n; ++r) {
+    if (tree[r] == ~0ull) continue;
+    const uint64_t k = tree[r];
+    lo_out[nf] = (int64_t)((k >> b) & idmask);
+    hi_out[nf] = (int64_t)(k & idmask);
+    tw += (double)((int64_t)(k >> (2 * b)) + wmin);
+    ++nf;
+  }
+  *weight_out = tw;
+  free(comp); free(parent); free(best); free(tree);
+  return nf;
+}
+
+int orc_omp_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 /* Components of an undirected edge list, numbered by first occurrence

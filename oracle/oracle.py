@@ -40,6 +40,8 @@ def lib() -> C.CDLL:
             "orc_coassoc_capped": (C.c_int, [_P, _I, _I, C.c_int, _P, _I, _P, _P, _P, _P, _P,
                                              _P]),
             "orc_coassoc_full": (None, [_P, _I, _I, C.c_int, _P]),
+            "orc_boruvka_omp": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+            "orc_omp_threads": (C.c_int, []),
             "orc_argsort1d": (None, [_P, _I, _P]),
             "orc_slink": (None, [_P, _I, _P]),
             "orc_labels_from_Z": (None, [_P, _I, _I, _P]),
@@ -99,6 +101,22 @@ def boruvka_minho_seq(dest, weight, fe, od):
     nm = lib().orc_boruvka_minho_seq(_p(dest), _p(w), _p(fe), _p(od), nv, dest.size,
                                      _p(mst), _p(comps))
     return mst[:max(nv - 1, 0)], int(nm), comps
+
+
+def boruvka_omp(indptr, dest, weight):
+    """All-core CPU Borůvka (OpenMP) of a symmetric int32-weight CSR: the
+    canonical forest as (lo, hi) pairs, its weight, and the threads used."""
+    ip = _a(indptr, np.int64)
+    d = _a(dest, np.int32)
+    w = _a(weight, np.int32)
+    n = ip.size - 1
+    lo = np.zeros(max(n, 1), np.int64)
+    hi = np.zeros(max(n, 1), np.int64)
+    tw = C.c_double()
+    nf = lib().orc_boruvka_omp(_p(ip), _p(d), _p(w), n, _p(lo), _p(hi), C.byref(tw))
+    if nf < 0:
+        raise ValueError("weight range + ids exceed 64 bits")
+    return lo[:nf], hi[:nf], int(tw.value), int(lib().orc_omp_threads())
 
 
 def connected_comps_seq(dest, weight, fe, od):

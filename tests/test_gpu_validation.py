@@ -98,3 +98,25 @@ def test_indices_errors(torch):
         ConsistencyIndex(N=4).score(a, a, format='bin')
     with pytest.raises(ValueError):
         HungarianIndex(4).score(a, a, format='dict')
+    # {0, 2}: passes the reference's count check, but a row has no sample
+    with pytest.raises(ValueError):
+        ConsistencyIndex(N=4).score(np.array([0, 0, 2, 2]), np.array([0, 1, 2, 2]))
+    # a huge label is rejected before any table is allocated
+    with pytest.raises(ValueError):
+        ConsistencyIndex(N=2).score(np.array([0, 10 ** 9]), np.array([0, 1]))
+    # 'list' format: one sample listed in two clusters
+    with pytest.raises(ValueError):
+        HungarianIndex(4).score([np.array([0, 1]), np.array([1, 2, 3])],
+                                [np.array([0, 1]), np.array([2, 3])], format='list')
+
+
+def test_bin_format_with_an_empty_column(torch):
+    """A binary partition whose last sample is in no cluster: N is the matrix
+    width, the sample counts in no cluster (the 'bin' path used to size the
+    labels from the number of ones and fail)."""
+    from masters_final_amd.validation.contingency import contingency
+    b1 = np.array([[1, 1, 0, 0], [0, 0, 1, 0]], np.uint8)
+    b2 = np.array([[1, 0, 1, 0], [0, 1, 0, 0]], np.uint8)
+    T, s1, s2, n = contingency(b1, b2, format1='bin', format2='bin')
+    assert n == 4 and T.tolist() == [[1, 1], [1, 0]]
+    assert s1.tolist() == [2, 1] and s2.tolist() == [2, 1]

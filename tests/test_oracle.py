@@ -212,6 +212,35 @@ def test_sl_mst_lifetime_hand_cases():
         O.sl_mst_lifetime(np.array([0]), np.array([1]), np.array([1.]), 2)
 
 
+@pytest.mark.parametrize("seed,n,deg,disc", [(0, 300, 4, None), (1, 2000, 6, None),
+                                             (2, 800, 3, 5.0), (3, 500, 8, 1e9)])
+def test_sl_mst_lifetime_equals_scipy_rule_distinct_weights(seed, n, deg, disc):
+    """singlelink.py:14-65 written out in numpy over scipy's minimum spanning
+    forest, on graphs with distinct weights (a unique forest): sort the
+    forest weights, take the first largest lifetime a, and when it reaches
+    disconnect_weight - largest weight keep the a lightest forest edges
+    (the reference drops m - a); components of the kept edges."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components, minimum_spanning_tree
+    g = synth.random_graph(n, deg, seed, wmax=10, n_isolated=3, n_parts=2)
+    fe, od = g["indptr"][:-1], np.diff(g["indptr"])
+    u, v, _, _ = O.csr_edges(g["dest"], g["weight"], fe, od)
+    w = np.random.default_rng(seed).permutation(u.size).astype(np.float64) + 1.0
+    F = minimum_spanning_tree(sp.csr_matrix((w, (u, v)), shape=(n, n))).tocoo()
+    order = np.argsort(F.data)
+    fw, fu, fv = F.data[order], F.row[order], F.col[order]
+    lt = fw[1:] - fw[:-1]
+    a = int(np.argmax(lt))
+    d = w.max() if disc is None else disc
+    keep = fw.size
+    if lt[a] >= d - fw[-1]:
+        keep = a
+    m = sp.csr_matrix((np.ones(keep), (fu[:keep], fv[:keep])), shape=(n, n))
+    nc, want = connected_components(m, directed=False)
+    lab, onc, kept = O.sl_mst_lifetime(u, v, w, n, disconnect_weight=disc)
+    assert kept == keep and onc == nc and np.array_equal(lab, want)
+
+
 # ---------------------------------------------------------- co-association
 def _dense_coassoc(labels):
     P, n = labels.shape
