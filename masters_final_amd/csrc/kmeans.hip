@@ -426,8 +426,11 @@ void k_label_bf16x3(
 // partition's centroids; strict < in ascending k, lowest k on ties.
 // acc + diff^2 with separately rounded multiply and add in FP64 (the
 // oracle's arithmetic: no fused multiply-add), plain expression otherwise.
+// HIP's __dmul_rn / __dadd_rn are plain operators that the compiler may
+// contract into one fma; the pragma keeps the two roundings.
 __device__ __forceinline__ double sq_acc(double acc, double diff) {
-  return __dadd_rn(acc, __dmul_rn(diff, diff));
+#pragma clang fp contract(off)
+  return acc + diff * diff;
 }
 __device__ __forceinline__ float sq_acc(float acc, float diff) { return acc + diff * diff; }
 
@@ -726,6 +729,7 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
                                                      const int64_t* __restrict__ k_off, int P,
                                                      const int32_t* __restrict__ labels,
                                                      float* __restrict__ dists) {
+#pragma clang fp contract(off)  // the oracle's separately rounded products and sums
   constexpr int PPB = 256 / LPP;  // points per block and step
   const int lane = lane_id(), sub = lane & (LPP - 1), base = lane & ~(LPP - 1);
   const int64_t groups = (int64_t)gridDim.x * PPB;
@@ -756,23 +760,23 @@ __global__ __launch_bounds__(256) void k_exact_dists(const float* __restrict__ X
             const float4 cv = *reinterpret_cast<const float4*>(cr + jb);
             const double e0 = (double)xv.x - (double)cv.x, e1 = (double)xv.y - (double)cv.y;
             const double e2 = (double)xv.z - (double)cv.z, e3 = (double)xv.w - (double)cv.w;
-            sq[0] = __dmul_rn(e0, e0);
-            sq[1] = __dmul_rn(e1, e1);
-            sq[2] = __dmul_rn(e2, e2);
-            sq[3] = __dmul_rn(e3, e3);
+            sq[0] = e0 * e0;
+            sq[1] = e1 * e1;
+            sq[2] = e2 * e2;
+            sq[3] = e3 * e3;
           } else {
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const int j = jb + u;
               const double diff = j < d ? (double)xr[j] - (double)cr[j] : 0.0;
-              sq[u] = __dmul_rn(diff, diff);
+              sq[u] = diff * diff;
             }
           }
           const int steps = d - j0 >= 4 * LPP ? LPP : (d - j0 + 3) / 4;
           for (int st = 0; st < steps; ++st) {
             if (sub == st) {
 #pragma unroll
-              for (int u = 0; u < 4; ++u) acc = __dadd_rn(acc, sq[u]);
+              for (int u = 0; u < 4; ++u) acc = acc + sq[u];
             }
             acc = __shfl(acc, base + st, kWave);
           }
@@ -839,8 +843,17 @@ __global__ __launch_bounds__(256) void k_label_direct(const float* __restrict__ 
 // ... up to the last multiple of 8, combined as ((r0+r1)+(r2+r3)) +
 // ((r4+r5)+(r6+r7)), then the remaining elements added in order). Every
 // operation is a separately rounded float32 addition.
+// (x - c) ** 2 in float32: a rounded difference, then a rounded square; the
+// pragma keeps the square from being fused into the row sum's additions.
+__device__ __forceinline__ float np_square_diff(float x, float c) {
+#pragma clang fp contract(off)
+  const float df = x - c;
+  return df * df;
+}
+
 template <int DX>
 __device__ __forceinline__ float np_row_sum(const float* v, int d) {
+#pragma clang fp contract(off)
   if (d < 8) {
     float r = 0.0f;
 #pragma unroll
@@ -900,10 +913,7 @@ __global__ __launch_bounds__(256) void k_label_np(const float* __restrict__ X, i
         for (int r = 0; r < rows; ++r) {
           float sq[DX];
 #pragma unroll
-          for (int j = 0; j < DX; ++j) {
-            const float df = __fsub_rn(xr[j], s_c[r * DX + j]);
-            sq[j] = __fmul_rn(df, df);
-          }
+          for (int j = 0; j < DX; ++j) sq[j] = np_square_diff(xr[j], s_c[r * DX + j]);
           const float dist = np_row_sum<DX>(sq, d);
           if (bl < 0 || dist < best) { best = dist; bl = (int32_t)(c0 + r - kbeg); }
         }
