@@ -194,6 +194,7 @@ def extras(args):
     ns = _ap.Namespace(**vars(args))
     ns.n = 1_000_000  # each workload's own default size
     ns.steps, ns.warmup = min(args.steps, 5), 1
+    ns.knobs = ""  # every extra workload tunes its own knobs
     out = {}
     for key, fn in (("mst_knn_deg40", lambda a: mst_case(a, 37)),
                     ("kmeans_4m_d64_k2048", run_kmeans),
@@ -411,9 +412,14 @@ def run_coassoc(args):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     log(f"ensemble {t1 - t0:.2f}s, rescored {fit['n_rescored']}")
-    from masters_final_amd.tune import autotune_coassoc
-    tuning = autotune_coassoc(labels, ks, use_cache=not args.retune)
-    log(f"tuning: {tuning['knobs']} ({tuning['ms']:.1f} ms, cached={tuning['cached']})")
+    from masters_final_amd.tune import autotune_coassoc, set_knobs
+    if args.knobs:  # fixed settings (profiling runs): no candidate timing
+        knobs = {k: int(v) for k, v in (kv.split("=") for kv in args.knobs.split(","))}
+        set_knobs(**knobs)
+        tuning = {"knobs": knobs, "ms": None, "cached": None, "source": "--knobs"}
+    else:
+        tuning = autotune_coassoc(labels, ks, use_cache=not args.retune)
+    log(f"tuning: {tuning['knobs']} ({tuning['ms']} ms, cached={tuning['cached']})")
     L.check(lib.eac_profile_enable(1))
     (ip, ind, dat, mx), ms = timed(lambda: coassoc_device(labels, ks), args.steps, args.warmup,
                                    torch)
@@ -581,31 +587,20 @@ def run_sl_batched(args):
                "seconds": secs, "equal_to_gpu": ok}
         log(f"CPU sample {ng} graphs {secs:.2f}s, equal {ok}")
         # all cores: the same per-graph restatement on every graph of the
-        # batch at once, one host thread per core (the C calls release the
-        # GIL), graphs handed out largest first
-        import concurrent.futures as cf
-        threads = len(__import__("os").sched_getaffinity(0))
-        order = np.argsort(-np.diff(goff), kind="stable")
-
-        def one(g):
-            a, z = int(goff[g]), int(goff[g + 1])
-            e0, e1 = int(ip_h[a]), int(ip_h[z])
-            u, v, w, _ = O.csr_edges(dest_h[e0:e1] - a, w_h[e0:e1], ip_h[a:z] - e0,
-                                     np.diff(ip_h[a:z + 1]))
-            try:
-                O.sp_sl_lifetime(u, v, w, z - a)
-            except ValueError:
-                pass
-            return e1 - e0
-
+        # batch, graphs spread over OpenMP threads (largest first); a sample
+        # of the graphs checked against the GPU labels
         c0 = time.perf_counter()
-        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
-            all_e = sum(ex.map(one, order.tolist()))
+        ncs, labs, threads = O.sl_batched_omp(ip_h, dest_h, w_h, goff)
         c1 = time.perf_counter()
-        cpu_all = {"value": all_e / (c1 - c0) / 1e6, "unit": "Medges/s", "cores": threads,
-                   "kind": "port", "sample": f"all {G} graphs ({all_e} entries), the per-graph "
-                   "C restatement on one host thread per core", "seconds": c1 - c0}
-        log(f"all-core CPU {c1 - c0:.2f}s on {threads} threads")
+        okall = all((ncs[g] == -1 and gout_h[g, 3] == -1) or
+                    (gout_h[g, 1] == 1 and ncs[g] == 0) or
+                    np.array_equal(lab_h[goff[g]:goff[g + 1]], labs[goff[g]:goff[g + 1]])
+                    for g in range(G))
+        cpu_all = {"value": M / (c1 - c0) / 1e6, "unit": "Medges/s", "cores": threads,
+                   "kind": "port", "sample": f"all {G} graphs ({M} entries), the per-graph C "
+                   "restatement on OpenMP threads (oracle/eac_oracle.c orc_sl_batched_omp)",
+                   "seconds": c1 - c0, "equal_to_gpu": bool(okall)}
+        log(f"all-core CPU {c1 - c0:.2f}s on {threads} threads, equal {okall}")
     else:
         cpu_all = None
     out = base_line(args, M / (mg * 1e-3) / 1e6, "Medges/s", mg, "int32",
@@ -708,8 +703,9 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="default workload only: skip the other workloads' extra keys")
     ap.add_argument("--knobs", default="",
-                    help="fixed tuning knobs for the MST workload, e.g. light_x10=20,level_step=-1 "
-                         "(skips the autotuner)")
+                    help="fixed tuning knobs for the mst or coassoc workload, e.g. "
+                         "light_x10=20,level_step=-1 or co_window=32768,co_unroll=8 (skips the "
+                         "autotuner)")
     ap.add_argument("--retune", action="store_true",
                     help="ignore the autotuner cache and time the candidates again")
     ap.add_argument("--copy-calib", action="store_true",

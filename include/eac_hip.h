@@ -89,12 +89,15 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *                           smaller, else 32768)
  *   EAC_TUNE_LEVEL_STEP     growth of the light edges per vertex from one
  *                           weight level of the split to the next (default
- *                           4, up to 4 levels; 1 or -1 = one split only) */
+ *                           4, up to 4 levels; 1 or -1 = one split only)
+ *   EAC_TUNE_CO_UNROLL      co-association member loads in flight per lane
+ *                           while counting (4 by default, or 8) */
 #define EAC_TUNE_LIGHT_X10 0
 #define EAC_TUNE_FIRST_BATCH 1
 #define EAC_TUNE_FILTER_BPCU 2
 #define EAC_TUNE_CO_WINDOW 3
 #define EAC_TUNE_LEVEL_STEP 4
+#define EAC_TUNE_CO_UNROLL 5
 #define EAC_TUNE_COUNT 8
 int eac_tune_set(int32_t knob, int32_t value);
 int eac_tune_get(int32_t knob, int32_t* value);
@@ -160,6 +163,7 @@ int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
                              const int32_t* out_degree, int32_t n, int64_t m,
                              int32_t* labels, int64_t* h_ncomp, void* ws,
                              size_t ws_bytes, void* stream);
+
 
 /* ---- single-link lifetime cut -----------------------------------------
  * Minimum spanning forest (as eac_boruvka_mst), its edges in ascending

@@ -140,8 +140,9 @@ __global__ __launch_bounds__(256) void k_first_partition(const int32_t* __restri
 // write (col, value) at out_off. RANK (PASS 1 only): also write, per entry,
 // the processing rank of the first partition in which the pair co-occurs
 // (0 = the partition pl.first[0], then the others in index order), kept in a
-// second LDS window with one atomicMin per pair visit.
-template <int PASS, bool RANK = false>
+// second LDS window with one atomicMin per pair visit. U: member loads in
+// flight per lane while visiting (4 or 8, a tuning knob).
+template <int PASS, bool RANK = false, int U = 4>
 __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, int win,
                                                       unsigned long long* __restrict__ row_ctr,
                                                       int32_t* __restrict__ row_nnz,
@@ -271,11 +272,11 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       // partitions walk instead).
       if (tot >= 32 * P) {
         int p = 0;
-        for (int32_t f0 = t; f0 < tot; f0 += 4 * kCB) {
-          int32_t j[4];
-          int pu[4];
+        for (int32_t f0 = t; f0 < tot; f0 += U * kCB) {
+          int32_t j[U];
+          int pu[U];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < U; ++u) {
             const int32_t f = f0 + u * kCB;
             j[u] = -1;
             if (f < tot) {
@@ -285,7 +286,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
             pu[u] = p;
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < U; ++u) {
             if (j[u] < 0 || j[u] == (int32_t)i) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       } else {
         const int wv = t / kWave, ln = lane_id();
         int pw = 0;
-        for (int32_t base = wv * 4 * kWave; base < tot; base += 4 * kCB) {
+        for (int32_t base = wv * U * kWave; base < tot; base += U * kCB) {
           for (;;) {  // uniform: advance the cursor to the partition of `base`
             const int q = pw + 1 + ln;
             const int32_t b = q <= P ? s_pre[q] : INT32_MAX;
@@ -309,11 +310,11 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           }
           const int qb = pw + 1 + ln;
           const int32_t bnd = qb <= P ? s_pre[qb] : INT32_MAX;  // ascending by lane
-          const bool covered = __shfl(bnd, kWave - 1, kWave) > base + 4 * kWave - 1;
-          int32_t j[4];
-          int pu[4];
+          const bool covered = __shfl(bnd, kWave - 1, kWave) > base + U * kWave - 1;
+          int32_t j[U];
+          int pu[U];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < U; ++u) {
             const int32_t f = base + u * kWave + ln;
             int p = pw;
             if (covered) {
@@ -329,7 +330,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
             pu[u] = p;
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < U; ++u) {
             if (j[u] < 0 || j[u] == (int32_t)i) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
@@ -516,6 +517,9 @@ int co_window(int64_t n) {
   if (need < w) w = (int)need;
   return w < 64 ? 64 : (w & ~1);
 }
+// Member loads in flight per lane in the pair visits (EAC_TUNE_CO_UNROLL:
+// 4 by default, or 8).
+inline int co_unroll() { return tune_value(EAC_TUNE_CO_UNROLL) == 8 ? 8 : 4; }
 inline size_t co_lds(int win, int P, bool rank = false) {
   return (size_t)(win / 2 + (rank ? win + P : 0) + 5 * P + 1) * 4;
 }
@@ -537,16 +541,16 @@ int32_t* build_win_table(const CoBufs& B, int64_t n, int P, int64_t ktot, int wi
   return wtab;
 }
 // Every workgroup resident at once (rows are handed out dynamically).
-template <int PASS, bool RANK = false>
+template <int PASS, bool RANK = false, int U = 4>
 int co_grid(int win, int P) {
   int dev = 0, cus = 256, per = 1;
   if (hipGetDevice(&dev) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   // allow launch-time LDS beyond 64 KB (the window of large graphs)
   const size_t lds = co_lds(win, P, RANK);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_coassoc_rows<PASS, RANK>),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_coassoc_rows<PASS, RANK, U>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coassoc_rows<PASS, RANK>, kCB,
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_coassoc_rows<PASS, RANK, U>, kCB,
                                                    lds) != hipSuccess || per < 1)
     per = 1;
   return cus * per;
@@ -611,8 +615,12 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
   CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr};
   int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
-  k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(
-      pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
+  if (co_unroll() == 8)
+    k_coassoc_rows<0, false, 8><<<co_grid<0, false, 8>(win, P), kCB, co_lds(win, P), s>>>(
+        pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
+  else
+    k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(
+        pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
   EAC_LAUNCH_CHECK();
   prof_end(pr2, s);
   EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
@@ -659,6 +667,9 @@ int coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_
     EAC_LAUNCH_CHECK();
     k_coassoc_rows<1, true><<<co_grid<1, true>(win, P), kCB, co_lds(win, P, true), s>>>(
         pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, rank);
+  } else if (co_unroll() == 8) {
+    k_coassoc_rows<1, false, 8><<<co_grid<1, false, 8>(win, P), kCB, co_lds(win, P), s>>>(
+        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, nullptr);
   } else {
     k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(
         pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, nullptr);

@@ -4,7 +4,8 @@ The knobs change speed only, never results: the Borůvka weight-level split
 (light undirected edges per vertex x 10 below the first threshold, -1 = no
 split; growth of that count per further level, -1 = one split only), the
 number of rounds enqueued before the first read-back, the blocks per compute
-unit of the edge-list filter, and the co-association LDS window (columns).
+unit of the edge-list filter, the co-association LDS window (columns) and the
+co-association member loads in flight per lane.
 `autotune_mst` times a grid of candidate settings
 on the caller's own graph with HIP events, checks that every candidate gives
 the same forest, keeps the fastest, and caches the choice per device, weight
@@ -21,7 +22,8 @@ from pathlib import Path
 
 from . import _lib as L
 
-KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3, "level_step": 4}
+KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3, "level_step": 4,
+         "co_unroll": 5}
 
 DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),
@@ -121,26 +123,28 @@ def autotune_mst(dest, weight, indptr, grid=None, reps=3, use_cache=True):
             "candidates": [{"ms": ms, **k} for ms, k in results]}
 
 
-def autotune_coassoc(labels, ks, windows=(4096, 8192, 16384, 32768), use_cache=True):
-    """Picks the co-association LDS window for this ensemble (device labels
-    [P][n], cluster counts ks): timed candidates, identical output checked
-    (row offsets and the largest count). Returns dict(knobs, ms, ...)."""
+def autotune_coassoc(labels, ks, windows=(4096, 8192, 16384, 32768), unrolls=(4, 8),
+                     use_cache=True):
+    """Picks the co-association LDS window and the member loads in flight per
+    lane for this ensemble (device labels [P][n], cluster counts ks): timed
+    candidates, identical output checked (row offsets and the largest
+    count). Returns dict(knobs, ms, ...)."""
     from .EAC.sparse import coassoc_device
     t = L.torch()
     P, n = labels.shape
     dev = t.cuda.get_device_name(t.cuda.current_device())
-    key = "coassoc|%s|n2^%d|P%d|k%d" % (dev, int(n).bit_length(), int(P),
-                                         int(sum(int(k) for k in ks)).bit_length())
+    key = "coassoc2|%s|n2^%d|P%d|k%d" % (dev, int(n).bit_length(), int(P),
+                                          int(sum(int(k) for k in ks)).bit_length())
     cache = _load_cache() if use_cache else {}
     if key in cache:
         set_knobs(**cache[key]["knobs"])
         return {"knobs": cache[key]["knobs"], "ms": cache[key]["ms"], "candidates": None,
                 "cached": True}
     results, ref = [], None
-    for w in windows:
+    for w, u in itertools.product(windows, unrolls):
         if w > 32768:
             continue
-        set_knobs(co_window=w)
+        set_knobs(co_window=w, co_unroll=u)
         ev0, ev1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
         ev0.record()
         ip, ind, dat, mx = coassoc_device(labels, ks)
@@ -150,9 +154,10 @@ def autotune_coassoc(labels, ks, windows=(4096, 8192, 16384, 32768), use_cache=T
         if ref is None:
             ref = sig
         elif sig != ref:
-            raise RuntimeError(f"co_window={w} changed the co-association: {sig} vs {ref}")
+            raise RuntimeError(f"co_window={w}, co_unroll={u} changed the co-association: "
+                               f"{sig} vs {ref}")
         del ip, ind, dat
-        results.append((ev0.elapsed_time(ev1), {"co_window": w}))
+        results.append((ev0.elapsed_time(ev1), {"co_window": w, "co_unroll": u}))
     results.sort(key=lambda x: x[0])
     best_ms, best = results[0]
     set_knobs(**best)

@@ -13,8 +13,10 @@ def build() -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     # OpenMP only for the all-core baseline (orc_boruvka_omp); every checker
     # function is single-threaded
-    cmd = ["gcc", "-O2", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-o", str(tmp), str(SRC),
-           "-lm"]
+    # -ffp-contract=off: every product and sum rounded on its own, as the
+    # reference's arithmetic is
+    cmd = ["gcc", "-O2", "-std=gnu11", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared", "-o",
+           str(tmp), str(SRC), "-lm"]
     subprocess.run(cmd, check=True)
     tmp.replace(LIB)
     return LIB

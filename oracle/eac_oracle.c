@@ -341,6 +341,43 @@ int64_t orc_boruvka_omp(const int64_t* indptr, const int32_t* dest, const int32_
   return nf;
 }
 
+/* All-core CPU baseline of the batched cut: orc_sp_sl_lifetime (eac.py rule,
+ * max_dist None) on every graph of a disjoint union, graphs spread over
+ * OpenMP threads (largest first). Graph g owns vertices [goff[g],
+ * goff[g+1]) and their rows of the symmetric CSR (union ids in dest).
+ * Writes per graph the component count (0 where the reference returns empty
+ * labels, -1 where it raises) and the first-occurrence labels in
+ * labels[goff[g]...]. */
+int orc_sp_sl_lifetime(const int64_t* u, const int64_t* v, const double* w, int64_t ne,
+                       int64_t nv, double max_dist, int64_t n_clusters, int64_t* labels,
+                       int64_t* n_comps, int64_t* labels_empty, double* th_out,
+                       int64_t* nc_stable_out);
+
+void orc_sl_batched_omp(const int64_t* indptr, const int32_t* dest, const int32_t* weight,
+                        const int64_t* goff, int64_t G, const int64_t* order, int64_t* nc_out,
+                        int64_t* labels) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t t = 0; t < G; ++t) {
+    const int64_t g = order ? order[t] : t;
+    const int64_t a = goff[g], z = goff[g + 1], nv = z - a;
+    int64_t ne = 0;
+    for (int64_t r = a; r < z; ++r)
+      for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e) ne += dest[e] > r;
+    int64_t* u = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+    int64_t* v = malloc(sizeof(int64_t) * (ne > 0 ? ne : 1));
+    double* w = malloc(sizeof(double) * (ne > 0 ? ne : 1));
+    int64_t k = 0;
+    for (int64_t r = a; r < z; ++r)
+      for (int64_t e = indptr[r]; e < indptr[r + 1]; ++e)
+        if (dest[e] > r) { u[k] = r - a; v[k] = dest[e] - a; w[k] = (double)weight[e]; ++k; }
+    int64_t nc = 0, empty = 0, ncs = 0;
+    double th = 0.0;
+    const int rc = orc_sp_sl_lifetime(u, v, w, ne, nv, -1.0, 0, labels + a, &nc, &empty, &th, &ncs);
+    nc_out[g] = rc != ORC_OK ? -1 : (empty ? 0 : nc);
+    free(u); free(v); free(w);
+  }
+}
+
 int orc_omp_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -41,6 +41,7 @@ def lib() -> C.CDLL:
                                              _P]),
             "orc_coassoc_full": (None, [_P, _I, _I, C.c_int, _P]),
             "orc_boruvka_omp": (_I, [_P, _P, _P, _I, _P, _P, _P]),
+            "orc_sl_batched_omp": (None, [_P, _P, _P, _P, _I, _P, _P, _P]),
             "orc_omp_threads": (C.c_int, []),
             "orc_argsort1d": (None, [_P, _I, _P]),
             "orc_slink": (None, [_P, _I, _P]),
@@ -117,6 +118,22 @@ def boruvka_omp(indptr, dest, weight):
     if nf < 0:
         raise ValueError("weight range + ids exceed 64 bits")
     return lo[:nf], hi[:nf], int(tw.value), int(lib().orc_omp_threads())
+
+
+def sl_batched_omp(indptr, dest, weight, goff):
+    """All-core CPU cut of every graph of a disjoint union (OpenMP, largest
+    graphs first): (components per graph, 0 where the reference returns empty
+    labels, -1 where it raises; first-occurrence labels per graph; threads)."""
+    ip = _a(indptr, np.int64)
+    d = _a(dest, np.int32)
+    w = _a(weight, np.int32)
+    go = _a(goff, np.int64)
+    G = go.size - 1
+    order = np.argsort(-np.diff(go), kind="stable").astype(np.int64)
+    nc = np.zeros(max(G, 1), np.int64)
+    lab = np.zeros(max(int(go[-1]), 1), np.int64)
+    lib().orc_sl_batched_omp(_p(ip), _p(d), _p(w), _p(go), G, _p(order), _p(nc), _p(lab))
+    return nc[:G], lab[: int(go[-1])], int(lib().orc_omp_threads())
 
 
 def connected_comps_seq(dest, weight, fe, od):

@@ -50,6 +50,33 @@ def test_coassoc_condensed_vs_oracle(torch, n, P, kmin, kmax, seed):
     assert np.all(np.diff(gc)[np.diff(rows) == 0] > 0)
 
 
+@pytest.mark.parametrize("window", [0, 4096])
+def test_coassoc_unroll_knob_same_matrix(torch, window):
+    """The tuning knobs (LDS window, member loads in flight) change speed
+    only: every setting gives the oracle's matrix."""
+    from masters_final_amd.EAC.sparse import coassoc_device
+    from masters_final_amd.tune import get_knobs, set_knobs
+    rng = np.random.default_rng(17)
+    n, P = 9000, 24
+    ks = rng.integers(3, 40, P)
+    labels = np.stack([rng.integers(0, k, n) for k in ks]).astype(np.int32)
+    saved = get_knobs()
+    try:
+        outs = []
+        for u in (4, 8):
+            set_knobs(co_window=window, co_unroll=u)
+            gi, gc, gd, mx = coassoc_device(labels, ks, full_rows=(window == 0))
+            outs.append((gi.cpu().numpy(), gc.cpu().numpy(), gd.cpu().numpy(), mx))
+    finally:
+        set_knobs(**saved)
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+    ip, ind, cnt = O.coassoc_condensed(labels, ks) if window else (None, None, None)
+    if ip is not None:
+        assert np.array_equal(outs[1][0], ip) and np.array_equal(outs[1][1], ind)
+        assert np.array_equal(outs[1][2], cnt)
+
+
 def test_coassoc_full_rows_and_dissim(torch):
     from masters_final_amd.EAC.sparse import coassoc_device
     rng = np.random.default_rng(5)

@@ -52,6 +52,41 @@ def test_label_ties_lowest_index(torch):
     assert nres > 0  # tied points cannot be certified: they are re-scored
 
 
+# Point (X, X) against centroids (T0, T1) and (T1, T0): the differences are
+# (a, b) and (b, a), so the oracle's separately rounded sums a^2 + b^2 are
+# equal (a tie: the lower index wins), while a fused multiply-add over
+# ascending dimensions rounds b^2 + fl(a^2) and a^2 + fl(b^2), and for these
+# values makes the second centroid strictly closer. Found by an exact
+# (fractions) search over X ~ 1e3, T ~ 1e-4.
+FMA_SENSITIVE_TIES = [(680.3784790039062, -1.3656633200298529e-05, -3.7909856473561376e-05),
+                      (463.11016845703125, 8.245135541073978e-05, -2.0252986359992065e-05),
+                      (-1514.383544921875, 3.94981871068012e-05, -6.70565859763883e-05),
+                      (-1193.2025146484375, -0.00014924639253877103, 3.663782763396739e-06)]
+
+
+@pytest.mark.parametrize("d", [2, 16, 40])
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_label_ties_need_separate_roundings(torch, d, prec):
+    """Ties that exist only with separately rounded products and sums (the
+    oracle's FP64 arithmetic) resolve to the lower index; a fused
+    multiply-add would pick the other centroid. d = 16 and 40 send the
+    split-BF16 path's tied points through the FP64 re-scoring (zero padding
+    dimensions add exact zeros)."""
+    from masters_final_amd.cluster.KMeans import label_device
+    m = len(FMA_SENSITIVE_TIES)
+    X = np.zeros((m, d), np.float32)
+    C = np.zeros((2 * m, d), np.float32)
+    for i, (x, t0, t1) in enumerate(FMA_SENSITIVE_TIES):
+        X[i, :2] = x
+        C[2 * i, :2] = (t0, t1)
+        C[2 * i + 1, :2] = (t1, t0)
+    lab, dist, _ = label_device(X, C, k_list=[2] * m, precision=prec)
+    got = np.array([lab[i, i].item() for i in range(m)])
+    want = np.array([O.kmeans_label_f64(X[i:i + 1], C[2 * i:2 * i + 2])[0][0] for i in range(m)])
+    assert want.tolist() == [0] * m
+    assert np.array_equal(got, want)
+
+
 def test_rescore_many_flags_tiled(torch):
     """More than 65,536 flagged points (ties between exact duplicate
     centroids) take the partition-sorted, LDS-tiled FP64 re-scoring; labels
