@@ -164,6 +164,22 @@ int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
                              int32_t* labels, int64_t* h_ncomp, void* ws,
                              size_t ws_bytes, void* stream);
 
+/* ---- connected components with no allocation ----------------------------
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops). The same labels as
+ * eac_connected_components, from the kernels' scratch in a fixed device
+ * array of the library (eac_connected_components_noalloc_max_bytes() bytes,
+ * 64 MiB; every GPU the library runs on holds its own copy until the
+ * process ends). Graphs whose eac_cc_workspace_bytes(n, m) exceeds it are
+ * rejected (EAC_ERR_UNSUPPORTED). Nothing is allocated and nothing is
+ * synchronised: d_out (device int32[2]) receives the number of components
+ * and a non-zero flag when dest holds ids outside [0, n). Calls on one GPU
+ * share that GPU's scratch: calls that overlap on one GPU (two streams, or
+ * two host threads) race; calls in one stream are ordered. */
+size_t eac_connected_components_noalloc_max_bytes(void);
+int eac_connected_components_noalloc(const int32_t* dest, const int64_t* first_edge,
+                                     const int32_t* out_degree, int32_t n, int64_t m,
+                                     int32_t* labels, int32_t* d_out, void* stream);
 
 /* ---- single-link lifetime cut -----------------------------------------
  * Minimum spanning forest (as eac_boruvka_mst), its edges in ascending

@@ -45,6 +45,8 @@ SIGNATURES = {
                                   _p, _sz, _p]),
     "eac_cc_workspace_bytes": (_sz, [_i32, _i64]),
     "eac_connected_components": (C.c_int, [_p, _p, _p, _i32, _i64, _p, _p, _p, _sz, _p]),
+    "eac_connected_components_noalloc_max_bytes": (_sz, []),
+    "eac_connected_components_noalloc": (C.c_int, [_p, _p, _p, _i32, _i64, _p, _p, _p]),
     "eac_sl_lifetime_workspace_bytes": (_sz, [_i32, _i64]),
     "eac_sl_lifetime": (C.c_int, [_p, _p, _i32, _p, _p, _i32, _i64, _i32, _i32, C.c_double,
                                   _i32, _p, _p, _p, _sz, _p]),

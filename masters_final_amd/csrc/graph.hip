@@ -2549,6 +2549,76 @@ extern "C" size_t eac_cc_workspace_bytes(int32_t n, int64_t m) {
   return ar.used + eac::scan_ws_bytes(n, 4) + 4096;
 }
 
+namespace eac {
+namespace {
+// The components launch sequence over scratch carved from `ar`; d_nc[0]
+// receives the number of components, d_nc[1] is set when dest holds ids
+// outside [0, n). Enqueue only: no synchronisation, no host memory.
+int cc_enqueue(const int32_t* dest, const int64_t* first_edge, const int32_t* out_degree,
+               int32_t n, int64_t m, int32_t* labels, int32_t* d_nc, Arena& ar, bool profile,
+               hipStream_t s) {
+  int64_t nchunks = (m + kGTile - 1) / kGTile;
+  int32_t* chunk_row = ar.take<int32_t>(nchunks + 2);
+  int32_t* parent = ar.take<int32_t>(n);
+  int32_t* root = ar.take<int32_t>(n);
+  int32_t* flag = ar.take<int32_t>(n);
+  int32_t* rank = ar.take<int32_t>(n);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "cc: workspace too small");
+  EAC_TRY(fill<int32_t>(d_nc, 2, 0, s));
+  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n);
+  EAC_LAUNCH_CHECK();
+  if (m > 0) {
+    k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(first_edge, n, m, nchunks, chunk_row);
+    EAC_LAUNCH_CHECK();
+    int g = grid_for(nchunks, 1, persistent_blocks());
+    int pr = profile ? prof_begin(EAC_PROF_CC, 4 * m + 8 * (int64_t)n, s) : -1;
+    if (out_degree)
+      k_unite_csr<true><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
+                                          parent, (unsigned int*)(d_nc + 1));
+    else
+      k_unite_csr<false><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
+                                           parent, (unsigned int*)(d_nc + 1));
+    EAC_LAUNCH_CHECK();
+    if (profile) prof_end(pr, s);
+  }
+  k_uf_flatten<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n, root, flag);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY((scan_excl<int32_t, int32_t>(flag, rank, n, 0, d_nc, ar, s)));
+  k_gather_rank<<<grid_for(n, 256, 4096), 256, 0, s>>>(root, rank, n, labels);
+  EAC_LAUNCH_CHECK();
+  return EAC_OK;
+}
+
+// Scratch of eac_connected_components_noalloc: a fixed device array that
+// every GPU the library runs on holds until the process ends. Host code
+// never uses this name as an address; it asks the runtime for each
+// device's own copy (cc_scratch_ptr).
+constexpr size_t kCcScratchBytes = 64u << 20;
+static __device__ __align__(256) unsigned char scratch[kCcScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_cc_scratch[kMaxDevices] = {nullptr};
+std::mutex g_cc_scratch_mu;
+
+// The address of the current device's copy of `scratch`, looked up once per
+// device (hipGetSymbolAddress with that device current).
+int cc_scratch_ptr(void** out) {
+  int dev = 0;
+  EAC_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(EAC_ERR_UNSUPPORTED, "cc noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_cc_scratch_mu);
+  if (!g_cc_scratch[dev]) {
+    void* p = nullptr;
+    EAC_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(EAC_ERR_HIP, "cc noalloc: no address for the scratch array");
+    g_cc_scratch[dev] = p;
+  }
+  *out = g_cc_scratch[dev];
+  return EAC_OK;
+}
+}  // namespace
+}  // namespace eac
+
 extern "C" int eac_connected_components(const int32_t* dest, const int64_t* first_edge,
                                         const int32_t* out_degree, int32_t n, int64_t m,
                                         int32_t* labels, int64_t* h_ncomp, void* ws,
@@ -2564,41 +2634,34 @@ extern "C" int eac_connected_components(const int32_t* dest, const int64_t* firs
   if (!ws || ws_bytes < need)
     return fail(EAC_ERR_WORKSPACE, "cc: workspace %zu < %zu bytes", ws_bytes, need);
   Arena ar(ws, ws_bytes);
-  int64_t nchunks = (m + kGTile - 1) / kGTile;
-  int32_t* chunk_row = ar.take<int32_t>(nchunks + 2);
-  int32_t* parent = ar.take<int32_t>(n);
-  int32_t* root = ar.take<int32_t>(n);
-  int32_t* flag = ar.take<int32_t>(n);
-  int32_t* rank = ar.take<int32_t>(n);
   int32_t* d_nc = ar.take<int32_t>(4);
-  EAC_TRY(fill<int32_t>(d_nc, 4, 0, s));
-  k_iota<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n);
-  EAC_LAUNCH_CHECK();
-  if (m > 0) {
-    k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(first_edge, n, m, nchunks, chunk_row);
-    EAC_LAUNCH_CHECK();
-    int g = grid_for(nchunks, 1, persistent_blocks());
-    int pr = prof_begin(EAC_PROF_CC, 4 * m + 8 * (int64_t)n, s);
-    if (out_degree)
-      k_unite_csr<true><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
-                                          parent, (unsigned int*)(d_nc + 1));
-    else
-      k_unite_csr<false><<<g, kGB, 0, s>>>(dest, first_edge, out_degree, n, m, chunk_row, nchunks,
-                                           parent, (unsigned int*)(d_nc + 1));
-    EAC_LAUNCH_CHECK();
-    prof_end(pr, s);
-  }
-  k_uf_flatten<<<grid_for(n, 256, 4096), 256, 0, s>>>(parent, n, root, flag);
-  EAC_LAUNCH_CHECK();
-  EAC_TRY((scan_excl<int32_t, int32_t>(flag, rank, n, 0, d_nc, ar, s)));
-  k_gather_rank<<<grid_for(n, 256, 4096), 256, 0, s>>>(root, rank, n, labels);
-  EAC_LAUNCH_CHECK();
+  EAC_TRY(cc_enqueue(dest, first_edge, out_degree, n, m, labels, d_nc, ar, true, s));
   int32_t hc[2] = {0, 0};
   EAC_TRY(d2h_sync(hc, d_nc, sizeof(hc), s));
   prof_flush();
   if (hc[1]) return fail(EAC_ERR_ARG, "cc: dest holds ids outside [0, n)");
   if (h_ncomp) *h_ncomp = hc[0];
   return EAC_OK;
+}
+
+extern "C" size_t eac_connected_components_noalloc_max_bytes(void) { return kCcScratchBytes; }
+
+extern "C" int eac_connected_components_noalloc(const int32_t* dest, const int64_t* first_edge,
+                                                const int32_t* out_degree, int32_t n, int64_t m,
+                                                int32_t* labels, int32_t* d_out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (n < 1 || m < 0) return fail(EAC_ERR_ARG, "cc noalloc: need n >= 1, m >= 0");
+  if (!first_edge || !labels || !d_out || (m > 0 && !dest))
+    return fail(EAC_ERR_ARG, "cc noalloc: null array");
+  const size_t need = eac_cc_workspace_bytes(n, m);
+  if (need > kCcScratchBytes)
+    return fail(EAC_ERR_UNSUPPORTED,
+                "cc noalloc: this graph needs %zu bytes of scratch, over the fixed %zu", need,
+                kCcScratchBytes);
+  void* ws = nullptr;
+  EAC_TRY(cc_scratch_ptr(&ws));
+  Arena ar(ws, kCcScratchBytes);
+  return cc_enqueue(dest, first_edge, out_degree, n, m, labels, d_out, ar, false, s);
 }
 
 // ------------------------------------------------ first-occurrence relabel
