@@ -389,6 +389,14 @@ def run_kmeans(args):
     return out
 
 
+def co_window_py(n, knob):
+    """The co-association LDS window the library picks for n points (the
+    window knob, else 32768, at most the row range rounded up to 64)."""
+    w = knob if knob and knob > 0 else 32768
+    w = min(w, 32768, -(-n // 64) * 64)
+    return max(64, w & ~1)
+
+
 def run_coassoc(args):
     import numpy as np
     import torch
@@ -421,10 +429,12 @@ def run_coassoc(args):
         tuning = autotune_coassoc(labels, ks, use_cache=not args.retune)
     log(f"tuning: {tuning['knobs']} ({tuning['ms']} ms, cached={tuning['cached']})")
     L.check(lib.eac_profile_enable(1))
-    (ip, ind, dat, mx), ms = timed(lambda: coassoc_device(labels, ks), args.steps, args.warmup,
-                                   torch)
-    prof = read_prof(lib, L, {8: "plan", 9: "count", 10: "fill"})
+    info = {}
+    (ip, ind, dat, mx), ms = timed(lambda: coassoc_device(labels, ks, info=info), args.steps,
+                                   args.warmup, torch)
+    prof = read_prof(lib, L, {8: "plan", 9: "count", 10: "fill", 11: "stage", 12: "compact"})
     L.check(lib.eac_profile_enable(0))
+    log(f"build path: {info}")
     nnz = int(ind.numel())
     incr = 0
     lab_h = labels.cpu().numpy()
@@ -434,19 +444,43 @@ def run_coassoc(args):
     out = base_line(args, incr / (ms * 1e-3) / 1e9, "Gpair-increments/s", ms, "int32",
                     {"workload": "coassoc_batched_ensemble", "n": n, "d": 16, "partitions": P,
                      "k_range": [32, 2048], "seed": 3})
-    fill_ms = prof[10][0] / max(prof[10][1], 1)
-    fill_bytes = prof[10][2] / max(prof[10][1], 1)
+    def per_launch(slot):
+        return prof[slot][0] / max(prof[slot][1], 1) if prof[slot][1] else 0.0
+
     out["result"] = {"nnz_condensed": nnz, "max_count": mx, "pair_increments": incr,
                      "ensemble_seconds": t1 - t0, "rescored": fit["n_rescored"],
-                     "count_ms": prof[9][0] / max(prof[9][1], 1), "fill_ms": fill_ms,
-                     "plan_ms": prof[8][0] / max(prof[8][1], 1)}
+                     "build_path": info.get("path"),
+                     "sample_stride": info.get("sample_stride"),
+                     "nnz_estimate": info.get("estimate"),
+                     "plan_ms": per_launch(8), "count_ms": per_launch(9),
+                     "fill_ms": per_launch(10), "stage_ms": per_launch(11),
+                     "compact_ms": per_launch(12)}
     out["tuning"] = tuning
-    ach = fill_bytes / (fill_ms * 1e-3) / 1e9
     whole = 4.0 * P * n + 8.0 * nnz + 8.0 * (n + 1)  # labels read, (col, count) + offsets
-    out["roofline"] = {"bound": "hbm", "kernel": "k_coassoc_rows<fill>", "achieved": ach,
+    if info.get("path") == "one-pass":
+        # staged pass: member lists read once (4 B per (partition, point)),
+        # 4 B per staged entry, 8 B per (row, window) segment, 4 B per row
+        win = co_window_py(n, tuning["knobs"].get("co_window", 0))
+        nw, q = -(-n // win), n // win
+        # segment slots: sum over rows of nw - (i + 1) // win
+        slots = n * nw - (win * q * (q - 1) // 2 + q * (n - q * win + 1))
+        kname, k_ms = "k_coassoc_rows<stage>", per_launch(11)
+        k_bytes = 4.0 * P * n + 4.0 * nnz + 8.0 * slots + 4.0 * n
+        c_ms = per_launch(12)
+        c_bytes = 12.0 * nnz + 8.0 * slots + 8.0 * (n + 1)  # entries in, (col, value) out
+        out["compaction"] = {"kernel": "k_coassoc_compact", "avg_launch_ms": c_ms,
+                             "alg_bytes_per_launch": c_bytes,
+                             "achieved": c_bytes / (c_ms * 1e-3) / 1e9 if c_ms else None,
+                             "frac": c_bytes / (c_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if c_ms
+                             else None}
+    else:
+        kname, k_ms = "k_coassoc_rows<fill>", per_launch(10)
+        k_bytes = prof[10][2] / max(prof[10][1], 1)
+    ach = k_bytes / (k_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": ach,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                       "traffic": None, "avg_launch_ms": fill_ms,
-                       "alg_bytes_per_launch": fill_bytes,
+                       "traffic": None, "avg_launch_ms": k_ms,
+                       "alg_bytes_per_launch": k_bytes,
                        "whole_build_alg_bytes": whole,
                        "whole_build_frac": whole / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
     out["cpu_baseline"] = None

@@ -70,6 +70,8 @@ int eac_abi_version(void);
 #define EAC_PROF_CO_PLAN 8    /* member lists: 16 B per (partition, point)       */
 #define EAC_PROF_CO_COUNT 9   /* co-association count pass                       */
 #define EAC_PROF_CO_FILL 10   /* co-association fill: 8 B/nnz + 8 B/row written  */
+#define EAC_PROF_CO_STAGE 11  /* one-pass build: staged pass                     */
+#define EAC_PROF_CO_COMPACT 12 /* one-pass build: compaction                     */
 int eac_profile_enable(int32_t on);
 int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t* alg_bytes);
 
@@ -91,13 +93,16 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *                           weight level of the split to the next (default
  *                           4, up to 4 levels; 1 or -1 = one split only)
  *   EAC_TUNE_CO_UNROLL      co-association member loads in flight per lane
- *                           while counting (4 by default, or 8) */
+ *                           while counting (4 by default, or 8)
+ *   EAC_TUNE_CO_RANGES      co-association row ranges with a row counter
+ *                           each (1..8, default 8: one per XCD) */
 #define EAC_TUNE_LIGHT_X10 0
 #define EAC_TUNE_FIRST_BATCH 1
 #define EAC_TUNE_FILTER_BPCU 2
 #define EAC_TUNE_CO_WINDOW 3
 #define EAC_TUNE_LEVEL_STEP 4
 #define EAC_TUNE_CO_UNROLL 5
+#define EAC_TUNE_CO_RANGES 6
 #define EAC_TUNE_COUNT 8
 int eac_tune_set(int32_t knob, int32_t value);
 int eac_tune_get(int32_t knob, int32_t* value);
@@ -376,6 +381,30 @@ int eac_coassoc_fill_ranked(const int32_t* labels, int64_t n, const int64_t* h_k
                             int32_t P, int32_t full_rows, const int64_t* indptr,
                             int32_t* indices, int32_t* data, int32_t* rank, void* ws,
                             size_t ws_bytes, void* stream);
+/* One-pass build of the same matrix (the same reference functions): the
+ * pairs are visited once instead of twice.
+ *   eac_coassoc_estimate: plan + count pass over rows 0, s, 2s, ...
+ *     (s = sample_stride >= 1): h_estimate = estimated nnz (exact for s = 1);
+ *   the caller allocates a staging buffer of
+ *     eac_coassoc_stage_bytes(n, full_rows, P, pool_entries) bytes: room
+ *     for pool_entries staged entries (the estimate plus a margin) beyond
+ *     the workgroups' partly used chunks;
+ *   eac_coassoc_stage (same ws, unchanged): indptr[n+1], h_nnz, largest
+ *     count; h_nnz = -1 when the pool was too small (then build with
+ *     eac_coassoc_count + eac_coassoc_fill);
+ *   eac_coassoc_compact: indices[nnz], data[nnz] as eac_coassoc_fill writes
+ *     them, from the staging buffer (its header holds the layout). */
+int eac_coassoc_estimate(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                         int32_t P, int32_t full_rows, int64_t sample_stride,
+                         int64_t* h_estimate, void* ws, size_t ws_bytes, void* stream);
+size_t eac_coassoc_stage_bytes(int64_t n, int32_t full_rows, int32_t P, int64_t pool_entries);
+int eac_coassoc_stage(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P,
+                      int32_t full_rows, int64_t* indptr, void* stage, size_t stage_bytes,
+                      int64_t* h_nnz, int32_t* h_max_count, void* ws, size_t ws_bytes,
+                      void* stream);
+int eac_coassoc_compact(int64_t n, const int64_t* indptr, const void* stage,
+                        size_t stage_bytes, int32_t w_offset, int32_t* indices,
+                        int32_t* data, void* stream);
 
 /* ---- capacity-capped co-association (the reference's lossy matrix) ------
  * From the exact matrix (indptr, indices, count) and its ranks (above),

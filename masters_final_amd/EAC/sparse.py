@@ -36,10 +36,19 @@ def labels_from_ensemble(ensemble, n_samples):
     return labels, ks
 
 
-def coassoc_device(labels, k_list, full_rows=False, w_offset=0):
+ONE_PASS_MIN_N = 65536  # smaller ensembles: the count pass is cheap, two passes
+
+
+def coassoc_device(labels, k_list, full_rows=False, w_offset=0, one_pass=None,
+                   sample_stride=None, margin=1 / 8, info=None):
     """Co-association CSR of labels [P, n] on the GPU.
     Returns (indptr int64 [n+1], indices int32 [nnz], data int32 [nnz], max_count).
-    data = count, or w_offset - count when w_offset != 0."""
+    data = count, or w_offset - count when w_offset != 0.
+    one_pass (default: n >= ONE_PASS_MIN_N) visits the pairs once: a count
+    over every sample_stride-th row (default n // 2048, at most 32) estimates
+    nnz, the staged pass writes into a pool of estimate x (1 + margin)
+    entries and a compaction builds the CSR; a pool that proves too small
+    falls back to the two-pass build. info (a dict) receives the path taken."""
     t = L.torch()
     lib = L.load_library()
     dl = L.as_device(labels, t.int32)
@@ -51,6 +60,39 @@ def coassoc_device(labels, k_list, full_rows=False, w_offset=0):
     indptr = t.empty(n + 1, dtype=t.int64, device=dl.device)
     nnz, mx = C.c_int64(), C.c_int32()
     ws = L.workspace(lib.eac_coassoc_workspace_bytes(n, P, int(koff[-1])))
+    fr = 1 if full_rows else 0
+    kp = koff.ctypes.data_as(C.c_void_p)
+    if one_pass is None:
+        one_pass = n >= ONE_PASS_MIN_N
+    if info is not None:
+        info["path"] = "two-pass"
+    if one_pass:
+        stride = int(sample_stride) if sample_stride else max(1, min(32, n // 2048))
+        est = C.c_int64()
+        L.check(lib.eac_coassoc_estimate(L.ptr(dl), n, kp, P, fr, stride, C.byref(est), L.ptr(ws),
+                                         ws.numel(), L.stream_ptr()))
+        pool = max(int(est.value * (1 + margin)), 0)
+        nb = lib.eac_coassoc_stage_bytes(n, fr, P, pool)
+        stage = t.empty(nb, dtype=t.uint8, device=dl.device)
+        L.check(lib.eac_coassoc_stage(L.ptr(dl), n, kp, P, fr, L.ptr(indptr), L.ptr(stage), nb,
+                                      C.byref(nnz), C.byref(mx), L.ptr(ws), ws.numel(),
+                                      L.stream_ptr()))
+        if info is not None:
+            info.update(estimate=est.value, pool_entries=pool, sample_stride=stride)
+        if nnz.value >= 0:
+            m = max(nnz.value, 1)
+            indices = t.empty(m, dtype=t.int32, device=dl.device)
+            data = t.empty(m, dtype=t.int32, device=dl.device)
+            if w_offset == "max+1":
+                w_offset = mx.value + 1
+            L.check(lib.eac_coassoc_compact(n, L.ptr(indptr), L.ptr(stage), nb, int(w_offset),
+                                            L.ptr(indices), L.ptr(data), L.stream_ptr()))
+            if info is not None:
+                info["path"] = "one-pass"
+            return indptr, indices[: nnz.value], data[: nnz.value], mx.value
+        del stage
+        if info is not None:
+            info["path"] = "one-pass overflow, two-pass"
     L.check(lib.eac_coassoc_count(L.ptr(dl), n, koff.ctypes.data_as(C.c_void_p), P,
                                   1 if full_rows else 0, L.ptr(indptr), C.byref(nnz), C.byref(mx),
                                   L.ptr(ws), ws.numel(), L.stream_ptr()))

@@ -19,8 +19,8 @@
 //    point's position in its list. A point's later co-members in partition p
 //    are then the suffix of its cluster's list after its own position: no
 //    search and no pair materialisation.
-//  * Count/fill: one workgroup per row (rows handed out by an atomic
-//    counter). Columns are covered in windows of 32768; each window has a
+//  * Count/fill: one workgroup per row (rows handed out by atomic counters,
+//    one per XCD over its own stretch of the rows). Columns are covered in windows of 32768; each window has a
 //    dense array of u16 counters (two per u32 word, 64 KiB of LDS). Per window every partition's
 //    suffix cursor advances over the members inside the window (flattened
 //    over partitions so that lanes stream consecutive members), each member
@@ -41,6 +41,8 @@ namespace {
 constexpr int kCB = 256;       // threads per row workgroup
 constexpr int kWin = 32768;    // columns per LDS window (two u16 counters per u32 word)
 constexpr int kMaxP = 1024;    // partitions staged in LDS
+constexpr int kMaxRanges = 8;  // row ranges (one per XCD)
+constexpr int kCtrs = kMaxRanges + 2;  // row counters, then the sampled nnz sum
 
 // keys: global cluster id (unassigned points -> ktot, sorted last);
 // values: p * n + i
@@ -114,7 +116,43 @@ struct CoPlan {
   const int32_t* wtab;    // window table (nullptr: search the window ends)
   int nw;                 // windows of the table: ceil(n / win)
   const int32_t* first;   // ranked fill: the partition processed first (device)
+  int ranges;             // row ranges with a counter each (1..kMaxRanges)
+  int64_t row_stride;     // > 1: only rows 0, stride, 2*stride, ... (a sample)
 };
+
+// Outputs of the row kernel (pointers a pass does not use are null).
+struct CoOut {
+  unsigned long long* row_ctr;  // [kMaxRanges] row counters, zeroed
+  int32_t* row_nnz;             // PASS 0 / 2: nnz of each row
+  unsigned int* max_count;      // PASS 0 / 2: largest count
+  unsigned long long* nnz_sum;  // PASS 0: sum of the visited rows' nnz
+  const int64_t* indptr;        // PASS 1: row offsets
+  int32_t w_offset;             // PASS 1: data = w_offset - count when non-zero
+  int32_t* col;                 // PASS 1: columns
+  int32_t* val;                 // PASS 1: counts (or w_offset - count)
+  int32_t* rank;                // PASS 1 with RANK: first partition rank
+  uint32_t* pool;               // PASS 2: staged entries (count << 16 | column - c0)
+  uint64_t* seg;                // PASS 2: per (row, window) offset << 16 | entries
+  unsigned long long* pool_ctr; // PASS 2: next free pool chunk
+  int64_t pool_cap;             // PASS 2: pool entries
+  int64_t pool_chunk;           // PASS 2: entries per chunk (>= win)
+  unsigned int* overflow;       // PASS 2: set when the pool ran out
+};
+
+// Staged entries are taken from the pool in chunks (one global atomic per
+// chunk) of up to this many, or 16 windows when that is more.
+constexpr int64_t kPoolChunk = 65536;
+
+// Segment-table slot of (row i, first window of the row): rows keep the
+// windows from w0(i) = (i + 1) / win on (every window with full rows), so
+// the table packs them with sum_{r < i} (nw - w0(r)) slots before row i;
+// sum_{m=1..i} floor(m / win) = win * q (q - 1) / 2 + q (i - q win + 1)
+// with q = floor(i / win).
+__host__ __device__ inline int64_t seg_base(int64_t i, int64_t win, int64_t nw, int full_rows) {
+  if (full_rows) return i * nw;
+  const int64_t q = i / win;
+  return i * nw - (win * q * (q - 1) / 2 + q * (i - q * win + 1));
+}
 
 // The partition the reference processes first: the one with the fewest
 // non-empty clusters, the first such (sparse.py:158: np.argmin over the
@@ -137,21 +175,24 @@ __global__ __launch_bounds__(256) void k_first_partition(const int32_t* __restri
 
 // One row: walk every partition's co-members > i (or all co-members != i for
 // full rows) window by window. PASS 0: per-row nnz and max count. PASS 1:
-// write (col, value) at out_off. RANK (PASS 1 only): also write, per entry,
+// write (col, value) at indptr[i]. RANK (PASS 1 only): also write, per entry,
 // the processing rank of the first partition in which the pair co-occurs
 // (0 = the partition pl.first[0], then the others in index order), kept in a
-// second LDS window with one atomicMin per pair visit. U: member loads in
-// flight per lane while visiting (4 or 8, a tuning knob).
+// second LDS window with one atomicMin per pair visit. PASS 2 (one-pass
+// build): each window's entries go to the workgroup's current pool chunk as
+// (count << 16 | column - c0), the window's (pool offset, entries) to the
+// segment table, and the row's nnz and the max count as in PASS 0; a
+// workgroup that finds the pool used up sets the overflow flag and stops.
+// U: member loads in flight per lane while visiting (4 or 8, a tuning knob).
 template <int PASS, bool RANK = false, int U = 4>
 __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, int win,
-                                                      unsigned long long* __restrict__ row_ctr,
-                                                      int32_t* __restrict__ row_nnz,
-                                                      unsigned int* __restrict__ max_count,
-                                                      const int64_t* __restrict__ indptr,
-                                                      int32_t w_offset,
-                                                      int32_t* __restrict__ out_col,
-                                                      int32_t* __restrict__ out_val,
-                                                      int32_t* __restrict__ out_rank) {
+                                                      CoOut out) {
+  unsigned long long* __restrict__ row_ctr = out.row_ctr;
+  const int64_t* __restrict__ indptr = out.indptr;
+  const int32_t w_offset = out.w_offset;
+  int32_t* __restrict__ out_col = out.col;
+  int32_t* __restrict__ out_val = out.val;
+  int32_t* __restrict__ out_rank = out.rank;
   // launch-time LDS: window counters (win / 2 words), with RANK the rank
   // window (win words), then per partition the next member position, the
   // cluster end, this window's member end and the flattened-range prefix
@@ -167,6 +208,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   int32_t* s_prank = s_g + pl.P;
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
+  __shared__ unsigned long long s_chunk;
   __shared__ unsigned int s_max;
   const int t = threadIdx.x;
   const int64_t n = pl.n;
@@ -184,17 +226,45 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   // there: rows in flight at the same time then share most of their
   // clusters, so their member runs are read from cache. The output does not
   // depend on the order (each row writes its own segment).
+  // The row order is cut into pl.ranges contiguous ranges with a counter
+  // each; workgroup b starts on range b % ranges. Workgroups are dispatched
+  // to the XCDs round-robin, so with 8 ranges each XCD (own L2) works
+  // through its own stretch of the order and its resident rows share their
+  // clusters' member lists in that L2. A workgroup whose range is used up
+  // moves on to the next ranges (thread 0 keeps the cursor).
   const int32_t* order = nullptr;
   {
     const int32_t a = pl.cstart[pl.k_off[0]], b = pl.cstart[pl.k_off[1]];
     if ((int64_t)(b - a) == n) order = pl.members + a;
   }
+  // a sample (row_stride > 1) takes rows 0, stride, 2 * stride, ...
+  const int64_t stride = pl.row_stride > 1 ? pl.row_stride : 1;
+  const int64_t nrows = (n + stride - 1) / stride;
+  if (stride > 1) order = nullptr;
+  int xr = (int)(blockIdx.x % (unsigned)pl.ranges), tries = 0;
+  // PASS 2: the workgroup's pool chunk [chunk_pos, chunk_end) (uniform)
+  int64_t chunk_pos = 0, chunk_end = 0;
+  uint32_t mx_row = 0;  // PASS 2: largest count this lane emitted
+  if (t == 0) s_max = 0;  // a workgroup may get no row
   for (;;) {
-    if (t == 0) s_row = (int64_t)atomicAdd(row_ctr, 1ull);
+    if (t == 0) {
+      int64_t r = nrows;
+      while (tries < pl.ranges) {
+        const int64_t lo = nrows * xr / pl.ranges, hi = nrows * (xr + 1) / pl.ranges;
+        const int64_t q = lo + (int64_t)atomicAdd(&row_ctr[xr], 1ull);
+        if (q < hi) {
+          r = q;
+          break;
+        }
+        xr = xr + 1 == pl.ranges ? 0 : xr + 1;
+        ++tries;
+      }
+      s_row = r;
+    }
     __syncthreads();
     const int64_t r = s_row;
-    if (r >= n) break;
-    const int64_t i = order ? (int64_t)order[r] : r;
+    if (r >= nrows) break;
+    const int64_t i = order ? (int64_t)order[r] : r * stride;
     if (t == 0) s_max = 0;
     for (int p = t; p < P; p += kCB) {
       int32_t q = pl.pos[(int64_t)p * n + i];
@@ -210,6 +280,9 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
     }
     int64_t row_base = PASS == 1 ? indptr[i] : 0;
     int64_t written = 0;
+    // PASS 2: the row's first segment-table slot and its first window
+    const int64_t w_first = full_rows ? 0 : (i + 1) / win;
+    const int64_t seg0 = PASS == 2 ? seg_base(i, win, pl.nw, full_rows) : 0;
     uint32_t nnz_local = 0;
     // windows aligned to multiples of win (columns <= i get no counts)
     for (int64_t c0 = full_rows ? 0 : ((i + 1) / win) * win; c0 < n; c0 += win) {
@@ -376,7 +449,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         }
         nnz_local += c;
         if (mx) atomicMax(&s_max, mx);
-      } else {
+      } else {  // PASS 1 and 2
         // wave w emits groups [w*RG, (w+1)*RG) in column order
         const int w = t / kWave, l = lane_id();
         const int RG = ((groups + 4 * kWave - 1) / (4 * kWave)) * kWave;
@@ -399,6 +472,23 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           if (q < w) o += s_tmp[q];
           wtot += s_tmp[q];
         }
+        if (PASS == 2) {  // block-uniform: wtot, chunk_pos, chunk_end
+          if (chunk_pos + (int64_t)wtot > chunk_end) {
+            if (t == 0) s_chunk = atomicAdd(out.pool_ctr, (unsigned long long)out.pool_chunk);
+            __syncthreads();
+            chunk_pos = (int64_t)s_chunk;
+            chunk_end = chunk_pos + out.pool_chunk;
+            if (chunk_end > out.pool_cap) {
+              if (t == 0) atomicOr(out.overflow, 1u);
+              return;  // the whole workgroup: the caller rebuilds in two passes
+            }
+          }
+          o += chunk_pos - row_base - written;
+          if (t == 0)
+            out.seg[seg0 + c0 / win - w_first] =
+                wtot ? ((uint64_t)chunk_pos << 16) | (uint64_t)wtot : 0ull;
+          chunk_pos += wtot;
+        }
         for (int g0 = ga; g0 < gb; g0 += kWave) {
           const int g = g0 + l;
           uint32_t xw[4] = {0u, 0u, 0u, 0u};
@@ -408,6 +498,19 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           for (int q = 0; q < 4; ++q) cnt += ((xw[q] & 0xffffu) != 0) + ((xw[q] >> 16) != 0);
           const uint32_t incl = wave_incl_scan(cnt);
           int64_t at = o + (incl - cnt);
+          if (PASS == 2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const uint32_t lo = xw[q] & 0xffffu, hi = xw[q] >> 16;
+              const uint32_t cw = 2u * (uint32_t)(4 * g + q);  // column - c0
+              if (lo) out.pool[at++] = (lo << 16) | cw;
+              if (hi) out.pool[at++] = (hi << 16) | (cw + 1u);
+              mx_row = lo > mx_row ? lo : mx_row;
+              mx_row = hi > mx_row ? hi : mx_row;
+            }
+            o += __shfl(incl, kWave - 1, kWave);
+            continue;
+          }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const uint32_t lo = xw[q] & 0xffffu, hi = xw[q] >> 16;
@@ -441,11 +544,52 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       uint32_t rt;
       (void)block_excl_scan<uint32_t>(nnz_local, s_tmp, &rt);
       if (t == 0) {
-        row_nnz[i] = (int32_t)rt;
-        if (s_max) atomicMax(max_count, s_max);
+        out.row_nnz[i] = (int32_t)rt;
+        if (out.nnz_sum) atomicAdd(out.nnz_sum, (unsigned long long)rt);
+        if (s_max) atomicMax(out.max_count, s_max);
       }
     }
+    if (PASS == 2 && t == 0) out.row_nnz[i] = (int32_t)written;
     __syncthreads();
+  }
+  if (PASS == 2) {
+    if (mx_row) atomicMax(&s_max, mx_row);
+    __syncthreads();
+    if (t == 0 && s_max) atomicMax(out.max_count, s_max);
+  }
+}
+
+// One-pass build, last step: copies each row's staged segments, in window
+// order, to indices/data at indptr[i] (one wavefront per row; the segment
+// headers are wave-uniform loads).
+__global__ __launch_bounds__(256) void k_coassoc_compact(
+    const unsigned char* __restrict__ stage, const int64_t* __restrict__ indptr, int64_t n,
+    int32_t w_offset, int32_t* __restrict__ out_col, int32_t* __restrict__ out_val) {
+  const int64_t* hdr = reinterpret_cast<const int64_t*>(stage);
+  const int win = (int)hdr[0], nw = (int)hdr[1], full_rows = (int)hdr[5];
+  const uint64_t* __restrict__ seg = reinterpret_cast<const uint64_t*>(stage + 256);
+  const uint32_t* __restrict__ pool = reinterpret_cast<const uint32_t*>(stage + hdr[4]);
+  const int l = lane_id();
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  for (int64_t i = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < n;
+       i += waves) {
+    const int64_t w_first = full_rows ? 0 : (i + 1) / win;
+    const uint64_t* sg = seg + seg_base(i, win, nw, full_rows);
+    int64_t dst = indptr[i];
+    for (int64_t w = w_first; w < nw; ++w) {
+      const uint64_t h = sg[w - w_first];
+      const int cnt = (int)(h & 0xffffu);
+      if (!cnt) continue;
+      const uint32_t* src = pool + (h >> 16);
+      const int32_t c0 = (int32_t)(w * win);
+      for (int e = l; e < cnt; e += kWave) {
+        const uint32_t x = src[e];
+        const int32_t c = (int32_t)(x >> 16);
+        out_col[dst + e] = c0 + (int32_t)(x & 0xffffu);
+        out_val[dst + e] = w_offset ? w_offset - c : c;
+      }
+      dst += cnt;
+    }
   }
 }
 
@@ -460,7 +604,7 @@ struct CoBufs {
   int32_t* csize;
   int32_t* cstart;
   int32_t* row_nnz;
-  unsigned long long* ctr;
+  unsigned long long* ctr;  // [kCtrs]: row counters, [kMaxRanges] sampled nnz sum
   unsigned int* misc;  // [0] bad label, [1] max count
 };
 
@@ -473,7 +617,7 @@ CoBufs carve_co(Arena& ar, int64_t n, int P, int64_t ktot) {
   b.csize = ar.take<int32_t>(ktot + 2);
   b.cstart = ar.take<int32_t>(ktot + 2);
   b.row_nnz = ar.take<int32_t>(n + 1);
-  b.ctr = ar.take<unsigned long long>(2);
+  b.ctr = ar.take<unsigned long long>(kCtrs);
   b.misc = ar.take<unsigned int>(4);
   // sort staging last: reused by later passes
   b.key = ar.take<uint32_t>(items);
@@ -520,6 +664,12 @@ int co_window(int64_t n) {
 // Member loads in flight per lane in the pair visits (EAC_TUNE_CO_UNROLL:
 // 4 by default, or 8).
 inline int co_unroll() { return tune_value(EAC_TUNE_CO_UNROLL) == 8 ? 8 : 4; }
+// Row ranges of the co-association row loop (EAC_TUNE_CO_RANGES: 1..8,
+// default 8, one per XCD).
+inline int co_ranges() {
+  const int r = tune_value(EAC_TUNE_CO_RANGES);
+  return r <= 0 || r > kMaxRanges ? kMaxRanges : r;
+}
 inline size_t co_lds(int win, int P, bool rank = false) {
   return (size_t)(win / 2 + (rank ? win + P : 0) + 5 * P + 1) * 4;
 }
@@ -574,19 +724,20 @@ extern "C" size_t eac_coassoc_workspace_bytes(int64_t n, int32_t P, int64_t k_to
   return ar.used + m + 4096;
 }
 
-// Pass 1: builds the member plan in ws (keep ws unchanged for the fill),
-// writes indptr[n+1] (int64) and returns nnz and the largest count.
-extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t* h_koff,
-                                 int32_t P, int32_t full_rows, int64_t* indptr, int64_t* h_nnz,
-                                 int32_t* h_max_count, void* ws, size_t ws_bytes, void* stream) {
+namespace eac {
+namespace {
+// The member plan in ws (shared by every pass over the same labels):
+// global cluster keys, the stable sort of (cluster, point), cluster starts,
+// each point's position in its cluster's list; zeroes the row counters and
+// flags.
+int co_plan_build(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P,
+                  void* ws, size_t ws_bytes, CoBufs* out, size_t* mark_out, Arena& ar,
+                  hipStream_t s) {
   EAC_TRY(co_check(labels, n, h_koff, P));
-  if (!indptr) return fail(EAC_ERR_ARG, "coassoc: null indptr");
   const int64_t ktot = h_koff[P];
   size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
   if (!ws || ws_bytes < need)
     return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
-  hipStream_t s = (hipStream_t)stream;
-  Arena ar(ws, ws_bytes);
   CoBufs B = carve_co(ar, n, P, ktot);
   const size_t mark = ar.used;
   const int64_t items = (int64_t)P * n;
@@ -595,7 +746,7 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
   EAC_HIP_CHECK(hipStreamSynchronize(s));
   EAC_TRY(fill<int32_t>(B.csize, ktot + 2, 0, s));
   EAC_TRY(fill<int32_t>((int32_t*)B.misc, 4, 0, s));
-  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
+  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, kCtrs, 0, s));
   int pr = prof_begin(EAC_PROF_CO_PLAN, 16 * items, s);
   k_member_keys<<<grid_for(items, 256, 8192), 256, 0, s>>>(labels, n, P, B.k_off, ktot, B.key,
                                                            B.val, B.csize, B.misc);
@@ -610,18 +761,49 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
                                                           B.members, B.pos);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
+  *out = B;
+  *mark_out = mark;
+  return EAC_OK;
+}
+
+// Launches the row kernel for PASS 0 or 2 (unranked) with the unroll knob.
+template <int PASS>
+int co_launch(const CoPlan& pl, int full_rows, int win, const CoOut& o, hipStream_t s) {
+  if (co_unroll() == 8)
+    k_coassoc_rows<PASS, false, 8><<<co_grid<PASS, false, 8>(win, pl.P), kCB,
+                                     co_lds(win, pl.P), s>>>(pl, full_rows, win, o);
+  else
+    k_coassoc_rows<PASS><<<co_grid<PASS>(win, pl.P), kCB, co_lds(win, pl.P), s>>>(pl, full_rows,
+                                                                                 win, o);
+  EAC_LAUNCH_CHECK();
+  return EAC_OK;
+}
+}  // namespace
+}  // namespace eac
+
+// Pass 1: builds the member plan in ws (keep ws unchanged for the fill),
+// writes indptr[n+1] (int64) and returns nnz and the largest count.
+extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                 int32_t P, int32_t full_rows, int64_t* indptr, int64_t* h_nnz,
+                                 int32_t* h_max_count, void* ws, size_t ws_bytes, void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (!indptr) return fail(EAC_ERR_ARG, "coassoc: null indptr");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ktot = h_koff[P];
+  Arena ar(ws, ws_bytes);
+  CoBufs B;
+  size_t mark = 0;
+  EAC_TRY(co_plan_build(labels, n, h_koff, P, ws, ws_bytes, &B, &mark, ar, s));
   const int win = co_window(n);
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoOut o{};
+  o.row_ctr = B.ctr;
+  o.row_nnz = B.row_nnz;
+  o.max_count = B.misc + 1;
   int pr2 = prof_begin(EAC_PROF_CO_COUNT, 0, s);
-  if (co_unroll() == 8)
-    k_coassoc_rows<0, false, 8><<<co_grid<0, false, 8>(win, P), kCB, co_lds(win, P), s>>>(
-        pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
-  else
-    k_coassoc_rows<0><<<co_grid<0>(win, P), kCB, co_lds(win, P), s>>>(
-        pl, full_rows, win, B.ctr, B.row_nnz, B.misc + 1, nullptr, 0, nullptr, nullptr, nullptr);
-  EAC_LAUNCH_CHECK();
+  EAC_TRY(co_launch<0>(pl, full_rows, win, o, s));
   prof_end(pr2, s);
   EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
   ar.used = mark;
@@ -652,7 +834,7 @@ int coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_
     return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
   Arena ar(ws, ws_bytes);
   CoBufs B = carve_co(ar, n, P, ktot);
-  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, 2, 0, s));
+  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, kCtrs, 0, s));
   int64_t nnz = 0;
   EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
   int win = co_window(n);
@@ -660,19 +842,25 @@ int coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
   int32_t* d_first = reinterpret_cast<int32_t*>(B.misc + 2);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, d_first};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, d_first, co_ranges(), 1};
+  CoOut o{};
+  o.row_ctr = B.ctr;
+  o.indptr = indptr;
+  o.w_offset = w_offset;
+  o.col = indices;
+  o.val = data;
+  o.rank = rank;
   int pr = prof_begin(EAC_PROF_CO_FILL, 8 * nnz + 8 * (n + 1), s);
   if (rank) {
     k_first_partition<<<1, 256, 0, s>>>(B.cstart, B.k_off, P, d_first);
     EAC_LAUNCH_CHECK();
     k_coassoc_rows<1, true><<<co_grid<1, true>(win, P), kCB, co_lds(win, P, true), s>>>(
-        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, rank);
+        pl, full_rows, win, o);
   } else if (co_unroll() == 8) {
     k_coassoc_rows<1, false, 8><<<co_grid<1, false, 8>(win, P), kCB, co_lds(win, P), s>>>(
-        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, nullptr);
+        pl, full_rows, win, o);
   } else {
-    k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(
-        pl, full_rows, win, B.ctr, nullptr, nullptr, indptr, w_offset, indices, data, nullptr);
+    k_coassoc_rows<1><<<co_grid<1>(win, P), kCB, co_lds(win, P), s>>>(pl, full_rows, win, o);
   }
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
@@ -700,6 +888,184 @@ extern "C" int eac_coassoc_fill_ranked(const int32_t* labels, int64_t n, const i
   if (!rank) return fail(EAC_ERR_ARG, "coassoc: null rank output");
   return coassoc_fill(labels, n, h_koff, P, full_rows, indptr, 0, indices, data, rank, ws,
                       ws_bytes, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------- one-pass build
+// The count pass visits every pair once only to size the output; the
+// one-pass build visits them once: the plan plus a sampled count estimate
+// the nnz, the caller allocates a staging buffer from it, the staged pass
+// writes every window's entries (4 bytes each) to chunks of that buffer
+// and the row sizes, and a compaction copies the segments into the CSR.
+// A pool that turns out too small is reported (h_nnz = -1): the caller
+// then runs eac_coassoc_count + eac_coassoc_fill.
+namespace eac {
+namespace {
+// staging buffer: header (int64: win, nw, pool counter, overflow flag,
+// pool byte offset, full_rows), then the segment table, then the pool
+// (4-byte entries). The compaction takes the layout from the header.
+constexpr size_t kStageHdr = 256;
+__global__ void k_stage_header(int64_t* hdr, int win, int nw, int64_t pool_at, int full_rows) {
+  if (threadIdx.x == 0) {
+    hdr[0] = win;
+    hdr[1] = nw;
+    hdr[2] = 0;
+    hdr[3] = 0;
+    hdr[4] = pool_at;
+    hdr[5] = full_rows;
+  }
+}
+struct StageLayout {
+  int win, nw;
+  int64_t slots;     // segment-table entries
+  size_t pool_at;    // byte offset of the pool
+  int64_t chunk;     // pool entries per chunk: n / 8 rounded up to a power
+                     // of two, at most kPoolChunk, and at least 16 windows
+                     // (a chunk loses at most one window's width at its end)
+};
+StageLayout stage_layout(int64_t n, int full_rows) {
+  StageLayout L;
+  L.win = co_window(n);
+  L.nw = (int)((n + L.win - 1) / L.win);
+  L.slots = seg_base(n, L.win, L.nw, full_rows);
+  L.pool_at = kStageHdr + (((size_t)L.slots * 8 + 255) & ~(size_t)255);
+  int64_t c = 1;
+  while (c < n / 8 && c < kPoolChunk) c <<= 1;
+  L.chunk = c < 16 * (int64_t)L.win ? 16 * (int64_t)L.win : c;
+  return L;
+}
+}  // namespace
+}  // namespace eac
+
+// Plan (as eac_coassoc_count) plus the count pass over rows 0, stride,
+// 2 * stride, ...: h_estimate = their nnz sum x (rows / sampled rows),
+// rounded up. Keep ws for eac_coassoc_stage.
+extern "C" int eac_coassoc_estimate(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                    int32_t P, int32_t full_rows, int64_t sample_stride,
+                                    int64_t* h_estimate, void* ws, size_t ws_bytes,
+                                    void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (sample_stride < 1) return fail(EAC_ERR_ARG, "coassoc: sample stride must be >= 1");
+  if (!h_estimate) return fail(EAC_ERR_ARG, "coassoc: null estimate");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t ktot = h_koff[P];
+  Arena ar(ws, ws_bytes);
+  CoBufs B;
+  size_t mark = 0;
+  EAC_TRY(co_plan_build(labels, n, h_koff, P, ws, ws_bytes, &B, &mark, ar, s));
+  const int win = co_window(n);
+  int nw = 0;
+  const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
+  CoPlan pl{labels, n, P,  B.k_off,     B.members,    B.pos, B.cstart,
+            wtab,   nw, nullptr, co_ranges(), sample_stride};
+  CoOut o{};
+  o.row_ctr = B.ctr;
+  o.row_nnz = B.row_nnz;
+  o.max_count = B.misc + 1;
+  o.nnz_sum = B.ctr + kMaxRanges;
+  int pr = prof_begin(EAC_PROF_CO_COUNT, 0, s);
+  EAC_TRY(co_launch<0>(pl, full_rows, win, o, s));
+  prof_end(pr, s);
+  unsigned int misc[4];
+  EAC_TRY(d2h(misc, B.misc, sizeof(misc), s));
+  unsigned long long sum = 0;
+  EAC_TRY(d2h(&sum, B.ctr + kMaxRanges, sizeof(sum), s));
+  prof_flush();
+  if (misc[0]) return fail(EAC_ERR_ARG, "coassoc: label >= k of its partition");
+  const int64_t rows = (n + sample_stride - 1) / sample_stride;
+  *h_estimate = (int64_t)(((long double)sum * n + rows - 1) / rows);
+  return EAC_OK;
+}
+
+// Bytes of the staging buffer with room for pool_entries staged entries
+// beyond the partly used chunks of the workgroups (the segment table and
+// the chunk size depend on n, full_rows and the window knob).
+extern "C" size_t eac_coassoc_stage_bytes(int64_t n, int32_t full_rows, int32_t P,
+                                          int64_t pool_entries) {
+  if (n < 1) n = 1;
+  if (P < 1) P = 1;
+  if (pool_entries < 0) pool_entries = 0;
+  const StageLayout L = stage_layout(n, full_rows);
+  const int64_t grid = co_unroll() == 8 ? co_grid<2, false, 8>(L.win, P) : co_grid<2>(L.win, P);
+  return L.pool_at + (size_t)(pool_entries + (grid + 1) * L.chunk) * 4;
+}
+
+// Staged pass (ws from eac_coassoc_estimate): writes indptr[n+1] and, in
+// stage, every row's window segments; h_nnz = nnz and h_max_count the
+// largest count, or h_nnz = -1 when the pool was too small.
+extern "C" int eac_coassoc_stage(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                 int32_t P, int32_t full_rows, int64_t* indptr, void* stage,
+                                 size_t stage_bytes, int64_t* h_nnz, int32_t* h_max_count,
+                                 void* ws, size_t ws_bytes, void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (!indptr || !stage || !h_nnz) return fail(EAC_ERR_ARG, "coassoc: null argument");
+  const int64_t ktot = h_koff[P];
+  size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "coassoc: workspace %zu < %zu bytes", ws_bytes, need);
+  const StageLayout SL = stage_layout(n, full_rows);
+  if (stage_bytes < SL.pool_at + (size_t)SL.chunk * 4)
+    return fail(EAC_ERR_WORKSPACE, "coassoc: staging buffer %zu < %zu bytes", stage_bytes,
+                SL.pool_at + (size_t)SL.chunk * 4);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  CoBufs B = carve_co(ar, n, P, ktot);
+  const size_t mark = ar.used;
+  unsigned char* st = static_cast<unsigned char*>(stage);
+  int64_t* hdr = reinterpret_cast<int64_t*>(st);
+  k_stage_header<<<1, 64, 0, s>>>(hdr, SL.win, SL.nw, (int64_t)SL.pool_at, full_rows);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY(fill<int64_t>((int64_t*)B.ctr, kCtrs, 0, s));
+  EAC_TRY(fill<int32_t>((int32_t*)B.misc, 4, 0, s));
+  int nw = 0;
+  const int32_t* wtab = build_win_table(B, n, P, ktot, SL.win, &nw, s);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoOut o{};
+  o.row_ctr = B.ctr;
+  o.row_nnz = B.row_nnz;
+  o.max_count = B.misc + 1;
+  o.seg = reinterpret_cast<uint64_t*>(st + kStageHdr);
+  o.pool = reinterpret_cast<uint32_t*>(st + SL.pool_at);
+  o.pool_ctr = reinterpret_cast<unsigned long long*>(hdr + 2);
+  o.pool_cap = (int64_t)((stage_bytes - SL.pool_at) / 4);
+  o.pool_chunk = SL.chunk;
+  o.overflow = reinterpret_cast<unsigned int*>(hdr + 3);
+  int pr = prof_begin(EAC_PROF_CO_STAGE, 0, s);
+  EAC_TRY(co_launch<2>(pl, full_rows, SL.win, o, s));
+  prof_end(pr, s);
+  EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
+  ar.used = mark;
+  unsigned int misc[4];
+  EAC_TRY(d2h(misc, B.misc, sizeof(misc), s));
+  int64_t tail[2] = {0, 0};  // overflow flag (hdr[3])
+  EAC_TRY(d2h(tail, hdr + 2, sizeof(tail), s));
+  int64_t nnz = 0;
+  EAC_TRY(d2h(&nnz, indptr + n, sizeof(int64_t), s));
+  prof_flush();
+  *h_nnz = tail[1] ? -1 : nnz;
+  if (h_max_count) *h_max_count = (int32_t)misc[1];
+  return EAC_OK;
+}
+
+// Compaction of a staged build: indices[nnz] / data[nnz] as
+// eac_coassoc_fill writes them (data = count, or w_offset - count).
+extern "C" int eac_coassoc_compact(int64_t n, const int64_t* indptr, const void* stage,
+                                   size_t stage_bytes, int32_t w_offset, int32_t* indices,
+                                   int32_t* data, void* stream) {
+  if (n < 1) return fail(EAC_ERR_ARG, "coassoc: need n >= 1");
+  if (!indptr || !stage || !indices || !data) return fail(EAC_ERR_ARG, "coassoc: null argument");
+  if (stage_bytes < kStageHdr)
+    return fail(EAC_ERR_WORKSPACE, "coassoc: staging buffer %zu < %zu bytes", stage_bytes,
+                kStageHdr);
+  hipStream_t s = (hipStream_t)stream;
+  int pr = prof_begin(EAC_PROF_CO_COMPACT, 0, s);
+  const int64_t blocks = (n + 3) / 4;
+  k_coassoc_compact<<<(unsigned)(blocks < 65536 ? blocks : 65536), 256, 0, s>>>(
+      static_cast<const unsigned char*>(stage), indptr, n, w_offset, indices, data);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr, s);
+  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  prof_flush();
+  return EAC_OK;
 }
 
 // ------------------------------------------------ capacity-capped selection

@@ -23,7 +23,7 @@ from pathlib import Path
 from . import _lib as L
 
 KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3, "level_step": 4,
-         "co_unroll": 5}
+         "co_unroll": 5, "co_ranges": 6}
 
 DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),

@@ -667,12 +667,15 @@ int64_t orc_coassoc_condensed(const int64_t* labels, const int64_t* kp, int64_t 
   for (int64_t step = 0; step < P; ++step) {
     int64_t p = step == 0 ? first : (step <= first ? step - 1 : step);
     const int64_t* lab = labels + p * n;
+    /* a negative label: the point is in no cluster of this partition */
     memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
-    for (int64_t i = 0; i < n; ++i) cnt[lab[i]]++;
+    for (int64_t i = 0; i < n; ++i)
+      if (lab[i] >= 0) cnt[lab[i]]++;
     off[0] = 0;
     for (int64_t c = 0; c < kmax; ++c) off[c + 1] = off[c] + cnt[c];
     memset(cnt, 0, sizeof(int64_t) * (kmax + 1));
-    for (int64_t i = 0; i < n; ++i) mem[off[lab[i]] + cnt[lab[i]]++] = i;
+    for (int64_t i = 0; i < n; ++i)
+      if (lab[i] >= 0) mem[off[lab[i]] + cnt[lab[i]]++] = i;
     for (int64_t c = 0; c < kp[p]; ++c) {
       const int64_t* S = mem + off[c];
       int64_t s = off[c + 1] - off[c];

@@ -225,7 +225,8 @@ def sl_mst_lifetime(u, v, w, nv, disconnect_weight=None):
 
 
 def coassoc_condensed(labels, kp):
-    """labels: (P, n) ints in [0, k_p). Returns upper-triangle CSR (indptr, indices, counts)."""
+    """labels: (P, n) ints in [0, k_p), or < 0 for a point in no cluster of
+    partition p. Returns upper-triangle CSR (indptr, indices, counts)."""
     L = _a(labels, np.int64)
     kp = _a(kp, np.int64)
     P, n = L.shape

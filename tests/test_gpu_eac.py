@@ -52,8 +52,8 @@ def test_coassoc_condensed_vs_oracle(torch, n, P, kmin, kmax, seed):
 
 @pytest.mark.parametrize("window", [0, 4096])
 def test_coassoc_unroll_knob_same_matrix(torch, window):
-    """The tuning knobs (LDS window, member loads in flight) change speed
-    only: every setting gives the oracle's matrix."""
+    """The tuning knobs (LDS window, member loads in flight, row ranges)
+    change speed only: every setting gives the oracle's matrix."""
     from masters_final_amd.EAC.sparse import coassoc_device
     from masters_final_amd.tune import get_knobs, set_knobs
     rng = np.random.default_rng(17)
@@ -63,14 +63,15 @@ def test_coassoc_unroll_knob_same_matrix(torch, window):
     saved = get_knobs()
     try:
         outs = []
-        for u in (4, 8):
-            set_knobs(co_window=window, co_unroll=u)
+        for u, ranges in ((4, 0), (8, 1), (4, 3)):
+            set_knobs(co_window=window, co_unroll=u, co_ranges=ranges)
             gi, gc, gd, mx = coassoc_device(labels, ks, full_rows=(window == 0))
             outs.append((gi.cpu().numpy(), gc.cpu().numpy(), gd.cpu().numpy(), mx))
     finally:
         set_knobs(**saved)
-    for a, b in zip(outs[0], outs[1]):
-        assert np.array_equal(a, b)
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
     ip, ind, cnt = O.coassoc_condensed(labels, ks) if window else (None, None, None)
     if ip is not None:
         assert np.array_equal(outs[1][0], ip) and np.array_equal(outs[1][1], ind)

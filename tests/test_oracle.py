@@ -247,18 +247,24 @@ def _dense_coassoc(labels):
     C = np.zeros((n, n), np.int64)
     for p in range(P):
         for c in np.unique(labels[p]):
+            if c < 0:
+                continue
             idx = np.nonzero(labels[p] == c)[0]
             np.add.at(C, (idx[:, None], idx[None, :]), 1)
     np.fill_diagonal(C, 0)
     return C
 
 
-@pytest.mark.parametrize("seed", range(3))
-def test_coassoc_condensed_vs_dense(seed):
+@pytest.mark.parametrize("seed,unassigned", [(0, 0), (1, 0), (2, 0), (3, 500)])
+def test_coassoc_condensed_vs_dense(seed, unassigned):
+    """Brute-force dense counts; negative labels (points in no cluster of a
+    partition) count for nothing."""
     rng = np.random.default_rng(seed)
     n, P = 300, 12
     kp = rng.integers(3, 20, P)
     labels = np.stack([rng.integers(0, k, n) for k in kp])
+    if unassigned:
+        labels[rng.integers(0, P, unassigned), rng.integers(0, n, unassigned)] = -1
     indptr, ind, cnt = O.coassoc_condensed(labels, kp)
     D = np.triu(_dense_coassoc(labels), 1)
     r, c = np.nonzero(D)
