@@ -196,7 +196,8 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   // launch-time LDS: window counters (win / 2 words), with RANK the rank
   // window (win words), then per partition the next member position, the
   // cluster end, this window's member end and the flattened-range prefix
-  // (P + 1), the row's global cluster, with RANK each partition's rank
+  // (P + 1), the row's global cluster, this window's member position minus
+  // flattened index, with RANK each partition's rank
   extern __shared__ __align__(16) uint32_t s_dyn[];
   uint32_t* s_cnt = s_dyn;
   uint32_t* s_rank = s_dyn + win / 2;
@@ -205,7 +206,8 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   int32_t* s_hi = s_end + pl.P;
   int32_t* s_pre = s_hi + pl.P;
   int32_t* s_g = s_pre + pl.P + 1;  // global cluster of the row in each partition
-  int32_t* s_prank = s_g + pl.P;
+  int32_t* s_d = s_g + pl.P;        // this window: member position - flattened index
+  int32_t* s_prank = s_d + pl.P;
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
   __shared__ unsigned long long s_chunk;
@@ -325,7 +327,10 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           int p = p0 + t;
           int32_t len = p < P ? s_hi[p] - s_cur[p] : 0;
           int32_t inc = wave_incl_scan(len);
-          if (p < P) s_pre[p] = carry + inc - len;
+          if (p < P) {
+            s_pre[p] = carry + inc - len;
+            s_d[p] = s_cur[p] - (carry + inc - len);
+          }
           carry += __shfl(inc, kWave - 1, kWave);
         }
         if (t == 0) s_pre[P] = carry;
@@ -333,16 +338,18 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
       __syncthreads();
       const int32_t tot = s_pre[P];
       // Flattened member index f -> partition p with s_pre[p] <= f <
-      // s_pre[p+1]; four member loads in flight per lane, consecutive lanes
-      // on consecutive f (coalesced). Long runs (32+ members per partition
-      // on average): lanes stride the block (f = t, t + kCB, ...) and walk
-      // the few boundaries each step crosses. Short runs: each wavefront
-      // takes 256 consecutive f per step; a wave-uniform cursor finds the
-      // partition of the step's first f by a ballot over 64 boundaries at a
-      // time, and each lane locates its f among the next 64 boundaries (held
-      // one per lane) with a 6-step shuffle search, so no lane walks the
-      // boundaries one LDS read at a time (steps spanning more than 64
-      // partitions walk instead).
+      // s_pre[p+1], member position f + s_d[p]; U member loads in flight
+      // per lane, consecutive lanes on consecutive f (coalesced). Long runs
+      // (32+ members per partition on average): lanes stride the block
+      // (f = t, t + kCB, ...) and walk the few boundaries each step crosses.
+      // Short runs: each wavefront takes U * 64 consecutive f per step; a
+      // wave-uniform cursor finds the partition of the step's first f by a
+      // ballot over 64 boundaries at a time; the next 64 boundaries are held
+      // one per lane, and for each 64-wide sub-block one ballot counts
+      // those before it while the few falling inside it are compared one by
+      // one (a wave-uniform readlane each), so the search costs a few
+      // instructions per boundary crossed and no LDS traffic (steps
+      // spanning more than 64 partitions walk instead).
       if (tot >= 32 * P) {
         int p = 0;
         for (int32_t f0 = t; f0 < tot; f0 += U * kCB) {
@@ -354,13 +361,13 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
             j[u] = -1;
             if (f < tot) {
               while (s_pre[p + 1] <= f) ++p;
-              j[u] = pl.members[s_cur[p] + (f - s_pre[p])];
+              j[u] = pl.members[f + s_d[p]];
             }
             pu[u] = p;
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            if (j[u] < 0 || j[u] == (int32_t)i) continue;
+            if (j[u] < 0 || (full_rows && j[u] == (int32_t)i)) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
             if (RANK) atomicMin(&s_rank[jj], (uint32_t)s_prank[pu[u]]);
@@ -388,23 +395,29 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           int pu[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            const int32_t f = base + u * kWave + ln;
+            const int32_t f0 = base + u * kWave, f = f0 + ln;
             int p = pw;
             if (covered) {
-              int c = 0;  // boundaries <= f among the 64 held (at most 63 here)
-#pragma unroll
-              for (int step = kWave / 2; step > 0; step >>= 1)
-                if (__shfl(bnd, c + step - 1, kWave) <= f) c += step;
-              p += c;
+              // partition of f = pw + (held boundaries <= f): those <= f0
+              // count for every lane; the few inside (f0, f0 + 63] are read
+              // one by one (wave-uniform loop, one readlane each)
+              const uint64_t le0 = __ballot(bnd <= f0);
+              uint64_t mid = __ballot(bnd <= f0 + kWave - 1) & ~le0;
+              p += __popcll(le0);
+              while (mid) {
+                const int q = __builtin_ctzll(mid);
+                p += __builtin_amdgcn_readlane(bnd, q) <= f ? 1 : 0;
+                mid &= mid - 1;
+              }
             } else if (f < tot) {
               while (s_pre[p + 1] <= f) ++p;
             }
-            j[u] = f < tot ? pl.members[s_cur[p] + (f - s_pre[p])] : -1;
+            j[u] = f < tot ? pl.members[f + s_d[p]] : -1;
             pu[u] = p;
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
-            if (j[u] < 0 || j[u] == (int32_t)i) continue;
+            if (j[u] < 0 || (full_rows && j[u] == (int32_t)i)) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
             if (RANK) atomicMin(&s_rank[jj], (uint32_t)s_prank[pu[u]]);
@@ -671,7 +684,7 @@ inline int co_ranges() {
   return r <= 0 || r > kMaxRanges ? kMaxRanges : r;
 }
 inline size_t co_lds(int win, int P, bool rank = false) {
-  return (size_t)(win / 2 + (rank ? win + P : 0) + 5 * P + 1) * 4;
+  return (size_t)(win / 2 + (rank ? win + P : 0) + 6 * P + 1) * 4;
 }
 // The ranked fill keeps a 4-byte rank per column next to the counters: its
 // window is at most half as wide.
