@@ -242,7 +242,7 @@ def mst_case(args, knn):
         tuning = autotune_mst(d_dest, d_w, d_ip, use_cache=not args.retune)
     log(f"tuning: {tuning['knobs']} (cached={tuning['cached']})")
 
-    for _ in range(args.warmup):
+    for _ in range(max(args.warmup, 1)):  # at least one call: the run's shape
         r = step()
     torch.cuda.synchronize()
     log(f"warmup done: forest {r['n_mst']} edges, {r['n_components']} components, "

@@ -657,15 +657,18 @@ __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* 
 // min_insert from a wavefront: lanes holding the same root (the giant
 // component's, typically) are first reduced to one lane, for up to two
 // distinct roots per call, so a hot root costs one LDS atomic per wave
-// instead of one per lane.
+// instead of one per lane; a root held by one lane only skips the reduction.
 __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long long* s_min,
                                                 int32_t root, uint64_t key, bool active,
                                                 unsigned long long* best) {
   uint64_t pend = __ballot(active);
   for (int it = 0; it < 2 && pend; ++it) {
     const int leader = __ffsll((unsigned long long)pend) - 1;
-    const int32_t R = __shfl(root, leader, kWave);
+    const int32_t R = __builtin_amdgcn_readlane(root, leader);
     const bool mine = active && root == R;
+    // a root no other lane holds (most of them while components are
+    // small): every lane inserts its own, no reduction
+    if (__popcll(__ballot(mine)) < 2) break;
     uint64_t v = mine ? key : kInf;
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -679,86 +682,79 @@ __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long l
   if (active) min_insert(s_root, s_min, root, key, best);
 }
 
-// Filters one wavefront's run of chunks of `list`. ROUTE (the first round of
-// a weight level): `list` is the heavy list, whose cross keys below `lim`
-// (the level's upper threshold, on the key's weight bits) move to the same
-// chunk of the (empty) light list and become candidates, while heavier cross
-// keys stay in the heavy chunk; otherwise `list` is the light list, filtered
-// in place.
+// Filters one chunk `cc` (n_in keys) of `list` with one wavefront. ROUTE
+// (the first round of a weight level): `list` is the heavy list, whose cross
+// keys below `lim` (the level's upper threshold, on the key's weight bits)
+// move to the same chunk of the (empty) light list and become candidates,
+// while heavier cross keys stay in the heavy chunk; otherwise `list` is the
+// light list, filtered in place.
 template <bool ROUTE>
-__device__ __forceinline__ void filter_run(uint64_t* __restrict__ list, int32_t* __restrict__ cnt,
-                                           uint64_t* __restrict__ to, int32_t* __restrict__ cnt_to,
-                                           int64_t c_begin, int64_t c_end, uint64_t lim, int vb,
-                                           const int32_t* P, int32_t* s_root,
-                                           unsigned long long* s_min,
-                                           unsigned long long* __restrict__ best,
-                                           unsigned long long* sum_in,
-                                           unsigned long long* sum_out) {
+__device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
+                                             int32_t* __restrict__ cnt,
+                                             uint64_t* __restrict__ to,
+                                             int32_t* __restrict__ cnt_to, int64_t cc,
+                                             int32_t n_in, uint64_t lim, int vb, const int32_t* P,
+                                             int32_t* s_root, unsigned long long* s_min,
+                                             unsigned long long* __restrict__ best,
+                                             unsigned long long* sum_in,
+                                             unsigned long long* sum_out) {
   const int lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
-  for (int64_t g0 = c_begin; g0 < c_end; g0 += kWave) {
-    int32_t myn = g0 + lane < c_end ? cnt[g0 + lane] : 0;
-    myn = myn < 0 ? 0 : (myn > kGTile ? kGTile : myn);
-    uint64_t pending = __ballot(myn > 0);
-    while (pending) {
-      const int l = __ffsll((unsigned long long)pending) - 1;
-      pending &= pending - 1;
-      const int64_t cc = g0 + l;
-      const int32_t n_in = __shfl(myn, l, kWave);
-      uint64_t* lc = list + cc * kGTile;
-      uint64_t* tc = ROUTE ? to + cc * kGTile : nullptr;
-      int32_t out = 0, out_to = 0;
-      for (int base = 0; base < n_in; base += 4 * kWave) {
-        uint64_t k[4];
+  uint64_t* lc = list + cc * kGTile;
+  uint64_t* tc = ROUTE ? to + cc * kGTile : nullptr;
+  int32_t out = 0, out_to = 0;
+  for (int base = 0; base < n_in; base += 4 * kWave) {
+    uint64_t k[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int e = base + q * kWave + lane;
-          k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
-        }
-        int32_t x[8];  // roots of the endpoints of the 4 keys (parents are flat)
+    for (int q = 0; q < 4; ++q) {
+      const int e = base + q * kWave + lane;
+      k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
+    }
+    int32_t x[8];  // roots of the endpoints of the 4 keys (parents are flat)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          int32_t a = 0, b = 0;
-          if (k[q] != kInf) unpack_ends(k[q], vb, &a, &b);
-          x[2 * q] = a;
-          x[2 * q + 1] = b;
-        }
+    for (int q = 0; q < 4; ++q) {
+      int32_t a = 0, b = 0;
+      if (k[q] != kInf) unpack_ends(k[q], vb, &a, &b);
+      x[2 * q] = a;
+      x[2 * q + 1] = b;
+    }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
-        bool keep[4], cand[4];
+    for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
+    bool keep[4], cand[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
-          cand[q] = keep[q] && (!ROUTE || (k[q] >> (2 * vb)) < lim);
-        }
+    for (int q = 0; q < 4; ++q) {
+      keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
+      cand[q] = keep[q] && (!ROUTE || (k[q] >> (2 * vb)) < lim);
+    }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
+    for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          if (ROUTE) {
-            const uint64_t mt = __ballot(cand[q]);
-            if (cand[q]) tc[out_to + __popcll(mt & below)] = k[q];
-            out_to += __popcll(mt);
-            const bool stay = keep[q] && !cand[q];
-            const uint64_t ms = __ballot(stay);
-            if (stay) lc[out + __popcll(ms & below)] = k[q];
-            out += __popcll(ms);
-          } else {
-            const uint64_t m = __ballot(keep[q]);
-            if (keep[q]) lc[out + __popcll(m & below)] = k[q];
-            out += __popcll(m);
-          }
-        }
-      }
-      if (lane == 0) {
-        cnt[cc] = out;
-        if (ROUTE) cnt_to[cc] = out_to;
-        *sum_in += (unsigned long long)n_in;
-        *sum_out += (unsigned long long)(out + out_to);
+    for (int q = 0; q < 4; ++q) {
+      if (ROUTE) {
+        const uint64_t mt = __ballot(cand[q]);
+        if (cand[q]) tc[out_to + __popcll(mt & below)] = k[q];
+        out_to += __popcll(mt);
+        const bool stay = keep[q] && !cand[q];
+        const uint64_t ms = __ballot(stay);
+        if (stay) lc[out + __popcll(ms & below)] = k[q];
+        out += __popcll(ms);
+      } else {
+        const uint64_t m = __ballot(keep[q]);
+        if (keep[q]) lc[out + __popcll(m & below)] = k[q];
+        out += __popcll(m);
       }
     }
   }
+  if (lane == 0) {
+    cnt[cc] = out;
+    if (ROUTE) cnt_to[cc] = out_to;
+    *sum_in += (unsigned long long)n_in;
+    *sum_out += (unsigned long long)(out + out_to);
+  }
 }
+
+// Chunks a workgroup takes per batch from its share (b, b + G, b + 2G, ...).
+constexpr int kOwnBatch = kWave;
 
 __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
@@ -773,6 +769,8 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   __shared__ int32_t s_root[kMinSlots];
   __shared__ unsigned long long s_min[kMinSlots];
   __shared__ unsigned long long s_io[2 * (kGB / kWave)];
+  __shared__ int32_t s_work[kOwnBatch], s_work_n[kOwnBatch];
+  __shared__ int s_take, s_nwork;
   const RoundCtl prev = ctl[r & 1];
   const RoundCtl c = next_ctl(prev, r, had_edge[r], max_rounds, plan);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
@@ -784,32 +782,69 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   }
   __syncthreads();
   const int lane = lane_id();
-  // each wavefront owns a contiguous run of chunks and reads their counts
-  // 64 at a time, so empty chunks (most of them in late rounds) cost nothing
-  const int64_t nwaves = (int64_t)gridDim.x * (kGB / kWave);
-  const int64_t wid = (int64_t)blockIdx.x * (kGB / kWave) + threadIdx.x / kWave;
-  const int64_t per = (nchunks + nwaves - 1) / nwaves;
-  const int64_t c_begin = wid * per;
-  const int64_t c_end = c_begin + per < nchunks ? c_begin + per : nchunks;
+  // Workgroup b owns chunks b, b + G, b + 2G, ... (G = grid): the key
+  // density of the list varies along it, and interleaving gives every
+  // workgroup a sample of the whole list. Per batch of up to 64 owned
+  // chunks, the first wavefront lists the non-empty ones in LDS and the
+  // wavefronts take them one at a time from an LDS counter, so a wavefront
+  // that drew light chunks takes more and empty chunks cost nothing.
+  uint64_t* const list = route ? heavy : light;
+  int32_t* const cnt = route ? cnt_heavy : cnt_light;
+  const uint64_t lim = route ? plan->lim[c.phase] : kInf;
+  const int64_t G = gridDim.x;
+  const int64_t owned = nchunks > (int64_t)blockIdx.x ? (nchunks - 1 - blockIdx.x) / G + 1 : 0;
   unsigned long long sum_in = 0, sum_out = 0;
-  if (route)
-    filter_run<true>(heavy, cnt_heavy, light, cnt_light, c_begin, c_end, plan->lim[c.phase], vb,
-                     P, s_root, s_min, best, &sum_in, &sum_out);
-  else
-    filter_run<false>(light, cnt_light, nullptr, nullptr, c_begin, c_end, kInf, vb, P, s_root,
-                      s_min, best, &sum_in, &sum_out);
-  __syncthreads();
+  for (int64_t b0 = 0; b0 < owned; b0 += kOwnBatch) {
+    if (threadIdx.x < kWave) {
+      const int64_t j = b0 + lane;
+      const int64_t cc = blockIdx.x + j * G;
+      int32_t nk = j < owned ? cnt[cc] : 0;
+      nk = nk < 0 ? 0 : (nk > kGTile ? kGTile : nk);
+      const uint64_t m = __ballot(nk > 0);
+      if (nk > 0) {
+        const int at = __popcll(m & ((1ull << lane) - 1ull));
+        s_work[at] = (int32_t)j;
+        s_work_n[at] = nk;
+      }
+      if (lane == 0) {
+        s_take = 0;
+        s_nwork = __popcll(m);
+      }
+    }
+    __syncthreads();
+    for (;;) {
+      int w = 0;
+      if (lane == 0) w = atomicAdd(&s_take, 1);
+      w = __builtin_amdgcn_readfirstlane(w);
+      if (w >= s_nwork) break;
+      const int64_t cc = blockIdx.x + (int64_t)s_work[w] * G;
+      const int32_t n_in = s_work_n[w];
+      if (route)
+        filter_chunk<true>(list, cnt, light, cnt_light, cc, n_in, lim, vb, P, s_root, s_min,
+                           best, &sum_in, &sum_out);
+      else
+        filter_chunk<false>(list, cnt, nullptr, nullptr, cc, n_in, lim, vb, P, s_root, s_min,
+                            best, &sum_in, &sum_out);
+    }
+    __syncthreads();
+  }
+  // Flush: one global atomicMin per distinct root of the workgroup, skipped
+  // when a cached value (an upper bound of the current one) already wins.
+  // Every thread reads its slots' cached values together (one round trip).
+  {
+    constexpr int J = kMinSlots / kGB;
+    int32_t rt[J];
+    unsigned long long kk[J], cur[J];
 #pragma unroll
-  for (int j = 0; j < kMinSlots / kGB; ++j) {
-    const int i = j * kGB + threadIdx.x;
-    const int32_t root = s_root[i];
-    if (root < 0) continue;
-    const unsigned long long key = s_min[i];
-    // a cached value is an upper bound of the current one, a fresh read
-    // (past this XCD's L2) is exact: both skip losing atomics
-    if (key < best[root] &&
-        key < __hip_atomic_load(&best[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-      atomicMin(&best[root], key);
+    for (int j = 0; j < J; ++j) {
+      rt[j] = s_root[j * kGB + threadIdx.x];
+      kk[j] = s_min[j * kGB + threadIdx.x];
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) cur[j] = rt[j] >= 0 ? best[rt[j]] : 0ull;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (rt[j] >= 0 && kk[j] < cur[j]) atomicMin(&best[rt[j]], kk[j]);
   }
   if (rlog && r < kMaxRounds) {  // profiling only: per-round traffic
     const int w = threadIdx.x / kWave;
