@@ -37,12 +37,12 @@ MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)",
              2: "k_hook (union-find links)", 4: "forest output + labels"}
 
 
-def profile_traffic(kernel):
+def profile_traffic(kernel, summary="boruvka_mst_knn1m_hbm_traffic.md"):
     """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE /
     WRITE_SIZE (scaled by a copy-kernel calibration) and committed under
     profiles/; None when no summary covers it."""
     import re
-    path = Path(__file__).resolve().parent / "profiles" / "boruvka_mst_knn1m_hbm_traffic.md"
+    path = Path(__file__).resolve().parent / "profiles" / summary
     if not path.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch %s: (\d+)" % re.escape(kernel), path.read_text())
@@ -457,6 +457,7 @@ def run_coassoc(args):
                      "compact_ms": per_launch(12)}
     out["tuning"] = tuning
     whole = 4.0 * P * n + 8.0 * nnz + 8.0 * (n + 1)  # labels read, (col, count) + offsets
+    traffic = None
     if info.get("path") == "one-pass":
         # staged pass: member lists read once (4 B per (partition, point)),
         # 4 B per staged entry, 8 B per (row, window) segment, 4 B per row
@@ -465,6 +466,8 @@ def run_coassoc(args):
         # segment slots: sum over rows of nw - (i + 1) // win
         slots = n * nw - (win * q * (q - 1) // 2 + q * (n - q * win + 1))
         kname, k_ms = "k_coassoc_rows<stage>", per_launch(11)
+        if n == 500_000 and P == 256:  # the profiled workload
+            traffic = profile_traffic(kname, "coassoc_ensemble_n500k_p256_counters.md")
         k_bytes = 4.0 * P * n + 4.0 * nnz + 8.0 * slots + 4.0 * n
         c_ms = per_launch(12)
         c_bytes = 12.0 * nnz + 8.0 * slots + 8.0 * (n + 1)  # entries in, (col, value) out
@@ -479,7 +482,9 @@ def run_coassoc(args):
     ach = k_bytes / (k_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "kernel": kname, "achieved": ach,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                       "traffic": None, "avg_launch_ms": k_ms,
+                       "traffic": traffic,
+                       "traffic_source": "profiles/coassoc_ensemble_n500k_p256_counters.md"
+                       if traffic else None, "avg_launch_ms": k_ms,
                        "alg_bytes_per_launch": k_bytes,
                        "whole_build_alg_bytes": whole,
                        "whole_build_frac": whole / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
