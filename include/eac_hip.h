@@ -322,6 +322,25 @@ int eac_kmeans_fit(const float* X, int64_t n, int32_t d, float* C_inout,
                    int32_t mode, int32_t* labels, float* dists, int32_t* h_iters,
                    int64_t* h_out, void* ws, size_t ws_bytes, void* stream);
 
+/* eac_kmeans_fit for capture in a HIP graph (the ensemble of
+ * MyML/MyML/utils/partition.py:117-122 + KMeans.py:117-189 with the
+ * partitions batched and no host round trip): no host read, no host copy,
+ * no synchronisation. d_koff = h_koff already on the device. Exactly
+ * max_iter labelings run; a partition that meets the stopping rule keeps
+ * its centroids (its later labelings repeat its labels) and its count in
+ * d_iters (device int32[P]) stops, so labels, dists, centroids and
+ * iteration counts equal eac_kmeans_fit's. Flags are re-scored from a
+ * device count; empty clusters are reseeded on the device (the same rule).
+ * d_info (device int64[2]): flags re-scored, clusters reseeded; d_err
+ * (device uint32): 1 = flag list overflow, 2 = more than 2048 empty
+ * clusters in one partition (results then invalid: run eac_kmeans_fit). */
+size_t eac_kmeans_fit_graph_workspace_bytes(int64_t n, int32_t d, int64_t k_total, int32_t P);
+int eac_kmeans_fit_graph(const float* X, int64_t n, int32_t d, float* C_inout,
+                         const int64_t* h_koff, const int64_t* d_koff, int32_t P,
+                         int32_t max_iter, double tol, int32_t mode, int32_t* labels,
+                         float* dists, int32_t* d_iters, int64_t* d_info, uint32_t* d_err,
+                         void* ws, size_t ws_bytes, void* stream);
+
 /* numpy's float32 sum of each row of x (rows x n, row-major): out[r] equals
  * x[r].sum() of a contiguous float32 numpy array to the bit (numpy's add
  * reduction: chunks of its 8192-element buffer, each summed pairwise with

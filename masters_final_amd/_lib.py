@@ -79,6 +79,9 @@ SIGNATURES = {
     "eac_coassoc_fill": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _i32, _p, _p, _p, _sz, _p]),
     "eac_coassoc_fill_ranked": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _p, _p, _p, _p, _sz,
                                           _p]),
+    "eac_kmeans_fit_graph_workspace_bytes": (_sz, [_i64, _i32, _i64, _i32]),
+    "eac_kmeans_fit_graph": (C.c_int, [_p, _i64, _i32, _p, _p, _p, _i32, _i32, C.c_double, _i32,
+                                       _p, _p, _p, _p, _p, _p, _sz, _p]),
     "eac_coassoc_estimate": (C.c_int, [_p, _i64, _p, _i32, _i32, _i64, _p, _p, _sz, _p]),
     "eac_coassoc_stage_bytes": (_sz, [_i64, _i32, _i32, _i64]),
     "eac_coassoc_stage": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _p, _sz, _p, _p, _p, _sz,

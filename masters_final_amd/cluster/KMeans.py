@@ -106,6 +106,80 @@ def fit_batched(X, init_centroids, k_list, max_iter=3, tol=1e-4, precision="bf16
             "n_rescored": int(h[0]), "n_reseeded": int(h[1])}
 
 
+class BatchedFitGraph:
+    """fit_batched recorded once in a HIP graph (torch.cuda.CUDAGraph) and
+    replayed: the batched ensemble of partition.py:117-122 (one K_Means.fit
+    per partition, KMeans.py:117-189) with no host round trip. The data X,
+    the cluster counts, max_iter, tol and the precision are fixed at
+    capture; each run() copies new initial centroids into the graph's input
+    buffer and replays. Results equal fit_batched's (exactly max_iter
+    labelings run; a partition that stops keeps its centroids, so the
+    extra labelings repeat its labels). Buffers are owned by the object
+    (not the shared workspace cache) so that replays never see them move."""
+
+    def __init__(self, X, k_list, max_iter=3, tol=1e-4, precision="bf16x3"):
+        t = L.torch()
+        lib = L.load_library()
+        self.X = L.as_device(X, t.float32).contiguous()
+        n, d = self.X.shape
+        self.ks = list(k_list)
+        P = len(self.ks)
+        self.koff = _koff(self.ks)
+        ktot = int(self.koff[-1])
+        dev = self.X.device
+        self.n, self.d, self.P, self.max_iter = n, d, P, int(max_iter)
+        self.d_koff = t.from_numpy(self.koff).to(dev)
+        self.init = t.zeros((ktot, d), dtype=t.float32, device=dev)
+        self.cen = t.zeros((ktot, d), dtype=t.float32, device=dev)
+        self.labels = t.empty((P, n), dtype=t.int32, device=dev)
+        self.dists = t.empty((P, n), dtype=t.float32, device=dev)
+        self.iters = t.zeros(P, dtype=t.int32, device=dev)
+        self.info = t.zeros(2, dtype=t.int64, device=dev)
+        self.err = t.zeros(1, dtype=t.int32, device=dev)
+        self.ws = t.empty(lib.eac_kmeans_fit_graph_workspace_bytes(n, d, ktot, P),
+                          dtype=t.uint8, device=dev)
+        self._args = (float(tol), _PREC[precision])
+        side = t.cuda.Stream()
+        side.wait_stream(t.cuda.current_stream())
+        with t.cuda.stream(side):  # one eager call before capture
+            self._enqueue(lib)
+        t.cuda.current_stream().wait_stream(side)
+        t.cuda.synchronize()
+        self.graph = t.cuda.CUDAGraph()
+        with t.cuda.graph(self.graph):
+            self._enqueue(lib)
+
+    def _enqueue(self, lib):
+        tol, mode = self._args
+        self.cen.copy_(self.init)
+        L.check(lib.eac_kmeans_fit_graph(
+            L.ptr(self.X), self.n, self.d, L.ptr(self.cen),
+            self.koff.ctypes.data_as(C.c_void_p), L.ptr(self.d_koff), self.P, self.max_iter,
+            tol, mode, L.ptr(self.labels), L.ptr(self.dists), L.ptr(self.iters),
+            L.ptr(self.info), L.ptr(self.err), L.ptr(self.ws), self.ws.numel(), L.stream_ptr()))
+
+    def run(self, init_centroids):
+        """Replays the graph from these initial centroids (sum k, d); the
+        results stay in the object's device buffers (labels, dists, cen,
+        iters, info, err) until the next run."""
+        t = L.torch()
+        self.init.copy_(L.as_device(init_centroids, t.float32))
+        self.graph.replay()
+        return self
+
+    def result(self):
+        """dict as fit_batched returns (reads the error word: synchronises);
+        raises when the replay hit a limit of the device-side path."""
+        err = int(self.err.item())
+        if err:
+            raise RuntimeError(f"captured fit reported error bits {err} (flag list overflow = 1, "
+                               f"more than 2048 empty clusters in a partition = 2)")
+        info = self.info.cpu().numpy()
+        return {"labels": self.labels, "dists": self.dists, "centroids": self.cen,
+                "iters": self.iters.cpu().numpy(), "n_rescored": int(info[0]),
+                "n_reseeded": int(info[1])}
+
+
 def sum_f32_device(x):
     """numpy's float32 sum of each row of a 2-D device tensor (1-D: one row),
     bit-equal to ndarray.sum(): the inertia of KMeans.py:164."""

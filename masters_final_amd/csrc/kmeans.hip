@@ -466,8 +466,10 @@ __global__ __launch_bounds__(256) void k_rescore_f64(const float* __restrict__ X
                                                      int64_t ktot,
                                                      const int64_t* __restrict__ k_off,
                                                      const uint64_t* __restrict__ flags,
-                                                     int64_t nflag, int32_t* __restrict__ labels) {
+                                                     int64_t nflag, int32_t* __restrict__ labels,
+                                                     const int64_t* __restrict__ d_nflag = nullptr) {
   const int l = lane_id();
+  if (d_nflag) nflag = *d_nflag;  // count left on the device (capturable calls)
   for (int64_t f = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; f < nflag;
        f += ((int64_t)gridDim.x * blockDim.x) / kWave) {
     uint64_t fl = flags[f];
@@ -537,7 +539,7 @@ template <int DX>
 __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
     const float* __restrict__ X, int64_t n, int d, const float* __restrict__ C,
     const int64_t* __restrict__ k_off, const uint64_t* __restrict__ flags, int64_t nflag,
-    int32_t* __restrict__ labels) {
+    int32_t* __restrict__ labels, const int64_t* __restrict__ d_nflag = nullptr) {
   extern __shared__ __align__(16) float s_dyn[];
   __shared__ double s_best[kRtS][kRtF];
   __shared__ int32_t s_bk[kRtS][kRtF];
@@ -548,7 +550,8 @@ __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
   float* s_x = s_dyn;
   float* s_c = s_dyn + kRtF * RS;  // 16-byte aligned: kRtF * RS * 4 = 256 (d + 1) bytes
   const int t = threadIdx.x, g = t & (kRtF - 1), sub = t / kRtF;
-  const int64_t f0 = (int64_t)blockIdx.x * kRtF;
+  if (d_nflag) nflag = *d_nflag;  // count left on the device: tiles in a loop
+  for (int64_t f0 = (int64_t)blockIdx.x * kRtF; f0 < nflag; f0 += (int64_t)gridDim.x * kRtF) {
   const int cnt = nflag - f0 < kRtF ? (int)(nflag - f0) : kRtF;
   // screen bounds (double, then rounded up into float)
   const double ug = 1.1920928955078125e-07, u64 = 1.1102230246251565e-16;
@@ -684,13 +687,16 @@ __global__ __launch_bounds__(kRtF * kRtS) void k_rescore_tiled(
     __syncthreads();
     done = end;
   }
+  }  // tiles
 }
 
 // Concatenates the kFlagSlots flag regions (counts cnt[slot]) into one list.
 __global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t region,
                                const unsigned long long* __restrict__ cnt,
-                               uint64_t* __restrict__ out, int64_t total) {
+                               uint64_t* __restrict__ out, int64_t total,
+                               const int64_t* __restrict__ d_total = nullptr) {
   __shared__ unsigned long long s_pre[kFlagSlots + 1];
+  if (d_total) total = *d_total;
   if (threadIdx.x == 0) {
     unsigned long long a = 0;
     for (int q = 0; q < kFlagSlots; ++q) {
@@ -1264,6 +1270,158 @@ __global__ void k_reseed(const float* __restrict__ X, int d, const uint64_t* __r
   }
 }
 
+// ----------------------------------------- capturable fit (no host reads)
+// Total of the flag regions (clamped to the list), error bit 1 when a
+// region or the list overflowed. One wavefront.
+__global__ void k_flags_total(const unsigned long long* __restrict__ cnt, int64_t region,
+                              int64_t cap, int64_t* __restrict__ d_nf,
+                              unsigned int* __restrict__ err) {
+  const int l = lane_id();
+  const unsigned long long c = l < kFlagSlots ? cnt[l] : 0ull;
+  const bool over = (int64_t)c > region;
+  unsigned long long tot = c;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) tot += __shfl_xor(tot, off, kWave);
+  const bool any_over = __ballot(over) != 0ull;
+  if (l == 0) {
+    if (any_over || (int64_t)tot > cap) atomicOr(err, 1u);
+    d_nf[0] = (int64_t)tot < cap ? (int64_t)tot : cap;
+  }
+}
+
+// Counting sort of the flags by partition (flag = partition << 40 | row):
+// per-partition counts, their exclusive scan (one block), and a scatter
+// through per-partition cursors. The order inside a partition is free (each
+// flag is re-scored on its own).
+__global__ __launch_bounds__(256) void k_flag_pcount(const uint64_t* __restrict__ flags,
+                                                     const int64_t* __restrict__ d_nf, int P,
+                                                     int32_t* __restrict__ pcount) {
+  const int64_t nf = *d_nf;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    const int p = (int)(flags[f] >> 40);
+    if (p < P) atomicAdd(&pcount[p], 1);
+  }
+}
+__global__ __launch_bounds__(1024) void k_flag_pscan(const int32_t* __restrict__ pcount, int P,
+                                                     int32_t* __restrict__ pcursor) {
+  __shared__ int32_t s_tmp[1024 / kWave + 1];
+  int32_t carry = 0;
+  for (int p0 = 0; p0 < P; p0 += 1024) {
+    const int p = p0 + threadIdx.x;
+    const int32_t v = p < P ? pcount[p] : 0;
+    int32_t tot;
+    const int32_t pre = block_excl_scan<int32_t>(v, s_tmp, &tot);
+    if (p < P) pcursor[p] = carry + pre;
+    carry += tot;
+    __syncthreads();
+  }
+}
+__global__ __launch_bounds__(256) void k_flag_pscatter(const uint64_t* __restrict__ flags,
+                                                       const int64_t* __restrict__ d_nf, int P,
+                                                       int32_t* __restrict__ pcursor,
+                                                       uint64_t* __restrict__ out) {
+  const int64_t nf = *d_nf;
+  for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
+       f += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t fl = flags[f];
+    const int p = (int)(fl >> 40);
+    if (p < P) out[atomicAdd(&pcursor[p], 1)] = fl;
+  }
+}
+
+// Device reseeding of every partition's empty clusters (the same rule as
+// k_dist_keys + radix sort + k_reseed): the r-th empty cluster of partition
+// p (ascending) takes the point of the r-th smallest key (dist bits << 32 |
+// index). One workgroup per partition with empties: the e-th smallest key by
+// a radix select (eight 8-bit digit passes over the partition's dists), the
+// e keys at or below it collected in LDS and ranked among themselves. More
+// than kMaxReseed empties in one partition set error bit 2.
+constexpr int kMaxReseed = 2048;
+__global__ __launch_bounds__(1024) void k_reseed_dev(
+    const float* __restrict__ X, int d, const float* __restrict__ dists, int64_t n,
+    const int64_t* __restrict__ k_off, int P, const int32_t* __restrict__ count,
+    const int32_t* __restrict__ erank, float* __restrict__ Cnew, unsigned int* __restrict__ err) {
+  __shared__ unsigned int hist[256];
+  __shared__ unsigned long long s_sel[kMaxReseed];
+  __shared__ int32_t s_emp[kMaxReseed];
+  __shared__ unsigned long long s_prefix;
+  __shared__ int64_t s_want;
+  __shared__ int s_nsel;
+  const int t = threadIdx.x;
+  for (int p = blockIdx.x; p < P; p += gridDim.x) {
+    const int64_t kb = k_off[p], ke = k_off[p + 1];
+    const int64_t e = (int64_t)erank[ke] - erank[kb];
+    if (e == 0) continue;
+    if (e > n || e > kMaxReseed) {
+      if (t == 0) atomicOr(err, 2u);
+      continue;
+    }
+    const float* dp = dists + (int64_t)p * n;
+    if (t == 0) {
+      s_prefix = 0ull;
+      s_want = e;  // rank (1-based) of the wanted key among those left
+    }
+    for (int shift = 56; shift >= 0; shift -= 8) {
+      for (int b = t; b < 256; b += blockDim.x) hist[b] = 0u;
+      __syncthreads();
+      const unsigned long long pre = s_prefix;
+      for (int64_t i = t; i < n; i += blockDim.x) {
+        const unsigned long long key =
+            ((unsigned long long)__float_as_uint(dp[i]) << 32) | (unsigned long long)i;
+        if (shift == 56 || ((key ^ pre) >> (shift + 8)) == 0ull)
+          atomicAdd(&hist[(key >> shift) & 255u], 1u);
+      }
+      __syncthreads();
+      if (t == 0) {
+        int64_t want = s_want, cum = 0;
+        int b = 0;
+        for (; b < 255 && cum + (int64_t)hist[b] < want; ++b) cum += hist[b];
+        s_want = want - cum;
+        s_prefix = pre | ((unsigned long long)b << shift);
+      }
+      __syncthreads();
+    }
+    const unsigned long long T = s_prefix;  // the e-th smallest key (keys are unique)
+    if (t == 0) s_nsel = 0;
+    for (int64_t c = kb + t; c < ke; c += blockDim.x)
+      if (count[c] == 0) s_emp[erank[c] - erank[kb]] = (int32_t)c;
+    __syncthreads();
+    for (int64_t i = t; i < n; i += blockDim.x) {
+      const unsigned long long key =
+          ((unsigned long long)__float_as_uint(dp[i]) << 32) | (unsigned long long)i;
+      if (key <= T) {
+        const int at = atomicAdd(&s_nsel, 1);
+        if (at < kMaxReseed) s_sel[at] = key;
+      }
+    }
+    __syncthreads();
+    const int ns = s_nsel < (int)e ? s_nsel : (int)e;
+    for (int j = t; j < ns; j += blockDim.x) {
+      const unsigned long long key = s_sel[j];
+      int r = 0;
+      for (int q = 0; q < ns; ++q) r += s_sel[q] < key;
+      const int64_t c = s_emp[r];
+      const int64_t pt = (int64_t)(key & 0xffffffffull);
+      for (int jj = 0; jj < d; ++jj) Cnew[c * d + jj] = X[pt * d + jj];
+    }
+    __syncthreads();
+  }
+}
+
+// Running totals of a capturable fit: info[0] += flags re-scored,
+// info[1] += empty clusters reseeded (device; one thread), counted only
+// while some partition still runs (any[0], the last stopping-rule result),
+// as the fit that stops when none does counts them.
+__global__ void k_fit_tally(int64_t* __restrict__ info, const int64_t* __restrict__ d_nf,
+                            const int32_t* __restrict__ erank, int64_t ktot,
+                            const int32_t* __restrict__ any) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && any[0]) {
+    if (d_nf) info[0] += *d_nf;
+    if (erank) info[1] += erank[ktot];
+  }
+}
+
 int kd_for(int d) {
   if (d <= 32) return 32;
   if (d <= 64) return 64;
@@ -1333,13 +1491,28 @@ size_t kmeans_label_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
   return ar.used + 4096;
 }
 
+// Device-side state of a capturable fit: flag count, per-partition flag
+// counts and cursors, flags sorted by partition, running totals (info[0]
+// flags re-scored, info[1] clusters reseeded), error bits.
+struct KmGraph {
+  const int32_t* any;  // some partition still runs (the labeling counts only then)
+  int64_t* d_nf;
+  int32_t* pcount;
+  int32_t* pcursor;
+  uint64_t* sorted;
+  int64_t* info;
+  unsigned int* err;
+};
+
 // Label all partitions; dists (nullable) receive FP64-exact distances for
 // the F64 / BF16X3 modes and fp32 distances for F32; inertia[p] (device,
 // nullable) their per-partition numpy float32 sums.
+// With g (capturable calls) nothing is read back: the flag count stays on
+// the device and sizes the re-scoring from there.
 int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t* h_koff,
                  int P, int mode, int32_t* labels, float* dists, float* inertia_out,
                  int64_t* n_rescored, Arena& ar, hipStream_t s,
-                 const int64_t* d_koff = nullptr) {
+                 const int64_t* d_koff = nullptr, const KmGraph* kg = nullptr) {
   const int64_t ktot = h_koff[P];
   LabelBufs B = carve_label(ar, n, d, ktot, P);
   if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans: workspace too small");
@@ -1393,6 +1566,52 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   }
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
+  if (kg) {
+    // flags gathered, sorted by partition and re-scored with device counts
+    // (grids sized for the whole list, blocks past the count exit at once)
+    k_flags_total<<<1, kWave, 0, s>>>(B.nflag, B.region, B.flag_cap, kg->d_nf, kg->err);
+    EAC_LAUNCH_CHECK();
+    const int gp = grid_for(B.flag_cap, 256, 4096);
+    k_gather_flags<<<gp, 256, 0, s>>>(B.flags_r, B.region, B.nflag, B.flags, 0, kg->d_nf);
+    EAC_LAUNCH_CHECK();
+    const uint64_t* fl = B.flags;
+    const bool tiled = rescore_tiled_lds(d) <= 64 * 1024;
+    if (tiled && P > 1) {
+      EAC_TRY(fill<int32_t>(kg->pcount, P, 0, s));
+      k_flag_pcount<<<gp, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->pcount);
+      EAC_LAUNCH_CHECK();
+      k_flag_pscan<<<1, 1024, 0, s>>>(kg->pcount, P, kg->pcursor);
+      EAC_LAUNCH_CHECK();
+      k_flag_pscatter<<<gp, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->pcursor, kg->sorted);
+      EAC_LAUNCH_CHECK();
+      fl = kg->sorted;
+    }
+    int pr2 = prof_begin(EAC_PROF_KM_RESCORE, 0, s);
+    if (tiled) {
+      const size_t lds = rescore_tiled_lds(d);
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      const int64_t tiles = (B.flag_cap + kRtF - 1) / kRtF;
+      const unsigned nb = (unsigned)(tiles < 4 * (int64_t)cus ? tiles : 4 * (int64_t)cus);
+      auto go = [&](auto kern) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        kern<<<nb, kRtF * kRtS, lds, s>>>(X, n, d, C, B.k_off, fl, 0, labels, kg->d_nf);
+      };
+      if (d <= 16) go(k_rescore_tiled<16>);
+      else if (d <= 32) go(k_rescore_tiled<32>);
+      else if (d <= 64) go(k_rescore_tiled<64>);
+      else go(k_rescore_tiled<0>);
+    } else {
+      k_rescore_f64<<<grid_for(B.flag_cap * kWave, 256, 16384), 256, 0, s>>>(
+          X, n, d, B.Ct, ktot, B.k_off, B.flags, 0, labels, kg->d_nf);
+    }
+    EAC_LAUNCH_CHECK();
+    prof_end(pr2, s);
+    k_fit_tally<<<1, 64, 0, s>>>(kg->info, kg->d_nf, nullptr, 0, kg->any);
+    EAC_LAUNCH_CHECK();
+  } else {
   unsigned long long hcnt[kFlagSlots];
   EAC_TRY(d2h(hcnt, B.nflag, sizeof(hcnt), s));
   unsigned long long nf = 0;
@@ -1431,7 +1650,7 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
       auto go = [&](auto kern) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        kern<<<nb, kRtF * kRtS, lds, s>>>(X, n, d, C, B.k_off, fl, (int64_t)nf, labels);
+        kern<<<nb, kRtF * kRtS, lds, s>>>(X, n, d, C, B.k_off, fl, (int64_t)nf, labels, nullptr);
       };
       if (d <= 16) go(k_rescore_tiled<16>);
       else if (d <= 32) go(k_rescore_tiled<32>);
@@ -1439,12 +1658,13 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
       else go(k_rescore_tiled<0>);
     } else {
       k_rescore_f64<<<grid_for((int64_t)nf * kWave, 256, 65536), 256, 0, s>>>(
-          X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels);
+          X, n, d, B.Ct, ktot, B.k_off, B.flags, (int64_t)nf, labels, nullptr);
     }
     EAC_LAUNCH_CHECK();
     prof_end(pr2, s);
   }
   if (n_rescored) *n_rescored = (int64_t)nf;
+  }  // read-back path
   if (inertia_out && !dists) return fail(EAC_ERR_ARG, "kmeans: inertia needs dists");
   if (dists) {
     const int pgs = (P + kPG - 1) / kPG;
@@ -1489,9 +1709,10 @@ size_t kmeans_update_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
 
 // New centroids for every partition (KMeans.py:709-754); empty clusters take
 // the points of smallest (dist, index) in ascending cluster order.
+// With g (capturable calls) the empty clusters are reseeded on the device.
 int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const float* dists,
                   const int64_t* h_koff, int P, float* Cnew, int64_t* n_empty, Arena& ar,
-                  hipStream_t s, const int64_t* d_koff = nullptr) {
+                  hipStream_t s, const int64_t* d_koff = nullptr, const KmGraph* g = nullptr) {
   const int64_t ktot = h_koff[P];
   const int64_t items = (int64_t)P * n;
   if (items > (int64_t)UINT32_MAX || ktot >= (int64_t)UINT32_MAX)
@@ -1549,6 +1770,15 @@ int kmeans_update(const float* X, int64_t n, int d, const int32_t* labels, const
   EAC_LAUNCH_CHECK();
   EAC_TRY((scan_excl<int32_t, int32_t>(eflag, erank, ktot + 1, 0, nullptr, ar, s)));
   ar.used = mark;
+  if (g) {
+    k_reseed_dev<<<P < 4096 ? P : 4096, 1024, 0, s>>>(X, d, dists, n, k_off, P, count, erank,
+                                                      Cnew, g->err);
+    EAC_LAUNCH_CHECK();
+    k_fit_tally<<<1, 64, 0, s>>>(g->info, nullptr, erank, ktot, g->any);
+    EAC_LAUNCH_CHECK();
+    if (n_empty) *n_empty = 0;
+    return EAC_OK;
+  }
   std::vector<int32_t> er(ktot + 1);
   {
     // host needs only the per-partition empty counts
@@ -1645,6 +1875,99 @@ extern "C" int eac_kmeans_update(const float* X, int64_t n, int32_t d, const int
   EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, C_new, &ne, ar, s));
   EAC_HIP_CHECK(hipStreamSynchronize(s));
   if (h_out) h_out[0] = ne;
+  return EAC_OK;
+}
+
+// ------------------------------------------------- capturable batched fit
+namespace eac {
+namespace {
+// KmGraph buffers (flag count, per-partition counts and cursors, flags
+// sorted by partition); info and err are the caller's device arrays.
+KmGraph carve_graph(Arena& ar, int64_t n, int P, int64_t* info, unsigned int* err) {
+  KmGraph g;
+  g.d_nf = ar.take<int64_t>(1);
+  g.pcount = ar.take<int32_t>(P);
+  g.pcursor = ar.take<int32_t>(P);
+  g.sorted = ar.take<uint64_t>((size_t)P * n);
+  g.info = info;
+  g.err = err;
+  g.any = nullptr;
+  return g;
+}
+__global__ void k_graph_init(int64_t* info, unsigned int* err) {
+  if (threadIdx.x == 0) {
+    info[0] = 0;
+    info[1] = 0;
+    err[0] = 0u;
+  }
+}
+}  // namespace
+}  // namespace eac
+
+extern "C" size_t eac_kmeans_fit_graph_workspace_bytes(int64_t n, int32_t d, int64_t k_total,
+                                                       int32_t P) {
+  if (n < 1) n = 1;
+  if (P < 1) P = 1;
+  Arena ga(nullptr, 0, true);
+  (void)carve_graph(ga, n, P, nullptr, nullptr);
+  return eac_kmeans_workspace_bytes(n, d, k_total, P) + ga.used + 4096;
+}
+
+// eac_kmeans_fit for capture in a HIP graph: no host read, no host copy, no
+// synchronisation. d_koff: the partition offsets already on the device;
+// exactly max_iter labelings run (a partition that meets the stopping rule
+// keeps its centroids, so its later labelings repeat, and its iteration
+// count in d_iters stops); d_info (device int64[2]) receives the flags
+// re-scored and the clusters reseeded, d_err (device uint32) error bits
+// (1: flag list overflow, 2: more than 2048 empty clusters in one
+// partition): the labels are valid when d_err is 0 after the replay.
+extern "C" int eac_kmeans_fit_graph(const float* X, int64_t n, int32_t d, float* C_inout,
+                                    const int64_t* h_koff, const int64_t* d_koff, int32_t P,
+                                    int32_t max_iter, double tol, int32_t mode, int32_t* labels,
+                                    float* dists, int32_t* d_iters, int64_t* d_info,
+                                    uint32_t* d_err, void* ws, size_t ws_bytes, void* stream) {
+  EAC_TRY(km_check(X, n, d, C_inout, h_koff, P));
+  if (!labels || !dists || !d_koff || !d_iters || !d_info || !d_err)
+    return fail(EAC_ERR_ARG, "kmeans fit graph: null array");
+  if (max_iter < 1) return fail(EAC_ERR_ARG, "kmeans fit graph: max_iter < 1");
+  const int64_t ktot = h_koff[P];
+  size_t need = eac_kmeans_fit_graph_workspace_bytes(n, d, ktot, P);
+  if (!ws || ws_bytes < need)
+    return fail(EAC_ERR_WORKSPACE, "kmeans fit graph: workspace %zu < %zu bytes", ws_bytes,
+                need);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  KmGraph g = carve_graph(ar, n, P, d_info, d_err);
+  float* Cnew = ar.take<float>((size_t)ktot * d);
+  float* inertia = ar.take<float>(P);
+  float* prev = ar.take<float>(P);
+  int32_t* active = ar.take<int32_t>(P);
+  int32_t* upd = ar.take<int32_t>(P);
+  int32_t* d_any = ar.take<int32_t>(1);
+  if (ar.overflow()) return fail(EAC_ERR_WORKSPACE, "kmeans fit graph: workspace too small");
+  const size_t mark = ar.used;
+  g.any = d_any;
+  k_graph_init<<<1, 64, 0, s>>>(d_info, d_err);
+  EAC_LAUNCH_CHECK();
+  EAC_TRY(fill<int32_t>(d_any, 1, 1, s));  // the first labeling always counts
+  k_fit_init<<<grid_for(P, 256, 1024), 256, 0, s>>>(prev, active, d_iters, P);
+  EAC_LAUNCH_CHECK();
+  for (int it = 1; it <= max_iter; ++it) {
+    ar.used = mark;
+    int64_t nres = 0;
+    EAC_TRY(kmeans_label(X, n, d, C_inout, h_koff, P, mode, labels, dists, inertia, &nres, ar,
+                         s, d_koff, &g));
+    k_converge<<<1, 1024, 0, s>>>(inertia, prev, active, d_iters, upd, P, it, max_iter,
+                                  (float)tol, d_any);
+    EAC_LAUNCH_CHECK();
+    if (it >= max_iter) break;
+    ar.used = mark;
+    int64_t ne = 0;
+    EAC_TRY(kmeans_update(X, n, d, labels, dists, h_koff, P, Cnew, &ne, ar, s, d_koff, &g));
+    const dim3 gc(grid_for(ktot * d / P + 1, 256, 64), P < 65535 ? P : 65535);
+    k_copy_running<<<gc, 256, 0, s>>>(C_inout, Cnew, d, d_koff, P, upd);
+    EAC_LAUNCH_CHECK();
+  }
   return EAC_OK;
 }
 
