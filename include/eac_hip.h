@@ -424,6 +424,19 @@ int eac_coassoc_stage(const int32_t* labels, int64_t n, const int64_t* h_koff, i
 int eac_coassoc_compact(int64_t n, const int64_t* indptr, const void* stage,
                         size_t stage_bytes, int32_t w_offset, int32_t* indices,
                         int32_t* data, void* stream);
+/* The one-pass build with no host read, copy or synchronisation, for
+ * capture in a HIP graph (the same matrix): d_koff = h_koff on the device
+ * (h_koff still gives the sizes); the staging buffer is sized by the caller
+ * (eac_coassoc_stage_bytes; no estimate pass); indices/data hold up to
+ * cap_nnz entries; w_offset as in eac_coassoc_fill, or -1 for (largest
+ * count + 1) taken on the device. d_out (device int64[4]): nnz, largest
+ * count, error bits (1: pool too small, 2: nnz > cap_nnz, 4: a label >= k
+ * of its partition; then no entry is written), 0. */
+int eac_coassoc_graph(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                      const int64_t* d_koff, int32_t P, int32_t full_rows, int32_t w_offset,
+                      int64_t* indptr, int32_t* indices, int32_t* data, int64_t cap_nnz,
+                      void* stage, size_t stage_bytes, int64_t* d_out, void* ws,
+                      size_t ws_bytes, void* stream);
 
 /* ---- capacity-capped co-association (the reference's lossy matrix) ------
  * From the exact matrix (indptr, indices, count) and its ranks (above),

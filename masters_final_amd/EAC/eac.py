@@ -12,6 +12,8 @@ connected_components numbers them.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 
 from .. import _lib as L
@@ -137,6 +139,78 @@ class EAC:
         self.labels = (np.empty(0, dtype=np.int32) if labels is None
                        else labels.cpu().numpy())
         return self.labels
+
+
+class EnsembleCoassocGraph:
+    """The EAC hot path up to the dissimilarity matrix recorded in one HIP
+    graph: the batched K-Means ensemble (eac_kmeans_fit_graph) and the
+    one-pass co-association with (max + 1) - count dissimilarities
+    (eac_coassoc_graph, condensed rows), with no host round trip inside.
+    X, the cluster counts, iters and the precision are fixed at capture;
+    run(init) replays from new initial centroids and reads the matrix size
+    back once. cap_nnz (default: every condensed pair, n (n - 1) / 2)
+    bounds the matrix; the staging pool holds cap_nnz entries as well."""
+
+    def __init__(self, X, k_list, iters=3, tol=1e-4, precision="bf16x3", cap_nnz=None):
+        from ..cluster.KMeans import BatchedFitGraph
+        t = L.torch()
+        lib = L.load_library()
+        self.fit = BatchedFitGraph(X, k_list, max_iter=iters, tol=tol, precision=precision,
+                                   capture=False)
+        f = self.fit
+        n, P = f.n, f.P
+        cap = int(cap_nnz) if cap_nnz else max(n * (n - 1) // 2, 1)
+        dev = f.X.device
+        self.cap = cap
+        self.indptr = t.empty(n + 1, dtype=t.int64, device=dev)
+        self.indices = t.empty(cap, dtype=t.int32, device=dev)
+        self.data = t.empty(cap, dtype=t.int32, device=dev)
+        self.out = t.zeros(4, dtype=t.int64, device=dev)
+        nb = lib.eac_coassoc_stage_bytes(n, 0, P, cap)
+        self.stage = t.empty(nb, dtype=t.uint8, device=dev)
+        self.cws = t.empty(lib.eac_coassoc_workspace_bytes(n, P, int(f.koff[-1])), dtype=t.uint8,
+                           device=dev)
+        side = t.cuda.Stream()
+        side.wait_stream(t.cuda.current_stream())
+        with t.cuda.stream(side):
+            self._enqueue_coassoc(lib)
+        t.cuda.current_stream().wait_stream(side)
+        t.cuda.synchronize()
+        self.graph = t.cuda.CUDAGraph()
+        with t.cuda.graph(self.graph):
+            f._enqueue(lib)
+            self._enqueue_coassoc(lib)
+
+    def _enqueue_coassoc(self, lib):
+        f = self.fit
+        L.check(lib.eac_coassoc_graph(
+            L.ptr(f.labels), f.n, f.koff.ctypes.data_as(C.c_void_p), L.ptr(f.d_koff), f.P, 0, -1,
+            L.ptr(self.indptr), L.ptr(self.indices), L.ptr(self.data), self.cap,
+            L.ptr(self.stage), self.stage.numel(), L.ptr(self.out), L.ptr(self.cws),
+            self.cws.numel(), L.stream_ptr()))
+
+    def run(self, init_centroids):
+        """Replays from these initial centroids; returns (indptr, indices,
+        data, max_count, fit result) as eac_pipeline's matrix."""
+        t = L.torch()
+        self.fit.init.copy_(L.as_device(init_centroids, t.float32))
+        self.graph.replay()
+        nnz, mx, err, _ = (int(v) for v in self.out.cpu())
+        if err:
+            raise RuntimeError(f"captured co-association reported error bits {err} (pool too "
+                               f"small = 1, more than {self.cap} entries = 2, label >= k = 4)")
+        fit = self.fit.result()
+        return self.indptr, self.indices[:nnz], self.data[:nnz], mx, fit
+
+
+def eac_pipeline_graph(graph, init, n_clusters=0):
+    """eac_pipeline with the ensemble and the co-association replayed from
+    an EnsembleCoassocGraph, then the forest and the cut."""
+    indptr, indices, data, mx, fit = graph.run(init)
+    nc, lab, info = sp_sl_lifetime_device((indptr, indices, data), graph.fit.P,
+                                          n_clusters=n_clusters)
+    return {"n_clusters": nc, "labels": lab, "n_rescored": fit["n_rescored"],
+            "nnz": int(indices.numel()), "ensemble_labels": fit["labels"], "max_count": mx}
 
 
 def eac_pipeline(X, k_list, iters=3, seed=0, precision="bf16x3", init=None, n_clusters=0):

@@ -87,6 +87,8 @@ SIGNATURES = {
     "eac_coassoc_stage": (C.c_int, [_p, _i64, _p, _i32, _i32, _p, _p, _sz, _p, _p, _p, _sz,
                                     _p]),
     "eac_coassoc_compact": (C.c_int, [_i64, _p, _p, _sz, _i32, _p, _p, _p]),
+    "eac_coassoc_graph": (C.c_int, [_p, _i64, _p, _p, _i32, _i32, _i32, _p, _p, _p, _i64, _p, _sz,
+                                    _p, _p, _sz, _p]),
     "eac_coassoc_cap_workspace_bytes": (_sz, [_i64]),
     "eac_coassoc_cap": (C.c_int, [_p, _p, _p, _p, _i64, _i32, _i32, _p, _i64, _i64, _p, _p, _p,
                                   _p, _p, _sz, _p]),

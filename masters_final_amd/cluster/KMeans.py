@@ -117,7 +117,7 @@ class BatchedFitGraph:
     extra labelings repeat its labels). Buffers are owned by the object
     (not the shared workspace cache) so that replays never see them move."""
 
-    def __init__(self, X, k_list, max_iter=3, tol=1e-4, precision="bf16x3"):
+    def __init__(self, X, k_list, max_iter=3, tol=1e-4, precision="bf16x3", capture=True):
         t = L.torch()
         lib = L.load_library()
         self.X = L.as_device(X, t.float32).contiguous()
@@ -145,9 +145,11 @@ class BatchedFitGraph:
             self._enqueue(lib)
         t.cuda.current_stream().wait_stream(side)
         t.cuda.synchronize()
-        self.graph = t.cuda.CUDAGraph()
-        with t.cuda.graph(self.graph):
-            self._enqueue(lib)
+        self.graph = None
+        if capture:  # (capture=False: a larger graph records _enqueue itself)
+            self.graph = t.cuda.CUDAGraph()
+            with t.cuda.graph(self.graph):
+                self._enqueue(lib)
 
     def _enqueue(self, lib):
         tol, mode = self._args

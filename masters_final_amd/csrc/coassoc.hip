@@ -575,9 +575,17 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
 // One-pass build, last step: copies each row's staged segments, in window
 // order, to indices/data at indptr[i] (one wavefront per row; the segment
 // headers are wave-uniform loads).
+// With ctl (the capturable build's device results: nnz, max count, error
+// bits) nothing is written when an error bit is set, and w_offset = -1
+// takes max count + 1 from ctl.
 __global__ __launch_bounds__(256) void k_coassoc_compact(
     const unsigned char* __restrict__ stage, const int64_t* __restrict__ indptr, int64_t n,
-    int32_t w_offset, int32_t* __restrict__ out_col, int32_t* __restrict__ out_val) {
+    int32_t w_offset, int32_t* __restrict__ out_col, int32_t* __restrict__ out_val,
+    const int64_t* __restrict__ ctl = nullptr) {
+  if (ctl) {
+    if (ctl[2] != 0) return;
+    if (w_offset == -1) w_offset = (int32_t)ctl[1] + 1;
+  }
   const int64_t* hdr = reinterpret_cast<const int64_t*>(stage);
   const int win = (int)hdr[0], nw = (int)hdr[1], full_rows = (int)hdr[5];
   const uint64_t* __restrict__ seg = reinterpret_cast<const uint64_t*>(stage + 256);
@@ -745,7 +753,7 @@ namespace {
 // flags.
 int co_plan_build(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_t P,
                   void* ws, size_t ws_bytes, CoBufs* out, size_t* mark_out, Arena& ar,
-                  hipStream_t s) {
+                  hipStream_t s, const int64_t* d_koff = nullptr) {
   EAC_TRY(co_check(labels, n, h_koff, P));
   const int64_t ktot = h_koff[P];
   size_t need = eac_coassoc_workspace_bytes(n, P, ktot);
@@ -754,9 +762,14 @@ int co_plan_build(const int32_t* labels, int64_t n, const int64_t* h_koff, int32
   CoBufs B = carve_co(ar, n, P, ktot);
   const size_t mark = ar.used;
   const int64_t items = (int64_t)P * n;
-  EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1), hipMemcpyHostToDevice,
-                               s));
-  EAC_HIP_CHECK(hipStreamSynchronize(s));
+  if (d_koff) {  // already on the device: a device copy (capturable)
+    EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, d_koff, sizeof(int64_t) * (P + 1),
+                                 hipMemcpyDeviceToDevice, s));
+  } else {
+    EAC_HIP_CHECK(hipMemcpyAsync(B.k_off, h_koff, sizeof(int64_t) * (P + 1),
+                                 hipMemcpyHostToDevice, s));
+    EAC_HIP_CHECK(hipStreamSynchronize(s));
+  }
   EAC_TRY(fill<int32_t>(B.csize, ktot + 2, 0, s));
   EAC_TRY(fill<int32_t>((int32_t*)B.misc, 4, 0, s));
   EAC_TRY(fill<int64_t>((int64_t*)B.ctr, kCtrs, 0, s));
@@ -1073,11 +1086,96 @@ extern "C" int eac_coassoc_compact(int64_t n, const int64_t* indptr, const void*
   int pr = prof_begin(EAC_PROF_CO_COMPACT, 0, s);
   const int64_t blocks = (n + 3) / 4;
   k_coassoc_compact<<<(unsigned)(blocks < 65536 ? blocks : 65536), 256, 0, s>>>(
-      static_cast<const unsigned char*>(stage), indptr, n, w_offset, indices, data);
+      static_cast<const unsigned char*>(stage), indptr, n, w_offset, indices, data, nullptr);
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
   EAC_HIP_CHECK(hipStreamSynchronize(s));
   prof_flush();
+  return EAC_OK;
+}
+
+// ------------------------------------------- capturable one-pass build
+// Results of the capturable build in the caller's device int64[4]: nnz,
+// largest count, error bits (1: pool too small, 2: more than cap_nnz
+// entries, 4: a label >= k of its partition), 0.
+namespace eac {
+namespace {
+__global__ void k_co_finish(const int64_t* __restrict__ indptr, int64_t n,
+                            const unsigned int* __restrict__ misc,
+                            const int64_t* __restrict__ hdr, int64_t cap_nnz,
+                            int64_t* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int64_t nnz = indptr[n];
+  int64_t err = 0;
+  if (hdr[3]) err |= 1;
+  if (nnz > cap_nnz) err |= 2;
+  if (misc[0]) err |= 4;
+  out[0] = nnz;
+  out[1] = (int64_t)misc[1];
+  out[2] = err;
+  out[3] = 0;
+}
+}  // namespace
+}  // namespace eac
+
+// The one-pass build with no host read, copy or synchronisation (for
+// capture in a HIP graph): the member plan from d_koff (h_koff still gives
+// the sizes), the staged pass into `stage` (eac_coassoc_stage_bytes of the
+// pool the caller can afford; no estimate), the row offsets, and the
+// compaction into indices/data of capacity cap_nnz. w_offset as in
+// eac_coassoc_fill, or -1 for (largest count + 1) taken on the device.
+// d_out (device int64[4]): nnz, largest count, error bits (1: pool too
+// small, 2: nnz > cap_nnz, 4: label >= k; no entry is written then).
+extern "C" int eac_coassoc_graph(const int32_t* labels, int64_t n, const int64_t* h_koff,
+                                 const int64_t* d_koff, int32_t P, int32_t full_rows,
+                                 int32_t w_offset, int64_t* indptr, int32_t* indices,
+                                 int32_t* data, int64_t cap_nnz, void* stage,
+                                 size_t stage_bytes, int64_t* d_out, void* ws, size_t ws_bytes,
+                                 void* stream) {
+  EAC_TRY(co_check(labels, n, h_koff, P));
+  if (!d_koff || !indptr || !indices || !data || !stage || !d_out)
+    return fail(EAC_ERR_ARG, "coassoc graph: null argument");
+  if (w_offset < -1) return fail(EAC_ERR_ARG, "coassoc graph: w_offset < -1");
+  const int64_t ktot = h_koff[P];
+  const StageLayout SL = stage_layout(n, full_rows);
+  if (stage_bytes < SL.pool_at + (size_t)SL.chunk * 4)
+    return fail(EAC_ERR_WORKSPACE, "coassoc graph: staging buffer %zu < %zu bytes", stage_bytes,
+                SL.pool_at + (size_t)SL.chunk * 4);
+  hipStream_t s = (hipStream_t)stream;
+  Arena ar(ws, ws_bytes);
+  CoBufs B;
+  size_t mark = 0;
+  EAC_TRY(co_plan_build(labels, n, h_koff, P, ws, ws_bytes, &B, &mark, ar, s, d_koff));
+  unsigned char* st = static_cast<unsigned char*>(stage);
+  int64_t* hdr = reinterpret_cast<int64_t*>(st);
+  k_stage_header<<<1, 64, 0, s>>>(hdr, SL.win, SL.nw, (int64_t)SL.pool_at, full_rows);
+  EAC_LAUNCH_CHECK();
+  int nw = 0;
+  const int32_t* wtab = build_win_table(B, n, P, ktot, SL.win, &nw, s);
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoOut o{};
+  o.row_ctr = B.ctr;
+  o.row_nnz = B.row_nnz;
+  o.max_count = B.misc + 1;
+  o.seg = reinterpret_cast<uint64_t*>(st + kStageHdr);
+  o.pool = reinterpret_cast<uint32_t*>(st + SL.pool_at);
+  o.pool_ctr = reinterpret_cast<unsigned long long*>(hdr + 2);
+  o.pool_cap = (int64_t)((stage_bytes - SL.pool_at) / 4);
+  o.pool_chunk = SL.chunk;
+  o.overflow = reinterpret_cast<unsigned int*>(hdr + 3);
+  int pr = prof_begin(EAC_PROF_CO_STAGE, 0, s);
+  EAC_TRY(co_launch<2>(pl, full_rows, SL.win, o, s));
+  prof_end(pr, s);
+  EAC_TRY((scan_excl<int32_t, int64_t>(B.row_nnz, indptr, n + 1, 0, nullptr, ar, s)));
+  ar.used = mark;
+  k_co_finish<<<1, 64, 0, s>>>(indptr, n, B.misc, hdr, cap_nnz, d_out);
+  EAC_LAUNCH_CHECK();
+  int pr2 = prof_begin(EAC_PROF_CO_COMPACT, 0, s);
+  const int64_t blocks = (n + 3) / 4;
+  k_coassoc_compact<<<(unsigned)(blocks < 65536 ? blocks : 65536), 256, 0, s>>>(
+      st, indptr, n, w_offset, indices, data, d_out);
+  EAC_LAUNCH_CHECK();
+  prof_end(pr2, s);
   return EAC_OK;
 }
 
