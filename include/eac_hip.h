@@ -95,7 +95,10 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *   EAC_TUNE_CO_UNROLL      co-association member loads in flight per lane
  *                           while counting (4 by default, or 8)
  *   EAC_TUNE_CO_RANGES      co-association row ranges with a row counter
- *                           each (1..8, default 8: one per XCD) */
+ *                           each (1..8, default 8: one per XCD)
+ *   EAC_TUNE_KM_WAVES       waves per SIMD the split-BF16 labeling kernel
+ *                           is compiled for (2, 3 or 4; default 3 for
+ *                           d <= 64, else unconstrained) */
 #define EAC_TUNE_LIGHT_X10 0
 #define EAC_TUNE_FIRST_BATCH 1
 #define EAC_TUNE_FILTER_BPCU 2
@@ -103,6 +106,7 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
 #define EAC_TUNE_LEVEL_STEP 4
 #define EAC_TUNE_CO_UNROLL 5
 #define EAC_TUNE_CO_RANGES 6
+#define EAC_TUNE_KM_WAVES 7
 #define EAC_TUNE_COUNT 8
 int eac_tune_set(int32_t knob, int32_t value);
 int eac_tune_get(int32_t knob, int32_t* value);

@@ -129,7 +129,9 @@ class BatchedFitGraph:
         dev = self.X.device
         self.n, self.d, self.P, self.max_iter = n, d, P, int(max_iter)
         self.d_koff = t.from_numpy(self.koff).to(dev)
-        self.init = t.zeros((ktot, d), dtype=t.float32, device=dev)
+        # the warm-up call before capture starts from data rows (tied, all
+        # equal centroids would flag every point for re-scoring)
+        self.init = self.X.index_select(0, t.arange(ktot, device=dev) % n).contiguous()
         self.cen = t.zeros((ktot, d), dtype=t.float32, device=dev)
         self.labels = t.empty((P, n), dtype=t.int32, device=dev)
         self.dists = t.empty((P, n), dtype=t.float32, device=dev)

@@ -143,12 +143,12 @@ __device__ __forceinline__ float point_bound(float nx, float xn, float xhn, floa
   return (2.f * cross + epi + orc) * 1.0001f + 1e-30f;
 }
 
-// Waves per SIMD the compiler must fit: 3 for KD <= 64 (a few values
-// spill; measured faster than 2 waves without spills), otherwise no
-// constraint.
+// Waves per SIMD the compiler must fit (WPE): by default 3 for KD <= 64 (a
+// few values spill; measured faster than 2 waves without spills),
+// otherwise no constraint (1); the EAC_TUNE_KM_WAVES knob picks 2, 3 or 4.
 constexpr int label_waves(int KD) { return KD <= 64 ? 3 : 1; }
-template <int KD>
-__global__ __launch_bounds__(kLB) __attribute__((amdgpu_waves_per_eu(label_waves(KD))))
+template <int KD, int WPE = label_waves(KD)>
+__global__ __launch_bounds__(kLB) __attribute__((amdgpu_waves_per_eu(WPE)))
 void k_label_bf16x3(
     const float* __restrict__ X, int64_t n, int d, const uint16_t* __restrict__ Ch,
     const uint16_t* __restrict__ Cl, const float* __restrict__ ncf,
@@ -1289,44 +1289,57 @@ __global__ void k_flags_total(const unsigned long long* __restrict__ cnt, int64_
   }
 }
 
-// Counting sort of the flags by partition (flag = partition << 40 | row):
-// per-partition counts, their exclusive scan (one block), and a scatter
-// through per-partition cursors. The order inside a partition is free (each
-// flag is re-scored on its own).
-__global__ __launch_bounds__(256) void k_flag_pcount(const uint64_t* __restrict__ flags,
+// Counting sort of the flags by partition (flag = partition << 40 | row),
+// stable-free but deterministic: kSortBlocks workgroups each take the same
+// grid-stride share of the flags in both passes; pass 1 counts its share's
+// flags per partition in LDS (bcount[p][b]); one workgroup scans bcount in
+// (partition, workgroup) order into offsets; pass 2 places the share's flags
+// from LDS cursors started at those offsets. No global atomics (their
+// serialisation on a few hot partitions dominated).
+constexpr int kSortBlocks = 256;
+constexpr int kSortMaxP = 4096;  // partitions counted in LDS
+__global__ __launch_bounds__(256) void k_flag_bcount(const uint64_t* __restrict__ flags,
                                                      const int64_t* __restrict__ d_nf, int P,
-                                                     int32_t* __restrict__ pcount) {
+                                                     int32_t* __restrict__ bcount) {
+  __shared__ int32_t h[kSortMaxP];
+  for (int p = threadIdx.x; p < P; p += blockDim.x) h[p] = 0;
+  __syncthreads();
   const int64_t nf = *d_nf;
   for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
        f += (int64_t)gridDim.x * blockDim.x) {
     const int p = (int)(flags[f] >> 40);
-    if (p < P) atomicAdd(&pcount[p], 1);
+    if (p < P) atomicAdd(&h[p], 1);
   }
+  __syncthreads();
+  for (int p = threadIdx.x; p < P; p += blockDim.x) bcount[(int64_t)p * gridDim.x + blockIdx.x] = h[p];
 }
-__global__ __launch_bounds__(1024) void k_flag_pscan(const int32_t* __restrict__ pcount, int P,
-                                                     int32_t* __restrict__ pcursor) {
+__global__ __launch_bounds__(1024) void k_flag_bscan(int32_t* __restrict__ bcount, int64_t len) {
   __shared__ int32_t s_tmp[1024 / kWave + 1];
   int32_t carry = 0;
-  for (int p0 = 0; p0 < P; p0 += 1024) {
-    const int p = p0 + threadIdx.x;
-    const int32_t v = p < P ? pcount[p] : 0;
+  for (int64_t i0 = 0; i0 < len; i0 += 1024) {
+    const int64_t i = i0 + threadIdx.x;
+    const int32_t v = i < len ? bcount[i] : 0;
     int32_t tot;
     const int32_t pre = block_excl_scan<int32_t>(v, s_tmp, &tot);
-    if (p < P) pcursor[p] = carry + pre;
+    if (i < len) bcount[i] = carry + pre;
     carry += tot;
     __syncthreads();
   }
 }
-__global__ __launch_bounds__(256) void k_flag_pscatter(const uint64_t* __restrict__ flags,
+__global__ __launch_bounds__(256) void k_flag_bscatter(const uint64_t* __restrict__ flags,
                                                        const int64_t* __restrict__ d_nf, int P,
-                                                       int32_t* __restrict__ pcursor,
+                                                       const int32_t* __restrict__ boff,
                                                        uint64_t* __restrict__ out) {
+  __shared__ int32_t cur[kSortMaxP];
+  for (int p = threadIdx.x; p < P; p += blockDim.x)
+    cur[p] = boff[(int64_t)p * gridDim.x + blockIdx.x];
+  __syncthreads();
   const int64_t nf = *d_nf;
   for (int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; f < nf;
        f += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t fl = flags[f];
     const int p = (int)(fl >> 40);
-    if (p < P) out[atomicAdd(&pcursor[p], 1)] = fl;
+    if (p < P) out[atomicAdd(&cur[p], 1)] = fl;
   }
 }
 
@@ -1497,8 +1510,7 @@ size_t kmeans_label_ws_bytes(int64_t n, int d, int64_t ktot, int P) {
 struct KmGraph {
   const int32_t* any;  // some partition still runs (the labeling counts only then)
   int64_t* d_nf;
-  int32_t* pcount;
-  int32_t* pcursor;
+  int32_t* bcount;     // [P][kSortBlocks] flags per (partition, sorting workgroup)
   uint64_t* sorted;
   int64_t* info;
   unsigned int* err;
@@ -1550,19 +1562,29 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   EAC_LAUNCH_CHECK();
   int g = (int)((n + kBM - 1) / kBM);
   int pr = prof_begin(EAC_PROF_KM_LABEL, 2 * n * (int64_t)d * ktot, s);
-  switch (KD) {
-    case 32:
-      k_label_bf16x3<32><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                           labels, B.flags_r, B.nflag, B.region);
-      break;
-    case 64:
-      k_label_bf16x3<64><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                           labels, B.flags_r, B.nflag, B.region);
-      break;
-    default:
-      k_label_bf16x3<128><<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat,
-                                            labels, B.flags_r, B.nflag, B.region);
-      break;
+  {
+    // waves per SIMD the kernel is compiled for: the built-in choice, or
+    // EAC_TUNE_KM_WAVES (2, 3 or 4; the same labels, other register
+    // budgets and occupancy)
+    const int wk = tune_value(EAC_TUNE_KM_WAVES);
+    const int w = (wk >= 2 && wk <= 4) ? wk : 0;
+    auto go = [&](auto kern) {
+      kern<<<g, kLB, 0, s>>>(X, n, d, B.Ch, B.Cl, B.ncf, B.k_off, P, B.pstat, labels, B.flags_r,
+                             B.nflag, B.region);
+    };
+    if (KD == 32) {
+      if (w == 2) go(k_label_bf16x3<32, 2>);
+      else if (w == 4) go(k_label_bf16x3<32, 4>);
+      else go(k_label_bf16x3<32, 3>);
+    } else if (KD == 64) {
+      if (w == 2) go(k_label_bf16x3<64, 2>);
+      else if (w == 4) go(k_label_bf16x3<64, 4>);
+      else go(k_label_bf16x3<64, 3>);
+    } else {
+      if (w == 2) go(k_label_bf16x3<128, 2>);
+      else if (w == 3) go(k_label_bf16x3<128, 3>);
+      else go(k_label_bf16x3<128, 1>);
+    }
   }
   EAC_LAUNCH_CHECK();
   prof_end(pr, s);
@@ -1576,13 +1598,12 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
     EAC_LAUNCH_CHECK();
     const uint64_t* fl = B.flags;
     const bool tiled = rescore_tiled_lds(d) <= 64 * 1024;
-    if (tiled && P > 1) {
-      EAC_TRY(fill<int32_t>(kg->pcount, P, 0, s));
-      k_flag_pcount<<<gp, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->pcount);
+    if (tiled && P > 1 && P <= kSortMaxP) {
+      k_flag_bcount<<<kSortBlocks, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->bcount);
       EAC_LAUNCH_CHECK();
-      k_flag_pscan<<<1, 1024, 0, s>>>(kg->pcount, P, kg->pcursor);
+      k_flag_bscan<<<1, 1024, 0, s>>>(kg->bcount, (int64_t)P * kSortBlocks);
       EAC_LAUNCH_CHECK();
-      k_flag_pscatter<<<gp, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->pcursor, kg->sorted);
+      k_flag_bscatter<<<kSortBlocks, 256, 0, s>>>(B.flags, kg->d_nf, P, kg->bcount, kg->sorted);
       EAC_LAUNCH_CHECK();
       fl = kg->sorted;
     }
@@ -1592,8 +1613,10 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
       int dev = 0, cus = 256;
       if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      // as many workgroups as the eager launch of a few million flags
+      // would have; those past the device count exit at once
       const int64_t tiles = (B.flag_cap + kRtF - 1) / kRtF;
-      const unsigned nb = (unsigned)(tiles < 4 * (int64_t)cus ? tiles : 4 * (int64_t)cus);
+      const unsigned nb = (unsigned)(tiles < 64 * (int64_t)cus ? tiles : 64 * (int64_t)cus);
       auto go = [&](auto kern) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1886,8 +1909,7 @@ namespace {
 KmGraph carve_graph(Arena& ar, int64_t n, int P, int64_t* info, unsigned int* err) {
   KmGraph g;
   g.d_nf = ar.take<int64_t>(1);
-  g.pcount = ar.take<int32_t>(P);
-  g.pcursor = ar.take<int32_t>(P);
+  g.bcount = ar.take<int32_t>((size_t)P * kSortBlocks);
   g.sorted = ar.take<uint64_t>((size_t)P * n);
   g.info = info;
   g.err = err;

@@ -23,7 +23,7 @@ from pathlib import Path
 from . import _lib as L
 
 KNOBS = {"light_x10": 0, "first_batch": 1, "filter_bpcu": 2, "co_window": 3, "level_step": 4,
-         "co_unroll": 5, "co_ranges": 6}
+         "co_unroll": 5, "co_ranges": 6, "km_waves": 7}
 
 DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),
@@ -158,6 +158,47 @@ def autotune_coassoc(labels, ks, windows=(4096, 8192, 16384, 32768), unrolls=(4,
                                f"{sig} vs {ref}")
         del ip, ind, dat
         results.append((ev0.elapsed_time(ev1), {"co_window": w, "co_unroll": u}))
+    results.sort(key=lambda x: x[0])
+    best_ms, best = results[0]
+    set_knobs(**best)
+    cache[key] = {"knobs": best, "ms": best_ms}
+    if use_cache:
+        _store_cache(cache)
+    return {"knobs": best, "ms": best_ms, "cached": False,
+            "candidates": [{"ms": ms, **k} for ms, k in results]}
+
+
+def autotune_kmeans(X, centroids, k_list=None, waves=(0, 2, 3, 4), reps=3, use_cache=True):
+    """Picks the waves per SIMD the split-BF16 labeling kernel is compiled
+    for (km_waves; 0 = built-in) on this data shape: timed label passes,
+    identical labels checked. Returns dict(knobs, ms, cached, candidates)."""
+    from .cluster.KMeans import label_device
+    t = L.torch()
+    n, d = X.shape
+    ks = list(k_list) if k_list is not None else [int(centroids.shape[0])]
+    dev = t.cuda.get_device_name(t.cuda.current_device())
+    key = "kmeans|%s|n2^%d|d%d|k2^%d|P%d" % (dev, int(n).bit_length(), int(d),
+                                             int(sum(ks)).bit_length(), len(ks))
+    cache = _load_cache() if use_cache else {}
+    if key in cache:
+        set_knobs(**cache[key]["knobs"])
+        return {"knobs": cache[key]["knobs"], "ms": cache[key]["ms"], "candidates": None,
+                "cached": True}
+    results, ref = [], None
+    for w in waves:
+        set_knobs(km_waves=w)
+        lab, _, _ = label_device(X, centroids, ks, precision="bf16x3")  # warm
+        ev0, ev1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            lab, _, _ = label_device(X, centroids, ks, precision="bf16x3")
+        ev1.record()
+        ev1.synchronize()
+        if ref is None:
+            ref = lab
+        elif not t.equal(lab, ref):
+            raise RuntimeError(f"km_waves={w} changed the labels")
+        results.append((ev0.elapsed_time(ev1) / reps, {"km_waves": w}))
     results.sort(key=lambda x: x[0])
     best_ms, best = results[0]
     set_knobs(**best)

@@ -62,10 +62,10 @@ def test_tuning_knobs_roundtrip():
     before = tune.get_knobs()
     try:
         tune.set_knobs(light_x10=25, first_batch=9, filter_bpcu=3, co_window=4096,
-                       level_step=8, co_unroll=8, co_ranges=1)
+                       level_step=8, co_unroll=8, co_ranges=1, km_waves=2)
         assert tune.get_knobs() == {"light_x10": 25, "first_batch": 9, "filter_bpcu": 3,
                                     "co_window": 4096, "level_step": 8, "co_unroll": 8,
-                                    "co_ranges": 1}
+                                    "co_ranges": 1, "km_waves": 2}
         tune.set_knobs(light_x10=-1)
         assert tune.get_knobs()["light_x10"] == -1
         lib = __import__("masters_final_amd._lib", fromlist=["x"]).load_library()

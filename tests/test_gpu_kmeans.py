@@ -294,3 +294,26 @@ def test_fit_float32_stop_rule(torch):
     km = K_Means(n_clusters=2, max_iter=3, init=init)
     km.fit(X)
     assert km.iters_ == 2 and km.inertia_.tobytes() == inert.tobytes()
+
+
+@pytest.mark.parametrize("d", [20, 64, 100])
+def test_label_waves_knob_keeps_labels(torch, d):
+    """The waves-per-SIMD variants of the split-BF16 labeling kernel
+    (km_waves 2, 3, 4, built-in) give the same labels and dists."""
+    from masters_final_amd.cluster.KMeans import label_device
+    from masters_final_amd.tune import get_knobs, set_knobs
+    rng = np.random.default_rng(d)
+    X = rng.normal(size=(3000, d)).astype(np.float32)
+    Cc = X[rng.choice(3000, 150, replace=False)].copy()
+    dX, dC = torch.from_numpy(X).cuda(), torch.from_numpy(Cc).cuda()
+    saved = get_knobs()
+    outs = []
+    try:
+        for w in (0, 2, 3, 4):
+            set_knobs(km_waves=w)
+            lab, dist, _ = label_device(dX, dC, precision="bf16x3")
+            outs.append((lab.clone(), dist.clone()))
+    finally:
+        set_knobs(**saved)
+    for lab, dist in outs[1:]:
+        assert torch.equal(lab, outs[0][0]) and torch.equal(dist, outs[0][1])
