@@ -105,6 +105,11 @@ __global__ void k_win_table(const uint32_t* __restrict__ skey, const int32_t* __
   }
 }
 
+// 64-bit words of a window's run-start bitmap: the short-run path covers
+// fewer than 32 flattened members per partition (rounded up to an even
+// count: the counters after it stay 16-byte aligned).
+__host__ __device__ inline int co_bm_words(int P) { return ((P / 2 + 1) + 1) & ~1; }
+
 struct CoPlan {
   const int32_t* labels;
   int64_t n;
@@ -193,21 +198,29 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
   int32_t* __restrict__ out_col = out.col;
   int32_t* __restrict__ out_val = out.val;
   int32_t* __restrict__ out_rank = out.rank;
-  // launch-time LDS: window counters (win / 2 words), with RANK the rank
+  // launch-time LDS: this window's run-start bitmap (co_bm_words(P)
+  // 64-bit words), the window counters (win / 2 words), with RANK the rank
   // window (win words), then per partition the next member position, the
   // cluster end, this window's member end and the flattened-range prefix
   // (P + 1), the row's global cluster, this window's member position minus
-  // flattened index, with RANK each partition's rank
+  // flattened index, each partition's rank; per run (non-empty partition
+  // range of this window) the position delta and the partition's rank; per
+  // bitmap word the run starts before it
   extern __shared__ __align__(16) uint32_t s_dyn[];
-  uint32_t* s_cnt = s_dyn;
-  uint32_t* s_rank = s_dyn + win / 2;
-  int32_t* s_cur = reinterpret_cast<int32_t*>(s_dyn + win / 2 + (RANK ? win : 0));
+  uint64_t* s_bm = reinterpret_cast<uint64_t*>(s_dyn);
+  const int bmw = co_bm_words(pl.P);
+  uint32_t* s_cnt = s_dyn + 2 * bmw;
+  uint32_t* s_rank = s_cnt + win / 2;
+  int32_t* s_cur = reinterpret_cast<int32_t*>(s_cnt + win / 2 + (RANK ? win : 0));
   int32_t* s_end = s_cur + pl.P;
   int32_t* s_hi = s_end + pl.P;
   int32_t* s_pre = s_hi + pl.P;
   int32_t* s_g = s_pre + pl.P + 1;  // global cluster of the row in each partition
   int32_t* s_d = s_g + pl.P;        // this window: member position - flattened index
   int32_t* s_prank = s_d + pl.P;
+  int32_t* s_dr = s_prank + pl.P;   // per run: member position - flattened index
+  int32_t* s_rp = s_dr + pl.P;      // per run: rank of its partition
+  int32_t* s_wpre = s_rp + pl.P;    // per bitmap word: run starts in the words before
   __shared__ uint32_t s_tmp[kCB / kWave + 1];
   __shared__ int64_t s_row;
   __shared__ unsigned long long s_chunk;
@@ -289,6 +302,7 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
     // windows aligned to multiples of win (columns <= i get no counts)
     for (int64_t c0 = full_rows ? 0 : ((i + 1) / win) * win; c0 < n; c0 += win) {
       const int64_t c1 = c0 + win < n ? c0 + win : n;
+      for (int v = t; v < bmw; v += kCB) s_bm[v] = 0ull;  // read back after the next barrier
       // members of this window: [cur, hi) with member < c1 (lists ascending);
       // the last window takes every remaining member, no search. Otherwise
       // an exponential probe from cur (a window holds few members of each
@@ -320,20 +334,45 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         }
       }
       __syncthreads();
-      // flattened ranges: prefix of (hi - cur) over partitions
+      // flattened ranges: prefix of (hi - cur) over partitions; the runs
+      // (non-empty ranges) in partition order, each start marked in the
+      // bitmap over the flattened indices (short windows), and per bitmap
+      // word the run starts before it. One wavefront: its LDS operations
+      // complete in order, so its bitmap reads see its own marks.
       if (t < kWave) {
-        int32_t carry = 0;
+        int32_t carry = 0, rcarry = 0;
+        const uint64_t below = (1ull << t) - 1ull;
         for (int p0 = 0; p0 < P; p0 += kWave) {
           int p = p0 + t;
           int32_t len = p < P ? s_hi[p] - s_cur[p] : 0;
           int32_t inc = wave_incl_scan(len);
+          const uint64_t ne = __ballot(len > 0);
           if (p < P) {
-            s_pre[p] = carry + inc - len;
-            s_d[p] = s_cur[p] - (carry + inc - len);
+            const int32_t pre = carry + inc - len;
+            s_pre[p] = pre;
+            s_d[p] = s_cur[p] - pre;
+            if (len > 0) {
+              const int r = rcarry + __popcll(ne & below);
+              s_dr[r] = s_cur[p] - pre;
+              if (RANK) s_rp[r] = s_prank[p];
+              if (pre < 64 * bmw) atomicOr(&s_bm[pre >> 6], 1ull << (pre & 63));
+            }
           }
           carry += __shfl(inc, kWave - 1, kWave);
+          rcarry += __popcll(ne);
         }
         if (t == 0) s_pre[P] = carry;
+        if (carry < 32 * P) {  // the bitmap covers the window: word prefix counts
+          const int W = (carry + 63) >> 6;
+          int32_t wc = 0;
+          for (int w0 = 0; w0 < W; w0 += kWave) {
+            const int w = w0 + t;
+            const int32_t c = w < W ? (int32_t)__popcll(s_bm[w]) : 0;
+            const int32_t ci = wave_incl_scan(c);
+            if (w < W) s_wpre[w] = wc + ci - c;
+            wc += __shfl(ci, kWave - 1, kWave);
+          }
+        }
       }
       __syncthreads();
       const int32_t tot = s_pre[P];
@@ -374,53 +413,32 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           }
         }
       } else {
+        // short runs: each wavefront takes U * 64 consecutive f per step;
+        // f's run = (run starts at or before f) - 1: the starts before its
+        // 64-bit bitmap word (prefix) plus a popcount of the word up to f
         const int wv = t / kWave, ln = lane_id();
-        int pw = 0;
+        const uint64_t upto = ln == kWave - 1 ? ~0ull : ((2ull << ln) - 1ull);
         for (int32_t base = wv * U * kWave; base < tot; base += U * kCB) {
-          for (;;) {  // uniform: advance the cursor to the partition of `base`
-            const int q = pw + 1 + ln;
-            const int32_t b = q <= P ? s_pre[q] : INT32_MAX;
-            const uint64_t m = __ballot(b <= base);
-            if (m == ~0ull) {
-              pw += kWave;
-              continue;
-            }
-            pw += __popcll(m);
-            break;
-          }
-          const int qb = pw + 1 + ln;
-          const int32_t bnd = qb <= P ? s_pre[qb] : INT32_MAX;  // ascending by lane
-          const bool covered = __shfl(bnd, kWave - 1, kWave) > base + U * kWave - 1;
           int32_t j[U];
-          int pu[U];
+          int32_t rk[U];
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             const int32_t f0 = base + u * kWave, f = f0 + ln;
-            int p = pw;
-            if (covered) {
-              // partition of f = pw + (held boundaries <= f): those <= f0
-              // count for every lane; the few inside (f0, f0 + 63] are read
-              // one by one (wave-uniform loop, one readlane each)
-              const uint64_t le0 = __ballot(bnd <= f0);
-              uint64_t mid = __ballot(bnd <= f0 + kWave - 1) & ~le0;
-              p += __popcll(le0);
-              while (mid) {
-                const int q = __builtin_ctzll(mid);
-                p += __builtin_amdgcn_readlane(bnd, q) <= f ? 1 : 0;
-                mid &= mid - 1;
-              }
-            } else if (f < tot) {
-              while (s_pre[p + 1] <= f) ++p;
+            j[u] = -1;
+            rk[u] = 0;
+            if (f < tot) {
+              const int w = f0 >> 6;
+              const int r = s_wpre[w] + (int)__popcll(s_bm[w] & upto) - 1;
+              j[u] = pl.members[f + s_dr[r]];
+              if (RANK) rk[u] = s_rp[r];
             }
-            j[u] = f < tot ? pl.members[f + s_d[p]] : -1;
-            pu[u] = p;
           }
 #pragma unroll
           for (int u = 0; u < U; ++u) {
             if (j[u] < 0 || (full_rows && j[u] == (int32_t)i)) continue;
             const int jj = (int)(j[u] - c0);
             atomicAdd(&s_cnt[jj >> 1], 1u << (16 * (jj & 1)));
-            if (RANK) atomicMin(&s_rank[jj], (uint32_t)s_prank[pu[u]]);
+            if (RANK) atomicMin(&s_rank[jj], (uint32_t)rk[u]);
           }
         }
       }
@@ -692,7 +710,8 @@ inline int co_ranges() {
   return r <= 0 || r > kMaxRanges ? kMaxRanges : r;
 }
 inline size_t co_lds(int win, int P, bool rank = false) {
-  return (size_t)(win / 2 + (rank ? win + P : 0) + 6 * P + 1) * 4;
+  return (size_t)(2 * co_bm_words(P) + win / 2 + (rank ? win : 0) + 9 * P + 1 +
+                  (P / 2 + 1)) * 4;
 }
 // The ranked fill keeps a 4-byte rank per column next to the counters: its
 // window is at most half as wide.
