@@ -485,14 +485,26 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
         const int w = t / kWave, l = lane_id();
         const int RG = ((groups + 4 * kWave - 1) / (4 * kWave)) * kWave;
         const int ga = w * RG, gb = (w + 1) * RG < groups ? (w + 1) * RG : groups;
+        // non-zero 16-bit halves of a group: the high bit of each half of
+        // ((x & 0x7fff7fff) + 0x7fff7fff) | x is set iff the half is non-zero
+        auto nz_count = [](const uint32_t* xw) {
+          uint32_t c = 0;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            c += __popc((((xw[q] & 0x7fff7fffu) + 0x7fff7fffu) | xw[q]) & 0x80008000u);
+          return c;
+        };
         uint32_t c = 0;
         for (int g = ga + l; g < gb; g += kWave) {
+          uint32_t xw[4];
+          if (4 * g + 4 <= words) {
+            const uint4 x = s4[g];
+            xw[0] = x.x; xw[1] = x.y; xw[2] = x.z; xw[3] = x.w;
+          } else {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int v = 4 * g + q;
-            const uint32_t x = v < words ? s_cnt[v] : 0u;
-            c += ((x & 0xffffu) != 0) + ((x >> 16) != 0);
+            for (int q = 0; q < 4; ++q) xw[q] = 4 * g + q < words ? s_cnt[4 * g + q] : 0u;
           }
+          c += nz_count(xw);
         }
         c = wave_sum(c);
         if (l == 0) s_tmp[w] = c;
@@ -524,9 +536,8 @@ __global__ __launch_bounds__(kCB) void k_coassoc_rows(CoPlan pl, int full_rows, 
           const int g = g0 + l;
           uint32_t xw[4] = {0u, 0u, 0u, 0u};
           if (g < gb) read_group(g, xw);
-          uint32_t cnt = 0;
-#pragma unroll
-          for (int q = 0; q < 4; ++q) cnt += ((xw[q] & 0xffffu) != 0) + ((xw[q] >> 16) != 0);
+          const uint32_t cnt = nz_count(xw);
+          if (__ballot(cnt != 0u) == 0ull) continue;  // wave-uniform: no entry in these groups
           const uint32_t incl = wave_incl_scan(cnt);
           int64_t at = o + (incl - cnt);
           if (PASS == 2) {

@@ -682,6 +682,12 @@ __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long l
   if (active) min_insert(s_root, s_min, root, key, best);
 }
 
+// Keys per lane per step of the filter (their root lookups in flight together).
+#ifndef EAC_FILTER_KEYS
+#define EAC_FILTER_KEYS 4
+#endif
+constexpr int kFilterKeys = EAC_FILTER_KEYS;
+
 // Filters one chunk `cc` (n_in keys) of `list` with one wavefront. ROUTE
 // (the first round of a weight level): `list` is the heavy list, whose cross
 // keys below `lim` (the level's upper threshold, on the key's weight bits)
@@ -703,33 +709,35 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
   uint64_t* lc = list + cc * kGTile;
   uint64_t* tc = ROUTE ? to + cc * kGTile : nullptr;
   int32_t out = 0, out_to = 0;
-  for (int base = 0; base < n_in; base += 4 * kWave) {
-    uint64_t k[4];
+  constexpr int Q = kFilterKeys;
+  for (int base = 0; base < n_in; base += Q * kWave) {
+    uint64_t k[Q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       const int e = base + q * kWave + lane;
       k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
     }
-    int32_t x[8];  // roots of the endpoints of the 4 keys (parents are flat)
+    int32_t x[2 * Q];  // roots of the endpoints of the Q keys (parents are flat)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       int32_t a = 0, b = 0;
       if (k[q] != kInf) unpack_ends(k[q], vb, &a, &b);
       x[2 * q] = a;
       x[2 * q + 1] = b;
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) x[i] = P[x[i]];
-    bool keep[4], cand[4];
+    for (int i = 0; i < 2 * Q; ++i) x[i] = P[x[i]];
+    bool keep[Q], cand[Q];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
       cand[q] = keep[q] && (!ROUTE || (k[q] >> (2 * vb)) < lim);
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
+    for (int i = 0; i < 2 * Q; ++i)
+      wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < Q; ++q) {
       if (ROUTE) {
         const uint64_t mt = __ballot(cand[q]);
         if (cand[q]) tc[out_to + __popcll(mt & below)] = k[q];
