@@ -574,30 +574,8 @@ __device__ int32_t find_live(int32_t* P, int32_t x, int32_t n, unsigned int* ove
     }
   }
 }
-// Roots of two vertices with both walks in flight together, in a kernel
-// where hooks happen (fresh loads; a start vertex is then pointed at the
-// root it found, which is an ancestor whatever hooks follow).
-__device__ __forceinline__ void roots_live2(int32_t* P, int32_t* a, int32_t* b, int32_t n,
-                                            unsigned int* overflow) {
-  int32_t x = *a, y = *b;
-  for (int32_t steps = 0;; ++steps) {
-    const int32_t px = uf_ld(P, x), py = uf_ld(P, y);
-    if (px == x && py == y) break;
-    x = px;
-    y = py;
-    if (steps > n) {
-      atomicOr(overflow, 4u);
-      break;
-    }
-  }
-  if (x != *a && uf_ld(P, *a) != x) uf_st(P, *a, x);
-  if (y != *b && uf_ld(P, *b) != y) uf_st(P, *b, y);
-  *a = x;
-  *b = y;
-}
-
 // ------------------------------------------------------ device round control
-// A round is hook(r) -> flatten(r) -> filter(r). The control state of round
+// A round is hook(r) -> filter(r). The control state of round
 // r lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
 // from it and from had_edge[r] (some representative had a candidate edge in
 // hook(r)), and block 0 stores it in ctl[(r + 1) & 1]. Rounds filter the
@@ -699,11 +677,11 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
                                              int32_t* __restrict__ cnt,
                                              uint64_t* __restrict__ to,
                                              int32_t* __restrict__ cnt_to, int64_t cc,
-                                             int32_t n_in, uint64_t lim, int vb, const int32_t* P,
+                                             int32_t n_in, uint64_t lim, int vb, int32_t* P,
                                              int32_t* s_root, unsigned long long* s_min,
                                              unsigned long long* __restrict__ best,
                                              unsigned long long* sum_in,
-                                             unsigned long long* sum_out) {
+                                             unsigned long long* sum_out, bool* any_cand) {
   const int lane = lane_id();
   const uint64_t below = (1ull << lane) - 1ull;
   uint64_t* lc = list + cc * kGTile;
@@ -717,7 +695,7 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
       const int e = base + q * kWave + lane;
       k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
     }
-    int32_t x[2 * Q];  // roots of the endpoints of the Q keys (parents are flat)
+    int32_t x[2 * Q];  // roots of the endpoints of the Q keys
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       int32_t a = 0, b = 0;
@@ -725,14 +703,41 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
       x[2 * q] = a;
       x[2 * q + 1] = b;
     }
-#pragma unroll
-    for (int i = 0; i < 2 * Q; ++i) x[i] = P[x[i]];
+    // Light keys: P[x] is x's root when the round started (their ends are
+    // written back below) and that root's parent is its root after this
+    // round's hooks. A routing pass follows a round without hooks, whose
+    // hook pass left P flat.
     bool keep[Q], cand[Q];
+    if (ROUTE) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
-      cand[q] = keep[q] && (!ROUTE || (k[q] >> (2 * vb)) < lim);
+      for (int i = 0; i < 2 * Q; ++i) x[i] = P[x[i]];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        keep[q] = k[q] != kInf && x[2 * q] != x[2 * q + 1];
+        cand[q] = keep[q] && (k[q] >> (2 * vb)) < lim;
+      }
+    } else {
+      int32_t p[2 * Q];
+#pragma unroll
+      for (int i = 0; i < 2 * Q; ++i) p[i] = P[x[i]];
+      int32_t rt[2 * Q];
+#pragma unroll
+      for (int i = 0; i < 2 * Q; ++i) rt[i] = P[p[i]];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        keep[q] = k[q] != kInf && rt[2 * q] != rt[2 * q + 1];
+        cand[q] = keep[q];
+      }
+      // a kept key's ends point at their roots: the next round's hook
+      // reads them with one load
+#pragma unroll
+      for (int i = 0; i < 2 * Q; ++i) {
+        if (keep[i / 2] && rt[i] != p[i]) P[x[i]] = rt[i];
+        x[i] = rt[i];
+      }
     }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) *any_cand |= cand[q];
 #pragma unroll
     for (int i = 0; i < 2 * Q; ++i)
       wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
@@ -772,7 +777,8 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                const unsigned int* __restrict__ had_edge,
                                                const MstPlan* __restrict__ plan,
                                                int32_t* P, unsigned long long* __restrict__ best,
-                                               int vb, int max_rounds,
+                                               unsigned int* __restrict__ cand_next, int vb,
+                                               int max_rounds,
                                                unsigned long long* __restrict__ rlog) {
   __shared__ int32_t s_root[kMinSlots];
   __shared__ unsigned long long s_min[kMinSlots];
@@ -780,7 +786,19 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   __shared__ int32_t s_work[kOwnBatch], s_work_n[kOwnBatch];
   __shared__ int s_take, s_nwork;
   const RoundCtl prev = ctl[r & 1];
-  const RoundCtl c = next_ctl(prev, r, had_edge[r], max_rounds, plan);
+  const unsigned int had = had_edge[r];
+  // The first batch of chunk counts of both lists is read with the control
+  // word (one round trip instead of two).
+  const int lane = lane_id();
+  const int64_t G = gridDim.x;
+  const int64_t owned = nchunks > (int64_t)blockIdx.x ? (nchunks - 1 - blockIdx.x) / G + 1 : 0;
+  int32_t pre_l = 0, pre_h = 0;
+  if (threadIdx.x < kWave && lane < owned) {
+    const int64_t cc = blockIdx.x + (int64_t)lane * G;
+    pre_l = cnt_light[cc];
+    pre_h = cnt_heavy[cc];
+  }
+  const RoundCtl c = next_ctl(prev, r, had, max_rounds, plan);
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
   const bool route = c.phase != prev.phase;
@@ -789,7 +807,6 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
     s_min[i] = kInf;
   }
   __syncthreads();
-  const int lane = lane_id();
   // Workgroup b owns chunks b, b + G, b + 2G, ... (G = grid): the key
   // density of the list varies along it, and interleaving gives every
   // workgroup a sample of the whole list. Per batch of up to 64 owned
@@ -799,14 +816,13 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   uint64_t* const list = route ? heavy : light;
   int32_t* const cnt = route ? cnt_heavy : cnt_light;
   const uint64_t lim = route ? plan->lim[c.phase] : kInf;
-  const int64_t G = gridDim.x;
-  const int64_t owned = nchunks > (int64_t)blockIdx.x ? (nchunks - 1 - blockIdx.x) / G + 1 : 0;
   unsigned long long sum_in = 0, sum_out = 0;
+  bool any_cand = false;
   for (int64_t b0 = 0; b0 < owned; b0 += kOwnBatch) {
     if (threadIdx.x < kWave) {
       const int64_t j = b0 + lane;
       const int64_t cc = blockIdx.x + j * G;
-      int32_t nk = j < owned ? cnt[cc] : 0;
+      int32_t nk = b0 == 0 ? (route ? pre_h : pre_l) : (j < owned ? cnt[cc] : 0);
       nk = nk < 0 ? 0 : (nk > kGTile ? kGTile : nk);
       const uint64_t m = __ballot(nk > 0);
       if (nk > 0) {
@@ -829,13 +845,14 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
       const int32_t n_in = s_work_n[w];
       if (route)
         filter_chunk<true>(list, cnt, light, cnt_light, cc, n_in, lim, vb, P, s_root, s_min,
-                           best, &sum_in, &sum_out);
+                           best, &sum_in, &sum_out, &any_cand);
       else
         filter_chunk<false>(list, cnt, nullptr, nullptr, cc, n_in, lim, vb, P, s_root, s_min,
-                            best, &sum_in, &sum_out);
+                            best, &sum_in, &sum_out, &any_cand);
     }
     __syncthreads();
   }
+  if (__ballot(any_cand) && lane == 0) cand_next[r + 1] = 1u;
   // Flush: one global atomicMin per distinct root of the workgroup, skipped
   // when a cached value (an upper bound of the current one) already wins.
   // Every thread reads its slots' cached values together (one round trip).
@@ -870,66 +887,101 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
 }
 
 // Every root v whose slot holds a candidate k (its component's lightest edge
-// to another component) follows k to the other side's current root s. If s
-// picked the same edge and v < s, v stays a root (mst.py:282-305: the
-// smaller id keeps no edge); otherwise v hooks under s and k joins the
-// forest, recorded in v's slot (a root hooks at most once in its lifetime,
-// so forest slots are written once and their order does not depend on
-// timing). Slots of round r are read-only here (the next round's slots are
-// cleared instead), so the mirror test sees every choice of the round.
-// If s's side already hooked under v (the mirror case with v < s), v stays.
+// to another component) joins the component of the other side s. The
+// candidates of one round form a forest of pick trees whose only cycle is a
+// mirror pair (two roots that picked the same key, mst.py:282-305: the
+// smaller id keeps no edge and stays the root). v walks its tree along the
+// picks (v -> s -> s's pick -> ...) to that pair, points its parent straight
+// at the tree's root and records k in its forest slot (a root hooks at most
+// once in its lifetime, so forest slots are written once and their order
+// does not depend on timing). Slots of round r are read-only here (the next
+// round's slots are cleared instead).
+// The ends of every candidate key point straight at their roots when the
+// round starts (the filter that listed the key wrote them back), and only
+// roots' parents change here, each written once, by its own thread, with
+// its tree's root; so P[x] of a key end x reads either x's root or that
+// root's tree root, which lies in the same tree: a walk that reads the
+// latter jumps ahead and still ends at the pair, and v's own side always
+// reads v. Stale reads only make a walk longer. A walk longer than n can
+// only come from a graph passed as symmetric that is not (flag 4).
+// A round whose previous filter listed no candidate hooks nothing and
+// flattens instead: every vertex walks its parent chain (vertices that sat
+// in no light key since an earlier round lag behind) and points at its
+// root, so the routing pass that follows finds every root with one load.
 __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* __restrict__ ctl,
                                               const unsigned long long* __restrict__ best,
                                               unsigned long long* __restrict__ best_next,
                                               int32_t* P, uint64_t* __restrict__ mst_key,
-                                              unsigned int* __restrict__ had_edge, int vb,
+                                              unsigned int* __restrict__ had_edge,
+                                              const unsigned int* __restrict__ cand, int vb,
                                               unsigned int* __restrict__ overflow) {
+  const int64_t v0 = (int64_t)blockIdx.x * kGB + threadIdx.x;
+  const uint64_t k0 = v0 < n ? best[v0] : kInf;  // read with the control word
+  const unsigned int has_cand = r == 0 ? 1u : cand[r];
   if (!ctl[r & 1].active) return;
+  if (!has_cand) {
+    for (int64_t v = v0; v < n; v += (int64_t)gridDim.x * kGB) {
+      best_next[v] = kInf;
+      int32_t x = P[v];
+      const int32_t p0 = x;
+      for (int32_t steps = 0;; ++steps) {
+        const int32_t p = P[x];
+        if (p == x) break;
+        x = p;
+        if (steps > n) {
+          atomicOr(overflow, 4u);
+          break;
+        }
+      }
+      if (x != p0) P[v] = x;
+    }
+    return;
+  }
   bool any = false;
-  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kGB) {
+  for (int64_t v = v0; v < n; v += (int64_t)gridDim.x * kGB) {
     best_next[v] = kInf;
-    const uint64_t k = best[v];
+    const uint64_t k = v == v0 ? k0 : best[v];
     if (k == kInf) continue;
     any = true;
-    int32_t ra, rb;
-    unpack_ends(k, vb, &ra, &rb);
-    roots_live2(P, &ra, &rb, n, overflow);
-    const int32_t s = (ra == (int32_t)v) ? rb : ra;
-    if (s == (int32_t)v) continue;
-    if (best[s] == k && (int32_t)v < s) continue;  // mirror pick: v stays
-    uf_st(P, (int32_t)v, s);
-    mst_key[v] = k;
-  }
-  if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
-}
-
-// After the links of round r every vertex is pointed straight at its root:
-// each thread walks its own vertex's chain (plain loads; no link happens
-// here and parents only ever move up) and writes only that vertex, so when
-// the kernel ends the parents are exactly flat and the filter finds a root
-// with one load.
-__global__ __launch_bounds__(kGB) void k_compress(int r, int32_t n,
-                                                  const RoundCtl* __restrict__ ctl,
-                                                  const unsigned int* __restrict__ had_edge,
-                                                  int32_t* P, unsigned int* __restrict__ overflow) {
-  if (!ctl[r & 1].active || !had_edge[r]) return;  // nothing linked: still flat
-  for (int64_t v = (int64_t)blockIdx.x * kGB + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * kGB) {
-    const int32_t x0 = (int32_t)v;
-    int32_t x = P[x0];
-    if (x == x0) continue;
+    int32_t a, b;
+    unpack_ends(k, vb, &a, &b);
+    const int32_t pa = P[a], pb = P[b];
+    int32_t t = pa == (int32_t)v ? pb : pa;  // the other side
+    if (t == (int32_t)v) continue;
+    int32_t y = (int32_t)v, f;
+    uint64_t ky = k;
     for (int32_t steps = 0;; ++steps) {
-      const int32_t p = P[x];
-      if (p == x) break;
-      x = p;
+      const uint64_t kt = best[t];
+      const int32_t pt = P[t];
+      if (kt == ky) {  // mirror pair: the smaller id is the root
+        f = y < t ? y : t;
+        break;
+      }
+      if (pt != t || kt == kInf) {  // t's walk already ended (or t does not hook)
+        f = pt;
+        break;
+      }
+      int32_t a2, b2;
+      unpack_ends(kt, vb, &a2, &b2);
+      const int32_t qa = P[a2], qb = P[b2];
+      if (qa == qb) {  // both ends read the tree's root
+        f = qa;
+        break;
+      }
+      y = t;
+      ky = kt;
+      t = qa == y ? qb : qa;
       if (steps > n) {
         atomicOr(overflow, 4u);
+        f = y;
         break;
       }
     }
-    P[x0] = x;
+    if (f == (int32_t)v) continue;  // v is the tree's root
+    P[v] = f;
+    mst_key[v] = k;
   }
+  if (__ballot(any) && lane_id() == 0) had_edge[r] = 1u;
 }
 
 // --------------------------------------------------------- forest output
@@ -1287,6 +1339,7 @@ struct MstState {
   unsigned long long range_hi[kSlots];
   unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
   unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
+  unsigned int cand[kMaxRounds + 1];  // cand[r]: filter(r - 1) listed a candidate
 };
 constexpr size_t kStateHead = sizeof(MstState);
 constexpr size_t kCtlHead = offsetof(MstState, range_lo);  // what the round loop reads
@@ -1492,8 +1545,9 @@ int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int
 // number of enqueued rounds. Rounds after convergence exit at once.
 int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStream_t s) {
   const int vb = vbits_for(n);
-  const int gh = grid_for(n, kGB, resident_grid(k_hook, kGB));
-  const int gc = grid_for(n, kGB, resident_grid(k_compress, kGB));
+  // one vertex per thread: a thread's walks run one after another, and a
+  // deep walk then holds up no other vertex
+  const int gh = grid_for(n, kGB);
   int gr_cap = resident_grid(k_round, kGB);
   if (tune_value(EAC_TUNE_FILTER_BPCU) > 0) {
     int dev = 0, cus = 256;
@@ -1507,15 +1561,13 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
   for (; r < r0 + count && r < kMaxRounds; ++r) {
     rec.hook.push_back(prof_begin(EAC_PROF_MST_HOOK, 0, s));
     k_hook<<<gh, kGB, 0, s>>>(r, n, B.st->ctl, B.best[r & 1], B.best[(r + 1) & 1], B.comp,
-                              B.mst_key, B.st->had_edge, vb, &B.st->overflow);
+                              B.mst_key, B.st->had_edge, B.st->cand, vb, &B.st->overflow);
     EAC_LAUNCH_CHECK();
     prof_end(rec.hook.back(), s);
-    k_compress<<<gc, kGB, 0, s>>>(r, n, B.st->ctl, B.st->had_edge, B.comp, &B.st->overflow);
-    EAC_LAUNCH_CHECK();
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
     k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
                                B.st->ctl, B.st->had_edge, &B.st->plan, B.comp,
-                               B.best[(r + 1) & 1], vb, kMaxRounds,
+                               B.best[(r + 1) & 1], B.st->cand, vb, kMaxRounds,
                                prof_on() ? &B.st->rlog[0][0][0] : nullptr);
     EAC_LAUNCH_CHECK();
     prof_end(rec.round.back(), s);

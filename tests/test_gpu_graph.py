@@ -168,6 +168,34 @@ def test_boruvka_edge_cases(torch):
     assert r["n_mst"] == 1 and r["ids"][0].item() == 1 and r["weights"][0].item() == 4
 
 
+@pytest.mark.parametrize("n,direction,extra_deg", [(30_000, -1, 0), (30_000, 1, 0),
+                                                   (20_000, -1, 8)])
+def test_boruvka_long_pick_chains(torch, n, direction, extra_deg):
+    """A path whose weights fall (or rise) along it: every vertex picks the
+    same neighbour side, so one round's picks form a single tree of depth
+    n - 2 (the deepest walk a hook can take). With extra heavier random
+    edges the path is the light level of the split."""
+    from masters_final_amd.graph.mst import boruvka_mst
+    i = np.arange(n - 1, dtype=np.int64)
+    lo, hi = i, i + 1
+    w = (n - i) if direction < 0 else (i + 1)
+    if extra_deg:
+        rng = np.random.default_rng(7)
+        m = n * extra_deg // 2
+        a = rng.integers(0, n, m)
+        b = rng.integers(0, n, m)
+        keep = np.abs(a - b) > 1
+        a, b = np.minimum(a[keep], b[keep]), np.maximum(a[keep], b[keep])
+        key = np.unique(a * n + b)
+        a, b = key // n, key % n
+        lo, hi = np.concatenate([lo, a]), np.concatenate([hi, b])
+        w = np.concatenate([w, rng.integers(n, 4 * n, a.size)])
+    indptr, dest, weight = synth.csr_from_undirected(n, lo, hi, w.astype(np.int32))
+    g = {"n": n, "indptr": indptr, "dest": dest, "weight": weight}
+    r = boruvka_mst(dest, weight, indptr, None, want_weights=True, want_labels=True)
+    _check_forest(g, r)
+
+
 def test_boruvka_knn_200k(torch):
     from masters_final_amd.graph.mst import boruvka_mst
     g = synth.knn_graph(n=200_000, k=20, seed=5)
