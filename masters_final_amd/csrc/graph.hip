@@ -105,10 +105,8 @@ __device__ __forceinline__ void unpack_ends(uint64_t key, int vb, int32_t* lo, i
 // chunk_row[c] = row holding entry c*kGTile (the last row whose first entry
 // is <= it), for c in [0, nchunks]; entries of chunk c belong to rows
 // chunk_row[c] .. chunk_row[c+1].
-__global__ void k_chunk_rows(const int64_t* __restrict__ fe, int32_t n, int64_t m,
-                             int64_t nchunks, int32_t* __restrict__ chunk_row) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c > nchunks) return;
+__device__ __forceinline__ int32_t chunk_row_of(const int64_t* __restrict__ fe, int32_t n,
+                                                int64_t m, int64_t c) {
   int64_t e = c * kGTile;
   if (e > m - 1) e = m - 1;
   int32_t lo = 0, hi = n;  // upper_bound over fe[0:n]
@@ -116,8 +114,14 @@ __global__ void k_chunk_rows(const int64_t* __restrict__ fe, int32_t n, int64_t 
     int32_t mid = lo + ((hi - lo) >> 1);
     if (fe[mid] <= e) lo = mid + 1; else hi = mid;
   }
-  int32_t r = lo - 1;
-  chunk_row[c] = r < 0 ? 0 : r;
+  const int32_t r = lo - 1;
+  return r < 0 ? 0 : r;
+}
+__global__ void k_chunk_rows(const int64_t* __restrict__ fe, int32_t n, int64_t m,
+                             int64_t nchunks, int32_t* __restrict__ chunk_row) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > nchunks) return;
+  chunk_row[c] = chunk_row_of(fe, n, m, c);
 }
 
 struct ChunkCtx {
@@ -180,28 +184,25 @@ __device__ __forceinline__ int32_t row_of(const ChunkCtx& cx, int64_t e, const i
 
 // ----------------------------------------------------- weight range pass
 // Device-scope atomics on one address serialise (tens of ns each), so every
-// per-block result below lands in one of kSlots slots (blockIdx mod kSlots)
-// and consumers fold the slots.
-constexpr int kSlots = 64;
-static_assert(kSlots == kWave, "slot folds use one wave");
+// block of the range pass stores its own (lo, hi) pair and the plan folds
+// them (at most kRangeBlocks pairs).
+constexpr int kRangeBlocks = 2048;
 
+// Part `bid` of `nblk` of the weight range: rng[2 * bid] = lowest weight
+// word, rng[2 * bid + 1] = highest (kInf, 0 when the part holds no entry).
 template <typename W, bool HAS_OD>
-__global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weight,
-                                                      const int64_t* __restrict__ fe,
-                                                      const int32_t* __restrict__ od,
-                                                      int32_t n, int64_t m,
-                                                      const int32_t* __restrict__ chunk_row,
-                                                      int64_t nchunks,
-                                                      unsigned long long* range_lo,
-                                                      unsigned long long* range_hi) {
+__device__ void range_part(const W* __restrict__ weight, const int64_t* __restrict__ fe,
+                           const int32_t* __restrict__ od, int32_t n, int64_t m,
+                           const int32_t* __restrict__ chunk_row, int64_t nchunks, int bid,
+                           int nblk, unsigned long long* __restrict__ rng) {
   __shared__ int32_t s_beg[kMaxRows], s_end[kMaxRows];
   __shared__ uint64_t s_mm[2 * (kGB / kWave)];
   uint64_t lo = kInf, hi = 0;
   if (!HAS_OD) {
     // rows tile [fe[0], fe[n]): one flat pass over that range (within [0, m))
     const int64_t b = fe[0] > 0 ? fe[0] : 0, e = fe[n] < m ? fe[n] : m;
-    const int64_t tid = (int64_t)blockIdx.x * kGB + threadIdx.x;
-    const int64_t stride = (int64_t)gridDim.x * kGB;
+    const int64_t tid = (int64_t)bid * kGB + threadIdx.x;
+    const int64_t stride = (int64_t)nblk * kGB;
     int64_t s0 = b;
     if (sizeof(W) == 4 && (reinterpret_cast<uintptr_t>(weight + b) & 15) == 0 && e > b) {
       // 16-byte loads over the aligned body
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
       hi = w > hi ? w : hi;
     }
   } else {
-    for (int64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    for (int64_t c = bid; c < nchunks; c += nblk) {
       ChunkCtx cx = stage_chunk<HAS_OD>(c, m, fe, od, chunk_row, s_beg, s_end);
       __syncthreads();
 #pragma unroll
@@ -256,27 +257,20 @@ __global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weig
       lo = s_mm[2 * i] < lo ? s_mm[2 * i] : lo;
       hi = s_mm[2 * i + 1] > hi ? s_mm[2 * i + 1] : hi;
     }
-    if (lo != kInf) {
-      const int slot = blockIdx.x & (kSlots - 1);
-      atomicMin(&range_lo[slot], (unsigned long long)lo);
-      atomicMax(&range_hi[slot], (unsigned long long)hi);
-    }
+    rng[2 * bid] = lo;
+    rng[2 * bid + 1] = hi;
   }
 }
 
-// Folds the range slots with one wave: (kInf, 0) when there is no entry.
-__device__ __forceinline__ void fold_range(const unsigned long long* range_lo,
-                                           const unsigned long long* range_hi, uint64_t* lo,
-                                           uint64_t* hi) {
-  uint64_t l = range_lo[lane_id()], h = range_hi[lane_id()];
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    uint64_t a = __shfl_xor(l, off, kWave), b = __shfl_xor(h, off, kWave);
-    l = a < l ? a : l;
-    h = b > h ? b : h;
-  }
-  *lo = l;
-  *hi = h;
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_weight_range(const W* __restrict__ weight,
+                                                      const int64_t* __restrict__ fe,
+                                                      const int32_t* __restrict__ od,
+                                                      int32_t n, int64_t m,
+                                                      const int32_t* __restrict__ chunk_row,
+                                                      int64_t nchunks,
+                                                      unsigned long long* __restrict__ rng) {
+  range_part<W, HAS_OD>(weight, fe, od, n, m, chunk_row, nchunks, blockIdx.x, gridDim.x, rng);
 }
 
 constexpr int kHistBins = 4096;
@@ -298,39 +292,64 @@ struct MstPlan {
 // weight quantiles frac, frac * step, frac * step^2 ... (levels - 1 of them)
 // from kPlanSamples evenly spaced entries with a 4096-bin histogram of the
 // range: the level thresholds of the split. They only steer the split; any
-// values give the same forest.
+// values give the same forest. The samples are gathered by the set-up pass
+// (spread over many workgroups: one workgroup alone fetching 8192 scattered
+// lines takes several microseconds).
 constexpr int kPlanThreads = 256;
 constexpr int kPlanSamples = 8192;
+
+// Samples bid, bid + nblk, ... (in units of kGB) of the plan's evenly spaced
+// weight entries, as weight words.
+template <typename W>
+__device__ void sample_part(const W* __restrict__ weight, int64_t m, int bid, int nblk,
+                            unsigned long long* __restrict__ samples) {
+  if (m <= 0) return;
+  for (int i = bid * kGB + threadIdx.x; i < kPlanSamples; i += nblk * kGB) {
+    const int64_t e = (int64_t)((double)i * (double)m / kPlanSamples);
+    samples[i] = word_of<W>(weight[e < m ? e : m - 1]);
+  }
+}
+
 template <typename W>
 __global__ __launch_bounds__(kPlanThreads) void k_plan(
-    const W* __restrict__ weight, int64_t m, const unsigned long long* __restrict__ range_lo,
-    const unsigned long long* __restrict__ range_hi, int vb, int levels, double frac,
+    int64_t m, const unsigned long long* __restrict__ rng, int nrng,
+    const unsigned long long* __restrict__ samples, int vb, int levels, double frac,
     double step, MstPlan* __restrict__ plan) {
   __shared__ unsigned int h[kHistBins];
   __shared__ unsigned int s_tmp[kPlanThreads / kWave + 1];
-  __shared__ uint64_t s_lo, s_hi;
+  __shared__ uint64_t s_mm[2 * (kPlanThreads / kWave)];
   __shared__ int s_found[kLevels];
-  // the sample loads go out first (they do not need the range), all at once
   constexpr int kPer = kPlanSamples / kPlanThreads;
   const bool sample = levels > 1 && m > 0;  // uniform over the block
   uint64_t sw[kPer];
 #pragma unroll
-  for (int q = 0; q < kPer; ++q) {
-    const int64_t e = (int64_t)((double)(threadIdx.x + q * kPlanThreads) * (double)m / kPlanSamples);
-    sw[q] = sample ? word_of<W>(weight[e < m ? e : m - 1]) : 0ull;
-  }
-  if (threadIdx.x < kWave) {
-    uint64_t l, hh;
-    fold_range(range_lo, range_hi, &l, &hh);
-    if (threadIdx.x == 0) {
-      s_lo = l;
-      s_hi = hh;
+  for (int q = 0; q < kPer; ++q) sw[q] = sample ? samples[threadIdx.x + q * kPlanThreads] : 0ull;
+  {
+    uint64_t l = kInf, hh = 0;
+    for (int i = threadIdx.x; i < nrng; i += kPlanThreads) {
+      const uint64_t a = rng[2 * i], b = rng[2 * i + 1];
+      l = a < l ? a : l;
+      hh = b > hh ? b : hh;
+    }
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+      const uint64_t a = __shfl_xor(l, off, kWave), b = __shfl_xor(hh, off, kWave);
+      l = a < l ? a : l;
+      hh = b > hh ? b : hh;
+    }
+    if (lane_id() == 0) {
+      s_mm[2 * (threadIdx.x / kWave)] = l;
+      s_mm[2 * (threadIdx.x / kWave) + 1] = hh;
     }
   }
   if (threadIdx.x < kLevels) s_found[threadIdx.x] = kHistBins;
   for (int i = threadIdx.x; i < kHistBins; i += kPlanThreads) h[i] = 0;
   __syncthreads();
-  uint64_t wlo = s_lo, whi = s_hi;
+  uint64_t wlo = kInf, whi = 0;
+  for (int i = 0; i < kPlanThreads / kWave; ++i) {
+    wlo = s_mm[2 * i] < wlo ? s_mm[2 * i] : wlo;
+    whi = s_mm[2 * i + 1] > whi ? s_mm[2 * i + 1] : whi;
+  }
   if (wlo == kInf) { wlo = 0; whi = 0; }
   int wbits = 0;
   while (wbits < 64 && ((whi - wlo) >> wbits) != 0) ++wbits;
@@ -556,24 +575,6 @@ __device__ void uf_unite(int32_t* P, int32_t a, int32_t b) {
   }
 }
 
-// Root lookups of the Borůvka rounds. Hooks only ever point a root at
-// another current root along the round's lightest-edge choices, which
-// contain no cycle for a symmetric input; a walk longer than n therefore
-// means a graph passed as symmetric is not, and the walk stops with flag 4.
-__device__ int32_t find_live(int32_t* P, int32_t x, int32_t n, unsigned int* overflow) {
-  for (int32_t steps = 0;; ++steps) {
-    int32_t p = uf_ld(P, x);
-    if (p == x) return x;
-    int32_t gp = uf_ld(P, p);
-    if (gp == p) return p;
-    uf_st(P, x, gp);  // path halving
-    x = gp;
-    if (steps > n) {
-      atomicOr(overflow, 4u);
-      return x;
-    }
-  }
-}
 // ------------------------------------------------------ device round control
 // A round is hook(r) -> filter(r). The control state of round
 // r lives in ctl[r & 1]; every block of filter(r) derives round r+1's state
@@ -945,8 +946,14 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
     any = true;
     int32_t a, b;
     unpack_ends(k, vb, &a, &b);
-    const int32_t pa = P[a], pb = P[b];
-    int32_t t = pa == (int32_t)v ? pb : pa;  // the other side
+    // the other side (round 0: every vertex is a root, the ends are roots)
+    int32_t t;
+    if (r == 0) {
+      t = a == (int32_t)v ? b : a;
+    } else {
+      const int32_t pa = P[a], pb = P[b];
+      t = pa == (int32_t)v ? pb : pa;
+    }
     if (t == (int32_t)v) continue;
     int32_t y = (int32_t)v, f;
     uint64_t ky = k;
@@ -963,14 +970,18 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
       }
       int32_t a2, b2;
       unpack_ends(kt, vb, &a2, &b2);
-      const int32_t qa = P[a2], qb = P[b2];
-      if (qa == qb) {  // both ends read the tree's root
-        f = qa;
-        break;
-      }
       y = t;
       ky = kt;
-      t = qa == y ? qb : qa;
+      if (r == 0) {
+        t = a2 == y ? b2 : a2;
+      } else {
+        const int32_t qa = P[a2], qb = P[b2];
+        if (qa == qb) {  // both ends read the tree's root
+          f = qa;
+          break;
+        }
+        t = qa == y ? qb : qa;
+      }
       if (steps > n) {
         atomicOr(overflow, 4u);
         f = y;
@@ -985,46 +996,6 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
 }
 
 // --------------------------------------------------------- forest output
-// Tile-wise compaction of non-sentinel keys in index order (deterministic):
-// pass 1 counts per tile, a scan turns counts into offsets, pass 2 writes.
-__global__ __launch_bounds__(kGB) void k_count_keys(const uint64_t* __restrict__ keys,
-                                                    int64_t n, int32_t* __restrict__ tile_cnt) {
-  __shared__ uint32_t s_tmp[kGB / kWave + 1];
-  int64_t base = (int64_t)blockIdx.x * kGTile;
-  uint32_t c = 0;
-#pragma unroll
-  for (int j = 0; j < kGI; ++j) {
-    int64_t i = base + j * kGB + threadIdx.x;
-    if (i < n && keys[i] != kInf) ++c;
-  }
-  uint32_t total;
-  (void)block_excl_scan<uint32_t>(c, s_tmp, &total);
-  if (threadIdx.x == 0) tile_cnt[blockIdx.x] = (int32_t)total;
-}
-
-__global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__ keys,
-                                                    int64_t n,
-                                                    const int32_t* __restrict__ tile_off,
-                                                    uint64_t* __restrict__ out) {
-  __shared__ uint32_t s_tmp[kGB / kWave + 1];
-  // blocked layout inside the tile keeps index order
-  int64_t base = (int64_t)blockIdx.x * kGTile + (int64_t)threadIdx.x * kGI;
-  uint64_t v[kGI];
-  uint32_t c = 0;
-#pragma unroll
-  for (int j = 0; j < kGI; ++j) {
-    int64_t i = base + j;
-    v[j] = i < n ? keys[i] : kInf;
-    c += v[j] != kInf;
-  }
-  uint32_t total;
-  uint32_t pre = block_excl_scan<uint32_t>(c, s_tmp, &total);
-  int64_t o = (int64_t)tile_off[blockIdx.x] + pre;
-#pragma unroll
-  for (int j = 0; j < kGI; ++j)
-    if (v[j] != kInf) out[o++] = v[j];
-}
-
 // CSR position (in the smaller endpoint's row) and weight of each forest
 // edge: 8 lanes per edge scan the row 32 entries at a time (each load
 // instruction reads 8 consecutive entries of 8 rows), keeping the first
@@ -1032,12 +1003,12 @@ __global__ __launch_bounds__(kGB) void k_write_keys(const uint64_t* __restrict__
 // flight; more costs occupancy for no gain). n_f is bounded by
 // *d_nf when given (the count left on the device).
 template <typename W, bool HAS_OD>
-__global__ __launch_bounds__(kGB) void k_forest_positions(
+__device__ void positions_part(
     const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
     const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb,
     const MstPlan* __restrict__ plan, int64_t* __restrict__ ids, W* __restrict__ wout,
-    unsigned int* __restrict__ missing) {
+    unsigned int* __restrict__ missing, int bid, int nblk) {
   const uint64_t word_lo = plan->wlo;
   constexpr int kG = 8;  // lanes per forest edge
   constexpr int kU = 1;  // edges per lane group in flight
@@ -1047,8 +1018,8 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
     n_f = c < n_f ? c : n_f;
   }
   const int sub = threadIdx.x & (kG - 1);
-  const int64_t groups = (int64_t)gridDim.x * (kGB / kG);
-  for (int64_t g = ((int64_t)blockIdx.x * kGB + threadIdx.x) / kG; g < n_f; g += groups * kU) {
+  const int64_t groups = (int64_t)nblk * (kGB / kG);
+  for (int64_t g = ((int64_t)bid * kGB + threadIdx.x) / kG; g < n_f; g += groups * kU) {
     int32_t bcol[kU];
     uint64_t wk[kU];
     int64_t beg[kU], end[kU], pos[kU];
@@ -1132,6 +1103,197 @@ __global__ __launch_bounds__(kGB) void k_forest_positions(
   }
 }
 
+// --------------------------------------------------------- call tail
+// The tail of a Borůvka call in five launches, over tiles of kGTile
+// vertices (thread t of a tile's block holds its vertices t * kGI ... in
+// the blocked layout, so index order is kept by one block scan):
+//   count   forest keys per tile; with labels, every vertex's root (parent
+//           walk) and first[] = INT32_MAX
+//   first   first occurrence of every root (smallest vertex of each
+//           component); the last block scans the tile counts into offsets
+//           and the forest count
+//   write   forest keys in vertex order (slots past the count get ~0); with
+//           labels, fr[v] = first[root(v)], the rank of each first vertex
+//           inside its tile and the tile's number of first vertices
+//   pos     forest positions and weights; the last block scans the tiles'
+//           first-vertex counts
+//   label   labels[v] = rank of fr[v] among first vertices; block 0 packs
+//           the call's read-back record
+
+constexpr int kTailSlots = 256;  // per-block (root, first vertex) table
+
+// In-place exclusive scan of a[0 : len) by one block; *total = the sum.
+__device__ void block_scan_inplace(int32_t* __restrict__ a, int64_t len, int32_t* total) {
+  __shared__ int32_t s_tmp[kGB / kWave + 1];
+  int32_t carry = 0;
+  for (int64_t b0 = 0; b0 < len; b0 += kGB) {
+    const int64_t i = b0 + threadIdx.x;
+    const int32_t v = i < len ? a[i] : 0;
+    int32_t tot;
+    const int32_t pre = block_excl_scan<int32_t>(v, s_tmp, &tot);
+    if (i < len) a[i] = carry + pre;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
+template <bool LABELS>
+__global__ __launch_bounds__(kGB) void k_tail_count(const uint64_t* __restrict__ mst_key,
+                                                    int32_t n, const int32_t* __restrict__ P,
+                                                    int32_t* __restrict__ rootv,
+                                                    int32_t* __restrict__ first,
+                                                    int32_t* __restrict__ fcnt,
+                                                    unsigned int* __restrict__ overflow) {
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  const int64_t t0 = (int64_t)blockIdx.x * kGTile;
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j) {
+    const int64_t i = t0 + j * kGB + threadIdx.x;
+    if (i < n) {
+      c += mst_key[i] != kInf;
+      if (LABELS) {
+        int32_t x = P[i];
+        for (int32_t steps = 0;; ++steps) {
+          const int32_t p = P[x];
+          if (p == x) break;
+          x = p;
+          if (steps > n) {
+            atomicOr(overflow, 4u);
+            break;
+          }
+        }
+        rootv[i] = x;
+        first[i] = INT32_MAX;
+      }
+    }
+  }
+  uint32_t total;
+  (void)block_excl_scan<uint32_t>(c, s_tmp, &total);
+  if (threadIdx.x == 0) fcnt[blockIdx.x] = (int32_t)total;
+}
+
+// Grid: the tiles (first occurrences, labels only) + one scan block.
+template <bool LABELS>
+__global__ __launch_bounds__(kGB) void k_tail_first(const int32_t* __restrict__ rootv,
+                                                    int32_t n, int32_t* first,
+                                                    int32_t* __restrict__ fcnt, int64_t tiles,
+                                                    int32_t* __restrict__ d_nf) {
+  if ((int64_t)blockIdx.x == (int64_t)gridDim.x - 1) {
+    block_scan_inplace(fcnt, tiles, d_nf);
+    return;
+  }
+  if (!LABELS) return;
+  __shared__ int32_t s_key[kTailSlots], s_min[kTailSlots];
+  const int64_t t0 = (int64_t)blockIdx.x * kGTile;
+  for (int i = threadIdx.x; i < kTailSlots; i += kGB) {
+    s_key[i] = -1;
+    s_min[i] = INT32_MAX;
+  }
+  __syncthreads();
+  for (int j = 0; j < kGTile; j += kGB) {
+    const int64_t i = t0 + j + threadIdx.x;
+    const bool ok = i < n;
+    const int32_t l = ok ? rootv[i] : -1;
+    // lanes with the same root elect their lowest (smallest index) lane
+    uint64_t pending = __ballot(ok);
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      const int32_t ll = __shfl(l, leader, kWave);
+      pending &= ~__ballot(ok && l == ll);
+      if (lane_id() == leader) {
+        uint32_t h = ((uint32_t)ll * 2654435761u) >> 24;  // 8 bits: kTailSlots
+        bool done = false;
+        for (int p = 0; p < 8 && !done; ++p, h = (h + 1) & (kTailSlots - 1)) {
+          const int32_t k = atomicCAS(&s_key[h], -1, ll);
+          if (k == -1 || k == ll) {
+            atomicMin(&s_min[h], (int32_t)i);
+            done = true;
+          }
+        }
+        if (!done) atomicMin(first + ll, (int32_t)i);
+      }
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kTailSlots; q += kGB) {
+    const int32_t k = s_key[q];
+    if (k >= 0) atomicMin(first + k, s_min[q]);
+  }
+}
+
+template <bool LABELS>
+__global__ __launch_bounds__(kGB) void k_tail_write(
+    const uint64_t* __restrict__ mst_key, int32_t n, const int32_t* __restrict__ foff,
+    const int32_t* __restrict__ d_nf, uint64_t* __restrict__ fkeys,
+    const int32_t* __restrict__ rootv, const int32_t* __restrict__ first,
+    int32_t* __restrict__ fr, int32_t* __restrict__ rloc, int32_t* __restrict__ lcnt) {
+  __shared__ uint32_t s_tmp[kGB / kWave + 1];
+  const int64_t t0 = (int64_t)blockIdx.x * kGTile;
+  const int64_t base = t0 + (int64_t)threadIdx.x * kGI;
+  uint64_t v[kGI];
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j) {
+    const int64_t i = base + j;
+    v[j] = i < n ? mst_key[i] : kInf;
+    c += v[j] != kInf;
+  }
+  int32_t f[kGI];
+  uint32_t cl = 0;
+  if (LABELS) {
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      const int64_t i = base + j;
+      f[j] = i < n ? first[rootv[i]] : -1;
+      cl += f[j] == (int32_t)i;
+    }
+  }
+  const int32_t nf = *d_nf;
+  uint32_t total;
+  const uint32_t pre = block_excl_scan<uint32_t>(c, s_tmp, &total);
+  int64_t o = (int64_t)foff[blockIdx.x] + pre;
+#pragma unroll
+  for (int j = 0; j < kGI; ++j)
+    if (v[j] != kInf) fkeys[o++] = v[j];
+  // slots [t0, t0 + kGTile) past the forest count hold no key
+#pragma unroll
+  for (int j = 0; j < kGI; ++j) {
+    const int64_t q = t0 + j * kGB + threadIdx.x;
+    if (q >= nf && q < n) fkeys[q] = kInf;
+  }
+  if (LABELS) {
+    uint32_t tl;
+    uint32_t r = block_excl_scan<uint32_t>(cl, s_tmp, &tl);
+#pragma unroll
+    for (int j = 0; j < kGI; ++j) {
+      const int64_t i = base + j;
+      if (i >= n) continue;
+      fr[i] = f[j];
+      if (f[j] == (int32_t)i) rloc[i] = (int32_t)r++;
+    }
+    if (threadIdx.x == 0) lcnt[blockIdx.x] = (int32_t)tl;
+  }
+}
+
+// Grid: position blocks (none when no positions are wanted) + one block
+// that scans the tiles' first-vertex counts (labels only).
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_tail_pos(
+    const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
+    const int32_t* __restrict__ dest, const W* __restrict__ weight,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb,
+    const MstPlan* __restrict__ plan, int64_t* __restrict__ ids, W* __restrict__ wout,
+    unsigned int* __restrict__ missing, int32_t* __restrict__ lcnt, int64_t tiles,
+    int32_t* __restrict__ d_ncomp) {
+  if (lcnt && blockIdx.x == gridDim.x - 1) {
+    block_scan_inplace(lcnt, tiles, d_ncomp);
+    return;
+  }
+  positions_part<W, HAS_OD>(fkeys, n_f, d_nf, dest, weight, fe, od, vb, plan, ids, wout,
+                            missing, blockIdx.x, gridDim.x - (lcnt ? 1 : 0));
+}
+
 // ----------------------------------------------------- union-find kernels
 __global__ void k_iota(int32_t* p, int32_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -1201,13 +1363,6 @@ __global__ void k_uf_flatten(int32_t* P, int32_t n, int32_t* __restrict__ root,
 
 // Roots of the Borůvka parents (which need not satisfy parent <= child):
 // bounded walks, flag 4 on a cycle.
-__global__ void k_flatten_mst(int32_t* P, int32_t n, int32_t* __restrict__ root,
-                              unsigned int* __restrict__ overflow) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    root[i] = find_live(P, (int32_t)i, n, overflow);
-}
-
 __global__ void k_gather_rank(const int32_t* __restrict__ root,
                               const int32_t* __restrict__ rank, int32_t n,
                               int32_t* __restrict__ out) {
@@ -1335,14 +1490,12 @@ struct MstState {
   MstPlan plan;
   RoundCtl ctl[2];
   unsigned int had_edge[kMaxRounds + 1];
-  unsigned long long range_lo[kSlots];  // weight-word range, spread over slots
-  unsigned long long range_hi[kSlots];
   unsigned long long tot0[2][kLogSlots];                 // round-0 light / heavy keys
   unsigned long long rlog[kMaxRounds][2][kLogSlots];     // per round: keys in / kept
   unsigned int cand[kMaxRounds + 1];  // cand[r]: filter(r - 1) listed a candidate
 };
 constexpr size_t kStateHead = sizeof(MstState);
-constexpr size_t kCtlHead = offsetof(MstState, range_lo);  // what the round loop reads
+constexpr size_t kCtlHead = offsetof(MstState, tot0);  // what the round loop reads
 
 inline unsigned long long fold_slots(const unsigned long long* v) {
   unsigned long long t = 0;
@@ -1366,6 +1519,8 @@ struct MstBuffers {
   int32_t* tile_cnt;
   int32_t* flag;
   int32_t* rank;
+  unsigned long long* rng;      // per-block weight ranges (2 x kRangeBlocks)
+  unsigned long long* samples;  // the plan's weight samples (kPlanSamples)
   MstState* st;
 };
 
@@ -1390,6 +1545,8 @@ MstBuffers carve_mst(Arena& ar, int32_t n, int64_t m, int symmetric) {
   b.tile_cnt = ar.take<int32_t>((n + kGTile - 1) / kGTile + 2);  // + forest count, + labels count
   b.flag = ar.take<int32_t>(n);
   b.rank = ar.take<int32_t>(n);
+  b.rng = ar.take<unsigned long long>(2 * kRangeBlocks);
+  b.samples = ar.take<unsigned long long>(kPlanSamples);
   b.st = ar.take<MstState>(1);
   return b;
 }
@@ -1428,18 +1585,13 @@ struct ForestInfo {
   const MstPlan* plan;   // device plan (wlo, whi, wbits)
 };
 
-__global__ void k_state_init(MstState* st) {
-  // every field starts at zero except the ones set below
+// One block: every field starts at zero except the first control word.
+__device__ void state_init_block(MstState* st) {
   uint32_t* w = reinterpret_cast<uint32_t*>(st);
   const size_t words = sizeof(MstState) / 4;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
-       i += (size_t)gridDim.x * blockDim.x)
-    w[i] = 0u;
+  for (size_t i = threadIdx.x; i < words; i += blockDim.x) w[i] = 0u;
   __syncthreads();
-  if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < kSlots; i += blockDim.x) st->range_lo[i] = kInf;
-    if (threadIdx.x == 0) st->ctl[0].active = 1;
-  }
+  if (threadIdx.x == 0) st->ctl[0].active = 1;
 }
 
 inline int vbits_for(int32_t n) {
@@ -1488,14 +1640,33 @@ inline int mst_round_bound(int32_t n, int levels) {
   return r < kMaxRounds ? r : kMaxRounds;
 }
 
-// best[0] = ~0, mst_key = ~0, comp = identity (union-find parents).
-__global__ void k_vertex_init(int32_t n, unsigned long long* __restrict__ best,
-                              uint64_t* __restrict__ mst_key, int32_t* __restrict__ comp) {
-  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * blockDim.x) {
-    best[v] = kInf;
-    mst_key[v] = kInf;
-    comp[v] = (int32_t)v;
+// The independent set-up work of one call in one launch, by block range:
+// [0, nr) the weight range (plain CSR only: with out-degrees it needs the
+// chunk rows, and runs as its own pass) and the plan's samples; [nr, nr +
+// nv) best[0] = ~0, mst_key = ~0, comp = identity (union-find parents);
+// [nr + nv, nr + nv + nc) the chunk rows; the last block clears the state.
+template <typename W, bool HAS_OD>
+__global__ __launch_bounds__(kGB) void k_mst_setup(
+    const W* __restrict__ weight, const int64_t* __restrict__ fe, const int32_t* __restrict__ od,
+    int32_t n, int64_t m, int64_t nchunks, int nr, int nv, int nc,
+    int32_t* __restrict__ chunk_row, unsigned long long* __restrict__ rng,
+    unsigned long long* __restrict__ samples, unsigned long long* __restrict__ best,
+    uint64_t* __restrict__ mst_key, int32_t* __restrict__ comp, MstState* st) {
+  const int b = blockIdx.x;
+  if (b < nr) {
+    if (!HAS_OD) range_part<W, false>(weight, fe, od, n, m, chunk_row, nchunks, b, nr, rng);
+    sample_part<W>(weight, m, b, nr, samples);
+  } else if (b < nr + nv) {
+    for (int64_t v = (int64_t)(b - nr) * kGB + threadIdx.x; v < n; v += (int64_t)nv * kGB) {
+      best[v] = kInf;
+      mst_key[v] = kInf;
+      comp[v] = (int32_t)v;
+    }
+  } else if (b < nr + nv + nc) {
+    const int64_t c = (int64_t)(b - nr - nv) * kGB + threadIdx.x;
+    if (c <= nchunks) chunk_row[c] = chunk_row_of(fe, n, m, c);
+  } else {
+    state_init_block(st);
   }
 }
 
@@ -1513,20 +1684,23 @@ int mst_begin(const int32_t* dest, const W* weight, const int64_t* fe, const int
               int32_t n, int64_t m, int symmetric, MstBuffers& B, MstRecs& rec, hipStream_t s) {
   const int64_t nchunks = B.nchunks;
   const int vb = vbits_for(n);
-  k_state_init<<<1, 1024, 0, s>>>(B.st);
-  EAC_LAUNCH_CHECK();
-  k_chunk_rows<<<grid_for(nchunks + 1, 256), 256, 0, s>>>(fe, n, m, nchunks, B.chunk_row);
-  EAC_LAUNCH_CHECK();
   const int gchunks = grid_for(nchunks, 1, resident_grid(k_round0<W, HAS_OD>, kGB));
-  const int grange = grid_for(nchunks, 1, resident_grid(k_weight_range<W, HAS_OD>, kGB));
-  k_weight_range<W, HAS_OD><<<grange, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
-                                                    B.st->range_lo, B.st->range_hi);
+  int cap_r = resident_grid(k_weight_range<W, HAS_OD>, kGB);
+  cap_r = cap_r < kRangeBlocks ? cap_r : kRangeBlocks;
+  const int grange = grid_for(nchunks, 1, cap_r);
+  const int nv = grid_for(n, kGB, 4096), nc = grid_for(nchunks + 1, kGB);
+  k_mst_setup<W, HAS_OD><<<grange + nv + nc + 1, kGB, 0, s>>>(
+      weight, fe, od, n, m, nchunks, grange, nv, nc, B.chunk_row, B.rng, B.samples, B.best[0],
+      B.mst_key, B.comp, B.st);
   EAC_LAUNCH_CHECK();
+  if (HAS_OD) {
+    k_weight_range<W, HAS_OD><<<grange, kGB, 0, s>>>(weight, fe, od, n, m, B.chunk_row, nchunks,
+                                                      B.rng);
+    EAC_LAUNCH_CHECK();
+  }
   const SplitPlan sp = split_plan(n, m, symmetric);
-  k_plan<W><<<1, kPlanThreads, 0, s>>>(weight, m, B.st->range_lo, B.st->range_hi, vb, sp.levels,
-                                       sp.frac, sp.step, &B.st->plan);
-  EAC_LAUNCH_CHECK();
-  k_vertex_init<<<grid_for(n, 256, 4096), 256, 0, s>>>(n, B.best[0], B.mst_key, B.comp);
+  k_plan<W><<<1, kPlanThreads, 0, s>>>(m, B.rng, grange, B.samples, vb, sp.levels, sp.frac,
+                                       sp.step, &B.st->plan);
   EAC_LAUNCH_CHECK();
   rec.round0_bytes = (int64_t)(4 + sizeof(W)) * m + 8 * ((int64_t)n + 1) +
                      (HAS_OD ? 4 * (int64_t)n : 0) + 8 * (int64_t)n;
@@ -1615,13 +1789,15 @@ void mst_prof_bytes(const MstState& hs, int enq, int32_t n, int64_t nchunks, Mst
 
 // Forest keys in vertex order (slots past the count stay ~0); the count
 // goes to *d_nf on the device.
-int mst_forest(MstBuffers& B, int32_t n, int32_t* d_nf, Arena& ar, hipStream_t s) {
+int mst_forest(MstBuffers& B, int32_t n, int32_t* d_nf, hipStream_t s) {
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
-  EAC_TRY(fill<uint64_t>(B.fkeys, n, kInf, s));
-  k_count_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt);
+  k_tail_count<false><<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.comp, nullptr, nullptr,
+                                                  B.tile_cnt, &B.st->overflow);
   EAC_LAUNCH_CHECK();
-  EAC_TRY((scan_excl<int32_t, int32_t>(B.tile_cnt, B.tile_cnt, ftiles, 0, d_nf, ar, s)));
-  k_write_keys<<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, B.fkeys);
+  k_tail_first<false><<<1, kGB, 0, s>>>(nullptr, n, nullptr, B.tile_cnt, ftiles, d_nf);
+  EAC_LAUNCH_CHECK();
+  k_tail_write<false><<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, d_nf, B.fkeys,
+                                                  nullptr, nullptr, nullptr, nullptr, nullptr);
   EAC_LAUNCH_CHECK();
   return EAC_OK;
 }
@@ -1662,7 +1838,7 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   if (!graph_mode) mst_prof_bytes(hs, enq, n, B.nchunks, rec);
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
   int32_t* d_nf = B.tile_cnt + ftiles;
-  EAC_TRY(mst_forest(B, n, d_nf, ar, s));
+  EAC_TRY(mst_forest(B, n, d_nf, s));
   fi->vb = vb;
   fi->d_nf = d_nf;
   fi->plan = &B.st->plan;
@@ -1690,8 +1866,8 @@ int mst_core(const int32_t* dest, const W* weight, const int64_t* fe, const int3
 struct MstTail {
   int32_t n_f, missing, overflow, plan_err, wbits, active, overflow_rounds, rounds;
 };
-__global__ void k_pack_tail(const MstState* __restrict__ st, int enq,
-                            const int32_t* __restrict__ d_nf, MstTail* __restrict__ out) {
+__device__ void pack_tail(const MstState* __restrict__ st, int enq,
+                          const int32_t* __restrict__ d_nf, MstTail* __restrict__ out) {
   const RoundCtl c = st->ctl[enq & 1];
   out->n_f = *d_nf;
   out->missing = (int32_t)st->missing;
@@ -1703,39 +1879,71 @@ __global__ void k_pack_tail(const MstState* __restrict__ st, int enq,
   out->rounds = c.rounds;
 }
 
-// Forest positions and weights plus first-occurrence component labels,
-// enqueued after the forest keys (nothing is read back).
-template <typename W, bool HAS_OD>
-int mst_outputs(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-                int32_t n, MstBuffers& B, int32_t* d_nf, int64_t* ids, W* wout,
-                int32_t* labels, Arena& ar, hipStream_t s) {
-  const int vb = vbits_for(n);
-  // the forest count stays on the device: positions run over the bound n-1
-  if (n > 1 && (ids || wout)) {
-    k_forest_positions<W, HAS_OD><<<grid_for(8 * (int64_t)(n - 1), kGB,
-                                             resident_grid(k_forest_positions<W, HAS_OD>, kGB)),
-                                    kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od, vb,
-                                                 &B.st->plan, ids, wout, &B.st->missing);
-    EAC_LAUNCH_CHECK();
+__global__ void k_pack_tail(const MstState* __restrict__ st, int enq,
+                            const int32_t* __restrict__ d_nf, MstTail* __restrict__ out) {
+  pack_tail(st, enq, d_nf, out);
+}
+
+__global__ __launch_bounds__(kGB) void k_tail_label(const int32_t* __restrict__ fr,
+                                                    const int32_t* __restrict__ rloc,
+                                                    const int32_t* __restrict__ lpre, int32_t n,
+                                                    int32_t* __restrict__ labels,
+                                                    const MstState* __restrict__ st, int enq,
+                                                    const int32_t* __restrict__ d_nf,
+                                                    MstTail* __restrict__ tail) {
+  for (int64_t i = (int64_t)blockIdx.x * kGB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kGB) {
+    const int32_t u = fr[i];
+    labels[i] = lpre[u / kGTile] + rloc[u];
   }
+  if (tail && blockIdx.x == 0 && threadIdx.x == 0) pack_tail(st, enq, d_nf, tail);
+}
+
+// The forest keys, their positions and weights, first-occurrence component
+// labels and the read-back record, enqueued after the rounds (nothing is
+// read back here).
+template <typename W, bool HAS_OD>
+int mst_tail(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
+             int32_t n, MstBuffers& B, int32_t* d_nf, int64_t* ids, W* wout, int32_t* labels,
+             int enq, MstTail* tail, hipStream_t s) {
+  const int vb = vbits_for(n);
+  const int64_t ftiles = (n + kGTile - 1) / kGTile;
+  int32_t* rootv = B.roots[1];
+  int32_t* first = B.roots[0];
+  int32_t* fr = B.hook;
+  int32_t* lcnt = B.flag;
+  int32_t* d_ncomp = B.tile_cnt + ftiles + 1;  // component count after the forest count
+  const bool pos = n > 1 && (ids || wout);
+  // the forest count stays on the device: positions run over the bound n-1
+  const int gpos = pos ? grid_for(8 * (int64_t)(n - 1), kGB,
+                                  resident_grid(k_tail_pos<W, HAS_OD>, kGB))
+                       : 0;
   if (labels) {
-    // flatten the parents to roots, then number components by their
-    // smallest vertex (first occurrence)
-    int32_t* rootv = B.roots[1];
-    int32_t* first = B.roots[0];
-    int32_t* root = B.hook;
-    const int64_t ftiles = (n + kGTile - 1) / kGTile;
-    int g = grid_for(n, 256, 4096);
-    k_flatten_mst<<<g, 256, 0, s>>>(B.comp, n, rootv, &B.st->overflow);
+    k_tail_count<true><<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.comp, rootv, first,
+                                                   B.tile_cnt, &B.st->overflow);
     EAC_LAUNCH_CHECK();
-    EAC_TRY(fill<int32_t>(first, n, INT32_MAX, s));
-    k_first_occ<<<grid_for(n, kFoTile, 4096), 256, 0, s>>>(rootv, n, n, first, &B.st->missing);
+    k_tail_first<true><<<(int)ftiles + 1, kGB, 0, s>>>(rootv, n, first, B.tile_cnt, ftiles,
+                                                       d_nf);
     EAC_LAUNCH_CHECK();
-    k_first_flag<<<g, 256, 0, s>>>(rootv, first, n, n, root, B.flag);
+    k_tail_write<true><<<(int)ftiles, kGB, 0, s>>>(B.mst_key, n, B.tile_cnt, d_nf, B.fkeys,
+                                                   rootv, first, fr, B.rank, lcnt);
     EAC_LAUNCH_CHECK();
-    // component count in the slot after the forest count
-    EAC_TRY((scan_excl<int32_t, int32_t>(B.flag, B.rank, n, 0, B.tile_cnt + ftiles + 1, ar, s)));
-    k_gather_rank<<<g, 256, 0, s>>>(root, B.rank, n, labels);
+    k_tail_pos<W, HAS_OD><<<gpos + 1, kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od,
+                                                   vb, &B.st->plan, ids, wout, &B.st->missing,
+                                                   lcnt, ftiles, d_ncomp);
+    EAC_LAUNCH_CHECK();
+    k_tail_label<<<grid_for(n, kGB, 4096), kGB, 0, s>>>(fr, B.rank, lcnt, n, labels, B.st, enq,
+                                                        d_nf, tail);
+    EAC_LAUNCH_CHECK();
+  } else {
+    EAC_TRY(mst_forest(B, n, d_nf, s));
+    if (pos) {
+      k_tail_pos<W, HAS_OD><<<gpos, kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od, vb,
+                                                 &B.st->plan, ids, wout, &B.st->missing,
+                                                 nullptr, 0, nullptr);
+      EAC_LAUNCH_CHECK();
+    }
+    k_pack_tail<<<1, 1, 0, s>>>(B.st, enq, d_nf, tail);
     EAC_LAUNCH_CHECK();
   }
   return EAC_OK;
@@ -1763,13 +1971,10 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   MstTail ht;
   for (int attempt = 0;; ++attempt) {
     ar.used = mark;
-    EAC_TRY(mst_forest(B, n, d_nf, ar, s));
     int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
-    EAC_TRY((mst_outputs<W, HAS_OD>(dest, weight, fe, od, n, B, d_nf, ids, wout, labels, ar,
-                                    s)));
+    EAC_TRY((mst_tail<W, HAS_OD>(dest, weight, fe, od, n, B, d_nf, ids, wout, labels, enq, tail,
+                                 s)));
     prof_end(pr, s);
-    k_pack_tail<<<1, 1, 0, s>>>(B.st, enq, d_nf, tail);
-    EAC_LAUNCH_CHECK();
     EAC_TRY(d2h_sync(&ht, tail, sizeof(ht), s));
     if (ht.plan_err)
       return fail(EAC_ERR_KEY_WIDTH,
