@@ -996,22 +996,82 @@ __global__ __launch_bounds__(kGB) void k_hook(int r, int32_t n, const RoundCtl* 
 }
 
 // --------------------------------------------------------- forest output
+// Weight whose word (word_of) is `word` (exact except that word_of maps
+// -0.0f and +0.0f to one word; callers read a zero float from the graph).
+template <typename W>
+__device__ __forceinline__ W weight_of_word(uint64_t word) {
+  if (std::is_same<W, float>::value) {
+    uint32_t u = (uint32_t)word;
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    const float f = __uint_as_float(u);
+    W w;
+    memcpy(&w, &f, sizeof(W));
+    return w;
+  } else if (std::is_signed<W>::value) {
+    return (W)(int64_t)(word ^ (1ull << 63));
+  } else {
+    return (W)word;
+  }
+}
+
+// First entry of row [beg, end) with column `col` (and, with CHECK_W, weight
+// word `wk`) scanned by the kG lanes of a group, 32 entries per step (each
+// load instruction reads 8 consecutive entries of 8 rows); *matches gets
+// the number of entries with that column seen (every one when the row is
+// scanned to its end, i.e. without CHECK_W).
+template <typename W, bool CHECK_W, int kG>
+__device__ __forceinline__ int64_t row_find(const int32_t* __restrict__ dest,
+                                            const W* __restrict__ weight, int64_t beg,
+                                            int64_t end, int32_t col, uint64_t wk,
+                                            uint64_t word_lo, int sub, int* matches) {
+  constexpr int64_t kNone = INT64_MAX;
+  int64_t pos = kNone;
+  int cnt = 0;
+  for (int64_t p0 = beg; p0 < end && (!CHECK_W || pos == kNone); p0 += 4 * kG) {
+    int32_t d[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t e = p0 + q * kG + sub;
+      d[q] = e < end ? dest[e] : -1;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t e = p0 + q * kG + sub;
+      if (d[q] == col) {
+        ++cnt;
+        if (e < pos && (!CHECK_W || word_of<W>(weight[e]) - word_lo == wk)) pos = e;
+      }
+    }
+#pragma unroll
+    for (int off = kG / 2; off > 0; off >>= 1) {
+      const int64_t o = __shfl_xor(pos, off, kG);
+      pos = o < pos ? o : pos;
+    }
+  }
+#pragma unroll
+  for (int off = kG / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, kG);
+  *matches = cnt;
+  return pos;
+}
+
 // CSR position (in the smaller endpoint's row) and weight of each forest
-// edge: 8 lanes per edge scan the row 32 entries at a time (each load
-// instruction reads 8 consecutive entries of 8 rows), keeping the first
-// entry with the edge's column and weight (kU edges per lane group in
-// flight; more costs occupancy for no gain). n_f is bounded by
-// *d_nf when given (the count left on the device).
+// edge: 8 lanes per edge scan the row, keeping the first entry with the
+// edge's column and weight. A symmetric graph's keys come from the smaller
+// endpoint's row (round 0 lists the entries with dest > row), so there a
+// column that occurs once in the row is the edge: the row's columns are
+// scanned alone and the weights are read only for a repeated column; the
+// output weight is the key's (a zero float is read from the graph, -0.0f
+// and +0.0f sharing a key word). n_f is bounded by *d_nf when given (the
+// count left on the device).
 template <typename W, bool HAS_OD>
 __device__ void positions_part(
     const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
-    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, int symmetric,
     const MstPlan* __restrict__ plan, int64_t* __restrict__ ids, W* __restrict__ wout,
     unsigned int* __restrict__ missing, int bid, int nblk) {
   const uint64_t word_lo = plan->wlo;
   constexpr int kG = 8;  // lanes per forest edge
-  constexpr int kU = 1;  // edges per lane group in flight
   constexpr int64_t kNone = INT64_MAX;
   if (d_nf) {
     const int64_t c = *d_nf;
@@ -1019,85 +1079,32 @@ __device__ void positions_part(
   }
   const int sub = threadIdx.x & (kG - 1);
   const int64_t groups = (int64_t)nblk * (kGB / kG);
-  for (int64_t g = ((int64_t)bid * kGB + threadIdx.x) / kG; g < n_f; g += groups * kU) {
-    int32_t bcol[kU];
-    uint64_t wk[kU];
-    int64_t beg[kU], end[kU], pos[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int64_t i = g + u * groups;
-      pos[u] = kNone;
-      beg[u] = 0;
-      end[u] = 0;
-      bcol[u] = -2;
-      wk[u] = 0;
-      if (i < n_f) {
-        const uint64_t k = fkeys[i];
-        int32_t a, b;
-        unpack_ends(k, vb, &a, &b);
-        bcol[u] = b;
-        wk[u] = k >> (2 * vb);
-        beg[u] = fe[a];
-        end[u] = HAS_OD ? beg[u] + od[a] : fe[a + 1];
-      }
+  for (int64_t i = ((int64_t)bid * kGB + threadIdx.x) / kG; i < n_f; i += groups) {
+    const uint64_t k = fkeys[i];
+    int32_t a, b;
+    unpack_ends(k, vb, &a, &b);
+    const uint64_t wk = k >> (2 * vb);
+    const int64_t beg = fe[a];
+    const int64_t end = HAS_OD ? beg + od[a] : fe[a + 1];
+    int matches = 0;
+    int64_t p = kNone;
+    bool from_key = false;
+    if (symmetric) {
+      p = row_find<W, false, kG>(dest, weight, beg, end, b, wk, word_lo, sub, &matches);
+      from_key = matches == 1;
     }
-    // first 32 entries of every row, all loads issued together
-    int32_t d[kU][4];
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t e = beg[u] + q * kG + sub;
-        d[u][q] = e < end[u] ? dest[e] : -1;
-      }
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t e = beg[u] + q * kG + sub;
-        if (d[u][q] == bcol[u] && e < pos[u] && word_of<W>(weight[e]) - word_lo == wk[u])
-          pos[u] = e;
-      }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-#pragma unroll
-      for (int off = kG / 2; off > 0; off >>= 1) {
-        const int64_t o = __shfl_xor(pos[u], off, kG);
-        pos[u] = o < pos[u] ? o : pos[u];
-      }
-      // longer rows: further windows (trip count uniform over the group)
-      for (int64_t p0 = beg[u] + 4 * kG; pos[u] == kNone && p0 < end[u]; p0 += 4 * kG) {
-        int32_t dd[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int64_t e = p0 + q * kG + sub;
-          dd[q] = e < end[u] ? dest[e] : -1;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int64_t e = p0 + q * kG + sub;
-          if (dd[q] == bcol[u] && e < pos[u] && word_of<W>(weight[e]) - word_lo == wk[u])
-            pos[u] = e;
-        }
-#pragma unroll
-        for (int off = kG / 2; off > 0; off >>= 1) {
-          const int64_t o = __shfl_xor(pos[u], off, kG);
-          pos[u] = o < pos[u] ? o : pos[u];
-        }
-      }
-    }
+    if (!from_key)
+      p = row_find<W, true, kG>(dest, weight, beg, end, b, wk, word_lo, sub, &matches);
     if (sub == 0) {
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int64_t i = g + u * groups;
-        if (i >= n_f) continue;
-        int64_t p = pos[u];
-        if (p == kNone) {
-          atomicOr(missing, 1u);
-          p = 0;
-        }
-        if (ids) ids[i] = p;
-        if (wout) wout[i] = weight[p];
+      if (p == kNone) {
+        atomicOr(missing, 1u);
+        p = 0;
+      }
+      if (ids) ids[i] = p;
+      if (wout) {
+        W w = weight_of_word<W>(wk + word_lo);
+        if (!from_key || (std::is_same<W, float>::value && w == W(0))) w = weight[p];
+        wout[i] = w;
       }
     }
   }
@@ -1282,7 +1289,7 @@ template <typename W, bool HAS_OD>
 __global__ __launch_bounds__(kGB) void k_tail_pos(
     const uint64_t* __restrict__ fkeys, int64_t n_f, const int32_t* __restrict__ d_nf,
     const int32_t* __restrict__ dest, const W* __restrict__ weight,
-    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb,
+    const int64_t* __restrict__ fe, const int32_t* __restrict__ od, int vb, int symmetric,
     const MstPlan* __restrict__ plan, int64_t* __restrict__ ids, W* __restrict__ wout,
     unsigned int* __restrict__ missing, int32_t* __restrict__ lcnt, int64_t tiles,
     int32_t* __restrict__ d_ncomp) {
@@ -1290,8 +1297,8 @@ __global__ __launch_bounds__(kGB) void k_tail_pos(
     block_scan_inplace(lcnt, tiles, d_ncomp);
     return;
   }
-  positions_part<W, HAS_OD>(fkeys, n_f, d_nf, dest, weight, fe, od, vb, plan, ids, wout,
-                            missing, blockIdx.x, gridDim.x - (lcnt ? 1 : 0));
+  positions_part<W, HAS_OD>(fkeys, n_f, d_nf, dest, weight, fe, od, vb, symmetric, plan, ids,
+                            wout, missing, blockIdx.x, gridDim.x - (lcnt ? 1 : 0));
 }
 
 // ----------------------------------------------------- union-find kernels
@@ -1904,8 +1911,8 @@ __global__ __launch_bounds__(kGB) void k_tail_label(const int32_t* __restrict__ 
 // read back here).
 template <typename W, bool HAS_OD>
 int mst_tail(const int32_t* dest, const W* weight, const int64_t* fe, const int32_t* od,
-             int32_t n, MstBuffers& B, int32_t* d_nf, int64_t* ids, W* wout, int32_t* labels,
-             int enq, MstTail* tail, hipStream_t s) {
+             int32_t n, int symmetric, MstBuffers& B, int32_t* d_nf, int64_t* ids, W* wout,
+             int32_t* labels, int enq, MstTail* tail, hipStream_t s) {
   const int vb = vbits_for(n);
   const int64_t ftiles = (n + kGTile - 1) / kGTile;
   int32_t* rootv = B.roots[1];
@@ -1929,8 +1936,8 @@ int mst_tail(const int32_t* dest, const W* weight, const int64_t* fe, const int3
                                                    rootv, first, fr, B.rank, lcnt);
     EAC_LAUNCH_CHECK();
     k_tail_pos<W, HAS_OD><<<gpos + 1, kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od,
-                                                   vb, &B.st->plan, ids, wout, &B.st->missing,
-                                                   lcnt, ftiles, d_ncomp);
+                                                   vb, symmetric, &B.st->plan, ids, wout,
+                                                   &B.st->missing, lcnt, ftiles, d_ncomp);
     EAC_LAUNCH_CHECK();
     k_tail_label<<<grid_for(n, kGB, 4096), kGB, 0, s>>>(fr, B.rank, lcnt, n, labels, B.st, enq,
                                                         d_nf, tail);
@@ -1939,8 +1946,8 @@ int mst_tail(const int32_t* dest, const W* weight, const int64_t* fe, const int3
     EAC_TRY(mst_forest(B, n, d_nf, s));
     if (pos) {
       k_tail_pos<W, HAS_OD><<<gpos, kGB, 0, s>>>(B.fkeys, n - 1, d_nf, dest, weight, fe, od, vb,
-                                                 &B.st->plan, ids, wout, &B.st->missing,
-                                                 nullptr, 0, nullptr);
+                                                 symmetric, &B.st->plan, ids, wout,
+                                                 &B.st->missing, nullptr, 0, nullptr);
       EAC_LAUNCH_CHECK();
     }
     k_pack_tail<<<1, 1, 0, s>>>(B.st, enq, d_nf, tail);
@@ -1972,8 +1979,8 @@ int mst_impl(const int32_t* dest, const W* weight, const int64_t* fe, const int3
   for (int attempt = 0;; ++attempt) {
     ar.used = mark;
     int pr = prof_begin(EAC_PROF_MST_FINAL, 16 * (int64_t)n, s);
-    EAC_TRY((mst_tail<W, HAS_OD>(dest, weight, fe, od, n, B, d_nf, ids, wout, labels, enq, tail,
-                                 s)));
+    EAC_TRY((mst_tail<W, HAS_OD>(dest, weight, fe, od, n, symmetric, B, d_nf, ids, wout, labels,
+                                 enq, tail, s)));
     prof_end(pr, s);
     EAC_TRY(d2h_sync(&ht, tail, sizeof(ht), s));
     if (ht.plan_err)
