@@ -696,12 +696,22 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
   uint64_t* tc = ROUTE ? to + cc * kGTile : nullptr;
   int32_t out = 0, out_to = 0;
   constexpr int Q = kFilterKeys;
+  // the next step's keys are loaded while this step is filtered (they lie
+  // past every write of this step: survivors go to [out, base + Q x 64))
+  uint64_t kn[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int e = q * kWave + lane;
+    kn[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
+  }
   for (int base = 0; base < n_in; base += Q * kWave) {
     uint64_t k[Q];
 #pragma unroll
+    for (int q = 0; q < Q; ++q) k[q] = kn[q];
+#pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const int e = base + q * kWave + lane;
-      k[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
+      const int e = base + Q * kWave + q * kWave + lane;
+      kn[q] = e < n_in ? __builtin_nontemporal_load(lc + e) : kInf;
     }
     int32_t x[2 * Q];  // roots of the endpoints of the Q keys
 #pragma unroll
