@@ -217,40 +217,42 @@ void k_label_bf16x3(
   constexpr int NV = kBN * VPR / kLB;
   uint4 ph[NV], pl[NV];
   float pnc;
-  // Rows past the partition are stored as zeros with ||c||^2 = +inf, which
-  // keeps them out of the epilogue. Fetch and store are branch-free: the
-  // loads are unconditional (row clamped to the partition's last centroid;
-  // every partition has one, the host checks k >= 1), every thread loads a
-  // norm (kLB / kBN threads store the same value to each entry) and the
-  // defaults are chosen when the tile is stored. A register write to a load's
-  // destination next to the load, or a load skipped on some lanes, would make
-  // the wave wait for the new tile there, before the MFMAs, and undo the
-  // double buffering.
-  int64_t f_c0 = 0, f_end = 0;
+  // Rows past the partition repeat its last centroid (every partition has
+  // one; the host checks k >= 1) and get ||c||^2 = +inf, so their D is +inf
+  // and the epilogue never takes them. Fetch and store are branch-free: the
+  // loads are unconditional, every thread loads a norm (kLB / kBN threads
+  // store the same value to each entry) and the +inf default is chosen when
+  // the tile is stored. A register write to a load's destination next to
+  // the load, or a load skipped on some lanes, would make the wave wait for
+  // the new tile there, before the MFMAs, and undo the double buffering.
+  // Row indices are 32-bit offsets from the tile's first centroid.
+  int32_t f_rem = 0;
   auto fetch = [&](int64_t c0, int64_t kend) {
+    const int32_t rem = (int32_t)(kend - c0);  // >= 1 rows left in the range
+    const uint4* bh = reinterpret_cast<const uint4*>(Ch + c0 * KD);
+    const uint4* bl = reinterpret_cast<const uint4*>(Cl + c0 * KD);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int v = t + i * kLB;
-      const int64_t c = min(c0 + v / VPR, kend - 1);
-      ph[i] = reinterpret_cast<const uint4*>(Ch + c * KD)[v % VPR];
-      pl[i] = reinterpret_cast<const uint4*>(Cl + c * KD)[v % VPR];
+      const int32_t r = min(v / VPR, rem - 1);
+      ph[i] = bh[r * VPR + v % VPR];
+      pl[i] = bl[r * VPR + v % VPR];
     }
-    pnc = ncf[min(c0 + (t % kBN), kend - 1)];
-    f_c0 = c0;
-    f_end = kend;
+    pnc = ncf[c0 + min(t % kBN, rem - 1)];
+    f_rem = rem;
   };
-  // (a mask, not a select of the two values: a select of array elements can
-  // be turned into a load through a selected pointer, i.e. a stack array)
+  // (the stored values pass through a mask: stored straight from the load
+  // registers, the compiler keeps the prefetched tile in scratch memory)
   auto masked = [](uint4 a, uint32_t m) { return make_uint4(a.x & m, a.y & m, a.z & m, a.w & m); };
   auto stash = [&](int b) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int v = t + i * kLB;
-      const uint32_t m = f_c0 + v / VPR < f_end ? ~0u : 0u;
+      const uint32_t m = v / VPR < f_rem ? ~0u : 0u;
       *reinterpret_cast<uint4*>(&s_h[b][(v / VPR) * LROW + (v % VPR) * 8]) = masked(ph[i], m);
       *reinterpret_cast<uint4*>(&s_l[b][(v / VPR) * LROW + (v % VPR) * 8]) = masked(pl[i], m);
     }
-    s_nc[b][t % kBN] = f_c0 + (t % kBN) < f_end ? pnc : INFINITY;
+    s_nc[b][t % kBN] = (t % kBN) < f_rem ? pnc : INFINITY;
   };
   int p = 0, buf = 0;
   int64_t c0 = k_off[0];
