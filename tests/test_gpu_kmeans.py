@@ -296,22 +296,26 @@ def test_fit_float32_stop_rule(torch):
     assert km.iters_ == 2 and km.inertia_.tobytes() == inert.tobytes()
 
 
-@pytest.mark.parametrize("d", [20, 64, 100])
-def test_label_waves_knob_keeps_labels(torch, d):
+@pytest.mark.parametrize("d,k_list", [(20, None), (64, None), (100, None),
+                                      (64, [1, 63, 64, 65, 150, 7, 200]), (16, [33, 2, 129])])
+def test_label_waves_knob_keeps_labels(torch, d, k_list):
     """The waves-per-SIMD variants of the split-BF16 labeling kernel
-    (km_waves 2, 3, 4, built-in) give the same labels and dists."""
+    (km_waves 2, 3, 4, built-in) give the same labels and dists, for one
+    centroid set and for several (partition sizes below, at and past the
+    64-centroid tile)."""
     from masters_final_amd.cluster.KMeans import label_device
     from masters_final_amd.tune import get_knobs, set_knobs
     rng = np.random.default_rng(d)
     X = rng.normal(size=(3000, d)).astype(np.float32)
-    Cc = X[rng.choice(3000, 150, replace=False)].copy()
+    k = 150 if k_list is None else sum(k_list)
+    Cc = X[rng.choice(3000, k, replace=False)].copy()
     dX, dC = torch.from_numpy(X).cuda(), torch.from_numpy(Cc).cuda()
     saved = get_knobs()
     outs = []
     try:
         for w in (0, 2, 3, 4):
             set_knobs(km_waves=w)
-            lab, dist, _ = label_device(dX, dC, precision="bf16x3")
+            lab, dist, _ = label_device(dX, dC, k_list=k_list, precision="bf16x3")
             outs.append((lab.clone(), dist.clone()))
     finally:
         set_knobs(**saved)
