@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
 MST_SLOTS = {0: "k_round0 (CSR stream)", 1: "k_round (edge-list pass)",
-             2: "k_hook (union-find links)", 4: "forest output + labels"}
+             2: "k_hook (pick-tree walks)", 4: "forest output + labels"}
 
 
 def profile_traffic(kernel, summary="boruvka_mst_knn1m_hbm_traffic.md"):

@@ -61,7 +61,7 @@ int eac_abi_version(void);
 #define EAC_PROF_SLOTS 16
 #define EAC_PROF_MST_ROUND0 0 /* CSR stream: 8 B/entry + 16 B/vertex + 8 B/edge out */
 #define EAC_PROF_MST_ROUND 1  /* list pass: 16 B/key in (key + 2 root lookups) + 8 B/kept key */
-#define EAC_PROF_MST_HOOK 2   /* slot scan + links: 8 B per vertex */
+#define EAC_PROF_MST_HOOK 2   /* slot scan + pick-tree walks: 8 B per vertex */
 #define EAC_PROF_MST_JUMP 3   /* unused (links and root lookups need no jump pass) */
 #define EAC_PROF_MST_FINAL 4  /* forest output + labels */
 #define EAC_PROF_CC 5         /* union-find over the CSR: 4 B/entry + 8 B/vertex */
