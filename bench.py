@@ -178,7 +178,9 @@ def run_mst(args):
     out = mst_case(args, args.knn)
     if not args.no_eac:
         log("end-to-end EAC: batched+BF16 vs plain per-call FP32 (n=10,000, P=100)")
-        out["eac_small"] = eac_compare(cpu=not args.no_cpu)
+        # 10 host-timed repetitions: with fewer, launch and first-call
+        # costs of a ~2 ms pipeline show in the mean
+        out["eac_small"] = eac_compare(cpu=not args.no_cpu, reps=10)
         log(f"EAC speedup {out['eac_small']['speedup']:.2f}x")
     if not args.no_extras:
         out.update(extras(args))
