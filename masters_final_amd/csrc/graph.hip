@@ -25,18 +25,26 @@
 //    offsets of the chunk staged in LDS), so vertex degree skew does not
 //    unbalance waves. For a symmetric CSR the per-vertex minimum is an LDS
 //    segmented min with no global atomics for rows inside the chunk.
-//  * Later rounds: per surviving edge, look up both endpoint components and
-//    atomicMin the key into each component's slot (agent-scope 64-bit
-//    atomics, skipped when a plain read already shows a smaller key), and
-//    compact the cross-component edges into the next list (one atomic per
-//    block; list order is irrelevant because atomicMin over unique keys is
-//    order independent, so every output is deterministic).
-//  * Hooking writes a separate hook[] array; a pointer-jumping pass then
-//    updates every vertex's component and resets the per-root minima.
-//  * Component labels use a lock-free union-find (hook larger root under
-//    smaller root with atomicCAS, path halving) over the forest edges, so
-//    every root is its component's smallest vertex and first-occurrence
-//    labels follow from one scan of root flags.
+//  * Edge lists are chunk-local (chunk c of the CSR owns slots
+//    [2048 c, 2048 c + count[c]) of the light and heavy lists), so no pass
+//    needs a shared output counter; list order is irrelevant because
+//    atomicMin over unique keys is order independent, so every output is
+//    deterministic.
+//  * A round is two kernels. The hook: every root with a candidate walks
+//    its pick tree (root -> other side of its lightest edge -> ...) to the
+//    tree's mirror pair and points its parent at the tree's root, recording
+//    its edge in its forest slot. The filter: per listed edge, both ends'
+//    roots by two parent jumps, the kept edges' ends written back, the
+//    cross edges' keys merged per root in an LDS table and flushed with one
+//    atomicMin per root and workgroup. Weight levels (a light list filtered
+//    round after round, the heavy list routed in when a level is done) keep
+//    the early rounds small.
+//  * The call's tail gives forest keys in vertex order, their CSR positions
+//    and first-occurrence component labels (each root's smallest vertex by
+//    a per-tile LDS table and one atomicMin per root and tile). The
+//    connected-components entry points use a lock-free union-find instead
+//    (larger root under smaller root with atomicCAS, path halving), whose
+//    roots are component minima.
 #include <climits>
 #include <cstddef>
 #include <cstdio>
