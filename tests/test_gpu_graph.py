@@ -196,6 +196,25 @@ def test_boruvka_long_pick_chains(torch, n, direction, extra_deg):
     _check_forest(g, r)
 
 
+def test_boruvka_asymmetric_input_passed_as_symmetric_is_reported(torch):
+    """Rows that list an edge only from one side can make a round's picks a
+    cycle with no mirror pair (a picks (a,b), b picks (b,c), c picks (c,a)):
+    the pick-tree walk stops after n steps and the call reports the misuse
+    instead of returning a forest."""
+    from masters_final_amd._lib import EACError
+    from masters_final_amd.graph.mst import boruvka_mst
+    # row 0: (0,1); row 1: (1,2); row 2: (2,0), all weight 1
+    dest = np.array([1, 2, 0], np.int32)
+    w = np.array([1, 1, 1], np.int32)
+    with pytest.raises(EACError, match="not symmetric"):
+        boruvka_mst(dest, w, np.array([0, 1, 2, 3]), None, symmetric=True, want_labels=True)
+    # the triangle in the condensed form (symmetric=False: upper triangle,
+    # every entry an edge) gives its forest
+    r = boruvka_mst(np.array([1, 2, 2], np.int32), w, np.array([0, 2, 3, 3]), None,
+                    symmetric=False, want_labels=True)
+    assert r["n_mst"] == 2 and r["n_components"] == 1
+
+
 def test_boruvka_knn_200k(torch):
     from masters_final_amd.graph.mst import boruvka_mst
     g = synth.knn_graph(n=200_000, k=20, seed=5)
