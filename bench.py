@@ -137,6 +137,11 @@ def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu
     from masters_final_amd.EAC.eac import EnsembleCoassocGraph, eac_pipeline_graph
     gpipe = EnsembleCoassocGraph(dX, ks, iters=iters, precision="bf16x3")
     rg, tg = timed(lambda: eac_pipeline_graph(gpipe, dinit), reps, 1, torch)
+    # the whole path recorded: the cut as a second HIP graph sized on one
+    # eager pass (capacity nnz x 1.25, rounds tuned on the first replay)
+    from masters_final_amd.EAC.eac import EACPipelineGraph
+    rec = EACPipelineGraph(dX, ks, dinit, iters=iters, precision="bf16x3")
+    rr, tr = timed(lambda: rec.run(dinit), reps, 1, torch)
     rp, tp = timed(plain, reps, 1, torch)
     out = {"n": n, "partitions": P, "k_range": [kmin, kmax], "iters": iters,
            "batched_labeling": "FP64 direct difference: d = 2 is below the d >= 16 where the "
@@ -148,6 +153,10 @@ def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu
            "graph_replay_scope": "ensemble (3 labelings) + one-pass co-association replayed "
                                  "from one HIP graph, then one read-back of the matrix size "
                                  "and the forest + cut launched eagerly",
+           "recorded_ms": tr, "recorded_speedup_vs_plain": tp / tr,
+           "recorded_scope": "ensemble + co-association graph, then the forest and the cut "
+                             "as a recorded batched-cut graph; one read-back at the end",
+           "recorded_rounds": rec.rounds,
            "batched_pts_per_s": n / (tb * 1e-3), "plain_pts_per_s": n / (tp * 1e-3),
            "n_clusters": rb["n_clusters"], "nnz": rb["nnz"], "rescored": rb["n_rescored"]}
     lb = rb["labels"].cpu().numpy() if rb["labels"] is not None else np.zeros(0)
@@ -156,6 +165,9 @@ def eac_compare(n=10_000, P=100, kmin=50, kmax=100, iters=3, seed=0, reps=3, cpu
     lg = rg["labels"].cpu().numpy() if rg["labels"] is not None else np.zeros(0)
     out["graph_equals_batched"] = bool(np.array_equal(lb, lg) and
                                        rg["n_clusters"] == rb["n_clusters"])
+    lr = rr["labels"].cpu().numpy() if rr["labels"] is not None else np.zeros(0)
+    out["recorded_equals_batched"] = bool(np.array_equal(lb, lr) and
+                                          rr["n_clusters"] == rb["n_clusters"])
     if cpu:
         from oracle import oracle as O
         c0 = time.perf_counter()

@@ -151,7 +151,8 @@ class EnsembleCoassocGraph:
     back once. cap_nnz (default: every condensed pair, n (n - 1) / 2)
     bounds the matrix; the staging pool holds cap_nnz entries as well."""
 
-    def __init__(self, X, k_list, iters=3, tol=1e-4, precision="bf16x3", cap_nnz=None):
+    def __init__(self, X, k_list, iters=3, tol=1e-4, precision="bf16x3", cap_nnz=None,
+                 pad_tail=False):
         from ..cluster.KMeans import BatchedFitGraph
         t = L.torch()
         lib = L.load_library()
@@ -162,7 +163,17 @@ class EnsembleCoassocGraph:
         cap = int(cap_nnz) if cap_nnz else max(n * (n - 1) // 2, 1)
         dev = f.X.device
         self.cap = cap
-        self.indptr = t.empty(n + 1, dtype=t.int64, device=dev)
+        # pad_tail: the matrix as a graph of n + 1 vertices whose last one
+        # owns the unused capacity as self loops (indptr[n + 1] = cap, the
+        # entries past nnz set on the device after each build), the input
+        # shape of a recorded lifetime cut. The loops carry the largest
+        # dissimilarity (the max count): the cut's weight-level split samples
+        # weights evenly over all entries, and loops at the smallest weight
+        # would pull its light level below every real edge.
+        self.pad_tail = bool(pad_tail)
+        self.indptr_ext = t.empty(n + 2, dtype=t.int64, device=dev)
+        self.indptr_ext[n + 1] = cap
+        self.indptr = self.indptr_ext[:n + 1]
         self.indices = t.empty(cap, dtype=t.int32, device=dev)
         self.data = t.empty(cap, dtype=t.int32, device=dev)
         self.out = t.zeros(4, dtype=t.int64, device=dev)
@@ -176,10 +187,17 @@ class EnsembleCoassocGraph:
             self._enqueue_coassoc(lib)
         t.cuda.current_stream().wait_stream(side)
         t.cuda.synchronize()
+        if self.pad_tail:
+            self._slot = t.arange(cap, dtype=t.int64, device=dev)
         self.graph = t.cuda.CUDAGraph()
         with t.cuda.graph(self.graph):
             f._enqueue(lib)
             self._enqueue_coassoc(lib)
+            if self.pad_tail:
+                past = self._slot >= self.indptr[n]
+                self.indices.masked_fill_(past, n)
+                top = self.out[1].to(t.int32)  # on the device: no read-back
+                self.data.copy_(t.where(past, top, self.data))
 
     def _enqueue_coassoc(self, lib):
         f = self.fit
@@ -201,6 +219,103 @@ class EnsembleCoassocGraph:
                                f"small = 1, more than {self.cap} entries = 2, label >= k = 4)")
         fit = self.fit.result()
         return self.indptr, self.indices[:nnz], self.data[:nnz], mx, fit
+
+
+class EACPipelineGraph:
+    """The whole EAC hot path with no host round trip before its one
+    read-back: an EnsembleCoassocGraph replay (ensemble, co-association,
+    dissimilarities) followed on the same stream by the lifetime cut
+    recorded as a batched-cut HIP graph (eac_sl_lifetime_batched_graph_*:
+    the matrix as graph 0, a padding vertex that owns the unused matrix
+    capacity as self loops as graph 1, whose entries past nnz are set on the
+    device inside the first graph). The capacity is the nnz of one eager
+    pass from `init` times `margin`; a replay whose matrix outgrows it is
+    reported (error bit 2), never cut. The cut's round count is tuned on the
+    first replay (its rounds plus `round_margin`; a replay that needs more
+    is detected on the device and run again at the bound)."""
+
+    def __init__(self, X, k_list, init, iters=3, tol=1e-4, precision="bf16x3", n_clusters=0,
+                 margin=1.25, round_margin=3):
+        t = L.torch()
+        self._lib = L.load_library()
+        first = eac_pipeline(X, k_list, iters=iters, precision=precision, init=init,
+                             n_clusters=n_clusters)
+        cap = int(first["nnz"] * margin) + 4096
+        self.coassoc = EnsembleCoassocGraph(X, k_list, iters=iters, tol=tol, precision=precision,
+                                            cap_nnz=cap, pad_tail=True)
+        g = self.coassoc
+        n, P = g.fit.n, g.fit.P
+        dev = g.fit.X.device
+        self.n, self.P, self.n_clusters = n, P, int(n_clusters)
+        self.goff = t.tensor([0, n, n + 1], dtype=t.int64, device=dev)
+        self.labels = t.zeros(n + 1, dtype=t.int32, device=dev)
+        self.gout = t.zeros((2, 4), dtype=t.int64, device=dev)
+        self.summary = t.zeros(8, dtype=t.int64, device=dev)
+        nb = self._lib.eac_sl_lifetime_batched_workspace_bytes(n + 1, cap, 2)
+        self.ws = t.empty(int(nb), dtype=t.uint8, device=dev)
+        self._h = None
+        self.bound_rounds = int(self._lib.eac_mst_round_bound(n + 1, cap, 0))
+        self._capture(0)
+        self.run(init)  # the first replay tunes the round count
+        want = min(self._last_rounds + int(round_margin), self.bound_rounds)
+        if want < self.rounds:
+            self._capture(want)
+
+    def _capture(self, max_rounds):
+        from ..graph.singlelink import EAC_CUT_EAC
+        self.close()
+        g = self.coassoc
+        h = C.c_void_p()
+        L.torch().cuda.current_stream().synchronize()
+        L.check(self._lib.eac_sl_lifetime_batched_graph_create(
+            L.ptr(g.indices), L.ptr(g.data), L.weight_code(g.data.dtype), L.ptr(g.indptr_ext),
+            None, self.n + 1, g.cap, 0, L.ptr(self.goff), 2, EAC_CUT_EAC, float(self.P),
+            self.n_clusters, int(max_rounds), L.ptr(self.labels), L.ptr(self.gout),
+            L.ptr(self.summary), L.ptr(self.ws), self.ws.numel(), C.byref(h)))
+        self._h = h
+        self.rounds = int(self._lib.eac_sl_lifetime_batched_graph_rounds(h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            self._lib.eac_sl_lifetime_batched_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, init_centroids):
+        """Replays the pipeline from these initial centroids; returns
+        eac_pipeline's dict."""
+        t = L.torch()
+        g = self.coassoc
+        g.fit.init.copy_(L.as_device(init_centroids, t.float32))
+        g.graph.replay()
+        L.check(self._lib.eac_sl_lifetime_batched_graph_launch(self._h, L.stream_ptr()))
+        head = t.cat([g.out, self.summary[:5], self.gout[0]]).cpu().tolist()
+        nnz, mx, err = int(head[0]), int(head[1]), int(head[2])
+        if err:
+            raise RuntimeError(f"recorded co-association reported error bits {err} (pool too "
+                               f"small = 1, more than {g.cap} entries = 2, label >= k = 4)")
+        flags = int(head[8])
+        if flags & 16 and self.rounds < self.bound_rounds:
+            self._capture(0)  # the cut needed more rounds than recorded
+            L.check(self._lib.eac_sl_lifetime_batched_graph_launch(self._h, L.stream_ptr()))
+            head = t.cat([g.out, self.summary[:5], self.gout[0]]).cpu().tolist()
+            flags = int(head[8])
+        if flags:
+            raise RuntimeError(f"recorded lifetime cut: device error flags {flags}")
+        self._last_rounds = int(head[6])
+        if head[12] < 0:
+            raise ValueError("lifetime cut: the requested number of clusters cannot be cut "
+                             "from the forest")
+        nc, empty = int(head[9]), int(head[10])
+        fit = g.fit.result()
+        return {"n_clusters": nc, "labels": None if empty else self.labels[:self.n],
+                "n_rescored": fit["n_rescored"], "nnz": nnz, "ensemble_labels": fit["labels"],
+                "max_count": mx}
 
 
 def eac_pipeline_graph(graph, init, n_clusters=0):

@@ -64,3 +64,53 @@ def test_graph_pipeline_reports_small_capacity(torch):
     g = EnsembleCoassocGraph(torch.from_numpy(X).cuda(), ks, iters=3, cap_nnz=1000)
     with pytest.raises(RuntimeError, match="error bits"):
         g.run(torch.from_numpy(_init(X, ks, 1)).cuda())
+
+
+@pytest.mark.parametrize("n,d,P,kmin,kmax,nclu", [
+    (3000, 2, 30, 10, 40, 0),     # lifetime rule picks the number of clusters
+    (2500, 20, 12, 8, 60, 0),
+    (2000, 2, 16, 5, 30, 4),      # a requested number of clusters
+])
+def test_recorded_pipeline_equals_eager(torch, n, d, P, kmin, kmax, nclu):
+    """EACPipelineGraph (ensemble, co-association and the lifetime cut in
+    two recorded graphs, one read-back) gives eac_pipeline's clustering on
+    every replay, including replays whose matrix differs from the one the
+    capacity and the round count were sized on."""
+    from masters_final_amd.EAC.eac import EACPipelineGraph, eac_pipeline
+    X, _ = synth.gmm_points(n, d, 6, 5, mean_scale=10.0)
+    X = X.astype(np.float32)
+    ks = list(np.random.default_rng(6).integers(kmin, kmax + 1, P))
+    dX = torch.from_numpy(X).cuda()
+    pg = EACPipelineGraph(dX, ks, torch.from_numpy(_init(X, ks, 0)).cuda(), n_clusters=nclu)
+    assert 0 < pg.rounds <= pg.bound_rounds
+    for seed in (1, 2, 3, 0):
+        init = torch.from_numpy(_init(X, ks, seed)).cuda()
+        ref = eac_pipeline(dX, ks, init=init, n_clusters=nclu)
+        got = pg.run(init)
+        for key in ("n_clusters", "n_rescored", "nnz", "max_count"):
+            assert got[key] == ref[key], key
+        assert torch.equal(got["ensemble_labels"], ref["ensemble_labels"])
+        if ref["labels"] is None:
+            assert got["labels"] is None
+        else:
+            assert torch.equal(got["labels"], ref["labels"])
+        _ = dX.sum().item()  # eager work and a synchronisation between replays
+
+
+def test_recorded_pipeline_rounds_overflow_is_rerun(torch):
+    """A replay that needs more Borůvka rounds than were recorded is
+    detected on the device and cut again at the bound, same result."""
+    from masters_final_amd.EAC.eac import EACPipelineGraph, eac_pipeline
+    X, _ = synth.gmm_points(2000, 2, 5, 5, mean_scale=10.0)
+    X = X.astype(np.float32)
+    ks = list(np.random.default_rng(3).integers(5, 40, 20))
+    dX = torch.from_numpy(X).cuda()
+    init = torch.from_numpy(_init(X, ks, 1)).cuda()
+    pg = EACPipelineGraph(dX, ks, init)
+    pg._capture(1)  # one round: every real matrix needs more
+    got = pg.run(init)
+    assert pg.rounds == pg.bound_rounds
+    ref = eac_pipeline(dX, ks, init=init)
+    assert got["n_clusters"] == ref["n_clusters"]
+    if ref["labels"] is not None:
+        assert torch.equal(got["labels"], ref["labels"])
