@@ -327,7 +327,9 @@ def mst_case(args, knn):
     out["result"] = {"forest_edges": r["n_mst"], "forest_weight": total_w,
                      "components": r["n_components"], "rounds": r["rounds"],
                      "labels_checksum": labels_crc}
-    kname = {0: "k_round0", 1: "k_round"}.get(dom)
+    from masters_final_amd.tune import get_knobs
+    big = get_knobs()["filter_bpcu"] < 0  # the 1024-thread filter variant
+    kname = {0: "k_round0<int, false>", 1: "k_round<1024>" if big else "k_round<256>"}.get(dom)
     out["roofline"] = {"bound": "hbm", "kernel": MST_SLOTS[dom], "achieved": achieved,
                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                        # the committed counter summary covers the k=20 graph only

@@ -84,8 +84,9 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *                           edges per vertex x 10 (default 15; -1 = no split)
  *   EAC_TUNE_FIRST_BATCH    rounds enqueued before the first read-back
  *                           (default 3/4 of bits(n-1), at least 6)
- *   EAC_TUNE_FILTER_BPCU    blocks per compute unit of the edge-list filter
- *                           (default: as many as are resident)
+ *   EAC_TUNE_FILTER_BPCU    256-thread blocks per compute unit of the
+ *                           edge-list filter (default: as many as are
+ *                           resident); -1: one 1024-thread block per unit
  *   EAC_TUNE_CO_WINDOW      co-association LDS window in columns (even,
  *                           <= 32768; default: the row range when n is
  *                           smaller, else 32768)

@@ -631,14 +631,15 @@ __device__ __forceinline__ RoundCtl next_ctl(RoundCtl c, int r, unsigned int had
 // further slots costs more LDS atomics than the global ones it saves
 // (measured on the 1M-vertex k-NN graph: 8 probes 0.83 ms per call, 1 probe
 // 0.79; 512 to 1024 slots are equal, 256 or 2048 and more are slower).
-constexpr int kMinLogSlots = 10;
-constexpr int kMinSlots = 1 << kMinLogSlots;
+// The table has four slots per thread (1024 for the 256-thread filter
+// block, 4096 for the 1024-thread one).
 constexpr int kMinProbes = 1;  // table slots tried before the direct atomic
+template <int LOGS>
 __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* s_min,
                                            int32_t root, uint64_t key,
                                            unsigned long long* best) {
-  uint32_t h = ((uint32_t)root * 2654435761u) >> (32 - kMinLogSlots);
-  for (int p = 0; p < kMinProbes; ++p, h = (h + 1) & (kMinSlots - 1)) {
+  uint32_t h = ((uint32_t)root * 2654435761u) >> (32 - LOGS);
+  for (int p = 0; p < kMinProbes; ++p, h = (h + 1) & ((1u << LOGS) - 1)) {
     const int32_t k = atomicCAS(&s_root[h], -1, root);
     if (k == -1 || k == root) {
       atomicMin(&s_min[h], (unsigned long long)key);
@@ -652,6 +653,7 @@ __device__ __forceinline__ void min_insert(int32_t* s_root, unsigned long long* 
 // component's, typically) are first reduced to one lane, for up to two
 // distinct roots per call, so a hot root costs one LDS atomic per wave
 // instead of one per lane; a root held by one lane only skips the reduction.
+template <int LOGS>
 __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long long* s_min,
                                                 int32_t root, uint64_t key, bool active,
                                                 unsigned long long* best) {
@@ -669,11 +671,11 @@ __device__ __forceinline__ void wave_min_insert(int32_t* s_root, unsigned long l
       const uint64_t o = __shfl_xor(v, off, kWave);
       v = o < v ? o : v;
     }
-    if (lane_id() == leader) min_insert(s_root, s_min, R, v, best);
+    if (lane_id() == leader) min_insert<LOGS>(s_root, s_min, R, v, best);
     pend &= ~__ballot(mine);
     if (mine) active = false;
   }
-  if (active) min_insert(s_root, s_min, root, key, best);
+  if (active) min_insert<LOGS>(s_root, s_min, root, key, best);
 }
 
 // Keys per lane per step of the filter (their root lookups in flight together).
@@ -688,7 +690,7 @@ constexpr int kFilterKeys = EAC_FILTER_KEYS;
 // move to the same chunk of the (empty) light list and become candidates,
 // while heavier cross keys stay in the heavy chunk; otherwise `list` is the
 // light list, filtered in place.
-template <bool ROUTE>
+template <bool ROUTE, int LOGS>
 __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
                                              int32_t* __restrict__ cnt,
                                              uint64_t* __restrict__ to,
@@ -766,7 +768,7 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
     for (int q = 0; q < Q; ++q) *any_cand |= cand[q];
 #pragma unroll
     for (int i = 0; i < 2 * Q; ++i)
-      wave_min_insert(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
+      wave_min_insert<LOGS>(s_root, s_min, x[i], k[i / 2], cand[i / 2], best);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       if (ROUTE) {
@@ -795,7 +797,8 @@ __device__ __forceinline__ void filter_chunk(uint64_t* __restrict__ list,
 // Chunks a workgroup takes per batch from its share (b, b + G, b + 2G, ...).
 constexpr int kOwnBatch = kWave;
 
-__global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ light,
+template <int RB>
+__global__ __launch_bounds__(RB) void k_round(int r, uint64_t* __restrict__ light,
                                                uint64_t* __restrict__ heavy,
                                                int32_t* __restrict__ cnt_light,
                                                int32_t* __restrict__ cnt_heavy,
@@ -806,9 +809,12 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
                                                unsigned int* __restrict__ cand_next, int vb,
                                                int max_rounds,
                                                unsigned long long* __restrict__ rlog) {
-  __shared__ int32_t s_root[kMinSlots];
-  __shared__ unsigned long long s_min[kMinSlots];
-  __shared__ unsigned long long s_io[2 * (kGB / kWave)];
+  constexpr int LOGS = RB == 1024 ? 12 : 10;
+  constexpr int kSlots = 1 << LOGS;
+  static_assert(kSlots == 4 * RB, "four table slots per thread");
+  __shared__ int32_t s_root[kSlots];
+  __shared__ unsigned long long s_min[kSlots];
+  __shared__ unsigned long long s_io[2 * (RB / kWave)];
   __shared__ int32_t s_work[kOwnBatch], s_work_n[kOwnBatch];
   __shared__ int s_take, s_nwork;
   const RoundCtl prev = ctl[r & 1];
@@ -828,7 +834,7 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[(r + 1) & 1] = c;
   if (!c.active) return;
   const bool route = c.phase != prev.phase;
-  for (int i = threadIdx.x; i < kMinSlots; i += kGB) {
+  for (int i = threadIdx.x; i < kSlots; i += RB) {
     s_root[i] = -1;
     s_min[i] = kInf;
   }
@@ -870,10 +876,10 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
       const int64_t cc = blockIdx.x + (int64_t)s_work[w] * G;
       const int32_t n_in = s_work_n[w];
       if (route)
-        filter_chunk<true>(list, cnt, light, cnt_light, cc, n_in, lim, vb, P, s_root, s_min,
+        filter_chunk<true, LOGS>(list, cnt, light, cnt_light, cc, n_in, lim, vb, P, s_root, s_min,
                            best, &sum_in, &sum_out, &any_cand);
       else
-        filter_chunk<false>(list, cnt, nullptr, nullptr, cc, n_in, lim, vb, P, s_root, s_min,
+        filter_chunk<false, LOGS>(list, cnt, nullptr, nullptr, cc, n_in, lim, vb, P, s_root, s_min,
                             best, &sum_in, &sum_out, &any_cand);
     }
     __syncthreads();
@@ -883,13 +889,13 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
   // when a cached value (an upper bound of the current one) already wins.
   // Every thread reads its slots' cached values together (one round trip).
   {
-    constexpr int J = kMinSlots / kGB;
+    constexpr int J = kSlots / RB;
     int32_t rt[J];
     unsigned long long kk[J], cur[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      rt[j] = s_root[j * kGB + threadIdx.x];
-      kk[j] = s_min[j * kGB + threadIdx.x];
+      rt[j] = s_root[j * RB + threadIdx.x];
+      kk[j] = s_min[j * RB + threadIdx.x];
     }
 #pragma unroll
     for (int j = 0; j < J; ++j) cur[j] = rt[j] >= 0 ? best[rt[j]] : 0ull;
@@ -902,7 +908,7 @@ __global__ __launch_bounds__(kGB) void k_round(int r, uint64_t* __restrict__ lig
     if (lane == 0) { s_io[2 * w] = sum_in; s_io[2 * w + 1] = sum_out; }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int i = 1; i < kGB / kWave; ++i) { sum_in += s_io[2 * i]; sum_out += s_io[2 * i + 1]; }
+      for (int i = 1; i < RB / kWave; ++i) { sum_in += s_io[2 * i]; sum_out += s_io[2 * i + 1]; }
       const int slot = blockIdx.x & (kLogSlots - 1);
       if (sum_in) {
         atomicAdd(&rlog[(2 * r) * kLogSlots + slot], sum_in);
@@ -1754,12 +1760,18 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
   // one vertex per thread: a thread's walks run one after another, and a
   // deep walk then holds up no other vertex
   const int gh = grid_for(n, kGB);
-  int gr_cap = resident_grid(k_round, kGB);
-  if (tune_value(EAC_TUNE_FILTER_BPCU) > 0) {
+  // filter: 256-thread workgroups (as many per compute unit as fit, or the
+  // knob's count), or with the knob at -1 one 1024-thread workgroup per
+  // unit, whose root table is four times larger and whose flush reaches
+  // each root from four times fewer workgroups
+  const int bpcu = tune_value(EAC_TUNE_FILTER_BPCU);
+  const bool big = bpcu < 0;
+  int gr_cap = big ? resident_grid(k_round<1024>, 1024) : resident_grid(k_round<kGB>, kGB);
+  if (bpcu != 0) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int want = cus * tune_value(EAC_TUNE_FILTER_BPCU);
+    const int want = cus * (big ? 1 : bpcu);
     gr_cap = want < gr_cap ? want : gr_cap;
   }
   const int gr = grid_for(B.nchunks, 1, gr_cap);
@@ -1771,10 +1783,16 @@ int mst_rounds(MstBuffers& B, int32_t n, int r0, int count, MstRecs& rec, hipStr
     EAC_LAUNCH_CHECK();
     prof_end(rec.hook.back(), s);
     rec.round.push_back(prof_begin(EAC_PROF_MST_ROUND, 0, s));
-    k_round<<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy, B.nchunks,
-                               B.st->ctl, B.st->had_edge, &B.st->plan, B.comp,
-                               B.best[(r + 1) & 1], B.st->cand, vb, kMaxRounds,
-                               prof_on() ? &B.st->rlog[0][0][0] : nullptr);
+    if (big)
+      k_round<1024><<<gr, 1024, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy,
+                                        B.nchunks, B.st->ctl, B.st->had_edge, &B.st->plan,
+                                        B.comp, B.best[(r + 1) & 1], B.st->cand, vb, kMaxRounds,
+                                        prof_on() ? &B.st->rlog[0][0][0] : nullptr);
+    else
+      k_round<kGB><<<gr, kGB, 0, s>>>(r, B.light, B.heavy, B.cnt_light, B.cnt_heavy,
+                                      B.nchunks, B.st->ctl, B.st->had_edge, &B.st->plan,
+                                      B.comp, B.best[(r + 1) & 1], B.st->cand, vb, kMaxRounds,
+                                      prof_on() ? &B.st->rlog[0][0][0] : nullptr);
     EAC_LAUNCH_CHECK();
     prof_end(rec.round.back(), s);
   }

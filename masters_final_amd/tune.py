@@ -4,7 +4,7 @@ The knobs change speed only, never results: the Borůvka weight-level split
 (light undirected edges per vertex x 10 below the first threshold, -1 = no
 split; growth of that count per further level, -1 = one split only), the
 number of rounds enqueued before the first read-back, the blocks per compute
-unit of the edge-list filter, the co-association LDS window (columns) and the
+unit of the edge-list filter (-1: one 1024-thread block per unit), the co-association LDS window (columns) and the
 co-association member loads in flight per lane.
 `autotune_mst` times a grid of candidate settings
 on the caller's own graph with HIP events, checks that every candidate gives
@@ -29,7 +29,7 @@ DEFAULT_GRID = {
     "light_x10": (-1, 10, 15, 20, 30),
     "level_step": (-1, 0, 8),
     "first_batch": (0,),
-    "filter_bpcu": (0, 4),
+    "filter_bpcu": (0, 4, -1),
 }
 
 

@@ -298,13 +298,15 @@ def test_graph_from_edges_matches_oracle(torch, elt4):
 
 @pytest.mark.parametrize("light_x10,first_batch,bpcu,step,wmax", [
     (-1, 0, 0, 0, 60), (5, 2, 1, 2, 60), (40, 3, 2, 0, 60), (15, 30, 0, -1, 60),
-    (10, 0, 0, 2, 3), (5, 4, 0, 3, 100000)])
+    (10, 0, 0, 2, 3), (5, 4, 0, 3, 100000),
+    (10, 0, -1, 8, 60), (-1, 0, -1, 0, 60), (5, 2, -1, 3, 100000)])
 def test_boruvka_tuning_knobs_keep_the_forest(torch, light_x10, first_batch, bpcu, step, wmax):
     """Every tuning knob setting gives the canonical forest (knobs change
     speed only): no split, one split, up to four weight levels (also with
     fewer distinct weights than levels), tiny and large light fractions, a
     first round batch far too short (rounds continue after the read-back)
-    or long."""
+    or long; 256-thread filter workgroups or one 1024-thread workgroup
+    per compute unit."""
     from masters_final_amd import tune
     from masters_final_amd.graph.mst import boruvka_mst
     g = synth.random_graph(30000, 16, 21, wmax=wmax, n_parts=3)
