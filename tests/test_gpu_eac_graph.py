@@ -114,3 +114,16 @@ def test_recorded_pipeline_rounds_overflow_is_rerun(torch):
     assert got["n_clusters"] == ref["n_clusters"]
     if ref["labels"] is not None:
         assert torch.equal(got["labels"], ref["labels"])
+
+
+def test_recorded_pipeline_reports_small_capacity(torch):
+    """A matrix larger than the recorded capacity is reported (the
+    co-association's capacity error bits), never cut from a truncated
+    matrix."""
+    from masters_final_amd.EAC.eac import EACPipelineGraph
+    X, _ = synth.gmm_points(1500, 2, 3, 7, mean_scale=10.0)
+    X = X.astype(np.float32)
+    ks = [4, 5, 6, 7, 8]
+    init = torch.from_numpy(_init(X, ks, 1)).cuda()
+    with pytest.raises(RuntimeError, match="error bits [23]"):
+        EACPipelineGraph(torch.from_numpy(X).cuda(), ks, init, margin=0.5)
