@@ -573,6 +573,12 @@ def run_coassoc(args):
     return out
 
 
+# Borůvka knob candidates for the batched cut's 669M-entry union graph (a
+# few settings: each candidate costs about 0.15 s there)
+SL_BATCHED_GRID = {"light_x10": (0, 10, 20), "level_step": (0, 8), "first_batch": (0,),
+                   "filter_bpcu": (0, -1)}
+
+
 def run_sl_batched(args):
     """512 independent symmetric CSR graphs (vertex counts log-uniform in
     [1,000, vmax], 3-D k-NN graphs with k = 0.75 x a target mean degree
@@ -595,6 +601,18 @@ def run_sl_batched(args):
     d_dest, d_w, d_ip, goff = b["dest"], b["weight"], b["indptr"], b["goff"]
     N, M = b["n"], int(d_dest.numel())
     log(f"batch: {N} vertices, {M} entries, {time.perf_counter() - t0:.1f}s")
+    # Borůvka knobs timed on this union graph (the settings left by another
+    # workload's tuning are up to 30% slower here)
+    from masters_final_amd.tune import autotune_mst, set_knobs
+    if args.knobs:
+        knobs = {k: int(v) for k, v in (kv.split("=") for kv in args.knobs.split(","))}
+        set_knobs(**knobs)
+        tuning = {"knobs": knobs, "ms": None, "cached": None, "source": "--knobs"}
+    else:
+        tuning = autotune_mst(d_dest, d_w, d_ip, grid=SL_BATCHED_GRID, reps=2,
+                              use_cache=not args.retune)
+        tuning.pop("candidates", None)
+    log(f"tuning: {tuning['knobs']} (cached={tuning['cached']})")
     (lab, gout, info), ms = timed(lambda: lifetime_cut_batched(d_dest, d_w, d_ip, goff, rule=0),
                                   args.steps, args.warmup, torch)
     log(f"batched eager {ms:.2f} ms")
@@ -707,6 +725,7 @@ def run_sl_batched(args):
                      "graph_equal_to_batched": same_graph, "batched_equal_to_per_graph": same,
                      **info}
     out["roofline"] = roof
+    out["tuning"] = tuning
     out["cpu_baseline"] = cpu
     out["cpu_baseline_all_cores"] = cpu_all
     return out
