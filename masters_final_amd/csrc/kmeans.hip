@@ -720,8 +720,10 @@ __global__ void k_gather_flags(const uint64_t* __restrict__ regions, int64_t reg
 }
 
 // FP64 distance of every point to its labelled centroid (oracle arithmetic),
-// stored as float32 like the reference's dists array. LPP lanes share a
-// point (4, 8 or 16: about d/4):
+// stored as float32 like the reference's dists array. LPP = 4 lanes share a
+// point (the addition chain has d/4 steps whatever LPP is, and fewer lanes
+// per point run more points per wave: on 4M x 64 points, 303 us against 342
+// with 8 lanes, 500 with 16, 416 with 2 and 785 with 1):
 // lane s loads dimensions 4s..4s+3 of each (4 LPP)-dimension chunk of the
 // point's row and its centroid's row (coalesced) and squares the
 // differences in parallel; the running sum then visits the lanes in order
@@ -1694,15 +1696,8 @@ int kmeans_label(const float* X, int64_t n, int d, const float* C, const int64_t
   if (dists) {
     const int pgs = (P + kPG - 1) / kPG;
     const int py = pgs < 65535 ? pgs : 65535;
-    if (d <= 16)
-      k_exact_dists<4><<<dim3(grid_for(n, 64, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                          labels, dists);
-    else if (d <= 32)
-      k_exact_dists<8><<<dim3(grid_for(n, 32, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                          labels, dists);
-    else
-      k_exact_dists<16><<<dim3(grid_for(n, 16, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
-                                                                           labels, dists);
+    k_exact_dists<4><<<dim3(grid_for(n, 64, 16384), py), 256, 0, s>>>(X, n, d, C, B.k_off, P,
+                                                                        labels, dists);
     EAC_LAUNCH_CHECK();
   }
   if (inertia_out) EAC_TRY(np_sum_rows(dists, n, P, B.csum, inertia_out, s));
