@@ -288,7 +288,8 @@ class EACPipelineGraph:
 
     def run(self, init_centroids):
         """Replays the pipeline from these initial centroids; returns
-        eac_pipeline's dict."""
+        eac_pipeline's dict (its labels a copy; its ensemble labels the
+        recorded graph's buffer, overwritten by the next replay)."""
         t = L.torch()
         g = self.coassoc
         g.fit.init.copy_(L.as_device(init_centroids, t.float32))
@@ -313,7 +314,7 @@ class EACPipelineGraph:
                              "from the forest")
         nc, empty = int(head[9]), int(head[10])
         fit = g.fit.result()
-        return {"n_clusters": nc, "labels": None if empty else self.labels[:self.n],
+        return {"n_clusters": nc, "labels": None if empty else self.labels[:self.n].clone(),
                 "n_rescored": fit["n_rescored"], "nnz": nnz, "ensemble_labels": fit["labels"],
                 "max_count": mx}
 
