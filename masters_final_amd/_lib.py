@@ -130,6 +130,15 @@ def load_library() -> C.CDLL:
         if not _LIB_PATH.exists():
             raise EACUnavailable(
                 f"{_LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        # torch first: its HIP libraries ask for the runtime as libamdhip64.so
+        # and this library as libamdhip64.so.7, so loaded the other way round
+        # the process holds two HIP (and HSA) runtimes, and calls on the
+        # second fail (a pinned host allocation, for one). Loaded after
+        # torch's, the name resolves to that same runtime.
+        try:
+            import torch as _torch  # noqa: F401
+        except ImportError:  # standalone use of the C library
+            pass
         lib = C.CDLL(str(_LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -94,3 +94,19 @@ def test_round_bound_counts_weight_levels():
         assert lib.eac_mst_round_bound(0, 10, 1) == 0
     finally:
         tune.set_knobs(**before)
+
+
+def test_one_hip_runtime_when_the_library_loads_first():
+    """Loading the library before torch leaves one HIP runtime in the
+    process (torch's), not a second copy of it."""
+    import subprocess
+    import sys
+    code = ("from masters_final_amd import _lib; _lib.load_library(); import torch; "
+            "print(sorted({l.split()[-1] for l in open('/proc/self/maps') "
+            "if 'libamdhip64' in l}))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    import ast
+    libs = ast.literal_eval(out.stdout.strip().splitlines()[-1])
+    assert len(libs) == 1, libs
