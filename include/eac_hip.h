@@ -96,7 +96,8 @@ int eac_profile_read(int32_t slot, double* total_ms, int64_t* launches, int64_t*
  *   EAC_TUNE_CO_UNROLL      co-association member loads in flight per lane
  *                           while counting (4 by default, or 8)
  *   EAC_TUNE_CO_RANGES      co-association row ranges with a row counter
- *                           each (1..8, default 8: one per XCD)
+ *                           each (1..8, default 8: one per XCD; 1 below
+ *                           65536 rows)
  *   EAC_TUNE_KM_WAVES       waves per SIMD the split-BF16 labeling kernel
  *                           is compiled for (2, 3 or 4; default 3 for
  *                           d <= 64, else unconstrained) */

@@ -714,11 +714,15 @@ int co_window(int64_t n) {
 // Member loads in flight per lane in the pair visits (EAC_TUNE_CO_UNROLL:
 // 4 by default, or 8).
 inline int co_unroll() { return tune_value(EAC_TUNE_CO_UNROLL) == 8 ? 8 : 4; }
-// Row ranges of the co-association row loop (EAC_TUNE_CO_RANGES: 1..8,
-// default 8, one per XCD).
-inline int co_ranges() {
+// Row ranges of the co-association row loop (EAC_TUNE_CO_RANGES: 1..8;
+// default one per XCD, or one range below 65536 rows: there every
+// workgroup takes few rows and the end of the pass, where each row fetch
+// tries every range's counter, dominates; 10,000 rows x 100 partitions:
+// the EAC pipeline's graph replay 1.51 ms with one range, 1.74 with eight).
+inline int co_ranges(int64_t n) {
   const int r = tune_value(EAC_TUNE_CO_RANGES);
-  return r <= 0 || r > kMaxRanges ? kMaxRanges : r;
+  if (r > 0 && r <= kMaxRanges) return r;
+  return n < 65536 ? 1 : kMaxRanges;
 }
 inline size_t co_lds(int win, int P, bool rank = false) {
   return (size_t)(2 * co_bm_words(P) + win / 2 + (rank ? win : 0) + 9 * P + 1 +
@@ -853,7 +857,7 @@ extern "C" int eac_coassoc_count(const int32_t* labels, int64_t n, const int64_t
   const int win = co_window(n);
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(n), 1};
   CoOut o{};
   o.row_ctr = B.ctr;
   o.row_nnz = B.row_nnz;
@@ -898,7 +902,7 @@ int coassoc_fill(const int32_t* labels, int64_t n, const int64_t* h_koff, int32_
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
   int32_t* d_first = reinterpret_cast<int32_t*>(B.misc + 2);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, d_first, co_ranges(), 1};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, d_first, co_ranges(n), 1};
   CoOut o{};
   o.row_ctr = B.ctr;
   o.indptr = indptr;
@@ -1012,7 +1016,7 @@ extern "C" int eac_coassoc_estimate(const int32_t* labels, int64_t n, const int6
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, win, &nw, s);
   CoPlan pl{labels, n, P,  B.k_off,     B.members,    B.pos, B.cstart,
-            wtab,   nw, nullptr, co_ranges(), sample_stride};
+            wtab,   nw, nullptr, co_ranges(n), sample_stride};
   CoOut o{};
   o.row_ctr = B.ctr;
   o.row_nnz = B.row_nnz;
@@ -1074,7 +1078,7 @@ extern "C" int eac_coassoc_stage(const int32_t* labels, int64_t n, const int64_t
   EAC_TRY(fill<int32_t>((int32_t*)B.misc, 4, 0, s));
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, SL.win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(n), 1};
   CoOut o{};
   o.row_ctr = B.ctr;
   o.row_nnz = B.row_nnz;
@@ -1182,7 +1186,7 @@ extern "C" int eac_coassoc_graph(const int32_t* labels, int64_t n, const int64_t
   EAC_LAUNCH_CHECK();
   int nw = 0;
   const int32_t* wtab = build_win_table(B, n, P, ktot, SL.win, &nw, s);
-  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(), 1};
+  CoPlan pl{labels, n, P, B.k_off, B.members, B.pos, B.cstart, wtab, nw, nullptr, co_ranges(n), 1};
   CoOut o{};
   o.row_ctr = B.ctr;
   o.row_nnz = B.row_nnz;
