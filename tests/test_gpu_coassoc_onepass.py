@@ -56,6 +56,24 @@ def test_one_pass_condensed_equals_oracle(torch, n, P, kmin, kmax, window, strid
     assert mx == (int(cnt.max()) if cnt.size else 0)
 
 
+@pytest.mark.parametrize("ranges", [1, 3, 8])
+def test_one_pass_row_ranges_keep_the_matrix(torch, ranges):
+    """Any number of row ranges (the default is one below 65536 rows, one
+    per XCD above) gives the oracle's matrix, one-pass and two-pass."""
+    from masters_final_amd.tune import get_knobs, set_knobs
+    labels, ks = _labels(6000, 30, 4, 60, 11, unassigned=500)
+    saved = get_knobs()
+    try:
+        set_knobs(co_ranges=ranges)
+        a = _build(labels, ks, one_pass=True)
+        b = _build(labels, ks, one_pass=False)
+    finally:
+        set_knobs(**saved)
+    ip, ind, cnt = O.coassoc_condensed(labels, ks)
+    for gi, gc, gd in (a[:3], b[:3]):
+        assert np.array_equal(gi, ip) and np.array_equal(gc, ind) and np.array_equal(gd, cnt)
+
+
 def test_one_pass_full_rows_dissim_equals_two_pass(torch):
     from masters_final_amd.tune import get_knobs, set_knobs
     labels, ks = _labels(2500, 20, 3, 30, 5, unassigned=300)
